@@ -1,0 +1,181 @@
+#!/usr/bin/env python3
+"""Benchmark of the asqtad Dslash-and-solver hot path on one MI355X.
+
+Workload (asqtad 32^4): seeded random SU(3) links -> staggered phases -> fat7
+(5/8, -1/16, 1/64, -1/384) + Naik (-1/24), all generated on the GPU; one
+unit-variance Gaussian source on even sites; QLL mass 0.02 (m = 0.01 of
+D^+D + 4m^2).
+
+A step is one even-odd asqtad Dslash call over the 524,288 even sites (FP64,
+the reference's double-precision path).  Printed as one JSON line:
+  value       = FP64 EO Dslash GFLOP/s (1146 flop per output site)
+  roofline    = the FP64 Dslash kernel vs the 8 TB/s HBM peak (2400 B/site)
+  extra       = FP32 / FP16 Dslash rates and the FP64-CG vs mixed-precision
+                CG solves/s at FP64 true relative residual 1e-10
+  cpu_baseline= the oracle's OpenMP Dslash (the dslash2T loop nest) on this
+                host, on the same lattice with random links
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FLOPS_ASQTAD = 1146          # per EO output site
+BYTES_SITE = {0: 2400, 1: 1200, 2: 608}   # algorithmic bytes per EO output site
+HBM_PEAK_GBS = 8000.0
+DIMS = (32, 32, 32, 32)
+QLL_MASS = 0.02
+RSQ = 1e-20
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cg", action="store_true", help="skip the solver comparison")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--dims", type=str, default=None, help="override lattice, e.g. 16,16,16,16")
+    return ap.parse_args()
+
+
+def cpu_baseline(dims, nsec_target=10.0):
+    """Oracle OpenMP restatement of QLL's dslash2T loop (256-site blocks, 16
+    terms), timed on the host cores, on random SU(3) links of the same shape."""
+    import oracle as O
+    import numpy as np
+    nth = min(os.cpu_count() or 1, 16)
+    os.environ.setdefault("OMP_NUM_THREADS", str(nth))
+    t0 = time.time()
+    u = O.random_su3_field(dims, 2024)
+    links = O.stag_doublestore(dims, u, u)     # 16 link terms per site
+    del u
+    y = O.gauss_vector(dims, 7, 100, parity=1)
+    gen_s = time.time() - t0
+    t1, nthreads = O.time_dslash(dims, links, y, 1)      # warm-up + size the sample
+    nrep = max(1, min(50, int(nsec_target / max(t1, 1e-6))))
+    t, nthreads = O.time_dslash(dims, links, y, nrep)
+    vh = O.volume(dims) // 2
+    return {
+        "value": round(FLOPS_ASQTAD * vh / t / 1e9, 3),
+        "unit": "GFLOP/s",
+        "cores": nthreads,
+        "kind": "port",
+        "sample": "%d EO Dslash calls at %s (%.2f s each; links generated in %.1f s)" % (
+            nrep, "x".join(map(str, dims)), t, gen_s),
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (every box here has one GPU); "
+              "--gpus %d is not supported" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import qll_amd as Q
+    dims = tuple(int(x) for x in args.dims.split(",")) if args.dims else DIMS
+    Q.init()
+    lat = Q.Lattice(dims)
+    vh = lat.V // 2
+    t0 = time.time()
+    L64 = Q.asqtad_links_device(lat, seed=1)
+    L32 = L64.converted(Q.F32)
+    L16 = L64.converted(Q.F16)
+    Q.sync()
+    setup_s = time.time() - t0
+    print("setup (links: SU(3)+phases+fat7+Naik+doublestore, 3 precisions): %.2f s" % setup_s,
+          file=sys.stderr, flush=True)
+
+    rates = {}
+    prof = {}
+    for prec, L in ((0, L64), (1, L32), (2, L16)):
+        yin = Q.Field(lat, prec, 3, 1, Q.ODD).gaussian(5, 100)
+        out = Q.Field(lat, prec, 3, 1, Q.EVEN)
+        Q.time_dslash(L, out, yin, args.warmup)
+        Q.sync()
+        ms = Q.time_dslash(L, out, yin, args.steps)
+        Q.sync()
+        rates[prec] = ms
+        gbs = BYTES_SITE[prec] * vh / (ms * 1e-3) / 1e9
+        prof[prec] = {"ms_per_call": round(ms, 5),
+                      "GFLOP/s": round(FLOPS_ASQTAD * vh / (ms * 1e-3) / 1e9, 1),
+                      "GB/s": round(gbs, 1),
+                      "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+        print("dslash prec=%d: %s" % (prec, prof[prec]), file=sys.stderr, flush=True)
+
+    ms64 = rates[0]
+    gflops = FLOPS_ASQTAD * vh / (ms64 * 1e-3) / 1e9
+    achieved = BYTES_SITE[0] * vh / (ms64 * 1e-3) / 1e9
+
+    solver = None
+    if not args.no_cg:
+        b = Q.Field(lat, Q.F64, 3, 1, Q.EVEN).gaussian(11, 100)
+        x = Q.Field(lat, Q.F64, 3, 1, Q.EVEN)
+        Ap = Q.Field(lat, Q.F64, 3, 1, Q.EVEN)
+        t = Q.Field(lat, Q.F64, 3, 1, Q.ODD)
+        bsq = b.norm2()
+
+        def true_rsq():
+            Q.stag_normal(L64, Ap, x, t, QLL_MASS)
+            Ap.axpby(1.0, b, -1.0)
+            return Ap.norm2() / bsq
+
+        solver = {}
+        Q.sync()
+        t0 = time.time()
+        info = Q.stag_cg(L64, x, b, QLL_MASS, RSQ, 50000)
+        Q.sync()
+        dt = time.time() - t0
+        solver["fp64_cg"] = {"seconds": round(dt, 4), "iterations": info["iterations"],
+                             "true_rel_resid": true_rsq() ** 0.5, "solves_per_s": round(1 / dt, 4)}
+        print("fp64 cg: %s" % solver["fp64_cg"], file=sys.stderr, flush=True)
+        for name, Ls in (("mixed_fp32", L32), ("mixed_fp16", L16)):
+            x.zero()
+            Q.sync()
+            t0 = time.time()
+            info = Q.stag_cg_mixed(L64, Ls, x, b, QLL_MASS, RSQ, 50000, 0.1)
+            Q.sync()
+            dt = time.time() - t0
+            solver[name] = {"seconds": round(dt, 4), "iterations": info["iterations"],
+                            "reliable_updates": info["reliable_updates"],
+                            "true_rel_resid": true_rsq() ** 0.5, "solves_per_s": round(1 / dt, 4),
+                            "speedup_vs_fp64": round(solver["fp64_cg"]["seconds"] / dt, 3)}
+            print("%s: %s" % (name, solver[name]), file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(dims)
+        print("cpu baseline: %s" % cpu, file=sys.stderr, flush=True)
+
+    line = {
+        "metric": "asqtad Dslash GFLOP/s + %HBM roofline; mixed-prec CG solves/s vs FP64 CG @1e-10",
+        "value": round(gflops, 2),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms64, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded SU(3) + fat7/Naik generated on the GPU)",
+        "config": {"workload": "asqtad even-odd Dslash, %s lattice (%d output sites), FP64, 1 rhs"
+                               % ("x".join(map(str, dims)), vh),
+                   "lattice": list(dims), "mass_qll": QLL_MASS, "flops_per_site": FLOPS_ASQTAD},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+        "cpu_baseline": cpu,
+        "dslash": {"f64": prof[0], "f32": prof[1], "f16": prof[2]},
+        "solvers": solver,
+        "device": Q.device_name(),
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,179 @@
+/*
+ * qll_amd.h -- device-resident C ABI of libqll_amd (MI355X / gfx950).
+ *
+ * The QLL drop-in (qll.h, solvers.h and the qop* entry points) is built on
+ * this layer; it is also the interface for callers that keep their fields in
+ * HBM between calls (benchmarks, batched solvers, mixed precision).  Plain C
+ * types only: opaque handles, pointers and sizes.
+ *
+ * Which reference interface each group replaces is noted per group; entry
+ * points marked "no reference counterpart" are extensions for device-resident
+ * callers.
+ *
+ * Errors: functions returning int give 0 on success and -1 on failure;
+ * functions returning a handle give NULL on failure.  qamd_last_error()
+ * describes the most recent failure.  Nothing here ever falls back to a host
+ * computation: when no usable GPU or kernel image is present every entry point
+ * fails.
+ */
+#ifndef QLL_AMD_H
+#define QLL_AMD_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { QAMD_F64 = 0, QAMD_F32 = 1, QAMD_F16 = 2 };
+enum { QAMD_EVEN = 0, QAMD_ODD = 1 };
+
+typedef struct qamd_lattice qamd_lattice;   /* 4-d periodic lattice       */
+typedef struct qamd_field qamd_field;       /* one parity, nrhs x ncomp    */
+typedef struct qamd_gauge qamd_gauge;       /* 4 directions of 3x3 links   */
+typedef struct qamd_links qamd_links;       /* doublestored Dslash links   */
+
+/* ---- runtime (replaces the OpenMP team set-up of thread.h:20-29 and the
+ *      aligned allocator lib/common.c:9-28 for device data) ---- */
+const char *qamd_last_error(void);
+int qamd_init(void);
+int qamd_sync(void);
+int qamd_device_name(char *buf, int len);
+void *qamd_malloc(size_t bytes);
+void qamd_free(void *p);
+int qamd_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int qamd_memcpy_d2h(void *dst, const void *src, size_t bytes);
+
+/* ---- lattice (replaces layoutSetup lib/layout.c:21-80 for the device
+ *      layout; all extents even and >= 4) ---- */
+qamd_lattice *qamd_lattice_create(const int dims[4]);
+void qamd_lattice_destroy(qamd_lattice *lat);
+int64_t qamd_lattice_sites(const qamd_lattice *lat);
+
+/* ---- fields (replaces fieldNew/fieldFree/fieldZero, lib/field.c:3-37).
+ *      ncomp = 3 (staggered colour vector) or 12 (Wilson spinor). ---- */
+qamd_field *qamd_field_create(qamd_lattice *lat, int prec, int ncomp, int nrhs, int parity);
+void qamd_field_destroy(qamd_field *f);
+int qamd_field_zero(qamd_field *f);
+int qamd_field_copy(qamd_field *dst, const qamd_field *src);  /* converts precision */
+/* host <-> device in lexicographic full-lattice order (x fastest), nreal =
+ * 2*ncomp doubles per site per rhs, rhs slowest; only the field's parity is
+ * read/written */
+int qamd_field_from_lex(qamd_field *f, const double *host);
+int qamd_field_to_lex(const qamd_field *f, double *host);
+/* Gaussian source, unit variance per complex entry, counter-based RNG keyed by
+ * (seed, stream+rhs, global lexicographic site) -- no reference counterpart */
+int qamd_field_gaussian(qamd_field *f, uint64_t seed, uint64_t stream);
+/* squared norm (per rhs) and real dot, accumulated in FP64 (replaces
+ * fieldNorm2 lib/field.c:50-63 and r_veq_norm2_X lib/vla.c:656-700) */
+int qamd_field_norm2(const qamd_field *f, double *out /* [nrhs] */);
+int qamd_field_redot(const qamd_field *a, const qamd_field *b, double *out /* [nrhs] */);
+/* y = a*x + b*y (all rhs), precisions may differ; replaces the BLAS-1 set
+ * X_veq_X_plus_r_times_X / X_eq_scale_plus_X (lib/vla.c:760-825) */
+int qamd_field_axpby(double a, const qamd_field *x, double b, qamd_field *y);
+
+/* ---- QLL host layout <-> device (replaces the layout/shift machinery of
+ *      lib/layout.c:120-205; reads innerGeom/outerGeom/innerCb of the QLL
+ *      Layout, so any VLEN is accepted) ---- */
+typedef struct {
+  int physGeom[4], innerGeom[4], outerGeom[4];
+  int nSitesOuter, nSitesInner, nEvenOuter, innerCb, innerCbDir;
+} qamd_qll_layout;
+/* host array in QLL order a[(nelem*NI)*oi + NI*e + ii], element type double
+ * (host_prec QAMD_F64) or float (QAMD_F32); nelem = 2*ncomp */
+int qamd_field_from_qll(qamd_field *f, const qamd_qll_layout *l, const void *host, int host_prec);
+int qamd_field_to_qll(const qamd_field *f, const qamd_qll_layout *l, void *host, int host_prec);
+
+/* ---- gauge fields (device), FP64 ---- */
+qamd_gauge *qamd_gauge_create(qamd_lattice *lat);
+void qamd_gauge_destroy(qamd_gauge *u);
+/* host [4][V][18] lexicographic, column-major 3x3 */
+int qamd_gauge_from_lex(qamd_gauge *u, const double *host);
+int qamd_gauge_to_lex(const qamd_gauge *u, double *host);
+/* from four QLL-layout arrays u[mu] (18 reals per site) */
+int qamd_gauge_from_qll(qamd_gauge *u, const qamd_qll_layout *l, const void *const host[4], int host_prec);
+int qamd_gauge_to_qll(const qamd_gauge *u, const qamd_qll_layout *l, void *const host[4], int host_prec);
+/* seeded SU(3): complex Gaussian entries -> Gram-Schmidt columns -> det = 1
+ * (no reference counterpart; column order of makeU lib/vla.c:1635-1654) */
+int qamd_gauge_random_su3(qamd_gauge *u, uint64_t seed);
+/* multiply U_mu(x) by eta_mu(x) = (-1)^(sum_{nu<mu} x_nu) */
+int qamd_gauge_stag_phases(qamd_gauge *u);
+/* fat7 + Naik smearing (replaces smearFat7 lib/gauge.c:264-416);
+ * coef = {one_link, three_staple, five_staple, seven_staple, lepage, naik};
+ * ll may be NULL */
+int qamd_fat7(qamd_gauge *fl, qamd_gauge *ll, const double coef[6],
+              const qamd_gauge *u, const qamd_gauge *ulong);
+/* plaquettes in the pair order of gaugePlaq (lib/gauge.c:114-183) */
+int qamd_gauge_plaquette(const qamd_gauge *u, double plaq[6]);
+
+/* ---- Dslash link sets (replaces createStaggeredLinks lib/stagDslash.c:422-462
+ *      and createWilsonLinks lib/wilsonDslash.c:181-202) ----
+ * kind: 0 = asqtad/staggered (8 or 16 terms), 1 = Wilson (8 terms) */
+qamd_links *qamd_links_create(qamd_lattice *lat, int kind, int prec, int nterms);
+void qamd_links_destroy(qamd_links *l);
+/* host [nterms][V][18] lexicographic, QLL term order */
+int qamd_links_from_lex(qamd_links *l, const double *host);
+/* QLL arrays: staggered u[8] (+ u3[8] for 16 terms, term 2k=u[k], 2k+1=u3[k]);
+ * Wilson u[8] */
+int qamd_links_from_qll(qamd_links *l, const qamd_qll_layout *lay, const void *const u[],
+                        const void *const u3[], int host_prec);
+/* doublestore from device fat/long links: t[2mu]=F(x), t[2mu+1]=sgn*F(x-mu)^+
+ * (sgn = -1 staggered, +1 Wilson), long links likewise with x-3mu */
+int qamd_links_from_gauge(qamd_links *l, const qamd_gauge *fat, const qamd_gauge *lng);
+/* copy with precision conversion (same kind and term count) */
+int qamd_links_convert(qamd_links *dst, const qamd_links *src);
+
+/* ---- Dslash (replaces dslash2 lib/stagDslash.c:531-544 and wilsonDslash
+ *      lib/wilsonDslash.c:272-285 for one output parity) ----
+ * out = a*w + b*D in, on out's parity; w may be NULL (then a is ignored).
+ * norm2_out (may be NULL): receives |out|^2 per rhs, FP64 accumulated. */
+int qamd_dslash(const qamd_links *l, qamd_field *out, const qamd_field *in,
+                double a, const qamd_field *w, double b, double *norm2_out);
+
+/* ---- normal operator (replaces dslashp2 lib/stagDslash.c:546-575):
+ *      tmp = D_oe p, Ap = m^2 p - D_eo tmp, pAp = m^2|p|^2 + |tmp|^2 ---- */
+int qamd_stag_normal(const qamd_links *l, qamd_field *Ap, const qamd_field *p,
+                     qamd_field *tmp, double mass, double *pAp);
+
+/* ---- solvers ---- */
+typedef struct {
+  int iterations;          /* total sloppy + precise iterations          */
+  int reliable_updates;
+  double true_rsq;         /* |b - A x|^2 / |b|^2 in FP64 at exit          */
+  double iter_rsq;         /* the recurrence's own relative residual^2    */
+  double seconds;
+  int converged;
+} qamd_solve_info;
+
+/* QLL CG semantics (cgSolveT lib/cg.c:43-122 with dslashp2): solves
+ * (m^2 - D_eo D_oe) x = b on even sites to |r|^2 < rsq |b|^2 (iterated),
+ * all vectors at the links' precision, x starts at 0.  Not rescaled by mass. */
+int qamd_stag_cg(const qamd_links *l, qamd_field *x, const qamd_field *b, double mass,
+                 double rsq, int maxits, qamd_solve_info *info);
+/* Mixed precision CG with reliable updates (no reference counterpart):
+ * outer FP64 (links l64, x, b), inner sloppy links ls (FP32 or FP16); stops on
+ * the FP64 true residual |b - A x|^2 <= rsq |b|^2.  delta: reliable-update
+ * threshold on |r| relative to its maximum since the last update. */
+int qamd_stag_cg_mixed(const qamd_links *l64, const qamd_links *ls, qamd_field *x,
+                       const qamd_field *b, double mass, double rsq, int maxits,
+                       double delta, qamd_solve_info *info);
+/* QLL multishift CG semantics (cgmsSolveT lib/cgms.c:6-179): shifts[0] is
+ * taken as 0, stop on the base residual, vectors at the links' precision. */
+int qamd_stag_cgms(const qamd_links *l, qamd_field *const x[], const qamd_field *b,
+                   double mass, const double shifts[], int nshift, double rsq,
+                   int maxits, qamd_solve_info *info);
+/* Multishift with sloppy inner links followed by per-shift mixed-precision
+ * refinement to the FP64 true residual rsq on every shift (no reference
+ * counterpart). */
+int qamd_stag_cgms_mixed(const qamd_links *l64, const qamd_links *ls, qamd_field *const x[],
+                         const qamd_field *b, double mass, const double shifts[], int nshift,
+                         double rsq, int maxits, qamd_solve_info *info /* [nshift] */);
+
+/* ---- timing helpers (hipEvents on the library stream) ---- */
+int qamd_time_dslash(const qamd_links *l, qamd_field *out, const qamd_field *in,
+                     int nrep, double *ms_per_call);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

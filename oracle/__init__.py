@@ -1,0 +1,288 @@
+"""FP64 CPU oracle for the QLL Dslash-and-solver path.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker and as the timed CPU baseline.
+The product (qll_amd/) never imports it.
+
+Thin numpy/ctypes wrapper over oracle.c (built by oracle/Makefile).  Arrays use
+the oracle's natural layout: lexicographic sites with x fastest, 6 reals per
+colour vector, 24 per Wilson spinor, 18 per column-major 3x3 matrix.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+FAT7_BASELINE = (5.0 / 8, -1.0 / 16, 1.0 / 64, -1.0 / 384, 0.0, -1.0 / 24)
+
+_d = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        i4 = C.POINTER(C.c_int)
+        u64 = C.c_uint64
+        L.orc_rng64.restype = u64
+        L.orc_rng64.argtypes = [u64, u64, u64, u64]
+        L.orc_uniform.restype = C.c_double
+        L.orc_uniform.argtypes = [u64, u64, u64, u64]
+        L.orc_random_su3_field.argtypes = [i4, u64, _d]
+        L.orc_apply_stag_phases.argtypes = [i4, _d]
+        L.orc_gauss_vector.argtypes = [i4, u64, u64, C.c_int, C.c_int, _d]
+        L.orc_su3_from_gauss.argtypes = [_d, _d]
+        L.orc_fat7.argtypes = [i4, _d, _d, _d, _d, C.c_void_p]
+        L.orc_stag_doublestore.argtypes = [i4, _d, C.c_void_p, _d]
+        L.orc_wilson_doublestore.argtypes = [i4, _d, _d]
+        L.orc_plaquette.argtypes = [i4, _d, _d]
+        L.orc_dslash.argtypes = [i4, _d, C.c_int, C.c_double, _d, _d, C.c_int]
+        L.orc_normal_op.restype = C.c_double
+        L.orc_normal_op.argtypes = [i4, _d, C.c_int, C.c_double, _d, _d, _d]
+        L.orc_cg.restype = C.c_int
+        L.orc_cg.argtypes = [i4, _d, C.c_int, C.c_double, _d, _d, C.c_double,
+                             C.c_int, C.c_int, C.POINTER(C.c_double)]
+        L.orc_cgms.restype = C.c_int
+        L.orc_cgms.argtypes = [i4, _d, C.c_int, _d, C.c_int, _d,
+                               C.POINTER(C.c_void_p), C.c_double, C.c_int, C.c_int]
+        L.orc_norm2_parity.restype = C.c_double
+        L.orc_norm2_parity.argtypes = [i4, _d, C.c_int, C.c_int]
+        L.orc_gamma.argtypes = [C.c_int, _d]
+        L.orc_wilson_dslash.argtypes = [i4, _d, C.c_double, _d, _d, C.c_int]
+        L.orc_wilson_meo.argtypes = [i4, _d, C.c_double, _d, _d, C.c_int, _d]
+        L.orc_wilson_cgne.restype = C.c_int
+        L.orc_wilson_cgne.argtypes = [i4, _d, C.c_double, _d, _d, C.c_double,
+                                      C.c_int, C.POINTER(C.c_double)]
+        L.orc_layout_setup.restype = C.c_int
+        L.orc_layout_setup.argtypes = [C.c_void_p, i4, i4]
+        L.orc_layout_index.restype = C.c_int
+        L.orc_layout_index.argtypes = [C.c_void_p, i4]
+        L.orc_layout_coord.argtypes = [C.c_void_p, i4, C.c_int]
+        L.orc_to_qll.argtypes = [C.c_void_p, _d, _d, C.c_int]
+        L.orc_from_qll.argtypes = [C.c_void_p, _d, _d, C.c_int]
+        L.orc_time_dslash.restype = C.c_double
+        L.orc_time_dslash.argtypes = [i4, _d, C.c_int, _d, _d, C.c_int,
+                                      C.POINTER(C.c_int)]
+        _LIB = L
+    return _LIB
+
+
+def _L(dims):
+    return (C.c_int * 4)(*[int(d) for d in dims])
+
+
+def volume(dims):
+    return int(np.prod(dims))
+
+
+def coords(dims):
+    """(V,4) integer coordinates in lexicographic order, x fastest."""
+    V = volume(dims)
+    s = np.arange(V)
+    out = np.empty((V, 4), dtype=np.int64)
+    for i, d in enumerate(dims):
+        out[:, i] = s % d
+        s = s // d
+    return out
+
+
+def parity(dims):
+    return coords(dims).sum(axis=1) & 1
+
+
+def random_su3_field(dims, seed):
+    u = np.empty((4, volume(dims), 18))
+    lib().orc_random_su3_field(_L(dims), seed, u.reshape(-1))
+    return u
+
+
+def apply_stag_phases(dims, u):
+    u = np.ascontiguousarray(u, dtype=np.float64).copy()
+    lib().orc_apply_stag_phases(_L(dims), u.reshape(-1))
+    return u
+
+
+def gauss_vector(dims, seed, stream, parity=0, nreal=6):
+    v = np.empty((volume(dims), nreal))
+    lib().orc_gauss_vector(_L(dims), seed, stream, parity, nreal, v.reshape(-1))
+    return v
+
+
+def fat7(dims, u, ulong=None, coef=FAT7_BASELINE, want_long=True):
+    V = volume(dims)
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    ulong = u if ulong is None else np.ascontiguousarray(ulong, dtype=np.float64)
+    fl = np.empty((4, V, 18))
+    ll = np.empty((4, V, 18)) if want_long else None
+    lib().orc_fat7(_L(dims), np.asarray(coef, dtype=np.float64), u.reshape(-1),
+                   ulong.reshape(-1), fl.reshape(-1),
+                   ll.ctypes.data if ll is not None else None)
+    return fl, ll
+
+
+def stag_doublestore(dims, fl, ll):
+    nt = 16 if ll is not None else 8
+    links = np.empty((nt, volume(dims), 18))
+    llp = None
+    if ll is not None:
+        ll = np.ascontiguousarray(ll, dtype=np.float64)
+        llp = ll.ctypes.data
+    lib().orc_stag_doublestore(_L(dims), np.ascontiguousarray(fl), llp, links.reshape(-1))
+    return links
+
+
+def wilson_doublestore(dims, u):
+    links = np.empty((8, volume(dims), 18))
+    lib().orc_wilson_doublestore(_L(dims), np.ascontiguousarray(u), links.reshape(-1))
+    return links
+
+
+def plaquette(dims, u):
+    p = np.empty(6)
+    lib().orc_plaquette(_L(dims), np.ascontiguousarray(u), p)
+    return p
+
+
+def asqtad_links(dims, seed, coef=FAT7_BASELINE):
+    """Seeded SU(3) links -> staggered phases -> fat7/Naik -> 16-term set."""
+    u = apply_stag_phases(dims, random_su3_field(dims, seed))
+    fl, ll = fat7(dims, u, u, coef)
+    return stag_doublestore(dims, fl, ll)
+
+
+SUB = {"all": 0, "even": 1, "odd": 2}
+
+
+def dslash(dims, links, mass, y, sub="all", x=None):
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    x = np.zeros_like(y) if x is None else x.copy()
+    lib().orc_dslash(_L(dims), np.ascontiguousarray(links).reshape(-1), links.shape[0],
+                     mass, y.reshape(-1), x.reshape(-1), SUB[sub])
+    return x
+
+
+def normal_op(dims, links, mass, p):
+    p = np.ascontiguousarray(p, dtype=np.float64)
+    Ap = np.zeros_like(p)
+    t = np.zeros_like(p)
+    an = lib().orc_normal_op(_L(dims), np.ascontiguousarray(links).reshape(-1),
+                             links.shape[0], mass, p.reshape(-1), Ap.reshape(-1),
+                             t.reshape(-1))
+    return Ap, an
+
+
+def cg(dims, links, mass, b, rsq, maxits, noscale=False):
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros_like(b)
+    fr = C.c_double()
+    its = lib().orc_cg(_L(dims), np.ascontiguousarray(links).reshape(-1), links.shape[0],
+                       mass, b.reshape(-1), x.reshape(-1), rsq, maxits, int(noscale),
+                       C.byref(fr))
+    return x, its, fr.value
+
+
+def cgms(dims, links, masses, b, rsq, maxits, noscale=False):
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    xs = [np.zeros_like(b) for _ in masses]
+    ptrs = (C.c_void_p * len(masses))(*[x.ctypes.data for x in xs])
+    its = lib().orc_cgms(_L(dims), np.ascontiguousarray(links).reshape(-1),
+                         links.shape[0], np.asarray(masses, dtype=np.float64),
+                         len(masses), b.reshape(-1), ptrs, rsq, maxits, int(noscale))
+    return xs, its
+
+
+def norm2_parity(dims, v, parity):
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    return lib().orc_norm2_parity(_L(dims), v.reshape(-1), v.shape[-1], parity)
+
+
+def gamma(mu):
+    g = np.empty(32)
+    lib().orc_gamma(mu, g)
+    return (g[0::2] + 1j * g[1::2]).reshape(4, 4)
+
+
+def wilson_dslash(dims, links, mass, y, sub="all", x=None):
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    x = np.zeros_like(y) if x is None else x.copy()
+    lib().orc_wilson_dslash(_L(dims), np.ascontiguousarray(links).reshape(-1), mass,
+                            y.reshape(-1), x.reshape(-1), SUB[sub])
+    return x
+
+
+def wilson_meo(dims, links, kappa, y, dagger=False):
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    x = np.zeros_like(y)
+    t = np.zeros_like(y)
+    lib().orc_wilson_meo(_L(dims), np.ascontiguousarray(links).reshape(-1), kappa,
+                         y.reshape(-1), x.reshape(-1), int(dagger), t.reshape(-1))
+    return x
+
+
+def wilson_cgne(dims, links, kappa, b, rsq, maxits):
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros_like(b)
+    fr = C.c_double()
+    its = lib().orc_wilson_cgne(_L(dims), np.ascontiguousarray(links).reshape(-1), kappa,
+                                b.reshape(-1), x.reshape(-1), rsq, maxits, C.byref(fr))
+    return x, its, fr.value
+
+
+class OrcLayout(C.Structure):
+    _fields_ = [("nd", C.c_int), ("physGeom", C.c_int * 4), ("innerGeom", C.c_int * 4),
+                ("outerGeom", C.c_int * 4), ("nSites", C.c_int), ("nSitesOuter", C.c_int),
+                ("nSitesInner", C.c_int), ("nEvenOuter", C.c_int), ("innerCb", C.c_int),
+                ("innerCbDir", C.c_int)]
+
+
+def layout(dims, inner=(1, 1, 1, 1)):
+    lo = OrcLayout()
+    rc = lib().orc_layout_setup(C.byref(lo), _L(dims), _L(inner))
+    if rc != 0:
+        raise ValueError("QLL layout rejects dims=%s inner=%s (code %d)" % (dims, inner, rc))
+    return lo
+
+
+def layout_index(lo, x):
+    return lib().orc_layout_index(C.byref(lo), _L(x))
+
+
+def layout_coord(lo, idx):
+    x = (C.c_int * 4)()
+    lib().orc_layout_coord(C.byref(lo), x, idx)
+    return list(x)
+
+
+def to_qll(lo, nat):
+    nat = np.ascontiguousarray(nat, dtype=np.float64)
+    q = np.empty_like(nat)
+    lib().orc_to_qll(C.byref(lo), nat.reshape(-1), q.reshape(-1), nat.shape[-1])
+    return q.reshape(-1)
+
+
+def from_qll(lo, q, nelem):
+    q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1)
+    nat = np.empty((q.size // nelem, nelem))
+    lib().orc_from_qll(C.byref(lo), q, nat.reshape(-1), nelem)
+    return nat
+
+
+def time_dslash(dims, links, y, nrep):
+    """Seconds per EO Dslash call of the OpenMP CPU restatement."""
+    x = np.zeros_like(y)
+    nth = C.c_int()
+    t = lib().orc_time_dslash(_L(dims), np.ascontiguousarray(links).reshape(-1),
+                              links.shape[0], np.ascontiguousarray(y).reshape(-1),
+                              x.reshape(-1), nrep, C.byref(nth))
+    return t, nth.value

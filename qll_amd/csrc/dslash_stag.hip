@@ -1,0 +1,248 @@
+// Asqtad / staggered even-odd Dslash for gfx950, and the link-set objects.
+//
+// One thread per output site of one parity.  The 16 (or 8) link matrices of
+// a site are streamed from the wavefront-blocked link array with
+// non-temporal loads (read once, kept out of L2), the neighbour colour
+// vectors are gathered through L2/MALL, and the sum is accumulated in
+// registers: out = a*w + b * sum_t L_t(x) in(x + d_t).
+//
+// Term order follows QLL's StaggeredLinks (lib/stagDslash.c:438-448, 820-826):
+// with Naik, term 2k is the 1-hop link u[k] and 2k+1 the 3-hop link u3[k];
+// k = 2mu gathers from x+mu, k = 2mu+1 from x-mu.
+#include "qamd_internal.h"
+#include "qamd_objects.h"
+#include "qamd_kernels.h"
+#include "qamd_vec3.h"
+
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+
+namespace qamd {
+
+template <int P> struct StagArgs {
+  typedef typename Prec<P>::S S;
+  Geom g;
+  const S *links;
+  const S *in;
+  const float *in_norm;
+  S *out;
+  float *out_norm;
+  const S *w;
+  const float *w_norm;
+  double a, b;
+  int par;            // output parity
+  float lscale[16];   // FP16 per-term link scale
+  RedScratch rs;
+  double *red_out;    // receives {|out|^2, |w|^2}
+  const int *skip;    // optional device flag: whole grid returns when set
+};
+
+struct FinStore2 {
+  double *dst;
+  __device__ void operator()(const double *t) const {
+    dst[0] = t[0];
+    dst[1] = t[1];
+  }
+};
+
+template <int P, int NT, bool RED>
+__global__ void __launch_bounds__(256) k_stag_dslash(StagArgs<P> A) {
+  typedef typename Prec<P>::R R;
+  typedef typename Prec<P>::C C;
+  typedef typename Prec<P>::S S;
+  const int64_t vp = A.g.vh_pad;
+  if (A.skip && *A.skip) return;   // uniform over the grid
+  int blk = xcd_block(blockIdx.x, gridDim.x);
+  int cb = blk * 256 + threadIdx.x;
+  double red[2] = {0.0, 0.0};
+  if (cb < A.g.vh) {
+    Site s = site_of(A.g, cb, A.par);
+    const S *lp = A.links + ((int64_t)(cb >> 6) * NT * 9 << 6) + (cb & 63);
+    C acc[3];
+#pragma unroll
+    for (int j = 0; j < 3; j++) acc[j].x = acc[j].y = (R)0;
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+      const int k = (NT == 16) ? t / 2 : t;
+      const int hop = (NT == 16 && (t & 1)) ? 3 : 1;
+      const int mu = k / 2;
+      const int h = (k & 1) ? -hop : hop;
+      int nb = nbr_cb(A.g, s, mu, h);
+      C v[3];
+      VecLd<P>::ld(A.in, A.in_norm, vp, nb, v, (P == QAMD_F16) ? A.lscale[t] : (R)1);
+      C m[9];
+#pragma unroll
+      for (int e = 0; e < 9; e++) m[e] = to_c(ntload(lp + (t * 9 + e) * 64), (R)0);
+      // column major: element (r,c) = m[3c+r]
+#pragma unroll
+      for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) cmac<C, R>(acc[r], m[3 * c + r], v[c]);
+    }
+    C o[3];
+    const R b = (R)A.b;
+    if (A.w) {
+      C w[3];
+      VecLd<P>::ld(A.w, A.w_norm, vp, cb, w, (R)1);
+      const R a = (R)A.a;
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        o[j].x = fma(a, w[j].x, b * acc[j].x);
+        o[j].y = fma(a, w[j].y, b * acc[j].y);
+        if (RED) red[1] += (double)w[j].x * w[j].x + (double)w[j].y * w[j].y;
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        o[j].x = b * acc[j].x;
+        o[j].y = b * acc[j].y;
+      }
+    }
+    if (RED) {
+#pragma unroll
+      for (int j = 0; j < 3; j++) red[0] += (double)o[j].x * o[j].x + (double)o[j].y * o[j].y;
+    }
+    VecSt<P>::st(A.out, A.out_norm, vp, cb, o);
+  }
+  if (RED) grid_sum<2>(red, A.rs, FinStore2{A.red_out});
+}
+
+template <int P, int NT>
+static void launch_t(const StagArgs<P> &A, bool red, hipStream_t st) {
+  int nb = (int)((A.g.vh + 255) / 256);
+  if (red)
+    hipLaunchKernelGGL((k_stag_dslash<P, NT, true>), dim3(nb), dim3(256), 0, st, A);
+  else
+    hipLaunchKernelGGL((k_stag_dslash<P, NT, false>), dim3(nb), dim3(256), 0, st, A);
+}
+
+template <int P>
+static int launch_p(const qamd_links *l, Field &out, const Field &in, double a, const Field *w,
+                    double b, double *red_out, hipStream_t st, const int *skip) {
+  typedef typename Prec<P>::S S;
+  StagArgs<P> A;
+  memset(&A, 0, sizeof A);
+  A.skip = skip;
+  A.g = l->lat->g;
+  A.links = (const S *)l->data[out.parity];
+  A.in = (const S *)in.data;
+  A.in_norm = in.norm;
+  A.out = (S *)out.data;
+  A.out_norm = out.norm;
+  A.w = w ? (const S *)w->data : nullptr;
+  A.w_norm = w ? w->norm : nullptr;
+  A.a = a;
+  A.b = b;
+  A.par = out.parity;
+  for (int t = 0; t < 16; t++) A.lscale[t] = l->scale[t];
+  if (red_out) {
+    int nb = (int)((A.g.vh + 255) / 256);
+    if (red_reserve(nb)) return -1;
+    A.rs = red_scratch();
+    A.red_out = red_out;
+  }
+  if (l->nterms == 16) launch_t<P, 16>(A, red_out != nullptr, st);
+  else launch_t<P, 8>(A, red_out != nullptr, st);
+  QAMD_HIP(hipGetLastError());
+  return 0;
+}
+
+int stag_dslash_launch(const qamd_links *l, Field &out, const Field &in, double a, const Field *w,
+                       double b, double *red_out, hipStream_t st, const int *skip) {
+  if (l->kind != 0) {
+    set_error("stag_dslash: links are not staggered");
+    return -1;
+  }
+  if (out.prec != l->prec || in.prec != l->prec || (w && w->prec != l->prec)) {
+    set_error("stag_dslash: field precision %d/%d does not match links %d", out.prec, in.prec, l->prec);
+    return -1;
+  }
+  if (out.ncomp != 3 || in.ncomp != 3 || out.nrhs != 1 || in.nrhs != 1) {
+    set_error("stag_dslash: needs single-rhs colour vectors");
+    return -1;
+  }
+  if (out.parity == in.parity) {
+    set_error("stag_dslash: input and output must have opposite parity");
+    return -1;
+  }
+  if (w && w->parity != out.parity) {
+    set_error("stag_dslash: w must have the output parity");
+    return -1;
+  }
+  switch (l->prec) {
+    case QAMD_F64: return launch_p<QAMD_F64>(l, out, in, a, w, b, red_out, st, skip);
+    case QAMD_F32: return launch_p<QAMD_F32>(l, out, in, a, w, b, red_out, st, skip);
+    default: return launch_p<QAMD_F16>(l, out, in, a, w, b, red_out, st, skip);
+  }
+}
+
+}  // namespace qamd
+
+using namespace qamd;
+
+extern "C" {
+
+int qamd_dslash(const qamd_links *l, qamd_field *out, const qamd_field *in, double a,
+                const qamd_field *w, double b, double *norm2_out) {
+  if (!l || !out || !in) {
+    set_error("dslash: NULL argument");
+    return -1;
+  }
+  double *slot = nullptr;
+  if (norm2_out) {
+    slot = scalar_slot();
+    if (!slot) return -1;
+  }
+  int rc = (l->kind == 1)
+               ? wilson_dslash_launch(l, out->f, in->f, a, w ? &w->f : nullptr, b, slot, stream(), nullptr)
+               : stag_dslash_launch(l, out->f, in->f, a, w ? &w->f : nullptr, b, slot, stream(), nullptr);
+  if (rc) return -1;
+  if (norm2_out) {
+    double t[2];
+    QAMD_HIP(hipMemcpyAsync(t, slot, sizeof t, hipMemcpyDeviceToHost, stream()));
+    QAMD_HIP(hipStreamSynchronize(stream()));
+    *norm2_out = t[0];
+  }
+  return 0;
+}
+
+int qamd_stag_normal(const qamd_links *l, qamd_field *Ap, const qamd_field *p, qamd_field *tmp,
+                     double mass, double *pAp) {
+  double *slot = scalar_slot();
+  if (!slot) return -1;
+  // tmp = D_oe p  (|tmp|^2 -> slot[0])
+  if (stag_dslash_launch(l, tmp->f, p->f, 0.0, nullptr, 1.0, slot, stream(), nullptr)) return -1;
+  // Ap = m^2 p - D_eo tmp  (|p|^2 -> slot[3])
+  if (stag_dslash_launch(l, Ap->f, tmp->f, mass * mass, &p->f, -1.0, slot + 2, stream(), nullptr)) return -1;
+  if (pAp) {
+    double t[4];
+    QAMD_HIP(hipMemcpyAsync(t, slot, sizeof t, hipMemcpyDeviceToHost, stream()));
+    QAMD_HIP(hipStreamSynchronize(stream()));
+    *pAp = mass * mass * t[3] + t[0];
+  }
+  return 0;
+}
+
+int qamd_time_dslash(const qamd_links *l, qamd_field *out, const qamd_field *in, int nrep,
+                     double *ms_per_call) {
+  hipEvent_t e0, e1;
+  QAMD_HIP(hipEventCreate(&e0));
+  QAMD_HIP(hipEventCreate(&e1));
+  QAMD_HIP(hipEventRecord(e0, stream()));
+  for (int i = 0; i < nrep; i++) {
+    int rc = (l->kind == 1) ? wilson_dslash_launch(l, out->f, in->f, 0.0, nullptr, 1.0, nullptr, stream())
+                            : stag_dslash_launch(l, out->f, in->f, 0.0, nullptr, 1.0, nullptr, stream(), nullptr);
+    if (rc) return -1;
+  }
+  QAMD_HIP(hipEventRecord(e1, stream()));
+  QAMD_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  QAMD_HIP(hipEventElapsedTime(&ms, e0, e1));
+  *ms_per_call = ms / nrep;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return 0;
+}
+
+}  // extern "C"

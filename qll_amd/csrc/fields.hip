@@ -1,0 +1,359 @@
+// Fields: creation, host import/export (lexicographic and QLL layouts),
+// Gaussian sources, FP64-accumulated norms/dots and axpby.
+#include "qamd_internal.h"
+#include "qamd_objects.h"
+#include "qamd_site.h"
+#include "qamd_kernels.h"
+
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+
+namespace qamd {
+
+size_t prec_bytes(int prec) {
+  return prec == QAMD_F64 ? 16 : prec == QAMD_F32 ? 8 : 4;
+}
+
+template <class H>
+__global__ void k_import(Field f, const H *src, SrcMap m) {
+  int cb = blockIdx.x * blockDim.x + threadIdx.x;
+  int rhs = blockIdx.y;
+  if (cb >= f.g->vh) return;
+  int64_t base, stride;
+  m.map(*f.g, rhs, f.parity, cb, base, stride);
+  double2 v[12];
+  for (int j = 0; j < f.ncomp; j++)
+    v[j] = make_double2((double)src[base + (2 * j) * stride], (double)src[base + (2 * j + 1) * stride]);
+  site_store(f, rhs, cb, v);
+}
+
+template <class H>
+__global__ void k_export(Field f, H *dst, SrcMap m) {
+  int cb = blockIdx.x * blockDim.x + threadIdx.x;
+  int rhs = blockIdx.y;
+  if (cb >= f.g->vh) return;
+  int64_t base, stride;
+  m.map(*f.g, rhs, f.parity, cb, base, stride);
+  double2 v[12];
+  site_load(f, rhs, cb, v);
+  for (int j = 0; j < f.ncomp; j++) {
+    dst[base + (2 * j) * stride] = (H)v[j].x;
+    dst[base + (2 * j + 1) * stride] = (H)v[j].y;
+  }
+}
+
+__global__ void k_gaussian(Field f, uint64_t seed, uint64_t stream) {
+  int cb = blockIdx.x * blockDim.x + threadIdx.x;
+  int rhs = blockIdx.y;
+  if (cb >= f.g->vh) return;
+  Site s = site_of(*f.g, cb, f.parity);
+  double2 v[12];
+  for (int j = 0; j < f.ncomp; j++) {
+    double a, b;
+    gauss2(seed, stream + rhs, (uint64_t)s.lex, j, a, b);
+    v[j] = make_double2(a * 0.70710678118654752440, b * 0.70710678118654752440);
+  }
+  site_store(f, rhs, cb, v);
+}
+
+// --------------------------------------------------------------- BLAS-1
+template <class Fin>
+__global__ void k_norm_dot(Field a, Field b, int rhs, int dot, Fin fin, RedScratch rs) {
+  double acc[1] = {0.0};
+  for (int cb = blockIdx.x * blockDim.x + threadIdx.x; cb < a.g->vh; cb += gridDim.x * blockDim.x) {
+    double2 va[12], vb[12];
+    site_load(a, rhs, cb, va);
+    if (dot) site_load(b, rhs, cb, vb);
+    for (int j = 0; j < a.ncomp; j++) {
+      if (dot) acc[0] += va[j].x * vb[j].x + va[j].y * vb[j].y;
+      else acc[0] += va[j].x * va[j].x + va[j].y * va[j].y;
+    }
+  }
+  grid_sum<1>(acc, rs, fin);
+}
+
+struct StoreTo {
+  double *dst;
+  __device__ void operator()(const double *t) const { *dst = t[0]; }
+};
+
+__global__ void k_axpby(double a, Field x, double b, Field y) {
+  int cb = blockIdx.x * blockDim.x + threadIdx.x;
+  int rhs = blockIdx.y;
+  if (cb >= y.g->vh) return;
+  double2 vx[12], vy[12];
+  site_load(x, rhs, cb, vx);
+  if (b != 0.0) site_load(y, rhs, cb, vy);
+  for (int j = 0; j < y.ncomp; j++)
+    vy[j] = (b != 0.0) ? make_double2(a * vx[j].x + b * vy[j].x, a * vx[j].y + b * vy[j].y)
+                       : make_double2(a * vx[j].x, a * vx[j].y);
+  site_store(y, rhs, cb, vy);
+}
+
+// ------------------------------------------------------------- helpers
+inline dim3 site_grid(const Geom &g, int nrhs, int bs = 256) {
+  return dim3((unsigned)((g.vh + bs - 1) / bs), (unsigned)nrhs);
+}
+
+double *scalar_slot() {
+  static double *d = nullptr;
+  if (!d) d = (double *)dev_alloc(64 * sizeof(double));
+  return d;
+}
+
+int reduce_field(const Field &a, const Field *b, int rhs, double *out) {
+  int nb = 1024;
+  if (red_reserve(nb)) return -1;
+  double *slot = scalar_slot();
+  if (!slot) return -1;
+  StoreTo fin{slot};
+  Field bb = b ? *b : a;
+  hipLaunchKernelGGL(k_norm_dot<StoreTo>, dim3(nb), dim3(256), 0, stream(), a, bb, rhs, b ? 1 : 0,
+                     fin, red_scratch());
+  QAMD_HIP(hipGetLastError());
+  QAMD_HIP(hipMemcpyAsync(out, slot, sizeof(double), hipMemcpyDeviceToHost, stream()));
+  QAMD_HIP(hipStreamSynchronize(stream()));
+  return 0;
+}
+
+}  // namespace qamd
+
+using namespace qamd;
+
+namespace qamd {
+const Geom *dev_geom(qamd_lattice *lat) {
+  // one device copy per distinct geometry, kept until the process ends
+  struct Ent { Geom h; Geom *d; };
+  static Ent tab[256];
+  static int n = 0;
+  for (int i = 0; i < n; i++)
+    if (memcmp(&tab[i].h, &lat->g, sizeof(Geom)) == 0) return tab[i].d;
+  if (n >= 256) {
+    set_error("more than 256 distinct lattice geometries");
+    return nullptr;
+  }
+  Geom *d = (Geom *)dev_alloc(sizeof(Geom));
+  if (!d) return nullptr;
+  if (hipMemcpy(d, &lat->g, sizeof(Geom), hipMemcpyHostToDevice) != hipSuccess) {
+    set_error("dev_geom: copy failed");
+    return nullptr;
+  }
+  tab[n].h = lat->g;
+  tab[n].d = d;
+  n++;
+  return d;
+}
+}  // namespace qamd
+
+extern "C" {
+
+qamd_field *qamd_field_create(qamd_lattice *lat, int prec, int ncomp, int nrhs, int parity) {
+  if (!lat) { set_error("field_create: NULL lattice"); return nullptr; }
+  if (prec < QAMD_F64 || prec > QAMD_F16) { set_error("field_create: bad precision %d", prec); return nullptr; }
+  if (ncomp != 3 && ncomp != 12) { set_error("field_create: ncomp must be 3 or 12"); return nullptr; }
+  if (nrhs < 1 || nrhs > 64) { set_error("field_create: nrhs must be 1..64"); return nullptr; }
+  if (parity != 0 && parity != 1) { set_error("field_create: parity must be 0 or 1"); return nullptr; }
+  const Geom *dg = dev_geom(lat);
+  if (!dg) return nullptr;
+  qamd_field *f = (qamd_field *)calloc(1, sizeof(qamd_field));
+  f->lat = lat;
+  f->f.prec = prec;
+  f->f.ncomp = ncomp;
+  f->f.nrhs = nrhs;
+  f->f.parity = parity;
+  f->f.g = dg;
+  f->f.vp = lat->g.vh_pad;
+  f->f.rhs_inner = (ncomp == 12 && nrhs > 1);
+  int64_t n = (int64_t)nrhs * ncomp * lat->g.vh_pad;
+  f->f.data = dev_alloc(n * prec_bytes(prec));
+  if (prec == QAMD_F16) f->f.norm = (float *)dev_alloc(sizeof(float) * nrhs * lat->g.vh_pad);
+  if (!f->f.data || (prec == QAMD_F16 && !f->f.norm)) {
+    dev_free(f->f.data);
+    dev_free(f->f.norm);
+    free(f);
+    return nullptr;
+  }
+  if (qamd_field_zero(f)) {
+    qamd_field_destroy(f);
+    return nullptr;
+  }
+  return f;
+}
+
+void qamd_field_destroy(qamd_field *f) {
+  if (!f) return;
+  (void)hipStreamSynchronize(stream());
+  dev_free(f->f.data);
+  dev_free(f->f.norm);
+  free(f);
+}
+
+int qamd_field_zero(qamd_field *f) {
+  int64_t n = f->f.elems();
+  QAMD_HIP(hipMemsetAsync(f->f.data, 0, n * prec_bytes(f->f.prec), stream()));
+  if (f->f.norm)
+    QAMD_HIP(hipMemsetAsync(f->f.norm, 0, sizeof(float) * f->f.nrhs * f->lat->g.vh_pad, stream()));
+  return 0;
+}
+
+int qamd_field_copy(qamd_field *dst, const qamd_field *src) {
+  if (dst->lat->g.V != src->lat->g.V || dst->f.ncomp != src->f.ncomp ||
+      dst->f.nrhs != src->f.nrhs || dst->f.parity != src->f.parity) {
+    set_error("field_copy: shape mismatch");
+    return -1;
+  }
+  if (dst->f.prec == src->f.prec) {
+    QAMD_HIP(hipMemcpyAsync(dst->f.data, src->f.data, src->f.elems() * prec_bytes(src->f.prec),
+                            hipMemcpyDeviceToDevice, stream()));
+    if (src->f.norm)
+      QAMD_HIP(hipMemcpyAsync(dst->f.norm, src->f.norm, sizeof(float) * src->f.nrhs * src->lat->g.vh_pad,
+                              hipMemcpyDeviceToDevice, stream()));
+    return 0;
+  }
+  return qamd_field_axpby(1.0, src, 0.0, dst);
+}
+
+static int import_common(qamd_field *f, const void *host, size_t bytes, int hprec, SrcMap m) {
+  void *stage = dev_alloc(bytes);
+  if (!stage) return -1;
+  hipError_t e = hipMemcpyAsync(stage, host, bytes, hipMemcpyHostToDevice, stream());
+  if (e != hipSuccess) {
+    dev_free(stage);
+    set_error("import copy: %s", hipGetErrorString(e));
+    return -1;
+  }
+  dim3 grid = site_grid(f->lat->g, f->f.nrhs);
+  if (hprec == QAMD_F64)
+    hipLaunchKernelGGL(k_import<double>, grid, dim3(256), 0, stream(), f->f, (const double *)stage, m);
+  else
+    hipLaunchKernelGGL(k_import<float>, grid, dim3(256), 0, stream(), f->f, (const float *)stage, m);
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(stream());
+  dev_free(stage);
+  if (e != hipSuccess) {
+    set_error("import kernel: %s", hipGetErrorString(e));
+    return -1;
+  }
+  return 0;
+}
+
+static int export_common(const qamd_field *f, void *host, size_t bytes, int hprec, SrcMap m, int prefill) {
+  void *stage = dev_alloc(bytes);
+  if (!stage) return -1;
+  hipError_t e = hipSuccess;
+  if (prefill) e = hipMemcpyAsync(stage, host, bytes, hipMemcpyHostToDevice, stream());
+  dim3 grid = site_grid(f->lat->g, f->f.nrhs);
+  if (e == hipSuccess) {
+    if (hprec == QAMD_F64)
+      hipLaunchKernelGGL(k_export<double>, grid, dim3(256), 0, stream(), f->f, (double *)stage, m);
+    else
+      hipLaunchKernelGGL(k_export<float>, grid, dim3(256), 0, stream(), f->f, (float *)stage, m);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(stream());
+  if (e == hipSuccess) e = hipMemcpyAsync(host, stage, bytes, hipMemcpyDeviceToHost, stream());
+  if (e == hipSuccess) e = hipStreamSynchronize(stream());
+  dev_free(stage);
+  if (e != hipSuccess) {
+    set_error("export: %s", hipGetErrorString(e));
+    return -1;
+  }
+  return 0;
+}
+
+int qamd_field_from_lex(qamd_field *f, const double *host) {
+  SrcMap m;
+  memset(&m, 0, sizeof m);
+  m.kind = 0;
+  m.nreal = 2 * f->f.ncomp;
+  size_t bytes = (size_t)f->f.nrhs * f->lat->g.V * m.nreal * sizeof(double);
+  return import_common(f, host, bytes, QAMD_F64, m);
+}
+
+int qamd_field_to_lex(const qamd_field *f, double *host) {
+  SrcMap m;
+  memset(&m, 0, sizeof m);
+  m.kind = 0;
+  m.nreal = 2 * f->f.ncomp;
+  size_t bytes = (size_t)f->f.nrhs * f->lat->g.V * m.nreal * sizeof(double);
+  // sites of the other parity keep the caller's values
+  return export_common(f, host, bytes, QAMD_F64, m, 1);
+}
+
+static int check_qll(const qamd_field *f, const qamd_qll_layout *l) {
+  for (int i = 0; i < 4; i++)
+    if (l->physGeom[i] != f->lat->g.L[i] || l->innerGeom[i] * l->outerGeom[i] != l->physGeom[i]) {
+      set_error("QLL layout does not match the lattice");
+      return -1;
+    }
+  if (f->f.nrhs != 1) {
+    set_error("QLL layout transfer needs nrhs = 1");
+    return -1;
+  }
+  return 0;
+}
+
+int qamd_field_from_qll(qamd_field *f, const qamd_qll_layout *l, const void *host, int hprec) {
+  if (check_qll(f, l)) return -1;
+  SrcMap m;
+  memset(&m, 0, sizeof m);
+  m.kind = 1;
+  m.nreal = 2 * f->f.ncomp;
+  m.lay = to_lay(l);
+  size_t esz = hprec == QAMD_F64 ? 8 : 4;
+  size_t half = (size_t)m.nreal * l->nSitesInner * l->nEvenOuter * esz;
+  const char *h = (const char *)host + (f->f.parity ? half : 0);
+  size_t bytes = (size_t)m.nreal * l->nSitesInner *
+                 (f->f.parity ? (l->nSitesOuter - l->nEvenOuter) : l->nEvenOuter) * esz;
+  return import_common(f, h, bytes, hprec, m);
+}
+
+int qamd_field_to_qll(const qamd_field *f, const qamd_qll_layout *l, void *host, int hprec) {
+  if (check_qll(f, l)) return -1;
+  SrcMap m;
+  memset(&m, 0, sizeof m);
+  m.kind = 1;
+  m.nreal = 2 * f->f.ncomp;
+  m.lay = to_lay(l);
+  size_t esz = hprec == QAMD_F64 ? 8 : 4;
+  size_t half = (size_t)m.nreal * l->nSitesInner * l->nEvenOuter * esz;
+  char *h = (char *)host + (f->f.parity ? half : 0);
+  size_t bytes = (size_t)m.nreal * l->nSitesInner *
+                 (f->f.parity ? (l->nSitesOuter - l->nEvenOuter) : l->nEvenOuter) * esz;
+  return export_common(f, h, bytes, hprec, m, 0);
+}
+
+int qamd_field_gaussian(qamd_field *f, uint64_t seed, uint64_t strm) {
+  hipLaunchKernelGGL(k_gaussian, site_grid(f->lat->g, f->f.nrhs), dim3(256), 0, stream(), f->f, seed, strm);
+  QAMD_HIP(hipGetLastError());
+  return 0;
+}
+
+int qamd_field_norm2(const qamd_field *f, double *out) {
+  for (int r = 0; r < f->f.nrhs; r++)
+    if (reduce_field(f->f, nullptr, r, &out[r])) return -1;
+  return 0;
+}
+
+int qamd_field_redot(const qamd_field *a, const qamd_field *b, double *out) {
+  if (a->f.ncomp != b->f.ncomp || a->f.nrhs != b->f.nrhs || a->lat->g.V != b->lat->g.V) {
+    set_error("redot: shape mismatch");
+    return -1;
+  }
+  for (int r = 0; r < a->f.nrhs; r++)
+    if (reduce_field(a->f, &b->f, r, &out[r])) return -1;
+  return 0;
+}
+
+int qamd_field_axpby(double a, const qamd_field *x, double b, qamd_field *y) {
+  if (x->f.ncomp != y->f.ncomp || x->f.nrhs != y->f.nrhs || x->lat->g.V != y->lat->g.V) {
+    set_error("axpby: shape mismatch");
+    return -1;
+  }
+  hipLaunchKernelGGL(k_axpby, site_grid(y->lat->g, y->f.nrhs), dim3(256), 0, stream(), a, x->f, b, y->f);
+  QAMD_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // extern "C"

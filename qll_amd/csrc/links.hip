@@ -1,0 +1,321 @@
+// Dslash link sets: allocation, import from host (lexicographic or QLL
+// layout), doublestore from device gauge fields, precision conversion.
+#include "qamd_internal.h"
+#include "qamd_objects.h"
+#include "qamd_site.h"
+#include "qamd_kernels.h"
+
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+
+namespace qamd {
+
+static int64_t link_elems(const qamd_links *l) { return l->lat->g.vh_pad * l->nterms * 9; }
+
+template <class S> __device__ inline S mk(double2 v);
+template <> __device__ inline double2 mk<double2>(double2 v) { return v; }
+template <> __device__ inline float2 mk<float2>(double2 v) { return make_float2((float)v.x, (float)v.y); }
+
+// one term from a host-ordered staging array into the blocked layout
+template <class H, class S>
+__global__ void k_link_import(S *d0, S *d1, const H *src, Geom g, int t, int nterm, SrcMap m) {
+  int cb = blockIdx.x * blockDim.x + threadIdx.x;
+  int par = blockIdx.y;
+  if (cb >= g.vh) return;
+  Site s = site_of(g, cb, par);
+  int64_t base, stride;
+  if (m.kind == 0) {
+    base = (int64_t)s.lex * 18;
+    stride = 1;
+  } else {
+    int oi, ii;
+    qll_index(m.lay, s.c, oi, ii);
+    base = (int64_t)18 * m.lay.NI * oi + ii;
+    stride = m.lay.NI;
+  }
+  S *d = par ? d1 : d0;
+  for (int e = 0; e < 9; e++)
+    d[link_idx(cb, t, e, nterm)] =
+        mk<S>(make_double2((double)src[base + 2 * e * stride], (double)src[base + (2 * e + 1) * stride]));
+}
+
+// doublestore from FP64 gauge fields: term for (k, naik) at output parity par
+template <class S>
+__global__ void k_link_from_gauge(S *d0, S *d1, const double2 *fat, const double2 *lng, Geom g,
+                                  int nterm, double sgn) {
+  int cb = blockIdx.x * blockDim.x + threadIdx.x;
+  int par = blockIdx.y;
+  if (cb >= g.vh) return;
+  Site s = site_of(g, cb, par);
+  S *d = par ? d1 : d0;
+  const int64_t vp = g.vh_pad;
+  for (int k = 0; k < 8; k++) {
+    int mu = k >> 1, bwd = k & 1;
+    for (int n = 0; n < (nterm == 16 ? 2 : 1); n++) {
+      const double2 *src = n ? lng : fat;
+      int t = (nterm == 16) ? 2 * k + n : k;
+      if (!bwd) {
+        const double2 *p = src + ((int64_t)(mu * 2 + par) * 9) * vp + cb;
+        for (int e = 0; e < 9; e++) d[link_idx(cb, t, e, nterm)] = mk<S>(p[e * vp]);
+      } else {
+        int hop = n ? 3 : 1;
+        int nb = nbr_cb(g, s, mu, -hop);
+        const double2 *p = src + ((int64_t)(mu * 2 + (1 - par)) * 9) * vp + nb;
+        for (int r = 0; r < 3; r++)
+          for (int c = 0; c < 3; c++) {
+            double2 v = p[(3 * r + c) * vp];   // element (c, r) of the source
+            d[link_idx(cb, t, 3 * c + r, nterm)] = mk<S>(make_double2(sgn * v.x, -sgn * v.y));
+          }
+      }
+    }
+  }
+}
+
+template <class S> __device__ inline double2 ld2(const S *p, int64_t i);
+template <> __device__ inline double2 ld2<double2>(const double2 *p, int64_t i) { return p[i]; }
+template <> __device__ inline double2 ld2<float2>(const float2 *p, int64_t i) {
+  return make_double2(p[i].x, p[i].y);
+}
+template <> __device__ inline double2 ld2<__half2>(const __half2 *p, int64_t i) {
+  float2 f = __half22float2(p[i]);
+  return make_double2(f.x, f.y);
+}
+
+// per-term max |component| as float bits (non-negative floats order as uints)
+template <class S>
+__global__ void k_link_absmax(const S *src, int64_t n, int nterm, unsigned *mx) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double2 v = ld2<S>(src, i);
+    float m = (float)fmax(fabs(v.x), fabs(v.y));
+    int t = (int)((i >> 6) / 9 % nterm);
+    atomicMax(&mx[t], __float_as_uint(m));
+  }
+}
+
+template <class SI, class SO>
+__global__ void k_link_convert(SO *dst, const SI *src, int64_t n, int nterm, const float *iscale,
+                               const float *oscale) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    int t = (int)((i >> 6) / 9 % nterm);
+    double2 v = ld2<SI>(src, i);
+    double f = (double)iscale[t] / (double)oscale[t];
+    v.x *= f;
+    v.y *= f;
+    if constexpr (sizeof(SO) == 4) dst[i] = __floats2half2_rn((float)v.x, (float)v.y);
+    else dst[i] = mk<SO>(v);
+  }
+}
+
+}  // namespace qamd
+
+using namespace qamd;
+
+extern "C" {
+
+qamd_links *qamd_links_create(qamd_lattice *lat, int kind, int prec, int nterms) {
+  if (!lat) { set_error("links_create: NULL lattice"); return nullptr; }
+  if (kind == 0 && nterms != 8 && nterms != 16) { set_error("staggered links need 8 or 16 terms"); return nullptr; }
+  if (kind == 1 && nterms != 8) { set_error("Wilson links need 8 terms"); return nullptr; }
+  if (kind != 0 && kind != 1) { set_error("links_create: bad kind"); return nullptr; }
+  if (prec < QAMD_F64 || prec > QAMD_F16) { set_error("links_create: bad precision"); return nullptr; }
+  qamd_links *l = (qamd_links *)calloc(1, sizeof(qamd_links));
+  l->lat = lat;
+  l->kind = kind;
+  l->prec = prec;
+  l->nterms = nterms;
+  for (int t = 0; t < 16; t++) l->scale[t] = 1.0f;
+  int64_t n = link_elems(l);
+  for (int p = 0; p < 2; p++) {
+    l->data[p] = dev_alloc(n * prec_bytes(prec));
+    if (!l->data[p]) {
+      qamd_links_destroy(l);
+      return nullptr;
+    }
+    if (hipMemsetAsync(l->data[p], 0, n * prec_bytes(prec), stream()) != hipSuccess) {
+      set_error("links_create: memset failed");
+      qamd_links_destroy(l);
+      return nullptr;
+    }
+  }
+  return l;
+}
+
+void qamd_links_destroy(qamd_links *l) {
+  if (!l) return;
+  (void)hipStreamSynchronize(stream());
+  dev_free(l->data[0]);
+  dev_free(l->data[1]);
+  free(l);
+}
+
+// import with an FP64/FP32 destination; FP16 goes through an FP64 copy
+static int import_terms(qamd_links *l, const void *const terms[], int hprec, SrcMap m, size_t term_bytes) {
+  qamd_links *dst = l;
+  if (l->prec == QAMD_F16) {
+    dst = qamd_links_create(l->lat, l->kind, QAMD_F64, l->nterms);
+    if (!dst) return -1;
+  }
+  void *stage = dev_alloc(term_bytes);
+  if (!stage) return -1;
+  const Geom &g = l->lat->g;
+  dim3 grid((unsigned)((g.vh + 255) / 256), 2);
+  int rc = 0;
+  for (int t = 0; t < l->nterms && !rc; t++) {
+    if (hipMemcpyAsync(stage, terms[t], term_bytes, hipMemcpyHostToDevice, stream()) != hipSuccess) {
+      set_error("links import: copy failed");
+      rc = -1;
+      break;
+    }
+    if (dst->prec == QAMD_F64) {
+      if (hprec == QAMD_F64)
+        hipLaunchKernelGGL((k_link_import<double, double2>), grid, dim3(256), 0, stream(), (double2 *)dst->data[0],
+                           (double2 *)dst->data[1], (const double *)stage, g, t, l->nterms, m);
+      else
+        hipLaunchKernelGGL((k_link_import<float, double2>), grid, dim3(256), 0, stream(), (double2 *)dst->data[0],
+                           (double2 *)dst->data[1], (const float *)stage, g, t, l->nterms, m);
+    } else {
+      if (hprec == QAMD_F64)
+        hipLaunchKernelGGL((k_link_import<double, float2>), grid, dim3(256), 0, stream(), (float2 *)dst->data[0],
+                           (float2 *)dst->data[1], (const double *)stage, g, t, l->nterms, m);
+      else
+        hipLaunchKernelGGL((k_link_import<float, float2>), grid, dim3(256), 0, stream(), (float2 *)dst->data[0],
+                           (float2 *)dst->data[1], (const float *)stage, g, t, l->nterms, m);
+    }
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(stream()) != hipSuccess) {
+      set_error("links import kernel failed");
+      rc = -1;
+    }
+  }
+  dev_free(stage);
+  if (!rc && dst != l) rc = qamd_links_convert(l, dst);
+  if (dst != l) qamd_links_destroy(dst);
+  return rc;
+}
+
+int qamd_links_from_lex(qamd_links *l, const double *host) {
+  SrcMap m;
+  memset(&m, 0, sizeof m);
+  m.kind = 0;
+  const void *terms[16];
+  size_t tb = (size_t)l->lat->g.V * 18 * sizeof(double);
+  for (int t = 0; t < l->nterms; t++) terms[t] = (const char *)host + t * tb;
+  return import_terms(l, terms, QAMD_F64, m, tb);
+}
+
+int qamd_links_from_qll(qamd_links *l, const qamd_qll_layout *lay, const void *const u[],
+                        const void *const u3[], int hprec) {
+  for (int i = 0; i < 4; i++)
+    if (lay->physGeom[i] != l->lat->g.L[i]) {
+      set_error("links_from_qll: layout does not match lattice");
+      return -1;
+    }
+  SrcMap m;
+  memset(&m, 0, sizeof m);
+  m.kind = 1;
+  m.lay = to_lay(lay);
+  const void *terms[16];
+  if (l->nterms == 16) {
+    if (!u3) {
+      set_error("links_from_qll: 16 terms need u3");
+      return -1;
+    }
+    for (int k = 0; k < 8; k++) {
+      terms[2 * k] = u[k];
+      terms[2 * k + 1] = u3[k];
+    }
+  } else {
+    for (int k = 0; k < 8; k++) terms[k] = u[k];
+  }
+  size_t tb = (size_t)l->lat->g.V * 18 * (hprec == QAMD_F64 ? 8 : 4);
+  return import_terms(l, terms, hprec, m, tb);
+}
+
+int qamd_links_from_gauge(qamd_links *l, const qamd_gauge *fat, const qamd_gauge *lng) {
+  if (l->nterms == 16 && !lng) {
+    set_error("links_from_gauge: 16 terms need long links");
+    return -1;
+  }
+  qamd_links *dst = l;
+  if (l->prec == QAMD_F16) {
+    dst = qamd_links_create(l->lat, l->kind, QAMD_F64, l->nterms);
+    if (!dst) return -1;
+  }
+  const Geom &g = l->lat->g;
+  dim3 grid((unsigned)((g.vh + 255) / 256), 2);
+  double sgn = l->kind == 0 ? -1.0 : 1.0;
+  const double2 *lp = lng ? lng->d : fat->d;
+  if (dst->prec == QAMD_F64)
+    hipLaunchKernelGGL(k_link_from_gauge<double2>, grid, dim3(256), 0, stream(), (double2 *)dst->data[0],
+                       (double2 *)dst->data[1], fat->d, lp, g, l->nterms, sgn);
+  else
+    hipLaunchKernelGGL(k_link_from_gauge<float2>, grid, dim3(256), 0, stream(), (float2 *)dst->data[0],
+                       (float2 *)dst->data[1], fat->d, lp, g, l->nterms, sgn);
+  int rc = 0;
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(stream()) != hipSuccess) {
+    set_error("links_from_gauge kernel failed");
+    rc = -1;
+  }
+  if (!rc && dst != l) rc = qamd_links_convert(l, dst);
+  if (dst != l) qamd_links_destroy(dst);
+  return rc;
+}
+
+int qamd_links_convert(qamd_links *dst, const qamd_links *src) {
+  if (dst->lat->g.V != src->lat->g.V || dst->nterms != src->nterms || dst->kind != src->kind) {
+    set_error("links_convert: mismatch");
+    return -1;
+  }
+  int64_t n = link_elems(src);
+  float *sc = (float *)dev_alloc(64 * sizeof(float));
+  unsigned *mx = (unsigned *)dev_alloc(16 * sizeof(unsigned));
+  if (!sc || !mx) return -1;
+  float oscale[16];
+  for (int t = 0; t < 16; t++) oscale[t] = 1.0f;
+  if (dst->prec == QAMD_F16) {
+    QAMD_HIP(hipMemsetAsync(mx, 0, 16 * sizeof(unsigned), stream()));
+    for (int p = 0; p < 2; p++) {
+      if (src->prec == QAMD_F64)
+        hipLaunchKernelGGL(k_link_absmax<double2>, dim3(2048), dim3(256), 0, stream(), (const double2 *)src->data[p], n, src->nterms, mx);
+      else if (src->prec == QAMD_F32)
+        hipLaunchKernelGGL(k_link_absmax<float2>, dim3(2048), dim3(256), 0, stream(), (const float2 *)src->data[p], n, src->nterms, mx);
+      else
+        hipLaunchKernelGGL(k_link_absmax<__half2>, dim3(2048), dim3(256), 0, stream(), (const __half2 *)src->data[p], n, src->nterms, mx);
+    }
+    unsigned hm[16];
+    QAMD_HIP(hipMemcpyAsync(hm, mx, sizeof hm, hipMemcpyDeviceToHost, stream()));
+    QAMD_HIP(hipStreamSynchronize(stream()));
+    for (int t = 0; t < src->nterms; t++) {
+      float m;
+      memcpy(&m, &hm[t], 4);
+      m *= src->scale[t];
+      oscale[t] = m > 0 ? m : 1.0f;
+    }
+  }
+  QAMD_HIP(hipMemcpyAsync(sc, src->scale, 16 * sizeof(float), hipMemcpyHostToDevice, stream()));
+  QAMD_HIP(hipMemcpyAsync(sc + 16, oscale, 16 * sizeof(float), hipMemcpyHostToDevice, stream()));
+  for (int p = 0; p < 2; p++) {
+#define CONV(SI, SO)                                                                                   \
+  hipLaunchKernelGGL((k_link_convert<SI, SO>), dim3(2048), dim3(256), 0, stream(), (SO *)dst->data[p], \
+                     (const SI *)src->data[p], n, src->nterms, sc, sc + 16)
+    int a = src->prec, b = dst->prec;
+    if (a == QAMD_F64 && b == QAMD_F64) CONV(double2, double2);
+    else if (a == QAMD_F64 && b == QAMD_F32) CONV(double2, float2);
+    else if (a == QAMD_F64 && b == QAMD_F16) CONV(double2, __half2);
+    else if (a == QAMD_F32 && b == QAMD_F64) CONV(float2, double2);
+    else if (a == QAMD_F32 && b == QAMD_F32) CONV(float2, float2);
+    else if (a == QAMD_F32 && b == QAMD_F16) CONV(float2, __half2);
+    else if (a == QAMD_F16 && b == QAMD_F64) CONV(__half2, double2);
+    else if (a == QAMD_F16 && b == QAMD_F32) CONV(__half2, float2);
+    else CONV(__half2, __half2);
+#undef CONV
+  }
+  QAMD_HIP(hipGetLastError());
+  QAMD_HIP(hipStreamSynchronize(stream()));
+  for (int t = 0; t < 16; t++) dst->scale[t] = oscale[t];
+  dev_free(sc);
+  dev_free(mx);
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,266 @@
+// Internal definitions shared by the HIP translation units of libqll_amd.
+//
+// Device data layout (HBM), per lattice parity p in {0 = even, 1 = odd}:
+//   * a site of parity p is addressed by its checkerboard index
+//       cb = lex >> 1,  lex = x + Lx*(y + Ly*(z + Lz*t))
+//     (all extents even), so the neighbour of a site is found by index
+//     arithmetic, with no tables;
+//   * colour vectors / spinors are structure-of-arrays: complex component j
+//     of rhs r at site cb lives at [(r*ncomp + j) * vh_pad + cb];  FP16 fields
+//     keep one FP32 max-norm per (rhs, site) beside them;
+//   * link sets are blocked by 64 sites (one wavefront): complex element k of
+//     term t for site cb lives at [((cb>>6)*nterm + t)*9 + k)*64 + (cb&63)],
+//     so a wavefront streams one contiguous block per site group.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <stdint.h>
+#include <stddef.h>
+
+#include "qll_amd.h"
+
+namespace qamd {
+
+constexpr int WAVE = 64;
+constexpr int NXCD = 8;   // MI355X: 8 accelerator complex dies, blocks round-robin
+
+// ------------------------------------------------------------ error handling
+void set_error(const char *fmt, ...);
+#define QAMD_HIP(call)                                                        \
+  do {                                                                        \
+    hipError_t _e = (call);                                                   \
+    if (_e != hipSuccess) {                                                   \
+      ::qamd::set_error("%s:%d %s -> %s", __FILE__, __LINE__, #call,          \
+                        hipGetErrorString(_e));                               \
+      return -1;                                                              \
+    }                                                                         \
+  } while (0)
+
+hipStream_t stream();
+int ensure_init();
+void *dev_alloc(size_t bytes);   // returns NULL and sets the error on failure
+void dev_free(void *p);
+
+// ---------------------------------------------- fast division by invariants
+// Quotient of a non-negative 31-bit integer by a fixed divisor with one
+// multiply-high and one shift (Granlund & Montgomery, PLDI 1994).
+struct FastDiv {
+  uint32_t d, mul, shr;
+  __host__ void init(uint32_t div) {
+    d = div;
+    uint32_t p = 0;
+    while ((1u << p) < div) p++;
+    if (div == 1) { mul = 0; shr = 0; }
+    else {
+      mul = (uint32_t)((((uint64_t)1 << (31 + p)) + div - 1) / div);
+      shr = p - 1;
+    }
+  }
+  __host__ __device__ inline uint32_t div(uint32_t n) const {
+#ifdef __HIP_DEVICE_COMPILE__
+    return d == 1 ? n : (__umulhi(n, mul) >> shr);
+#else
+    return d == 1 ? n : (uint32_t)(((uint64_t)n * mul) >> 32) >> shr;
+#endif
+  }
+};
+
+// ---------------------------------------------------------------- geometry
+struct Geom {
+  int L[4];
+  int64_t V, vh, vh_pad;   // volume, sites per parity, padded to 64
+  int stride[4];           // lexicographic strides
+  FastDiv dxh, dy, dz;     // divisors Lx/2, Ly, Lz
+};
+void geom_init(Geom &g, const int L[4]);
+
+struct Site {
+  int c[4];
+  int lex;
+};
+
+// coordinates of checkerboard site cb of parity par
+__device__ __forceinline__ Site site_of(const Geom &g, int cb, int par) {
+  Site s;
+  uint32_t q1 = g.dxh.div((uint32_t)cb);
+  int xh = cb - (int)q1 * (g.L[0] >> 1);
+  uint32_t q2 = g.dy.div(q1);
+  s.c[1] = (int)q1 - (int)q2 * g.L[1];
+  uint32_t q3 = g.dz.div(q2);
+  s.c[2] = (int)q2 - (int)q3 * g.L[2];
+  s.c[3] = (int)q3;
+  int xb = (s.c[1] + s.c[2] + s.c[3] + par) & 1;
+  s.c[0] = 2 * xh + xb;
+  s.lex = 2 * cb + xb;
+  return s;
+}
+
+// checkerboard index of the neighbour at hop h (|h| <= 3 < L) along mu
+__device__ __forceinline__ int nbr_cb(const Geom &g, const Site &s, int mu, int h) {
+  int c = s.c[mu] + h;
+  int L = g.L[mu];
+  c += (c < 0) ? L : 0;
+  c -= (c >= L) ? L : 0;
+  return (s.lex + (c - s.c[mu]) * g.stride[mu]) >> 1;
+}
+
+// blocked link index: element k (0..8) of term t for site cb
+__device__ __forceinline__ int64_t link_idx(int cb, int t, int k, int nterm) {
+  return ((((int64_t)(cb >> 6) * nterm + t) * 9 + k) << 6) + (cb & 63);
+}
+__host__ inline int64_t link_idx_h(int64_t cb, int t, int k, int nterm) {
+  return ((((cb >> 6) * nterm + t) * 9 + k) << 6) + (cb & 63);
+}
+
+// XCD-aware block order: block b runs on die b % 8; give each die one
+// contiguous range of logical blocks so that neighbouring sites share an L2.
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+  int xcd = b % NXCD, idx = b / NXCD;
+  int q = nb / NXCD, r = nb % NXCD;
+  return xcd * q + (xcd < r ? xcd : r) + idx;
+}
+
+// --------------------------------------------------------------- precisions
+// Complex loads/stores for the three storage precisions.  Accumulation is in
+// double for FP64 storage and float for FP32/FP16 storage.
+template <int P> struct Prec;
+template <> struct Prec<QAMD_F64> {
+  typedef double2 S;   // stored complex
+  typedef double R;    // arithmetic type
+  typedef double2 C;
+};
+template <> struct Prec<QAMD_F32> {
+  typedef float2 S;
+  typedef float R;
+  typedef float2 C;
+};
+template <> struct Prec<QAMD_F16> {
+  typedef __half2 S;
+  typedef float R;
+  typedef float2 C;
+};
+
+__device__ __forceinline__ double2 to_c(double2 v, double) { return v; }
+__device__ __forceinline__ float2 to_c(float2 v, float) { return v; }
+__device__ __forceinline__ float2 to_c(__half2 v, float) { return __half22float2(v); }
+
+// non-temporal (streaming) loads of data read exactly once
+typedef double dvec2_t __attribute__((ext_vector_type(2)));
+typedef float fvec2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ double2 ntload(const double2 *p) {
+  dvec2_t v = __builtin_nontemporal_load(reinterpret_cast<const dvec2_t *>(p));
+  return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ float2 ntload(const float2 *p) {
+  fvec2_t v = __builtin_nontemporal_load(reinterpret_cast<const fvec2_t *>(p));
+  return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ __half2 ntload(const __half2 *p) {
+  unsigned u = __builtin_nontemporal_load(reinterpret_cast<const unsigned *>(p));
+  return *reinterpret_cast<__half2 *>(&u);
+}
+
+// ---------------------------------------------------------------- fields
+struct Field {             // one parity of a colour-vector / spinor field
+  int prec;                // QAMD_F64/F32/F16
+  int ncomp;               // complex components per site and rhs (3 or 12)
+  int nrhs;
+  int parity;              // parity the field lives on (0/1)
+  int rhs_inner;           // 1: rhs fastest (spinor blocks sharing links)
+  int64_t vp;              // sites per parity, padded to 64
+  const Geom *g;           // device copy of the geometry
+  void *data;              // complex numbers, see cidx()
+  float *norm;             // FP16 only, see nidx()
+  __host__ __device__ int64_t elems() const { return (int64_t)nrhs * ncomp * vp; }
+  // complex component j of rhs r at checkerboard site cb
+  __host__ __device__ int64_t cidx(int r, int j, int64_t cb) const {
+    return rhs_inner ? ((int64_t)j * vp + cb) * nrhs + r : ((int64_t)r * ncomp + j) * vp + cb;
+  }
+  __host__ __device__ int64_t nidx(int r, int64_t cb) const {
+    return rhs_inner ? cb * nrhs + r : (int64_t)r * vp + cb;
+  }
+};
+
+// -------------------------------------------------------------- reductions
+// Deterministic grid reduction: each block stores its partial sums, the last
+// block to finish (device-scope atomic ticket) adds the partials in block
+// order and runs a scalar epilogue.  The ticket is reset by that block.
+constexpr int MAXRED = 8;
+struct RedScratch {
+  double *partials;     // [MAXRED][max_blocks]
+  unsigned *ticket;
+  int max_blocks;
+};
+RedScratch &red_scratch();
+int red_reserve(int nblocks);
+
+template <int N>
+__device__ __forceinline__ void wave_sum(double (&v)[N]) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+    for (int i = 0; i < N; i++) v[i] += __shfl_xor(v[i], o, 64);
+}
+
+// Reduce v over the block; thread 0 gets the block total.  blockDim.x must be
+// a multiple of 64 and at most 1024.
+template <int N>
+__device__ __forceinline__ void block_sum(double (&v)[N]) {
+  __shared__ double sh[16][N];
+  wave_sum(v);
+  int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  int nw = blockDim.x >> 6;
+  if (l == 0)
+#pragma unroll
+    for (int i = 0; i < N; i++) sh[w][i] = v[i];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+      double s = 0;
+      for (int k = 0; k < nw; k++) s += sh[k][i];
+      v[i] = s;
+    }
+  }
+  __syncthreads();
+}
+
+// Grid reduction; 'fin' is called by thread 0 of the last block with the
+// totals.  Every block of the grid must call this exactly once.
+template <int N, class Fin>
+__device__ __forceinline__ void grid_sum(double (&v)[N], RedScratch rs, Fin fin) {
+  __shared__ bool last;
+  __shared__ double tot[N];
+  block_sum(v);
+  int nb = gridDim.x;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < N; i++) rs.partials[i * rs.max_blocks + blockIdx.x] = v[i];
+    __threadfence();
+    unsigned t = atomicAdd(rs.ticket, 1u);
+    last = (t == (unsigned)nb - 1);
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  // fixed-order sum of the partials: thread k adds blocks k, k+bs, ... in
+  // order, then the block sum adds the per-thread results in thread order
+  double w[N];
+#pragma unroll
+  for (int i = 0; i < N; i++) {
+    double s = 0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x)
+      s += __builtin_nontemporal_load(&rs.partials[i * rs.max_blocks + b]);
+    w[i] = s;
+  }
+  block_sum(w);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < N; i++) tot[i] = w[i];
+    *rs.ticket = 0u;
+    __threadfence();
+    fin(tot);
+  }
+}
+
+}  // namespace qamd

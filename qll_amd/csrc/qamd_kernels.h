@@ -1,0 +1,18 @@
+// Cross-file internal entry points of libqll_amd.
+#pragma once
+#include "qamd_internal.h"
+#include "qamd_objects.h"
+
+namespace qamd {
+
+// fields.hip
+const Geom *dev_geom(qamd_lattice *lat);
+double *scalar_slot();
+
+// dslash_stag.hip / dslash_wilson.hip
+int stag_dslash_launch(const qamd_links *l, Field &out, const Field &in, double a, const Field *w,
+                       double b, double *red_out, hipStream_t st, const int *skip = nullptr);
+int wilson_dslash_launch(const qamd_links *l, Field &out, const Field &in, double a, const Field *w,
+                         double b, double *red_out, hipStream_t st, const int *skip = nullptr);
+
+}  // namespace qamd

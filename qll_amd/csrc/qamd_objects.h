@@ -1,0 +1,36 @@
+// Definitions of the opaque handles of qll_amd.h.
+#pragma once
+#include "qamd_internal.h"
+
+struct qamd_lattice {
+  qamd::Geom g;
+};
+
+struct qamd_field {
+  qamd::Field f;
+  qamd_lattice *lat;
+};
+
+// FP64 gauge field, layout [mu][parity][k 0..8][vh_pad] of double2
+struct qamd_gauge {
+  qamd_lattice *lat;
+  double2 *d;
+};
+
+// Dslash link set for both output parities; data[p] holds the terms used by
+// output sites of parity p in the blocked layout of qamd_internal.h.
+struct qamd_links {
+  qamd_lattice *lat;
+  int kind;     // 0 staggered, 1 Wilson
+  int prec;
+  int nterms;
+  void *data[2];
+  float scale[16];   // FP16 only: per-term scale (stored = link / scale)
+};
+
+namespace qamd {
+inline double2 *gauge_ptr(const qamd_gauge *u, int mu, int par) {
+  return u->d + ((int64_t)(mu * 2 + par) * 9) * u->lat->g.vh_pad;
+}
+size_t prec_bytes(int prec);   // bytes of one stored complex number
+}  // namespace qamd

@@ -1,0 +1,159 @@
+// Runtime: device selection, the library stream, allocation, errors, lattice
+// geometry and the deterministic-reduction scratch.
+#include "qamd_internal.h"
+#include "qamd_objects.h"
+
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+#include <stdlib.h>
+
+namespace qamd {
+
+static char g_err[1024] = "no error";
+static hipStream_t g_stream = nullptr;
+static int g_init = 0;
+static RedScratch g_red = {nullptr, nullptr, 0};
+
+void set_error(const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+  if (getenv("QAMD_VERBOSE")) fprintf(stderr, "qll_amd: %s\n", g_err);
+}
+
+int ensure_init() {
+  if (g_init) return 0;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n < 1) {
+    set_error("no HIP device available (%s)", e == hipSuccess ? "count 0" : hipGetErrorString(e));
+    return -1;
+  }
+  QAMD_HIP(hipSetDevice(0));
+  // default priority stream, non-blocking w.r.t. the null stream
+  QAMD_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+  g_init = 1;
+  return 0;
+}
+
+hipStream_t stream() { return g_stream; }
+
+void *dev_alloc(size_t bytes) {
+  if (ensure_init()) return nullptr;
+  void *p = nullptr;
+  if (bytes == 0) bytes = 256;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) {
+    set_error("hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    return nullptr;
+  }
+  return p;
+}
+
+void dev_free(void *p) {
+  if (p) (void)hipFree(p);
+}
+
+RedScratch &red_scratch() { return g_red; }
+
+int red_reserve(int nblocks) {
+  if (ensure_init()) return -1;
+  if (nblocks <= g_red.max_blocks) return 0;
+  // grow: wait for pending work that may still use the old scratch
+  QAMD_HIP(hipStreamSynchronize(g_stream));
+  dev_free(g_red.partials);
+  dev_free(g_red.ticket);
+  int mb = 1024;
+  while (mb < nblocks) mb *= 2;
+  g_red.partials = (double *)dev_alloc(sizeof(double) * MAXRED * mb);
+  g_red.ticket = (unsigned *)dev_alloc(256);
+  if (!g_red.partials || !g_red.ticket) return -1;
+  QAMD_HIP(hipMemsetAsync(g_red.ticket, 0, 256, g_stream));
+  QAMD_HIP(hipStreamSynchronize(g_stream));
+  g_red.max_blocks = mb;
+  return 0;
+}
+
+void geom_init(Geom &g, const int L[4]) {
+  g.V = 1;
+  for (int i = 0; i < 4; i++) {
+    g.L[i] = L[i];
+    g.V *= L[i];
+  }
+  g.vh = g.V / 2;
+  g.vh_pad = (g.vh + WAVE - 1) / WAVE * WAVE;
+  g.stride[0] = 1;
+  g.stride[1] = L[0];
+  g.stride[2] = L[0] * L[1];
+  g.stride[3] = L[0] * L[1] * L[2];
+  g.dxh.init(L[0] / 2);
+  g.dy.init(L[1]);
+  g.dz.init(L[2]);
+}
+
+}  // namespace qamd
+
+using namespace qamd;
+
+extern "C" {
+
+const char *qamd_last_error(void) { return g_err; }
+
+int qamd_init(void) { return ensure_init(); }
+
+int qamd_sync(void) {
+  if (ensure_init()) return -1;
+  QAMD_HIP(hipStreamSynchronize(g_stream));
+  return 0;
+}
+
+int qamd_device_name(char *buf, int len) {
+  if (ensure_init()) return -1;
+  hipDeviceProp_t p;
+  QAMD_HIP(hipGetDeviceProperties(&p, 0));
+  snprintf(buf, len, "%s (%s, %d CUs)", p.name, p.gcnArchName, p.multiProcessorCount);
+  return 0;
+}
+
+void *qamd_malloc(size_t bytes) { return dev_alloc(bytes); }
+void qamd_free(void *p) { dev_free(p); }
+
+int qamd_memcpy_h2d(void *dst, const void *src, size_t bytes) {
+  if (ensure_init()) return -1;
+  QAMD_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_stream));
+  QAMD_HIP(hipStreamSynchronize(g_stream));
+  return 0;
+}
+
+int qamd_memcpy_d2h(void *dst, const void *src, size_t bytes) {
+  if (ensure_init()) return -1;
+  QAMD_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_stream));
+  QAMD_HIP(hipStreamSynchronize(g_stream));
+  return 0;
+}
+
+qamd_lattice *qamd_lattice_create(const int dims[4]) {
+  for (int i = 0; i < 4; i++) {
+    if (dims[i] < 4 || (dims[i] & 1)) {
+      set_error("lattice extent %d = %d: extents must be even and >= 4", i, dims[i]);
+      return nullptr;
+    }
+  }
+  int64_t V = (int64_t)dims[0] * dims[1] * dims[2] * dims[3];
+  if (V / 2 >= ((int64_t)1 << 30)) {
+    set_error("lattice of %lld sites exceeds the 2^31 site limit", (long long)V);
+    return nullptr;
+  }
+  if (ensure_init()) return nullptr;
+  qamd_lattice *lat = (qamd_lattice *)calloc(1, sizeof(qamd_lattice));
+  geom_init(lat->g, dims);
+  return lat;
+}
+
+void qamd_lattice_destroy(qamd_lattice *lat) { free(lat); }
+
+int64_t qamd_lattice_sites(const qamd_lattice *lat) { return lat ? lat->g.V : 0; }
+
+}  // extern "C"

@@ -1,0 +1,769 @@
+// Krylov solvers on the device: QLL's CG and multishift CG recurrences, and
+// mixed-precision CG with reliable updates.
+//
+// Per CG iteration the work is two Dslash launches (the even-odd normal
+// operator, with |Dp|^2 and |p|^2 reduced in their epilogues) and two fused
+// vector kernels.  All scalars live in a device-side state block: the
+// reductions' last block updates them, the next kernels read them, so the host
+// never sits inside an iteration.  Every kernel checks the state's 'done' flag
+// and returns at once when it is set, which lets the host enqueue iterations
+// in batches and stop exactly where the reference's loop stops.
+#include "qamd_internal.h"
+#include "qamd_objects.h"
+#include "qamd_kernels.h"
+#include "qamd_vec3.h"
+
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+#include <chrono>
+
+namespace qamd {
+
+constexpr int MAXSHIFT = 32;
+
+struct SolverDev {
+  double rsq, rsq_old;     // current and previous |r|^2
+  double nrm[4];           // Dslash reductions: |t|^2, -, |Ap|^2, |p|^2
+  double m2;               // mass^2 of the normal operator
+  double rsqstop;
+  double alpha, beta;
+  int its, itnlim, done, mode;   // mode 0: QLL rescaled CG, 1: textbook CG
+  int auto_stop;
+  // multishift scalars (cgmsSolveT, lib/cgms.c:6-179)
+  int nshift;
+  double shifts[MAXSHIFT];
+  double a[MAXSHIFT], b[MAXSHIFT], s[MAXSHIFT], bo[MAXSHIFT];
+  double z[MAXSHIFT], zo[MAXSHIFT], zn[MAXSHIFT];
+  double pa[MAXSHIFT], pzn[MAXSHIFT];   // coefficients for the p update
+};
+
+template <int P> struct VT {
+  typedef typename Prec<P>::S S;
+  typedef typename Prec<P>::C C;
+  typedef typename Prec<P>::R R;
+  __device__ static void ld(const Field &f, int i, C (&v)[3]) {
+    VecLd<P>::ld((const S *)f.data, f.norm, f.g->vh_pad, i, v, (R)1);
+  }
+  __device__ static void st(Field &f, int i, const C (&v)[3]) {
+    VecSt<P>::st((S *)f.data, f.norm, f.g->vh_pad, i, v);
+  }
+};
+
+// ---------------------------------------------------------------- CG kernels
+template <int P>
+__global__ void __launch_bounds__(256) k_cg_p(Field p, Field r, SolverDev *S) {
+  typedef typename Prec<P>::C C;
+  typedef typename Prec<P>::R R;
+  if (S->done) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= p.g->vh) return;
+  C pv[3], rv[3];
+  VT<P>::ld(p, i, pv);
+  VT<P>::ld(r, i, rv);
+  if (S->mode == 0) {   // p += r / rsq  (lib/cg.c:86-87)
+    R be = (R)(1.0 / S->rsq);
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      pv[j].x = fma(be, rv[j].x, pv[j].x);
+      pv[j].y = fma(be, rv[j].y, pv[j].y);
+    }
+  } else {              // p = r + beta p
+    R be = (R)S->beta;
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      pv[j].x = fma(be, pv[j].x, rv[j].x);
+      pv[j].y = fma(be, pv[j].y, rv[j].y);
+    }
+  }
+  VT<P>::st(p, i, pv);
+}
+
+struct FinCG {
+  SolverDev *S;
+  __device__ void operator()(const double *t) const {
+    S->rsq_old = S->rsq;
+    S->rsq = t[0];
+    S->beta = S->rsq / S->rsq_old;
+    S->its += 1;
+    if (S->its >= S->itnlim || (S->auto_stop && S->rsq < S->rsqstop)) S->done = 1;
+  }
+};
+
+// x += alpha p; r -= alpha Ap; |r|^2   (lib/cg.c:96-110)
+template <int P, int PX>
+__global__ void __launch_bounds__(256) k_cg_xr(Field x, Field p, Field r, Field Ap, SolverDev *S, RedScratch rs) {
+  typedef typename Prec<P>::C C;
+  typedef typename Prec<P>::R R;
+  typedef typename Prec<PX>::C CX;
+  typedef typename Prec<PX>::R RX;
+  if (S->done) return;
+  double pAp = S->m2 * S->nrm[3] + S->nrm[0];
+  double alpha = (S->mode == 0) ? 1.0 / pAp : S->rsq / pAp;
+  double red[1] = {0.0};
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < p.g->vh) {
+    C pv[3], rv[3], av[3];
+    CX xv[3];
+    VT<P>::ld(p, i, pv);
+    VT<P>::ld(r, i, rv);
+    VT<P>::ld(Ap, i, av);
+    VT<PX>::ld(x, i, xv);
+    R al = (R)alpha;
+    RX alx = (RX)alpha;
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      xv[j].x = fma(alx, (RX)pv[j].x, xv[j].x);
+      xv[j].y = fma(alx, (RX)pv[j].y, xv[j].y);
+      rv[j].x = fma(-al, av[j].x, rv[j].x);
+      rv[j].y = fma(-al, av[j].y, rv[j].y);
+    }
+    VT<PX>::st(x, i, xv);
+    VT<P>::st(r, i, rv);
+    // the residual norm is taken of the stored (rounded) vector
+    C rr[3];
+    VT<P>::ld(r, i, rr);
+#pragma unroll
+    for (int j = 0; j < 3; j++) red[0] += (double)rr[j].x * rr[j].x + (double)rr[j].y * rr[j].y;
+  }
+  grid_sum<1>(red, rs, FinCG{S});
+}
+
+// reliable update, part 1: xhi += xs; xs = 0
+template <int PX>
+__global__ void k_ru_accum(Field xhi, Field xs) {
+  typedef typename Prec<PX>::C CX;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= xhi.g->vh) return;
+  double2 h[3];
+  CX s[3];
+  VT<QAMD_F64>::ld(xhi, i, h);
+  VT<PX>::ld(xs, i, s);
+#pragma unroll
+  for (int j = 0; j < 3; j++) {
+    h[j].x += (double)s[j].x;
+    h[j].y += (double)s[j].y;
+    s[j].x = 0;
+    s[j].y = 0;
+  }
+  VT<QAMD_F64>::st(xhi, i, h);
+  VT<PX>::st(xs, i, s);
+}
+
+struct FinTrue {
+  SolverDev *S;
+  __device__ void operator()(const double *t) const {
+    S->rsq = t[0];
+    S->beta = S->rsq / S->rsq_old;
+  }
+};
+
+// reliable update, part 2: rhi = b - Ax; rs = rhi; |rhi|^2
+template <int PS>
+__global__ void k_ru_resid(Field rhi, Field b, Field Ax, Field rs, SolverDev *S, RedScratch red_s) {
+  typedef typename Prec<PS>::C CS;
+  typedef typename Prec<PS>::R RS;
+  double red[1] = {0.0};
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < rhi.g->vh) {
+    double2 bv[3], av[3];
+    VT<QAMD_F64>::ld(b, i, bv);
+    VT<QAMD_F64>::ld(Ax, i, av);
+    CS sv[3];
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      bv[j].x -= av[j].x;
+      bv[j].y -= av[j].y;
+      red[0] += bv[j].x * bv[j].x + bv[j].y * bv[j].y;
+      sv[j].x = (RS)bv[j].x;
+      sv[j].y = (RS)bv[j].y;
+    }
+    VT<QAMD_F64>::st(rhi, i, bv);
+    VT<PS>::st(rs, i, sv);
+  }
+  grid_sum<1>(red, red_s, FinTrue{S});
+}
+
+// ------------------------------------------------------- multishift kernels
+// coefficients of cgmsSolveT (lib/cgms.c:78-89) from the current state
+__device__ inline void ms_coefs(const SolverDev *S, double pkp, double *b, double *zn) {
+  b[0] = S->rsq / pkp;
+  zn[0] = 1;
+  for (int i = 1; i < S->nshift; i++) {
+    double z = S->z[i], zo = S->zo[i], bo0 = S->bo[0];
+    double zni = z * zo * bo0;
+    double c1 = b[0] * S->a[0] * (zo - z);
+    c1 += zo * bo0 * (1 + S->shifts[i] * b[0]);
+    if (c1 != 0.0) zni /= c1;
+    else zni = 0;
+    double bi;
+    if (z != 0.0) bi = b[0] * zni / z;
+    else zni = bi = 0;
+    zn[i] = zni;
+    b[i] = bi;
+  }
+}
+
+struct MSPtrs {
+  void *x[MAXSHIFT];
+  float *xn[MAXSHIFT];
+  void *p[MAXSHIFT];
+  float *pn[MAXSHIFT];
+};
+
+struct FinMS {
+  SolverDev *S;
+  double pkp;
+  __device__ void operator()(const double *t) const {
+    double b[MAXSHIFT], zn[MAXSHIFT];
+    ms_coefs(S, pkp, b, zn);
+    double oldrsq = S->rsq;
+    S->rsq = t[0];
+    S->its += 1;
+    if (S->its >= S->itnlim || S->rsqstop <= 0 || S->rsq < S->rsqstop) S->done = 1;
+    double a0 = S->rsq / oldrsq;
+    S->pa[0] = a0;
+    for (int i = 1; i < S->nshift; i++) {
+      double c2 = S->z[i] * b[0];
+      S->pa[i] = (c2 != 0.0) ? a0 * zn[i] * b[i] / c2 : 0;
+    }
+    for (int i = 0; i < S->nshift; i++) {
+      S->pzn[i] = zn[i];
+      S->a[i] = S->pa[i];
+      S->bo[i] = b[i];
+      S->zo[i] = S->z[i];
+      S->z[i] = zn[i];
+    }
+  }
+};
+
+// x_i += b_i s_i p_i for all shifts; r -= b_0 s_0 Ap; |r|^2
+template <int P>
+__global__ void __launch_bounds__(256) k_ms_x(MSPtrs V, Field r, Field Ap, Field proto, SolverDev *S, RedScratch rs) {
+  typedef typename Prec<P>::C C;
+  typedef typename Prec<P>::R R;
+  typedef typename Prec<P>::S Sx;
+  if (S->done) return;
+  double pkp = (S->m2 * S->nrm[3] + S->nrm[0]) * S->s[0] * S->s[0];
+  if (pkp <= 0) {   // lib/cgms.c:76: loss of precision, leave
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      S->its += 1;
+      S->done = 1;
+    }
+    return;
+  }
+  double b[MAXSHIFT], zn[MAXSHIFT];
+  ms_coefs(S, pkp, b, zn);
+  const int64_t vp = proto.g->vh_pad;
+  double red[1] = {0.0};
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < proto.g->vh) {
+    for (int k = 0; k < S->nshift; k++) {
+      C xv[3], pv[3];
+      VecLd<P>::ld((const Sx *)V.x[k], V.xn[k], vp, i, xv, (R)1);
+      VecLd<P>::ld((const Sx *)V.p[k], V.pn[k], vp, i, pv, (R)1);
+      R t = (R)(b[k] * S->s[k]);
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        xv[j].x = fma(t, pv[j].x, xv[j].x);
+        xv[j].y = fma(t, pv[j].y, xv[j].y);
+      }
+      VecSt<P>::st((Sx *)V.x[k], V.xn[k], vp, i, xv);
+    }
+    C rv[3], av[3];
+    VT<P>::ld(r, i, rv);
+    VT<P>::ld(Ap, i, av);
+    R t0 = (R)(b[0] * S->s[0]);
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      rv[j].x = fma(-t0, av[j].x, rv[j].x);
+      rv[j].y = fma(-t0, av[j].y, rv[j].y);
+    }
+    VT<P>::st(r, i, rv);
+    C rr[3];
+    VT<P>::ld(r, i, rr);
+#pragma unroll
+    for (int j = 0; j < 3; j++) red[0] += (double)rr[j].x * rr[j].x + (double)rr[j].y * rr[j].y;
+  }
+  grid_sum<1>(red, rs, FinMS{S, pkp});
+}
+
+// p_0 = a_0 p_0 + r;  p_i = a_i p_i + zn_i r   (lib/cgms.c:141-155)
+template <int P>
+__global__ void __launch_bounds__(256) k_ms_p(MSPtrs V, Field r, SolverDev *S) {
+  typedef typename Prec<P>::C C;
+  typedef typename Prec<P>::R R;
+  typedef typename Prec<P>::S Sx;
+  if (S->done) return;
+  const int64_t vp = r.g->vh_pad;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= r.g->vh) return;
+  C rv[3];
+  VT<P>::ld(r, i, rv);
+  for (int k = 0; k < S->nshift; k++) {
+    if (k > 0 && S->s[k] == 0) continue;
+    C pv[3];
+    VecLd<P>::ld((const Sx *)V.p[k], V.pn[k], vp, i, pv, (R)1);
+    R a = (R)S->pa[k], z = (R)(k == 0 ? 1.0 : S->pzn[k]);
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      pv[j].x = fma(a, pv[j].x, z * rv[j].x);
+      pv[j].y = fma(a, pv[j].y, z * rv[j].y);
+    }
+    VecSt<P>::st((Sx *)V.p[k], V.pn[k], vp, i, pv);
+  }
+}
+
+// ------------------------------------------------------------ host helpers
+struct Work {
+  qamd_lattice *lat;
+  int prec;
+  qamd_field *t, *Ap, *r, *p;
+  SolverDev *S;       // device
+  SolverDev *Sh;      // pinned host mirror
+};
+
+static void work_free(Work &w) {
+  qamd_field_destroy(w.t);
+  qamd_field_destroy(w.Ap);
+  qamd_field_destroy(w.r);
+  qamd_field_destroy(w.p);
+  dev_free(w.S);
+  if (w.Sh) (void)hipHostFree(w.Sh);
+  memset(&w, 0, sizeof w);
+}
+
+static int work_init(Work &w, qamd_lattice *lat, int prec) {
+  memset(&w, 0, sizeof w);
+  w.lat = lat;
+  w.prec = prec;
+  w.t = qamd_field_create(lat, prec, 3, 1, 1);
+  w.Ap = qamd_field_create(lat, prec, 3, 1, 0);
+  w.r = qamd_field_create(lat, prec, 3, 1, 0);
+  w.p = qamd_field_create(lat, prec, 3, 1, 0);
+  w.S = (SolverDev *)dev_alloc(sizeof(SolverDev));
+  if (hipHostMalloc((void **)&w.Sh, sizeof(SolverDev), hipHostMallocDefault) != hipSuccess) w.Sh = nullptr;
+  if (!w.t || !w.Ap || !w.r || !w.p || !w.S || !w.Sh) {
+    if (!w.Sh) set_error("hipHostMalloc failed");
+    work_free(w);
+    return -1;
+  }
+  if (red_reserve((int)((lat->g.vh + 255) / 256))) {
+    work_free(w);
+    return -1;
+  }
+  return 0;
+}
+
+static int push_state(Work &w) {
+  QAMD_HIP(hipMemcpyAsync(w.S, w.Sh, sizeof(SolverDev), hipMemcpyHostToDevice, stream()));
+  return 0;
+}
+static int pull_state(Work &w) {
+  QAMD_HIP(hipMemcpyAsync(w.Sh, w.S, sizeof(SolverDev), hipMemcpyDeviceToHost, stream()));
+  QAMD_HIP(hipStreamSynchronize(stream()));
+  return 0;
+}
+
+static inline dim3 grid1(const qamd_lattice *lat) { return dim3((unsigned)((lat->g.vh + 255) / 256)); }
+
+// one normal-operator application with reductions into S->nrm
+static int normal_op(const qamd_links *l, Field &Ap, Field &p, Field &t, SolverDev *S, const int *skip) {
+  if (stag_dslash_launch(l, t, p, 0.0, nullptr, 1.0, &S->nrm[0], stream(), skip)) return -1;
+  // Ap = m^2 p - D tmp: the mass enters through w, a = m^2 from the state
+  return 0;
+}
+
+template <int P>
+static void launch_cg_p(Work &w, qamd_field *p, qamd_field *r) {
+  hipLaunchKernelGGL(k_cg_p<P>, grid1(w.lat), dim3(256), 0, stream(), p->f, r->f, w.S);
+}
+static int cg_p(Work &w, qamd_field *p, qamd_field *r) {
+  switch (p->f.prec) {
+    case QAMD_F64: launch_cg_p<QAMD_F64>(w, p, r); break;
+    case QAMD_F32: launch_cg_p<QAMD_F32>(w, p, r); break;
+    default: launch_cg_p<QAMD_F16>(w, p, r); break;
+  }
+  QAMD_HIP(hipGetLastError());
+  return 0;
+}
+
+static int cg_xr(Work &w, qamd_field *x, qamd_field *p, qamd_field *r, qamd_field *Ap) {
+  int P = p->f.prec, PX = x->f.prec;
+#define XR(A, B) hipLaunchKernelGGL((k_cg_xr<A, B>), grid1(w.lat), dim3(256), 0, stream(), x->f, p->f, r->f, Ap->f, w.S, red_scratch())
+  if (P == QAMD_F64 && PX == QAMD_F64) XR(QAMD_F64, QAMD_F64);
+  else if (P == QAMD_F32 && PX == QAMD_F32) XR(QAMD_F32, QAMD_F32);
+  else if (P == QAMD_F16 && PX == QAMD_F32) XR(QAMD_F16, QAMD_F32);
+  else if (P == QAMD_F16 && PX == QAMD_F16) XR(QAMD_F16, QAMD_F16);
+  else {
+    set_error("cg_xr: unsupported precision pair %d/%d", P, PX);
+    return -1;
+  }
+#undef XR
+  QAMD_HIP(hipGetLastError());
+  return 0;
+}
+
+// Ap = m^2 p - D_eo D_oe p with the reductions of dslashp2 (lib/stagDslash.c:546-575)
+static int apply_normal(const qamd_links *l, Work &w, qamd_field *Ap, qamd_field *p, qamd_field *t, double m2,
+                        const int *skip) {
+  if (stag_dslash_launch(l, t->f, p->f, 0.0, nullptr, 1.0, &w.S->nrm[0], stream(), skip)) return -1;
+  if (stag_dslash_launch(l, Ap->f, t->f, m2, &p->f, -1.0, &w.S->nrm[2], stream(), skip)) return -1;
+  return 0;
+}
+
+static double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+static int check_vec(const qamd_field *f, int prec, int parity, const char *what) {
+  if (!f || f->f.ncomp != 3 || f->f.nrhs != 1 || f->f.parity != parity || (prec >= 0 && f->f.prec != prec)) {
+    set_error("%s: expected a single-rhs colour vector of parity %d%s", what, parity,
+              prec >= 0 ? " at the links' precision" : "");
+    return -1;
+  }
+  return 0;
+}
+
+// FP64 true residual |b - (m2 - DD) x|^2 / |b|^2 using the work vectors
+static int true_resid(const qamd_links *l64, Work &w64, const qamd_field *x, const qamd_field *b, double m2,
+                      double *rel) {
+  if (apply_normal(l64, w64, w64.Ap, (qamd_field *)x, w64.t, m2, nullptr)) return -1;
+  if (qamd_field_copy(w64.r, b)) return -1;
+  if (qamd_field_axpby(-1.0, w64.Ap, 1.0, w64.r)) return -1;
+  double rr, bb;
+  if (qamd_field_norm2(w64.r, &rr) || qamd_field_norm2(b, &bb)) return -1;
+  *rel = bb > 0 ? rr / bb : 0.0;
+  return 0;
+}
+
+}  // namespace qamd
+
+using namespace qamd;
+
+extern "C" {
+
+int qamd_stag_cg(const qamd_links *l, qamd_field *x, const qamd_field *b, double mass, double rsq, int maxits,
+                 qamd_solve_info *info) {
+  if (!l || l->kind != 0) { set_error("stag_cg: staggered links required"); return -1; }
+  if (check_vec(x, l->prec, 0, "stag_cg x") || check_vec(b, l->prec, 0, "stag_cg b")) return -1;
+  double t0 = now_s();
+  Work w;
+  if (work_init(w, l->lat, l->prec)) return -1;
+  int rc = -1;
+  do {
+    // x = 0, p = 0, r = b, rsq = |b|^2   (lib/cg.c:64-71)
+    if (qamd_field_zero(x) || qamd_field_zero(w.p) || qamd_field_copy(w.r, b)) break;
+    double bsq;
+    if (qamd_field_norm2(w.r, &bsq)) break;
+    memset(w.Sh, 0, sizeof(SolverDev));
+    w.Sh->rsq = bsq;
+    w.Sh->m2 = mass * mass;
+    w.Sh->rsqstop = rsq * bsq;
+    w.Sh->itnlim = maxits > 0 ? maxits : 1;
+    w.Sh->mode = 0;
+    w.Sh->auto_stop = 1;
+    if (bsq == 0) {   // zero source: the reference divides by zero; return x = 0
+      w.Sh->done = 1;
+    }
+    if (push_state(w)) break;
+    const int batch = 16;
+    int err = 0;
+    while (!w.Sh->done) {
+      for (int k = 0; k < batch && !err; k++) {
+        err |= cg_p(w, w.p, w.r);
+        err |= apply_normal(l, w, w.Ap, w.p, w.t, mass * mass, &w.S->done);
+        err |= cg_xr(w, x, w.p, w.r, w.Ap);
+      }
+      if (err || pull_state(w)) {
+        err = 1;
+        break;
+      }
+    }
+    if (err) break;
+    if (info) {
+      memset(info, 0, sizeof *info);
+      info->iterations = w.Sh->its;
+      info->iter_rsq = bsq > 0 ? w.Sh->rsq / bsq : 0;
+      info->true_rsq = -1;
+      info->converged = bsq == 0 || w.Sh->rsq < w.Sh->rsqstop;
+      info->seconds = now_s() - t0;
+    }
+    rc = 0;
+  } while (0);
+  work_free(w);
+  return rc;
+}
+
+// Mixed-precision CG with reliable updates (textbook CG in the sloppy
+// precision; FP64 x and true residual at every update).  If x is nonzero on
+// entry it is used as the initial guess.
+static int cg_mixed_core(const qamd_links *l64, const qamd_links *ls, qamd_field *x, const qamd_field *b,
+                         double m2, double rsq, int maxits, double delta, qamd_solve_info *info) {
+  double t0 = now_s();
+  qamd_lattice *lat = l64->lat;
+  int ps = ls->prec;
+  int px = ps == QAMD_F16 ? QAMD_F32 : ps;
+  Work w64, ws;
+  if (work_init(w64, lat, QAMD_F64)) return -1;
+  if (work_init(ws, lat, ps)) {
+    work_free(w64);
+    return -1;
+  }
+  qamd_field *xs = qamd_field_create(lat, px, 3, 1, 0);
+  qamd_field *Ax = w64.Ap;
+  int rc = -1, nupd = 0, its = 0;
+  double bsq = 0, r2 = 0;
+  do {
+    if (!xs) break;
+    if (qamd_field_norm2(b, &bsq)) break;
+    if (bsq == 0) {
+      if (qamd_field_zero(x)) break;
+      rc = 0;
+      break;
+    }
+    double stop = rsq * bsq;
+    memset(ws.Sh, 0, sizeof(SolverDev));
+    ws.Sh->m2 = m2;
+    ws.Sh->itnlim = 1 << 30;
+    ws.Sh->mode = 1;
+    ws.Sh->auto_stop = 0;
+    if (push_state(ws)) break;
+    // initial true residual r = b - A x (x may be a guess)
+    auto reliable = [&](void) -> int {
+      if (apply_normal(l64, w64, Ax, x, w64.t, m2, nullptr)) return -1;
+      switch (ps) {
+        case QAMD_F32:
+          hipLaunchKernelGGL(k_ru_resid<QAMD_F32>, grid1(lat), dim3(256), 0, stream(), w64.r->f, b->f, Ax->f,
+                             ws.r->f, ws.S, red_scratch());
+          break;
+        default:
+          hipLaunchKernelGGL(k_ru_resid<QAMD_F16>, grid1(lat), dim3(256), 0, stream(), w64.r->f, b->f, Ax->f,
+                             ws.r->f, ws.S, red_scratch());
+          break;
+      }
+      QAMD_HIP(hipGetLastError());
+      return pull_state(ws);
+    };
+    ws.Sh->rsq_old = 1.0;
+    if (push_state(ws) || reliable()) break;
+    r2 = ws.Sh->rsq;
+    if (r2 < stop) {
+      rc = 0;
+      break;
+    }
+    // p = r
+    if (qamd_field_copy(ws.p, ws.r) || qamd_field_zero(xs)) break;
+    double maxrr = sqrt(r2);
+    int err = 0;
+    while (its < maxits) {
+      err |= apply_normal(ls, ws, ws.Ap, ws.p, ws.t, m2, nullptr);
+      err |= cg_xr(ws, xs, ws.p, ws.r, ws.Ap);
+      if (err || pull_state(ws)) {
+        err = 1;
+        break;
+      }
+      its++;
+      double r2s = ws.Sh->rsq;
+      double rn = sqrt(r2s);
+      if (rn > maxrr) maxrr = rn;
+      if (rn < delta * maxrr || r2s < stop || its >= maxits) {
+        // reliable update: x += xs, r = b - A x in FP64, r_sloppy = r
+        if (px == QAMD_F32)
+          hipLaunchKernelGGL(k_ru_accum<QAMD_F32>, grid1(lat), dim3(256), 0, stream(), x->f, xs->f);
+        else
+          hipLaunchKernelGGL(k_ru_accum<QAMD_F64>, grid1(lat), dim3(256), 0, stream(), x->f, xs->f);
+        if (hipGetLastError() != hipSuccess || reliable()) {
+          err = 1;
+          break;
+        }
+        nupd++;
+        r2 = ws.Sh->rsq;
+        if (r2 < stop) break;
+        maxrr = sqrt(r2);
+      }
+      if (cg_p(ws, ws.p, ws.r)) {
+        err = 1;
+        break;
+      }
+    }
+    if (err) break;
+    rc = 0;
+  } while (0);
+  if (info && rc == 0) {
+    memset(info, 0, sizeof *info);
+    info->iterations = its;
+    info->reliable_updates = nupd;
+    info->true_rsq = bsq > 0 ? r2 / bsq : 0;
+    info->iter_rsq = info->true_rsq;
+    info->converged = bsq == 0 || r2 < rsq * bsq;
+    info->seconds = now_s() - t0;
+  }
+  qamd_field_destroy(xs);
+  work_free(ws);
+  work_free(w64);
+  return rc;
+}
+
+int qamd_stag_cg_mixed(const qamd_links *l64, const qamd_links *ls, qamd_field *x, const qamd_field *b,
+                       double mass, double rsq, int maxits, double delta, qamd_solve_info *info) {
+  if (!l64 || !ls || l64->kind != 0 || ls->kind != 0 || l64->prec != QAMD_F64 || ls->prec == QAMD_F64 ||
+      l64->lat->g.V != ls->lat->g.V) {
+    set_error("stag_cg_mixed: need FP64 links and FP32/FP16 sloppy links on one lattice");
+    return -1;
+  }
+  if (check_vec(x, QAMD_F64, 0, "cg_mixed x") || check_vec(b, QAMD_F64, 0, "cg_mixed b")) return -1;
+  if (delta <= 0 || delta >= 1) delta = 0.1;
+  if (qamd_field_zero(x)) return -1;
+  return cg_mixed_core(l64, ls, x, b, mass * mass, rsq, maxits, delta, info);
+}
+
+int qamd_stag_cgms(const qamd_links *l, qamd_field *const x[], const qamd_field *b, double mass,
+                   const double shifts[], int n, double rsq, int maxits, qamd_solve_info *info) {
+  if (!l || l->kind != 0) { set_error("stag_cgms: staggered links required"); return -1; }
+  if (n < 1 || n > MAXSHIFT) { set_error("stag_cgms: 1..%d shifts", MAXSHIFT); return -1; }
+  if (check_vec(b, l->prec, 0, "stag_cgms b")) return -1;
+  for (int i = 0; i < n; i++)
+    if (check_vec(x[i], l->prec, 0, "stag_cgms x")) return -1;
+  double t0 = now_s();
+  Work w;
+  if (work_init(w, l->lat, l->prec)) return -1;
+  qamd_field *ps[MAXSHIFT] = {nullptr};
+  int rc = -1;
+  do {
+    ps[0] = w.p;
+    int ok = 1;
+    for (int i = 1; i < n; i++) {
+      ps[i] = qamd_field_create(l->lat, l->prec, 3, 1, 0);
+      if (!ps[i]) ok = 0;
+    }
+    if (!ok) break;
+    double insq;
+    if (qamd_field_norm2(b, &insq)) break;
+    int err = 0;
+    for (int i = 0; i < n; i++) err |= qamd_field_zero(x[i]);
+    if (err) break;
+    if (insq == 0) {   // lib/cgms.c:25-30
+      if (info) memset(info, 0, sizeof *info);
+      rc = 0;
+      break;
+    }
+    if (qamd_field_copy(w.r, b)) break;
+    for (int i = 0; i < n; i++) err |= qamd_field_copy(ps[i], w.r);
+    if (err) break;
+    memset(w.Sh, 0, sizeof(SolverDev));
+    w.Sh->nshift = n;
+    for (int i = 0; i < n; i++) {
+      w.Sh->shifts[i] = shifts[i];
+      w.Sh->zo[i] = w.Sh->z[i] = 1;
+      w.Sh->bo[i] = -1;
+      w.Sh->a[i] = 0;
+      w.Sh->s[i] = 1;
+    }
+    w.Sh->rsq = insq;
+    w.Sh->rsqstop = rsq * insq;
+    w.Sh->m2 = mass * mass;
+    w.Sh->itnlim = maxits > 0 ? maxits : 1;
+    if (push_state(w)) break;
+    MSPtrs V;
+    memset(&V, 0, sizeof V);
+    for (int i = 0; i < n; i++) {
+      V.x[i] = x[i]->f.data;
+      V.xn[i] = x[i]->f.norm;
+      V.p[i] = ps[i]->f.data;
+      V.pn[i] = ps[i]->f.norm;
+    }
+    const int batch = 8;
+    while (!w.Sh->done && !err) {
+      for (int k = 0; k < batch && !err; k++) {
+        err |= apply_normal(l, w, w.Ap, ps[0], w.t, mass * mass, &w.S->done);
+        switch (l->prec) {
+          case QAMD_F64:
+            hipLaunchKernelGGL(k_ms_x<QAMD_F64>, grid1(l->lat), dim3(256), 0, stream(), V, w.r->f, w.Ap->f, w.r->f, w.S, red_scratch());
+            hipLaunchKernelGGL(k_ms_p<QAMD_F64>, grid1(l->lat), dim3(256), 0, stream(), V, w.r->f, w.S);
+            break;
+          case QAMD_F32:
+            hipLaunchKernelGGL(k_ms_x<QAMD_F32>, grid1(l->lat), dim3(256), 0, stream(), V, w.r->f, w.Ap->f, w.r->f, w.S, red_scratch());
+            hipLaunchKernelGGL(k_ms_p<QAMD_F32>, grid1(l->lat), dim3(256), 0, stream(), V, w.r->f, w.S);
+            break;
+          default:
+            hipLaunchKernelGGL(k_ms_x<QAMD_F16>, grid1(l->lat), dim3(256), 0, stream(), V, w.r->f, w.Ap->f, w.r->f, w.S, red_scratch());
+            hipLaunchKernelGGL(k_ms_p<QAMD_F16>, grid1(l->lat), dim3(256), 0, stream(), V, w.r->f, w.S);
+            break;
+        }
+        if (hipGetLastError() != hipSuccess) err = 1;
+      }
+      if (err || pull_state(w)) err = 1;
+    }
+    if (err) break;
+    if (info) {
+      memset(info, 0, sizeof *info);
+      info->iterations = w.Sh->its;
+      info->iter_rsq = w.Sh->rsq / insq;
+      info->true_rsq = -1;
+      info->converged = w.Sh->rsq < w.Sh->rsqstop;
+      info->seconds = now_s() - t0;
+    }
+    rc = 0;
+  } while (0);
+  for (int i = 1; i < n; i++) qamd_field_destroy(ps[i]);
+  work_free(w);
+  return rc;
+}
+
+int qamd_stag_cgms_mixed(const qamd_links *l64, const qamd_links *ls, qamd_field *const x[], const qamd_field *b,
+                         double mass, const double shifts[], int n, double rsq, int maxits, qamd_solve_info *info) {
+  if (!l64 || !ls || l64->prec != QAMD_F64 || ls->prec == QAMD_F64 || l64->kind != 0 || ls->kind != 0) {
+    set_error("stag_cgms_mixed: need FP64 and sloppy staggered links");
+    return -1;
+  }
+  if (n < 1 || n > MAXSHIFT) { set_error("stag_cgms_mixed: 1..%d shifts", MAXSHIFT); return -1; }
+  for (int i = 0; i < n; i++)
+    if (check_vec(x[i], QAMD_F64, 0, "cgms_mixed x")) return -1;
+  if (check_vec(b, QAMD_F64, 0, "cgms_mixed b")) return -1;
+  qamd_lattice *lat = l64->lat;
+  // multishift in FP32 (FP16 sloppy links are promoted: the shifted
+  // recurrences need the FP32 vectors)
+  qamd_links *l32 = (qamd_links *)ls;
+  qamd_links *own = nullptr;
+  if (ls->prec != QAMD_F32) {
+    own = qamd_links_create(lat, 0, QAMD_F32, l64->nterms);
+    if (!own || qamd_links_convert(own, l64)) {
+      qamd_links_destroy(own);
+      return -1;
+    }
+    l32 = own;
+  }
+  qamd_field *bs = qamd_field_create(lat, QAMD_F32, 3, 1, 0);
+  qamd_field *xs[MAXSHIFT] = {nullptr};
+  int rc = -1;
+  do {
+    if (!bs || qamd_field_copy(bs, b)) break;
+    int ok = 1;
+    for (int i = 0; i < n; i++) ok &= (xs[i] = qamd_field_create(lat, QAMD_F32, 3, 1, 0)) != nullptr;
+    if (!ok) break;
+    qamd_solve_info msinfo;
+    // the FP32 recurrences cannot resolve residuals far below 1e-6; refine
+    double ms_rsq = rsq > 1e-12 ? rsq : 1e-12;
+    if (qamd_stag_cgms(l32, xs, bs, mass, shifts, n, ms_rsq, maxits, &msinfo)) break;
+    int err = 0;
+    for (int i = 0; i < n && !err; i++) {
+      err |= qamd_field_copy(x[i], xs[i]);
+      qamd_solve_info ri;
+      err = err || cg_mixed_core(l64, ls, x[i], b, mass * mass + shifts[i], rsq, maxits, 0.1, &ri);
+      if (info && !err) {
+        info[i] = ri;
+        info[i].iterations += msinfo.iterations;
+        info[i].seconds += (i == 0) ? msinfo.seconds : 0;
+      }
+    }
+    if (err) break;
+    rc = 0;
+  } while (0);
+  for (int i = 0; i < n; i++) qamd_field_destroy(xs[i]);
+  qamd_field_destroy(bs);
+  qamd_links_destroy(own);
+  return rc;
+}
+
+}  // extern "C"

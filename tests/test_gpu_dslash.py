@@ -1,0 +1,152 @@
+"""GPU parity: asqtad / staggered and Wilson even-odd Dslash against the FP64 oracle.
+
+Tolerances (relative L2 difference over the output parity):
+  FP64 1e-13, FP32 1e-5, FP16 5e-3 (10-bit mantissa
+  storage of links and vectors with per-site max-norm vectors).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import parity_mask, rel_err
+
+pytestmark = pytest.mark.gpu
+
+TOL = {0: 1e-13, 1: 1e-5, 2: 5e-3}
+DIMS = [(4, 4, 4, 4), (8, 8, 8, 8), (4, 6, 8, 10), (6, 4, 4, 12)]
+
+
+def _stag_case(dims, seed, nterms=16):
+    if nterms == 16:
+        links = O.asqtad_links(dims, seed)
+    else:
+        u = O.apply_stag_phases(dims, O.random_su3_field(dims, seed))
+        fl, _ = O.fat7(dims, u, u, want_long=False)
+        links = O.stag_doublestore(dims, fl, None)
+    y = O.gauss_vector(dims, seed + 1, 100, parity=-1)
+    return links, y
+
+
+@pytest.mark.parametrize("dims", DIMS)
+@pytest.mark.parametrize("prec", [0, 1, 2])
+def test_asqtad_eo_dslash(gpu, dims, prec):
+    print("case: asqtad EO dims=%s prec=%d" % (dims, prec), flush=True)
+    links, y = _stag_case(dims, 11)
+    lat = gpu.Lattice(dims)
+    L = gpu.Links(lat, gpu.STAG, 0, 16).set_lex(links)
+    if prec != 0:
+        L = L.converted(prec)
+    for p in (0, 1):
+        yin = gpu.Field(lat, prec, 3, 1, 1 - p).set_lex(y)
+        out = gpu.Field(lat, prec, 3, 1, p)
+        gpu.dslash(L, out, yin)
+        ref = O.dslash(dims, links, 0.0, y, sub="even" if p == 0 else "odd")
+        err = rel_err(out.get_lex(), ref, parity_mask(dims, p))
+        assert err < TOL[prec], (p, err)
+
+
+@pytest.mark.parametrize("nterms", [8, 16])
+def test_dslash_all_with_mass(gpu, nterms):
+    """QLL dslash2 on 'all': x = mass*y + D y, both parities (lib/stagDslash.c:313)."""
+    dims = (4, 4, 6, 8)
+    links, y = _stag_case(dims, 5, nterms)
+    mass = 0.37
+    lat = gpu.Lattice(dims)
+    L = gpu.Links(lat, gpu.STAG, 0, nterms).set_lex(links)
+    ref = O.dslash(dims, links, mass, y, sub="all")
+    got = np.zeros_like(y)
+    for p in (0, 1):
+        yo = gpu.Field(lat, 0, 3, 1, 1 - p).set_lex(y)
+        yp = gpu.Field(lat, 0, 3, 1, p).set_lex(y)
+        out = gpu.Field(lat, 0, 3, 1, p)
+        n2 = gpu.dslash(L, out, yo, a=mass, w=yp, b=1.0, want_norm=True)
+        got = out.get_lex(got)
+        m = parity_mask(dims, p)
+        assert abs(n2 - np.sum(ref[m] ** 2)) <= 1e-12 * np.sum(ref[m] ** 2)
+    assert rel_err(got, ref) < 1e-13
+
+
+def test_normal_operator(gpu):
+    """dslashp2: Ap = m^2 p - D_eo D_oe p, pAp = m^2|p|^2 + |D_oe p|^2."""
+    dims = (4, 4, 4, 8)
+    links, _ = _stag_case(dims, 3)
+    p = O.gauss_vector(dims, 9, 101, parity=0)
+    mass = 0.2
+    Ap_ref, an_ref = O.normal_op(dims, links, mass, p)
+    lat = gpu.Lattice(dims)
+    L = gpu.Links(lat, gpu.STAG, 0, 16).set_lex(links)
+    pf = gpu.Field(lat, 0, 3, 1, 0).set_lex(p)
+    Ap = gpu.Field(lat, 0, 3, 1, 0)
+    t = gpu.Field(lat, 0, 3, 1, 1)
+    an = gpu.stag_normal(L, Ap, pf, t, mass)
+    assert abs(an - an_ref) <= 1e-13 * an_ref
+    assert rel_err(Ap.get_lex(), Ap_ref, parity_mask(dims, 0)) < 1e-13
+    # p^+ A p equals Anorm2 (the identity QLL relies on)
+    assert abs(pf.redot(Ap) - an_ref) <= 1e-12 * an_ref
+
+
+@pytest.mark.parametrize("prec", [0, 1, 2])
+def test_wilson_eo_dslash(gpu, prec):
+    dims = (4, 4, 6, 8)
+    u = O.random_su3_field(dims, 21)
+    links = O.wilson_doublestore(dims, u)
+    y = O.gauss_vector(dims, 22, 200, parity=-1, nreal=24)
+    lat = gpu.Lattice(dims)
+    L = gpu.Links(lat, gpu.WILSON, 0, 8).set_lex(links)
+    if prec != 0:
+        L = L.converted(prec)
+    mass = 0.15
+    ref = O.wilson_dslash(dims, links, mass, y, sub="all")
+    for p in (0, 1):
+        yo = gpu.Field(lat, prec, 12, 1, 1 - p).set_lex(y)
+        yp = gpu.Field(lat, prec, 12, 1, p).set_lex(y)
+        out = gpu.Field(lat, prec, 12, 1, p)
+        gpu.dslash(L, out, yo, a=4 + mass, w=yp, b=-1.0)
+        err = rel_err(out.get_lex(), ref, parity_mask(dims, p))
+        assert err < TOL[prec], (p, err)
+
+
+@pytest.mark.parametrize("nrhs", [4, 16])
+def test_wilson_multi_rhs(gpu, nrhs):
+    """A block of right-hand sides sharing link loads equals nrhs single solves."""
+    dims = (4, 4, 4, 8)
+    links = O.wilson_doublestore(dims, O.random_su3_field(dims, 31))
+    ys = np.stack([O.gauss_vector(dims, 40 + r, 300, parity=-1, nreal=24) for r in range(nrhs)])
+    lat = gpu.Lattice(dims)
+    L = gpu.Links(lat, gpu.WILSON, 1, 8)
+    L64 = gpu.Links(lat, gpu.WILSON, 0, 8).set_lex(links)
+    L = L64.converted(1)
+    yin = gpu.Field(lat, 1, 12, nrhs, 1).set_lex(ys)
+    out = gpu.Field(lat, 1, 12, nrhs, 0)
+    gpu.dslash(L, out, yin)
+    got = out.get_lex()
+    m = parity_mask(dims, 0)
+    for r in range(nrhs):
+        ref = O.wilson_dslash(dims, links, 0.0, ys[r], sub="even")
+        assert rel_err(got[r], ref, m) < 1e-5
+
+
+@pytest.mark.parametrize("inner", [(1, 1, 1, 1), (2, 1, 1, 1), (1, 2, 2, 1), (2, 2, 2, 1)])
+def test_qll_layout_transfer(gpu, inner):
+    """Device import/export of QLL's SIMD layout (lib/layout.c:120-142) for several VLEN."""
+    dims = (4, 4, 4, 8)
+    lo = O.layout(dims, inner)
+    v = O.gauss_vector(dims, 3, 7, parity=-1)
+    q = O.to_qll(lo, v)
+    import ctypes as C
+    import qll_amd
+    ql = qll_amd.QllLayout()
+    for i in range(4):
+        ql.physGeom[i] = lo.physGeom[i]
+        ql.innerGeom[i] = lo.innerGeom[i]
+        ql.outerGeom[i] = lo.outerGeom[i]
+    ql.nSitesOuter, ql.nSitesInner, ql.nEvenOuter = lo.nSitesOuter, lo.nSitesInner, lo.nEvenOuter
+    ql.innerCb, ql.innerCbDir = lo.innerCb, lo.innerCbDir
+    lat = gpu.Lattice(dims)
+    back = np.zeros_like(q)
+    for p in (0, 1):
+        f = gpu.Field(lat, 0, 3, 1, p)
+        qll_amd._check(qll_amd.lib().qamd_field_from_qll(f.h, C.byref(ql), q.ctypes.data, 0))
+        assert rel_err(f.get_lex(), v, parity_mask(dims, p)) == 0.0
+        qll_amd._check(qll_amd.lib().qamd_field_to_qll(f.h, C.byref(ql), back.ctypes.data, 0))
+    assert np.array_equal(back, q)
