@@ -169,6 +169,41 @@ int qamd_stag_cgms_mixed(const qamd_links *l64, const qamd_links *ls, qamd_field
                          const qamd_field *b, double mass, const double shifts[], int nshift,
                          double rsq, int maxits, qamd_solve_info *info /* [nshift] */);
 
+/* Wilson solver on the even-odd preconditioned operator in the hopping
+ * normalisation, Mhat = 1 - kappa^2 H_eo H_oe (H the hopping sum of the Wilson
+ * Dslash above), by CG on the normal equations Mhat^+ Mhat x = Mhat^+ b with
+ * Mhat^+ = g5 Mhat g5.  All nrhs (<= 16) right-hand sides of b are solved
+ * together, sharing link loads.  With sloppy links ls (FP32/FP16) the inner
+ * CGNE runs in that precision inside FP64 defect correction; stops when every
+ * rhs has |b - Mhat x|^2 <= rsq |b|^2 in FP64.  info: nrhs entries.  Replaces
+ * the Wilson solve of lib/wilsonDslash.c:287-364 (see wilsonSolve in qll.h). */
+int qamd_wilson_cgne(const qamd_links *l64, const qamd_links *ls, qamd_field *x, const qamd_field *b,
+                     double kappa, double rsq, int maxits, qamd_solve_info *info);
+/* multiply a spinor field by gamma_5 = diag(1,1,-1,-1) (DeGrand-Rossi) */
+int qamd_field_gamma5(qamd_field *f);
+
+/* ---- QLL host-layout helpers used by the drop-in API ----
+ * Generic shift / transport on host arrays in QLL layout (replaces shiftT
+ * lib/stagDslash.c:25-101, the transporter lib/transporter.c:33-193 and Xadj
+ * lib/vla.c:827-854): for every site x of the outer-site range
+ * [beginOuter, endOuter),
+ *     dest(x) = op( [U(x) *] src(x - disp) )  (+ dest(x) if accumulate)
+ * nelem reals per site; with u != NULL and td > 0 the source is multiplied by
+ * the 3x3 matrix field u at x as r[i,j,k] += u[j,l] v[i,l,k] (td = 2 for
+ * complex data, lib/vla.c:1207-1225); adjoint != 0 stores the adjoint of a
+ * 3x3 result.  Arrays are element type host_prec (F64 double, F32 float). */
+int qamd_qll_shift(const qamd_qll_layout *l, const int disp[4], int nelem, int nc, int td,
+                   const void *u, const void *src, void *dest, int host_prec, int beginOuter,
+                   int endOuter, int accumulate, int adjoint);
+/* sum (squares = 0) or sum of squares (squares = 1) of n host reals, FP64
+ * accumulated on the GPU (replaces sumElem_X / norm2_X, lib/vla.c:656-700) */
+int qamd_host_sum(const void *host, int64_t n, int host_prec, int squares, double *out);
+
+/* ---- flat device vectors of n reals, prec F64 or F32 (the BLAS-1 of
+ *      lib/vla.c:656-825 for solvers driven by host Linop callbacks) ---- */
+int qamd_flat_axpby(int64_t n, int prec, double a, const void *x, double b, void *y); /* y = a x + b y */
+int qamd_flat_dot(int64_t n, int prec, const void *x, const void *y, double *out);   /* FP64 sum x.y */
+
 /* ---- timing helpers (hipEvents on the library stream) ---- */
 int qamd_time_dslash(const qamd_links *l, qamd_field *out, const qamd_field *in,
                      int nrep, double *ms_per_call);

@@ -185,7 +185,7 @@ struct Field {             // one parity of a colour-vector / spinor field
 // Deterministic grid reduction: each block stores its partial sums, the last
 // block to finish (device-scope atomic ticket) adds the partials in block
 // order and runs a scalar epilogue.  The ticket is reset by that block.
-constexpr int MAXRED = 8;
+constexpr int MAXRED = 16;
 struct RedScratch {
   double *partials;     // [MAXRED][max_blocks]
   unsigned *ticket;

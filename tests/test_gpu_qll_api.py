@@ -1,0 +1,41 @@
+"""GPU: the QLL C API (qll.h) driven from C the way QLL's testdslash.c and
+testwilson.c drive it, checked against the FP64 oracle in QLL's host layout
+(tests/qll_driver.c)."""
+import os
+import subprocess
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+SRC = os.path.join(ROOT, "tests", "qll_driver.c")
+BIN = os.path.join(ROOT, "tests", "qll_driver.bin")
+
+
+def _driver():
+    if not os.path.exists(BIN) or os.path.getmtime(BIN) < os.path.getmtime(SRC):
+        subprocess.run(["gcc", "-O1", "-fopenmp", "-std=gnu11", "-I", os.path.join(ROOT, "include"),
+                        "-I", os.path.join(ROOT, "oracle"), "-o", BIN, SRC,
+                        "-L", os.path.join(ROOT, "qll_amd", "lib"), "-lqll_amd",
+                        "-L", os.path.join(ROOT, "oracle"), "-loracle",
+                        "-Wl,-rpath," + os.path.join(ROOT, "qll_amd", "lib"),
+                        "-Wl,-rpath," + os.path.join(ROOT, "oracle"), "-lm"], check=True)
+    return BIN
+
+
+@pytest.mark.parametrize("prec,vlen,dims", [
+    ("D", 1, (4, 4, 4, 8)),
+    ("D", 4, (4, 4, 8, 8)),
+    ("F", 8, (4, 4, 8, 8)),
+    ("F", 1, (6, 4, 4, 8)),
+])
+def test_qll_api_driver(prec, vlen, dims):
+    print("case: qll_driver %s vlen=%d dims=%s" % (prec, vlen, dims), flush=True)
+    env = dict(os.environ, OMP_NUM_THREADS="4")
+    r = subprocess.run([_driver(), prec, str(vlen)] + [str(d) for d in dims], capture_output=True, text=True,
+                       timeout=300, env=env)
+    print(r.stdout[-6000:])
+    print(r.stderr[-2000:])
+    assert r.returncode == 0, r.stdout[-3000:]
+    assert "PASSED" in r.stdout
