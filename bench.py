@@ -40,6 +40,7 @@ def parse():
     ap.add_argument("--no-cg", action="store_true", help="skip the solver comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--dims", type=str, default=None, help="override lattice, e.g. 16,16,16,16")
+    ap.add_argument("--precs", type=str, default="0,1,2", help="Dslash precisions to time (0 f64, 1 f32, 2 f16)")
     return ap.parse_args()
 
 
@@ -57,7 +58,7 @@ def cpu_baseline(dims, nsec_target=10.0):
     y = O.gauss_vector(dims, 7, 100, parity=1)
     gen_s = time.time() - t0
     t1, nthreads = O.time_dslash(dims, links, y, 1)      # warm-up + size the sample
-    nrep = max(1, min(50, int(nsec_target / max(t1, 1e-6))))
+    nrep = max(1, min(5000, int(nsec_target / max(t1, 1e-6))))
     t, nthreads = O.time_dslash(dims, links, y, nrep)
     vh = O.volume(dims) // 2
     return {
@@ -90,9 +91,18 @@ def main():
     print("setup (links: SU(3)+phases+fat7+Naik+doublestore, 3 precisions): %.2f s" % setup_s,
           file=sys.stderr, flush=True)
 
+    # measured streaming-copy bandwidth (read + write) over 2 x 1 GiB
+    copy_gbs, _ = Q.stream_copy(1 << 30, 20)
+    print("stream copy: %.1f GB/s" % copy_gbs, file=sys.stderr, flush=True)
+
     rates = {}
     prof = {}
+    precs = [int(p) for p in args.precs.split(",")]
+    if 0 not in precs:
+        precs.insert(0, 0)
     for prec, L in ((0, L64), (1, L32), (2, L16)):
+        if prec not in precs:
+            continue
         yin = Q.Field(lat, prec, 3, 1, Q.ODD).gaussian(5, 100)
         out = Q.Field(lat, prec, 3, 1, Q.EVEN)
         Q.time_dslash(L, out, yin, args.warmup)
@@ -168,9 +178,11 @@ def main():
                                % ("x".join(map(str, dims)), vh),
                    "lattice": list(dims), "mass_qll": QLL_MASS, "flops_per_site": FLOPS_ASQTAD},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "measured_copy_gbs": round(copy_gbs, 1),
+                     "frac_of_copy": round(achieved / copy_gbs, 4)},
         "cpu_baseline": cpu,
-        "dslash": {"f64": prof[0], "f32": prof[1], "f16": prof[2]},
+        "dslash": {{0: "f64", 1: "f32", 2: "f16"}[p]: prof[p] for p in prof},
         "solvers": solver,
         "device": Q.device_name(),
     }

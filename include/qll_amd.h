@@ -207,6 +207,9 @@ int qamd_flat_dot(int64_t n, int prec, const void *x, const void *y, double *out
 /* ---- timing helpers (hipEvents on the library stream) ---- */
 int qamd_time_dslash(const qamd_links *l, qamd_field *out, const qamd_field *in,
                      int nrep, double *ms_per_call);
+/* streaming copy of 'bytes' repeated nrep times: read+write GB/s and ms per
+ * copy (the measured-bandwidth reference; no reference counterpart) */
+int qamd_stream_copy(size_t bytes, int nrep, double *gbs, double *ms_per_copy);
 
 #ifdef __cplusplus
 }

@@ -99,6 +99,10 @@ def lib():
             "qamd_stag_cgms_mixed": (C.c_int, [_vp, _vp, C.POINTER(_vp), _vp, C.c_double, _d, C.c_int, C.c_double,
                                                C.c_int, C.POINTER(SolveInfo)]),
             "qamd_time_dslash": (C.c_int, [_vp, _vp, _vp, C.c_int, C.POINTER(C.c_double)]),
+            "qamd_stream_copy": (C.c_int, [C.c_size_t, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+            "qamd_wilson_cgne": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_double, C.c_int,
+                                           C.POINTER(SolveInfo)]),
+            "qamd_field_gamma5": (C.c_int, [_vp]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(L, name)
@@ -304,6 +308,21 @@ def stag_cgms_mixed(l64, ls, xs, b, mass, shifts, rsq, maxits):
     _check(lib().qamd_stag_cgms_mixed(l64.h, ls.h, arr, b.h, mass, np.asarray(shifts, dtype=np.float64), len(xs),
                                       rsq, maxits, infos))
     return [i.as_dict() for i in infos]
+
+
+def wilson_cgne(l64, x, b, kappa, rsq, maxits, ls=None):
+    """Solve (1 - kappa^2 H_eo H_oe) x = b for every rhs of the block b."""
+    infos = (SolveInfo * b.nrhs)()
+    _check(lib().qamd_wilson_cgne(l64.h, ls.h if ls is not None else None, x.h, b.h, kappa, rsq, maxits,
+                                  infos))
+    return [i.as_dict() for i in infos]
+
+
+def stream_copy(nbytes, nrep=20):
+    """Measured copy bandwidth (read + write) in GB/s and ms per copy."""
+    g, ms = C.c_double(), C.c_double()
+    _check(lib().qamd_stream_copy(nbytes, nrep, C.byref(g), C.byref(ms)))
+    return g.value, ms.value
 
 
 def time_dslash(links, out, inp, nrep):

@@ -1,0 +1,60 @@
+// Measurement helpers: a plain streaming copy kernel of known size (the
+// STREAM-copy reference for the roofline, and the calibration kernel for
+// rocprof's FETCH_SIZE / WRITE_SIZE counters).
+#include "qamd_internal.h"
+
+namespace qamd {
+
+// each thread moves 4 x 16 B (four independent loads in flight), one tile of
+// 4 x 256 elements per block
+__global__ void __launch_bounds__(256) k_stream_copy(const double2 *__restrict__ a, double2 *__restrict__ b, int64_t n) {
+  int64_t base = (int64_t)blockIdx.x * 1024 + threadIdx.x;
+  double2 v[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+    if (base + k * 256 < n) v[k] = ntload(a + base + k * 256);
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+    if (base + k * 256 < n) b[base + k * 256] = v[k];
+}
+
+}  // namespace qamd
+
+using namespace qamd;
+
+extern "C" {
+
+// copy 'bytes' (rounded down to 16 B) nrep times; returns the copy bandwidth
+// counting read + write bytes, in GB/s, and the mean ms per copy
+int qamd_stream_copy(size_t bytes, int nrep, double *gbs, double *ms_per_copy) {
+  if (ensure_init()) return -1;
+  int64_t n = (int64_t)(bytes / 16);
+  double2 *a = (double2 *)dev_alloc(n * 16), *b = (double2 *)dev_alloc(n * 16);
+  if (!a || !b) {
+    dev_free(a);
+    dev_free(b);
+    return -1;
+  }
+  QAMD_HIP(hipMemsetAsync(a, 0, n * 16, stream()));
+  hipEvent_t e0, e1;
+  QAMD_HIP(hipEventCreate(&e0));
+  QAMD_HIP(hipEventCreate(&e1));
+  int nb = (int)((n + 1023) / 1024);
+  hipLaunchKernelGGL(k_stream_copy, dim3(nb), dim3(256), 0, stream(), a, b, n);   // warm-up
+  QAMD_HIP(hipEventRecord(e0, stream()));
+  for (int i = 0; i < nrep; i++) hipLaunchKernelGGL(k_stream_copy, dim3(nb), dim3(256), 0, stream(), a, b, n);
+  QAMD_HIP(hipEventRecord(e1, stream()));
+  QAMD_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  QAMD_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  dev_free(a);
+  dev_free(b);
+  double t = ms / nrep;
+  if (ms_per_copy) *ms_per_copy = t;
+  if (gbs) *gbs = 2.0 * n * 16 / (t * 1e-3) / 1e9;
+  return 0;
+}
+
+}  // extern "C"

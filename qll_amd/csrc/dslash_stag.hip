@@ -20,15 +20,18 @@
 
 namespace qamd {
 
-template <int P> struct StagArgs {
+// links and input in precision P, output and w in precision PO (PO = P, or
+// FP32 output from FP16 links/input for the mixed-precision solvers)
+template <int P, int PO = P> struct StagArgs {
   typedef typename Prec<P>::S S;
+  typedef typename Prec<PO>::S SO;
   Geom g;
   const S *links;
   const S *in;
   const float *in_norm;
-  S *out;
+  SO *out;
   float *out_norm;
-  const S *w;
+  const SO *w;
   const float *w_norm;
   double a, b;
   int par;            // output parity
@@ -46,8 +49,8 @@ struct FinStore2 {
   }
 };
 
-template <int P, int NT, bool RED>
-__global__ void __launch_bounds__(256) k_stag_dslash(StagArgs<P> A) {
+template <int P, int PO, int NT, bool RED>
+__global__ void __launch_bounds__(256) k_stag_dslash(StagArgs<P, PO> A) {
   typedef typename Prec<P>::R R;
   typedef typename Prec<P>::C C;
   typedef typename Prec<P>::S S;
@@ -84,7 +87,7 @@ __global__ void __launch_bounds__(256) k_stag_dslash(StagArgs<P> A) {
     const R b = (R)A.b;
     if (A.w) {
       C w[3];
-      VecLd<P>::ld(A.w, A.w_norm, vp, cb, w, (R)1);
+      VecLd<PO>::ld(A.w, A.w_norm, vp, cb, w, (R)1);
       const R a = (R)A.a;
 #pragma unroll
       for (int j = 0; j < 3; j++) {
@@ -103,59 +106,67 @@ __global__ void __launch_bounds__(256) k_stag_dslash(StagArgs<P> A) {
 #pragma unroll
       for (int j = 0; j < 3; j++) red[0] += (double)o[j].x * o[j].x + (double)o[j].y * o[j].y;
     }
-    VecSt<P>::st(A.out, A.out_norm, vp, cb, o);
+    VecSt<PO>::st(A.out, A.out_norm, vp, cb, o);
   }
-  if (RED) grid_sum<2>(red, A.rs, FinStore2{A.red_out});
+  if (RED) grid_partial<2>(red, A.rs);
 }
 
-template <int P, int NT>
-static void launch_t(const StagArgs<P> &A, bool red, hipStream_t st) {
+template <int P, int PO, int NT>
+static void launch_t(const StagArgs<P, PO> &A, bool red, hipStream_t st) {
   int nb = (int)((A.g.vh + 255) / 256);
   if (red)
-    hipLaunchKernelGGL((k_stag_dslash<P, NT, true>), dim3(nb), dim3(256), 0, st, A);
+    hipLaunchKernelGGL((k_stag_dslash<P, PO, NT, true>), dim3(nb), dim3(256), 0, st, A);
   else
-    hipLaunchKernelGGL((k_stag_dslash<P, NT, false>), dim3(nb), dim3(256), 0, st, A);
+    hipLaunchKernelGGL((k_stag_dslash<P, PO, NT, false>), dim3(nb), dim3(256), 0, st, A);
 }
 
-template <int P>
+template <int P, int PO = P>
 static int launch_p(const qamd_links *l, Field &out, const Field &in, double a, const Field *w,
-                    double b, double *red_out, hipStream_t st, const int *skip) {
+                    double b, double *red_out, hipStream_t st, const int *skip, int pset, bool finish) {
   typedef typename Prec<P>::S S;
-  StagArgs<P> A;
+  typedef typename Prec<PO>::S SO;
+  StagArgs<P, PO> A;
   memset(&A, 0, sizeof A);
   A.skip = skip;
   A.g = l->lat->g;
   A.links = (const S *)l->data[out.parity];
   A.in = (const S *)in.data;
   A.in_norm = in.norm;
-  A.out = (S *)out.data;
+  A.out = (SO *)out.data;
   A.out_norm = out.norm;
-  A.w = w ? (const S *)w->data : nullptr;
+  A.w = w ? (const SO *)w->data : nullptr;
   A.w_norm = w ? w->norm : nullptr;
   A.a = a;
   A.b = b;
   A.par = out.parity;
   for (int t = 0; t < 16; t++) A.lscale[t] = l->scale[t];
-  if (red_out) {
-    int nb = (int)((A.g.vh + 255) / 256);
+  const bool red = red_out != nullptr || !finish;
+  const int nb = (int)((A.g.vh + 255) / 256);
+  if (red) {
     if (red_reserve(nb)) return -1;
-    A.rs = red_scratch();
+    A.rs = red_set(pset);
     A.red_out = red_out;
   }
-  if (l->nterms == 16) launch_t<P, 16>(A, red_out != nullptr, st);
-  else launch_t<P, 8>(A, red_out != nullptr, st);
+  if (l->nterms == 16) launch_t<P, PO, 16>(A, red, st);
+  else launch_t<P, PO, 8>(A, red, st);
+  if (red && finish) launch_finish<2>(A.rs, nb, FinStore2{red_out}, st, skip);
   QAMD_HIP(hipGetLastError());
   return 0;
 }
 
 int stag_dslash_launch(const qamd_links *l, Field &out, const Field &in, double a, const Field *w,
-                       double b, double *red_out, hipStream_t st, const int *skip) {
+                       double b, double *red_out, hipStream_t st, const int *skip, int pset, bool finish) {
   if (l->kind != 0) {
     set_error("stag_dslash: links are not staggered");
     return -1;
   }
-  if (out.prec != l->prec || in.prec != l->prec || (w && w->prec != l->prec)) {
-    set_error("stag_dslash: field precision %d/%d does not match links %d", out.prec, in.prec, l->prec);
+  if (pset < 0 || pset >= NSETS) {
+    set_error("stag_dslash: bad partial-sum set %d", pset);
+    return -1;
+  }
+  bool half_to_single = (l->prec == QAMD_F16 && out.prec == QAMD_F32);
+  if (in.prec != l->prec || (out.prec != l->prec && !half_to_single) || (w && w->prec != out.prec)) {
+    set_error("stag_dslash: field precisions in %d / out %d do not fit links %d", in.prec, out.prec, l->prec);
     return -1;
   }
   if (out.ncomp != 3 || in.ncomp != 3 || out.nrhs != 1 || in.nrhs != 1) {
@@ -171,9 +182,12 @@ int stag_dslash_launch(const qamd_links *l, Field &out, const Field &in, double 
     return -1;
   }
   switch (l->prec) {
-    case QAMD_F64: return launch_p<QAMD_F64>(l, out, in, a, w, b, red_out, st, skip);
-    case QAMD_F32: return launch_p<QAMD_F32>(l, out, in, a, w, b, red_out, st, skip);
-    default: return launch_p<QAMD_F16>(l, out, in, a, w, b, red_out, st, skip);
+    case QAMD_F64: return launch_p<QAMD_F64>(l, out, in, a, w, b, red_out, st, skip, pset, finish);
+    case QAMD_F32: return launch_p<QAMD_F32>(l, out, in, a, w, b, red_out, st, skip, pset, finish);
+    default:
+      if (half_to_single)
+        return launch_p<QAMD_F16, QAMD_F32>(l, out, in, a, w, b, red_out, st, skip, pset, finish);
+      return launch_p<QAMD_F16>(l, out, in, a, w, b, red_out, st, skip, pset, finish);
   }
 }
 

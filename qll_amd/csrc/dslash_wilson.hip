@@ -178,7 +178,7 @@ __global__ void __launch_bounds__(256) k_wilson(WArgs<P> A) {
         SpIO<P>::st(A.out, A.out.cidx(r, 3 * sp + c, cb), v);
       }
   }
-  if (RED) grid_sum<1>(red, A.rs, FinStore1{A.red_out});
+  if (RED) grid_partial<1>(red, A.rs);
 }
 
 template <int P>
@@ -204,9 +204,10 @@ static int wlaunch(const qamd_links *l, Field &out, const Field &in, double a, c
   int nb = (int)((nthreads + 255) / 256);
   if (red_out) {
     if (red_reserve(nb)) return -1;
-    A.rs = red_scratch();
+    A.rs = red_set(0);
     A.red_out = red_out;
     hipLaunchKernelGGL((k_wilson<P, true>), dim3(nb), dim3(256), 0, st, A);
+    launch_finish<1>(A.rs, nb, FinStore1{red_out}, st, skip);
   } else {
     hipLaunchKernelGGL((k_wilson<P, false>), dim3(nb), dim3(256), 0, st, A);
   }

@@ -58,8 +58,7 @@ __global__ void k_gaussian(Field f, uint64_t seed, uint64_t stream) {
 }
 
 // --------------------------------------------------------------- BLAS-1
-template <class Fin>
-__global__ void k_norm_dot(Field a, Field b, int rhs, int dot, Fin fin, RedScratch rs) {
+__global__ void k_norm_dot(Field a, Field b, int rhs, int dot, RedScratch rs) {
   double acc[1] = {0.0};
   for (int cb = blockIdx.x * blockDim.x + threadIdx.x; cb < a.g->vh; cb += gridDim.x * blockDim.x) {
     double2 va[12], vb[12];
@@ -70,7 +69,7 @@ __global__ void k_norm_dot(Field a, Field b, int rhs, int dot, Fin fin, RedScrat
       else acc[0] += va[j].x * va[j].x + va[j].y * va[j].y;
     }
   }
-  grid_sum<1>(acc, rs, fin);
+  grid_partial<1>(acc, rs);
 }
 
 struct StoreTo {
@@ -109,8 +108,9 @@ int reduce_field(const Field &a, const Field *b, int rhs, double *out) {
   if (!slot) return -1;
   StoreTo fin{slot};
   Field bb = b ? *b : a;
-  hipLaunchKernelGGL(k_norm_dot<StoreTo>, dim3(nb), dim3(256), 0, stream(), a, bb, rhs, b ? 1 : 0,
-                     fin, red_scratch());
+  RedScratch rs = red_set(0);
+  hipLaunchKernelGGL(k_norm_dot, dim3(nb), dim3(256), 0, stream(), a, bb, rhs, b ? 1 : 0, rs);
+  launch_finish<1>(rs, nb, fin, stream());
   QAMD_HIP(hipGetLastError());
   QAMD_HIP(hipMemcpyAsync(out, slot, sizeof(double), hipMemcpyDeviceToHost, stream()));
   QAMD_HIP(hipStreamSynchronize(stream()));

@@ -76,13 +76,13 @@ struct FinOne {
 };
 
 template <class T>
-__global__ void k_flat_sum(const T *x, const T *y, int64_t n, int mode, RedScratch rs, FinOne fin) {
+__global__ void k_flat_sum(const T *x, const T *y, int64_t n, int mode, RedScratch rs) {
   double acc[1] = {0.0};
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     double a = (double)x[i];
     acc[0] += mode == 0 ? a : a * (double)(y ? y[i] : x[i]);
   }
-  grid_sum<1>(acc, rs, fin);
+  grid_partial<1>(acc, rs);
 }
 
 template <class T>
@@ -151,12 +151,14 @@ static int flat_reduce(int64_t n, int prec, const void *x, const void *y, int mo
   if (red_reserve(nb)) return -1;
   double *slot = scalar_slot();
   if (!slot) return -1;
+  RedScratch rs = red_set(0);
   if (prec == QAMD_F64)
     hipLaunchKernelGGL(k_flat_sum<double>, dim3(nb), dim3(256), 0, stream(), (const double *)x, (const double *)y,
-                       n, mode, red_scratch(), FinOne{slot + 8});
+                       n, mode, rs);
   else
     hipLaunchKernelGGL(k_flat_sum<float>, dim3(nb), dim3(256), 0, stream(), (const float *)x, (const float *)y, n,
-                       mode, red_scratch(), FinOne{slot + 8});
+                       mode, rs);
+  launch_finish<1>(rs, nb, FinOne{slot + 8}, stream());
   QAMD_HIP(hipGetLastError());
   QAMD_HIP(hipMemcpyAsync(out, slot + 8, sizeof(double), hipMemcpyDeviceToHost, stream()));
   QAMD_HIP(hipStreamSynchronize(stream()));

@@ -282,7 +282,7 @@ struct FinStore6 {
   }
 };
 
-__global__ void k_plaq(const double2 *u, Geom g, RedScratch rs, FinStore6 fin) {
+__global__ void k_plaq(const double2 *u, Geom g, RedScratch rs) {
   double acc[6] = {0, 0, 0, 0, 0, 0};
   GView U{u, g};
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < 2 * g.vh; i += (int64_t)gridDim.x * blockDim.x) {
@@ -301,7 +301,7 @@ __global__ void k_plaq(const double2 *u, Geom g, RedScratch rs, FinStore6 fin) {
         acc[l++] += P.e[0].x + P.e[4].x + P.e[8].x;
       }
   }
-  grid_sum<6>(acc, rs, fin);
+  grid_partial<6>(acc, rs);
 }
 
 // host transfer: [4][V][18] lexicographic or 4 QLL arrays
@@ -455,7 +455,9 @@ int qamd_gauge_plaquette(const qamd_gauge *u, double plaq[6]) {
   double *slot = scalar_slot();
   if (!slot) return -1;
   FinStore6 fin{slot, 1.0 / (3.0 * (double)g.V)};
-  hipLaunchKernelGGL(k_plaq, dim3(nb), dim3(256), 0, stream(), u->d, g, red_scratch(), fin);
+  RedScratch rs = red_set(0);
+  hipLaunchKernelGGL(k_plaq, dim3(nb), dim3(256), 0, stream(), u->d, g, rs);
+  launch_finish<6>(rs, nb, fin, stream());
   QAMD_HIP(hipGetLastError());
   QAMD_HIP(hipMemcpyAsync(plaq, slot, 6 * sizeof(double), hipMemcpyDeviceToHost, stream()));
   QAMD_HIP(hipStreamSynchronize(stream()));

@@ -82,14 +82,28 @@ template <> __device__ inline double2 ld2<__half2>(const __half2 *p, int64_t i) 
   return make_double2(f.x, f.y);
 }
 
-// per-term max |component| as float bits (non-negative floats order as uints)
+// per-term max |component| as float bits (non-negative floats order as
+// uints).  Each wavefront reduces one 64-site block (all terms) in registers
+// and issues one atomic per term.
 template <class S>
-__global__ void k_link_absmax(const S *src, int64_t n, int nterm, unsigned *mx) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    double2 v = ld2<S>(src, i);
-    float m = (float)fmax(fabs(v.x), fabs(v.y));
-    int t = (int)((i >> 6) / 9 % nterm);
-    atomicMax(&mx[t], __float_as_uint(m));
+__global__ void k_link_absmax(const S *src, int64_t nblk, int nterm, unsigned *mx) {
+  int lane = threadIdx.x & 63;
+  int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+  float m[16];
+  for (int t = 0; t < 16; t++) m[t] = 0.f;
+  for (int64_t b = wave; b < nblk; b += nwaves) {
+    const S *p = src + b * nterm * 9 * 64 + lane;
+    for (int t = 0; t < nterm; t++)
+      for (int e = 0; e < 9; e++) {
+        double2 v = ld2<S>(p, (t * 9 + e) * 64);
+        m[t] = fmaxf(m[t], (float)fmax(fabs(v.x), fabs(v.y)));
+      }
+  }
+  for (int t = 0; t < nterm; t++) {
+    float v = m[t];
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    if (lane == 0) atomicMax(&mx[t], __float_as_uint(v));
   }
 }
 
@@ -267,6 +281,7 @@ int qamd_links_convert(qamd_links *dst, const qamd_links *src) {
     return -1;
   }
   int64_t n = link_elems(src);
+  int64_t nblk = src->lat->g.vh_pad / 64;
   float *sc = (float *)dev_alloc(64 * sizeof(float));
   unsigned *mx = (unsigned *)dev_alloc(16 * sizeof(unsigned));
   if (!sc || !mx) return -1;
@@ -276,11 +291,11 @@ int qamd_links_convert(qamd_links *dst, const qamd_links *src) {
     QAMD_HIP(hipMemsetAsync(mx, 0, 16 * sizeof(unsigned), stream()));
     for (int p = 0; p < 2; p++) {
       if (src->prec == QAMD_F64)
-        hipLaunchKernelGGL(k_link_absmax<double2>, dim3(2048), dim3(256), 0, stream(), (const double2 *)src->data[p], n, src->nterms, mx);
+        hipLaunchKernelGGL(k_link_absmax<double2>, dim3(2048), dim3(256), 0, stream(), (const double2 *)src->data[p], nblk, src->nterms, mx);
       else if (src->prec == QAMD_F32)
-        hipLaunchKernelGGL(k_link_absmax<float2>, dim3(2048), dim3(256), 0, stream(), (const float2 *)src->data[p], n, src->nterms, mx);
+        hipLaunchKernelGGL(k_link_absmax<float2>, dim3(2048), dim3(256), 0, stream(), (const float2 *)src->data[p], nblk, src->nterms, mx);
       else
-        hipLaunchKernelGGL(k_link_absmax<__half2>, dim3(2048), dim3(256), 0, stream(), (const __half2 *)src->data[p], n, src->nterms, mx);
+        hipLaunchKernelGGL(k_link_absmax<__half2>, dim3(2048), dim3(256), 0, stream(), (const __half2 *)src->data[p], nblk, src->nterms, mx);
     }
     unsigned hm[16];
     QAMD_HIP(hipMemcpyAsync(hm, mx, sizeof hm, hipMemcpyDeviceToHost, stream()));

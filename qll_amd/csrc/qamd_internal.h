@@ -182,16 +182,18 @@ struct Field {             // one parity of a colour-vector / spinor field
 };
 
 // -------------------------------------------------------------- reductions
-// Deterministic grid reduction: each block stores its partial sums, the last
-// block to finish (device-scope atomic ticket) adds the partials in block
-// order and runs a scalar epilogue.  The ticket is reset by that block.
-constexpr int MAXRED = 16;
+// Deterministic grid reductions in two launches: every block of the reducing
+// kernel stores its block sums (plain stores, no fences or atomics), then a
+// one-block finisher kernel adds them in block order and runs the scalar
+// epilogue.  The kernel boundary makes the partial sums visible, which on a
+// multi-die GPU is far cheaper than a device-scope fence per block.
+constexpr int MAXRED = 16;   // values reduced by one kernel
+constexpr int NSETS = 4;     // independent partial-sum buffers
 struct RedScratch {
-  double *partials;     // [MAXRED][max_blocks]
-  unsigned *ticket;
+  double *part;              // [MAXRED][max_blocks] of one set
   int max_blocks;
 };
-RedScratch &red_scratch();
+RedScratch red_set(int set);   // host: scratch set 0..NSETS-1 (reserve first)
 int red_reserve(int nblocks);
 
 template <int N>
@@ -225,42 +227,53 @@ __device__ __forceinline__ void block_sum(double (&v)[N]) {
   __syncthreads();
 }
 
-// Grid reduction; 'fin' is called by thread 0 of the last block with the
-// totals.  Every block of the grid must call this exactly once.
-template <int N, class Fin>
-__device__ __forceinline__ void grid_sum(double (&v)[N], RedScratch rs, Fin fin) {
-  __shared__ bool last;
-  __shared__ double tot[N];
+// block sums of v into the partial-sum set; every block calls it once
+template <int N>
+__device__ __forceinline__ void grid_partial(double (&v)[N], RedScratch rs) {
   block_sum(v);
-  int nb = gridDim.x;
-  if (threadIdx.x == 0) {
+  if (threadIdx.x == 0)
 #pragma unroll
-    for (int i = 0; i < N; i++) rs.partials[i * rs.max_blocks + blockIdx.x] = v[i];
-    __threadfence();
-    unsigned t = atomicAdd(rs.ticket, 1u);
-    last = (t == (unsigned)nb - 1);
-  }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  // fixed-order sum of the partials: thread k adds blocks k, k+bs, ... in
-  // order, then the block sum adds the per-thread results in thread order
-  double w[N];
+    for (int i = 0; i < N; i++) rs.part[i * rs.max_blocks + blockIdx.x] = v[i];
+}
+
+// fixed-order total of N slots over nb blocks (thread k adds blocks k,
+// k+256, ... in order; the block sum adds the threads in order)
+template <int N>
+__device__ __forceinline__ void finish_sum(RedScratch rs, int nb, double (&w)[N]) {
 #pragma unroll
   for (int i = 0; i < N; i++) {
     double s = 0;
-    for (int b = threadIdx.x; b < nb; b += blockDim.x)
-      s += __builtin_nontemporal_load(&rs.partials[i * rs.max_blocks + b]);
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) s += rs.part[i * rs.max_blocks + b];
     w[i] = s;
   }
   block_sum(w);
+}
+
+template <int N, class Fin>
+__global__ void __launch_bounds__(256) k_finish(RedScratch rs, int nb, Fin fin, const int *skip) {
+  if (skip && *skip) return;
+  double w[N];
+  finish_sum(rs, nb, w);
+  if (threadIdx.x == 0) fin(w);
+}
+
+// totals of N0 slots of set a and N1 slots of set b, passed as one array
+template <int N0, int N1, class Fin>
+__global__ void __launch_bounds__(256) k_finish2(RedScratch a, RedScratch b, int nb, Fin fin, const int *skip) {
+  if (skip && *skip) return;
+  double w0[N0], w1[N1], t[N0 + N1];
+  finish_sum(a, nb, w0);
+  finish_sum(b, nb, w1);
   if (threadIdx.x == 0) {
-#pragma unroll
-    for (int i = 0; i < N; i++) tot[i] = w[i];
-    *rs.ticket = 0u;
-    __threadfence();
-    fin(tot);
+    for (int i = 0; i < N0; i++) t[i] = w0[i];
+    for (int i = 0; i < N1; i++) t[N0 + i] = w1[i];
+    fin(t);
   }
+}
+
+template <int N, class Fin>
+inline void launch_finish(RedScratch rs, int nb, Fin fin, hipStream_t st, const int *skip = nullptr) {
+  hipLaunchKernelGGL((k_finish<N, Fin>), dim3(1), dim3(256), 0, st, rs, nb, fin, skip);
 }
 
 }  // namespace qamd

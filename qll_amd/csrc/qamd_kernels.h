@@ -10,8 +10,12 @@ const Geom *dev_geom(qamd_lattice *lat);
 double *scalar_slot();
 
 // dslash_stag.hip / dslash_wilson.hip
+// Reduction: with red_out, {|out|^2, |w|^2} is reduced into partial-sum set
+// pset and (finish = true) a finisher stores the totals at red_out; with
+// finish = false the partial sums are left in set pset for a caller's finisher.
 int stag_dslash_launch(const qamd_links *l, Field &out, const Field &in, double a, const Field *w,
-                       double b, double *red_out, hipStream_t st, const int *skip = nullptr);
+                       double b, double *red_out, hipStream_t st, const int *skip = nullptr, int pset = 0,
+                       bool finish = true);
 int wilson_dslash_launch(const qamd_links *l, Field &out, const Field &in, double a, const Field *w,
                          double b, double *red_out, hipStream_t st, const int *skip = nullptr);
 

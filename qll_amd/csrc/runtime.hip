@@ -13,7 +13,8 @@ namespace qamd {
 static char g_err[1024] = "no error";
 static hipStream_t g_stream = nullptr;
 static int g_init = 0;
-static RedScratch g_red = {nullptr, nullptr, 0};
+static double *g_red_base = nullptr;   // NSETS partial-sum sets
+static int g_red_mb = 0;               // blocks per set
 
 void set_error(const char *fmt, ...) {
   va_list ap;
@@ -56,23 +57,27 @@ void dev_free(void *p) {
   if (p) (void)hipFree(p);
 }
 
-RedScratch &red_scratch() { return g_red; }
+RedScratch red_set(int set) {
+  RedScratch r;
+  r.part = g_red_base + (size_t)set * MAXRED * g_red_mb;
+  r.max_blocks = g_red_mb;
+  return r;
+}
 
 int red_reserve(int nblocks) {
   if (ensure_init()) return -1;
-  if (nblocks <= g_red.max_blocks) return 0;
+  if (nblocks <= g_red_mb) return 0;
   // grow: wait for pending work that may still use the old scratch
   QAMD_HIP(hipStreamSynchronize(g_stream));
-  dev_free(g_red.partials);
-  dev_free(g_red.ticket);
+  dev_free(g_red_base);
   int mb = 1024;
   while (mb < nblocks) mb *= 2;
-  g_red.partials = (double *)dev_alloc(sizeof(double) * MAXRED * mb);
-  g_red.ticket = (unsigned *)dev_alloc(256);
-  if (!g_red.partials || !g_red.ticket) return -1;
-  QAMD_HIP(hipMemsetAsync(g_red.ticket, 0, 256, g_stream));
-  QAMD_HIP(hipStreamSynchronize(g_stream));
-  g_red.max_blocks = mb;
+  g_red_base = (double *)dev_alloc(sizeof(double) * NSETS * MAXRED * mb);
+  if (!g_red_base) {
+    g_red_mb = 0;
+    return -1;
+  }
+  g_red_mb = mb;
   return 0;
 }
 

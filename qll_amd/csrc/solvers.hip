@@ -5,9 +5,10 @@
 // operator, with |Dp|^2 and |p|^2 reduced in their epilogues) and two fused
 // vector kernels.  All scalars live in a device-side state block: the
 // reductions' last block updates them, the next kernels read them, so the host
-// never sits inside an iteration.  Every kernel checks the state's 'done' flag
-// and returns at once when it is set, which lets the host enqueue iterations
-// in batches and stop exactly where the reference's loop stops.
+// never sits inside an iteration; it enqueues iterations in batches and reads
+// the state once per batch.  Every kernel of the plain solvers checks the
+// state's 'done' flag and returns at once when it is set, which stops the
+// batch exactly where the reference's loop stops.
 #include "qamd_internal.h"
 #include "qamd_objects.h"
 #include "qamd_kernels.h"
@@ -21,6 +22,7 @@
 namespace qamd {
 
 constexpr int MAXSHIFT = 32;
+constexpr int BATCH = 16;   // iterations enqueued between two reads of the state
 
 struct SolverDev {
   double rsq, rsq_old;     // current and previous |r|^2
@@ -28,6 +30,7 @@ struct SolverDev {
   double m2;               // mass^2 of the normal operator
   double rsqstop;
   double alpha, beta;
+  double maxrr;            // max |r| since the last reliable update
   int its, itnlim, done, mode;   // mode 0: QLL rescaled CG, 1: textbook CG
   int auto_stop;
   // multishift scalars (cgmsSolveT, lib/cgms.c:6-179)
@@ -43,16 +46,17 @@ template <int P> struct VT {
   typedef typename Prec<P>::C C;
   typedef typename Prec<P>::R R;
   __device__ static void ld(const Field &f, int i, C (&v)[3]) {
-    VecLd<P>::ld((const S *)f.data, f.norm, f.g->vh_pad, i, v, (R)1);
+    VecLd<P>::ld((const S *)f.data, f.norm, f.vp, i, v, (R)1);
   }
   __device__ static void st(Field &f, int i, const C (&v)[3]) {
-    VecSt<P>::st((S *)f.data, f.norm, f.g->vh_pad, i, v);
+    VecSt<P>::st((S *)f.data, f.norm, f.vp, i, v);
   }
 };
 
 // ---------------------------------------------------------------- CG kernels
-template <int P>
-__global__ void __launch_bounds__(256) k_cg_p(Field p, Field r, SolverDev *S) {
+// p update; with SH = 1 an FP16 copy of p (the Dslash input) is written too
+template <int P, int SH>
+__global__ void __launch_bounds__(256) k_cg_p(Field p, Field r, Field shadow, SolverDev *S) {
   typedef typename Prec<P>::C C;
   typedef typename Prec<P>::R R;
   if (S->done) return;
@@ -77,6 +81,12 @@ __global__ void __launch_bounds__(256) k_cg_p(Field p, Field r, SolverDev *S) {
     }
   }
   VT<P>::st(p, i, pv);
+  if (SH) {
+    float2 h[3];
+#pragma unroll
+    for (int j = 0; j < 3; j++) h[j] = make_float2((float)pv[j].x, (float)pv[j].y);
+    VT<QAMD_F16>::st(shadow, i, h);
+  }
 }
 
 struct FinCG {
@@ -85,6 +95,7 @@ struct FinCG {
     S->rsq_old = S->rsq;
     S->rsq = t[0];
     S->beta = S->rsq / S->rsq_old;
+    S->maxrr = fmax(S->maxrr, sqrt(S->rsq));
     S->its += 1;
     if (S->its >= S->itnlim || (S->auto_stop && S->rsq < S->rsqstop)) S->done = 1;
   }
@@ -100,6 +111,7 @@ __global__ void __launch_bounds__(256) k_cg_xr(Field x, Field p, Field r, Field 
   if (S->done) return;
   double pAp = S->m2 * S->nrm[3] + S->nrm[0];
   double alpha = (S->mode == 0) ? 1.0 / pAp : S->rsq / pAp;
+  if (blockIdx.x == 0 && threadIdx.x == 0) S->alpha = alpha;   // read only by the host
   double red[1] = {0.0};
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < p.g->vh) {
@@ -126,7 +138,7 @@ __global__ void __launch_bounds__(256) k_cg_xr(Field x, Field p, Field r, Field 
 #pragma unroll
     for (int j = 0; j < 3; j++) red[0] += (double)rr[j].x * rr[j].x + (double)rr[j].y * rr[j].y;
   }
-  grid_sum<1>(red, rs, FinCG{S});
+  grid_partial<1>(red, rs);
 }
 
 // reliable update, part 1: xhi += xs; xs = 0
@@ -155,6 +167,7 @@ struct FinTrue {
   __device__ void operator()(const double *t) const {
     S->rsq = t[0];
     S->beta = S->rsq / S->rsq_old;
+    S->maxrr = sqrt(S->rsq);
   }
 };
 
@@ -181,7 +194,7 @@ __global__ void k_ru_resid(Field rhi, Field b, Field Ax, Field rs, SolverDev *S,
     VT<QAMD_F64>::st(rhi, i, bv);
     VT<PS>::st(rs, i, sv);
   }
-  grid_sum<1>(red, red_s, FinTrue{S});
+  grid_partial<1>(red, red_s);
 }
 
 // ------------------------------------------------------- multishift kernels
@@ -213,8 +226,8 @@ struct MSPtrs {
 
 struct FinMS {
   SolverDev *S;
-  double pkp;
   __device__ void operator()(const double *t) const {
+    double pkp = (S->m2 * S->nrm[3] + S->nrm[0]) * S->s[0] * S->s[0];
     double b[MAXSHIFT], zn[MAXSHIFT];
     ms_coefs(S, pkp, b, zn);
     double oldrsq = S->rsq;
@@ -239,7 +252,7 @@ struct FinMS {
 
 // x_i += b_i s_i p_i for all shifts; r -= b_0 s_0 Ap; |r|^2
 template <int P>
-__global__ void __launch_bounds__(256) k_ms_x(MSPtrs V, Field r, Field Ap, Field proto, SolverDev *S, RedScratch rs) {
+__global__ void __launch_bounds__(256) k_ms_x(MSPtrs V, Field r, Field Ap, SolverDev *S, RedScratch rs) {
   typedef typename Prec<P>::C C;
   typedef typename Prec<P>::R R;
   typedef typename Prec<P>::S Sx;
@@ -254,10 +267,10 @@ __global__ void __launch_bounds__(256) k_ms_x(MSPtrs V, Field r, Field Ap, Field
   }
   double b[MAXSHIFT], zn[MAXSHIFT];
   ms_coefs(S, pkp, b, zn);
-  const int64_t vp = proto.g->vh_pad;
+  const int64_t vp = r.vp;
   double red[1] = {0.0};
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < proto.g->vh) {
+  if (i < r.g->vh) {
     for (int k = 0; k < S->nshift; k++) {
       C xv[3], pv[3];
       VecLd<P>::ld((const Sx *)V.x[k], V.xn[k], vp, i, xv, (R)1);
@@ -285,7 +298,7 @@ __global__ void __launch_bounds__(256) k_ms_x(MSPtrs V, Field r, Field Ap, Field
 #pragma unroll
     for (int j = 0; j < 3; j++) red[0] += (double)rr[j].x * rr[j].x + (double)rr[j].y * rr[j].y;
   }
-  grid_sum<1>(red, rs, FinMS{S, pkp});
+  grid_partial<1>(red, rs);
 }
 
 // p_0 = a_0 p_0 + r;  p_i = a_i p_i + zn_i r   (lib/cgms.c:141-155)
@@ -295,7 +308,7 @@ __global__ void __launch_bounds__(256) k_ms_p(MSPtrs V, Field r, SolverDev *S) {
   typedef typename Prec<P>::R R;
   typedef typename Prec<P>::S Sx;
   if (S->done) return;
-  const int64_t vp = r.g->vh_pad;
+  const int64_t vp = r.vp;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= r.g->vh) return;
   C rv[3];
@@ -315,82 +328,100 @@ __global__ void __launch_bounds__(256) k_ms_p(MSPtrs V, Field r, SolverDev *S) {
 }
 
 // ------------------------------------------------------------ host helpers
-struct Work {
-  qamd_lattice *lat;
-  int prec;
-  qamd_field *t, *Ap, *r, *p;
-  SolverDev *S;       // device
-  SolverDev *Sh;      // pinned host mirror
+struct State {
+  SolverDev *S;    // device
+  SolverDev *Sh;   // pinned host mirror
 };
 
-static void work_free(Work &w) {
-  qamd_field_destroy(w.t);
-  qamd_field_destroy(w.Ap);
-  qamd_field_destroy(w.r);
-  qamd_field_destroy(w.p);
-  dev_free(w.S);
-  if (w.Sh) (void)hipHostFree(w.Sh);
-  memset(&w, 0, sizeof w);
+static void state_free(State &s) {
+  dev_free(s.S);
+  if (s.Sh) (void)hipHostFree(s.Sh);
+  s.S = s.Sh = nullptr;
 }
 
-static int work_init(Work &w, qamd_lattice *lat, int prec) {
-  memset(&w, 0, sizeof w);
-  w.lat = lat;
-  w.prec = prec;
-  w.t = qamd_field_create(lat, prec, 3, 1, 1);
-  w.Ap = qamd_field_create(lat, prec, 3, 1, 0);
-  w.r = qamd_field_create(lat, prec, 3, 1, 0);
-  w.p = qamd_field_create(lat, prec, 3, 1, 0);
-  w.S = (SolverDev *)dev_alloc(sizeof(SolverDev));
-  if (hipHostMalloc((void **)&w.Sh, sizeof(SolverDev), hipHostMallocDefault) != hipSuccess) w.Sh = nullptr;
-  if (!w.t || !w.Ap || !w.r || !w.p || !w.S || !w.Sh) {
-    if (!w.Sh) set_error("hipHostMalloc failed");
-    work_free(w);
+static int state_init(State &s, qamd_lattice *lat) {
+  s.S = (SolverDev *)dev_alloc(sizeof(SolverDev));
+  if (hipHostMalloc((void **)&s.Sh, sizeof(SolverDev), hipHostMallocDefault) != hipSuccess) s.Sh = nullptr;
+  if (!s.S || !s.Sh) {
+    if (!s.Sh) set_error("hipHostMalloc failed");
+    state_free(s);
     return -1;
   }
-  if (red_reserve((int)((lat->g.vh + 255) / 256))) {
-    work_free(w);
-    return -1;
-  }
-  return 0;
+  memset(s.Sh, 0, sizeof(SolverDev));
+  return red_reserve((int)((lat->g.vh + 255) / 256));
 }
 
-static int push_state(Work &w) {
-  QAMD_HIP(hipMemcpyAsync(w.S, w.Sh, sizeof(SolverDev), hipMemcpyHostToDevice, stream()));
+static int push_state(State &s) {
+  QAMD_HIP(hipMemcpyAsync(s.S, s.Sh, sizeof(SolverDev), hipMemcpyHostToDevice, stream()));
   return 0;
 }
-static int pull_state(Work &w) {
-  QAMD_HIP(hipMemcpyAsync(w.Sh, w.S, sizeof(SolverDev), hipMemcpyDeviceToHost, stream()));
+static int pull_state(State &s) {
+  QAMD_HIP(hipMemcpyAsync(s.Sh, s.S, sizeof(SolverDev), hipMemcpyDeviceToHost, stream()));
   QAMD_HIP(hipStreamSynchronize(stream()));
   return 0;
 }
 
 static inline dim3 grid1(const qamd_lattice *lat) { return dim3((unsigned)((lat->g.vh + 255) / 256)); }
 
-// one normal-operator application with reductions into S->nrm
-static int normal_op(const qamd_links *l, Field &Ap, Field &p, Field &t, SolverDev *S, const int *skip) {
-  if (stag_dslash_launch(l, t, p, 0.0, nullptr, 1.0, &S->nrm[0], stream(), skip)) return -1;
-  // Ap = m^2 p - D tmp: the mass enters through w, a = m^2 from the state
+static double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+static int check_vec(const qamd_field *f, int prec, int parity, const char *what) {
+  if (!f || f->f.ncomp != 3 || f->f.nrhs != 1 || f->f.parity != parity || (prec >= 0 && f->f.prec != prec)) {
+    set_error("%s: expected a single-rhs colour vector of parity %d%s", what, parity,
+              prec >= 0 ? " at the required precision" : "");
+    return -1;
+  }
   return 0;
 }
 
-template <int P>
-static void launch_cg_p(Work &w, qamd_field *p, qamd_field *r) {
-  hipLaunchKernelGGL(k_cg_p<P>, grid1(w.lat), dim3(256), 0, stream(), p->f, r->f, w.S);
+// Ap = m^2 p - D_eo D_oe p with the reductions of dslashp2
+// (lib/stagDslash.c:546-575): |t|^2 -> nrm[0], |p|^2 -> nrm[3].  pin is the
+// Dslash input copy of p (an FP16 shadow for FP16 links), pw the full p.
+struct FinNormal {
+  SolverDev *S;
+  __device__ void operator()(const double *t) const {
+    for (int i = 0; i < 4; i++) S->nrm[i] = t[i];
+  }
+};
+
+static int apply_normal(const qamd_links *l, SolverDev *S, qamd_field *Ap, qamd_field *pin, qamd_field *pw,
+                        qamd_field *t, double m2, const int *skip) {
+  // partial sums of |t|^2 in set 1 and of |Ap|^2, |p|^2 in set 2, one finisher
+  if (stag_dslash_launch(l, t->f, pin->f, 0.0, nullptr, 1.0, nullptr, stream(), skip, 1, false)) return -1;
+  if (stag_dslash_launch(l, Ap->f, t->f, m2, &pw->f, -1.0, nullptr, stream(), skip, 2, false)) return -1;
+  int nb = (int)((l->lat->g.vh + 255) / 256);
+  hipLaunchKernelGGL((k_finish2<2, 2, FinNormal>), dim3(1), dim3(256), 0, stream(), red_set(1), red_set(2), nb,
+                     FinNormal{S}, skip);
+  QAMD_HIP(hipGetLastError());
+  return 0;
 }
-static int cg_p(Work &w, qamd_field *p, qamd_field *r) {
+
+template <int P, int SH>
+static void launch_cg_p(const qamd_lattice *lat, qamd_field *p, qamd_field *r, qamd_field *sh, SolverDev *S) {
+  hipLaunchKernelGGL((k_cg_p<P, SH>), grid1(lat), dim3(256), 0, stream(), p->f, r->f, sh ? sh->f : p->f, S);
+}
+static int cg_p(const qamd_lattice *lat, qamd_field *p, qamd_field *r, qamd_field *sh, SolverDev *S) {
+  if (sh && p->f.prec != QAMD_F32) {
+    set_error("cg_p: the FP16 shadow needs an FP32 p");
+    return -1;
+  }
   switch (p->f.prec) {
-    case QAMD_F64: launch_cg_p<QAMD_F64>(w, p, r); break;
-    case QAMD_F32: launch_cg_p<QAMD_F32>(w, p, r); break;
-    default: launch_cg_p<QAMD_F16>(w, p, r); break;
+    case QAMD_F64: launch_cg_p<QAMD_F64, 0>(lat, p, r, nullptr, S); break;
+    case QAMD_F32:
+      if (sh) launch_cg_p<QAMD_F32, 1>(lat, p, r, sh, S);
+      else launch_cg_p<QAMD_F32, 0>(lat, p, r, nullptr, S);
+      break;
+    default: launch_cg_p<QAMD_F16, 0>(lat, p, r, nullptr, S); break;
   }
   QAMD_HIP(hipGetLastError());
   return 0;
 }
 
-static int cg_xr(Work &w, qamd_field *x, qamd_field *p, qamd_field *r, qamd_field *Ap) {
+static int cg_xr(const qamd_lattice *lat, qamd_field *x, qamd_field *p, qamd_field *r, qamd_field *Ap, SolverDev *S) {
   int P = p->f.prec, PX = x->f.prec;
-#define XR(A, B) hipLaunchKernelGGL((k_cg_xr<A, B>), grid1(w.lat), dim3(256), 0, stream(), x->f, p->f, r->f, Ap->f, w.S, red_scratch())
+#define XR(A, B) hipLaunchKernelGGL((k_cg_xr<A, B>), grid1(lat), dim3(256), 0, stream(), x->f, p->f, r->f, Ap->f, S, red_set(0))
   if (P == QAMD_F64 && PX == QAMD_F64) XR(QAMD_F64, QAMD_F64);
   else if (P == QAMD_F32 && PX == QAMD_F32) XR(QAMD_F32, QAMD_F32);
   else if (P == QAMD_F16 && PX == QAMD_F32) XR(QAMD_F16, QAMD_F32);
@@ -400,40 +431,8 @@ static int cg_xr(Work &w, qamd_field *x, qamd_field *p, qamd_field *r, qamd_fiel
     return -1;
   }
 #undef XR
+  launch_finish<1>(red_set(0), (int)grid1(lat).x, FinCG{S}, stream(), &S->done);
   QAMD_HIP(hipGetLastError());
-  return 0;
-}
-
-// Ap = m^2 p - D_eo D_oe p with the reductions of dslashp2 (lib/stagDslash.c:546-575)
-static int apply_normal(const qamd_links *l, Work &w, qamd_field *Ap, qamd_field *p, qamd_field *t, double m2,
-                        const int *skip) {
-  if (stag_dslash_launch(l, t->f, p->f, 0.0, nullptr, 1.0, &w.S->nrm[0], stream(), skip)) return -1;
-  if (stag_dslash_launch(l, Ap->f, t->f, m2, &p->f, -1.0, &w.S->nrm[2], stream(), skip)) return -1;
-  return 0;
-}
-
-static double now_s() {
-  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
-}
-
-static int check_vec(const qamd_field *f, int prec, int parity, const char *what) {
-  if (!f || f->f.ncomp != 3 || f->f.nrhs != 1 || f->f.parity != parity || (prec >= 0 && f->f.prec != prec)) {
-    set_error("%s: expected a single-rhs colour vector of parity %d%s", what, parity,
-              prec >= 0 ? " at the links' precision" : "");
-    return -1;
-  }
-  return 0;
-}
-
-// FP64 true residual |b - (m2 - DD) x|^2 / |b|^2 using the work vectors
-static int true_resid(const qamd_links *l64, Work &w64, const qamd_field *x, const qamd_field *b, double m2,
-                      double *rel) {
-  if (apply_normal(l64, w64, w64.Ap, (qamd_field *)x, w64.t, m2, nullptr)) return -1;
-  if (qamd_field_copy(w64.r, b)) return -1;
-  if (qamd_field_axpby(-1.0, w64.Ap, 1.0, w64.r)) return -1;
-  double rr, bb;
-  if (qamd_field_norm2(w64.r, &rr) || qamd_field_norm2(b, &bb)) return -1;
-  *rel = bb > 0 ? rr / bb : 0.0;
   return 0;
 }
 
@@ -443,39 +442,43 @@ using namespace qamd;
 
 extern "C" {
 
+// QLL CG: all vectors at the links' precision, recurrences of cgSolveT
 int qamd_stag_cg(const qamd_links *l, qamd_field *x, const qamd_field *b, double mass, double rsq, int maxits,
                  qamd_solve_info *info) {
   if (!l || l->kind != 0) { set_error("stag_cg: staggered links required"); return -1; }
   if (check_vec(x, l->prec, 0, "stag_cg x") || check_vec(b, l->prec, 0, "stag_cg b")) return -1;
   double t0 = now_s();
-  Work w;
-  if (work_init(w, l->lat, l->prec)) return -1;
+  qamd_lattice *lat = l->lat;
+  State st;
+  memset(&st, 0, sizeof st);
+  qamd_field *t = qamd_field_create(lat, l->prec, 3, 1, 1);
+  qamd_field *Ap = qamd_field_create(lat, l->prec, 3, 1, 0);
+  qamd_field *r = qamd_field_create(lat, l->prec, 3, 1, 0);
+  qamd_field *p = qamd_field_create(lat, l->prec, 3, 1, 0);
   int rc = -1;
   do {
+    if (!t || !Ap || !r || !p || state_init(st, lat)) break;
     // x = 0, p = 0, r = b, rsq = |b|^2   (lib/cg.c:64-71)
-    if (qamd_field_zero(x) || qamd_field_zero(w.p) || qamd_field_copy(w.r, b)) break;
+    if (qamd_field_zero(x) || qamd_field_zero(p) || qamd_field_copy(r, b)) break;
     double bsq;
-    if (qamd_field_norm2(w.r, &bsq)) break;
-    memset(w.Sh, 0, sizeof(SolverDev));
-    w.Sh->rsq = bsq;
-    w.Sh->m2 = mass * mass;
-    w.Sh->rsqstop = rsq * bsq;
-    w.Sh->itnlim = maxits > 0 ? maxits : 1;
-    w.Sh->mode = 0;
-    w.Sh->auto_stop = 1;
-    if (bsq == 0) {   // zero source: the reference divides by zero; return x = 0
-      w.Sh->done = 1;
-    }
-    if (push_state(w)) break;
-    const int batch = 16;
+    if (qamd_field_norm2(r, &bsq)) break;
+    SolverDev *Sh = st.Sh;
+    Sh->rsq = bsq;
+    Sh->m2 = mass * mass;
+    Sh->rsqstop = rsq * bsq;
+    Sh->itnlim = maxits > 0 ? maxits : 1;
+    Sh->mode = 0;
+    Sh->auto_stop = 1;
+    if (bsq == 0) Sh->done = 1;   // zero source: the reference divides by zero; x = 0
+    if (push_state(st)) break;
     int err = 0;
-    while (!w.Sh->done) {
-      for (int k = 0; k < batch && !err; k++) {
-        err |= cg_p(w, w.p, w.r);
-        err |= apply_normal(l, w, w.Ap, w.p, w.t, mass * mass, &w.S->done);
-        err |= cg_xr(w, x, w.p, w.r, w.Ap);
+    while (!Sh->done) {
+      for (int k = 0; k < BATCH && !err; k++) {
+        err |= cg_p(lat, p, r, nullptr, st.S);
+        err |= apply_normal(l, st.S, Ap, p, p, t, mass * mass, &st.S->done);
+        err |= cg_xr(lat, x, p, r, Ap, st.S);
       }
-      if (err || pull_state(w)) {
+      if (err || pull_state(st)) {
         err = 1;
         break;
       }
@@ -483,108 +486,107 @@ int qamd_stag_cg(const qamd_links *l, qamd_field *x, const qamd_field *b, double
     if (err) break;
     if (info) {
       memset(info, 0, sizeof *info);
-      info->iterations = w.Sh->its;
-      info->iter_rsq = bsq > 0 ? w.Sh->rsq / bsq : 0;
+      info->iterations = Sh->its;
+      info->iter_rsq = bsq > 0 ? Sh->rsq / bsq : 0;
       info->true_rsq = -1;
-      info->converged = bsq == 0 || w.Sh->rsq < w.Sh->rsqstop;
+      info->converged = bsq == 0 || Sh->rsq < Sh->rsqstop;
       info->seconds = now_s() - t0;
     }
     rc = 0;
   } while (0);
-  work_free(w);
+  qamd_field_destroy(t);
+  qamd_field_destroy(Ap);
+  qamd_field_destroy(r);
+  qamd_field_destroy(p);
+  state_free(st);
   return rc;
 }
 
-// Mixed-precision CG with reliable updates (textbook CG in the sloppy
-// precision; FP64 x and true residual at every update).  If x is nonzero on
-// entry it is used as the initial guess.
+// Mixed-precision CG with reliable updates.  The Krylov vectors r, p, Ap and
+// the solution increment live in FP32; the Dslash runs at the sloppy links'
+// precision (FP32, or FP16 with an FP16 copy of p as input and FP32 output);
+// x, b and the true residual are FP64.  Reliable updates happen at batch
+// boundaries when |r| has dropped below delta * max|r| since the last one.
+// x is used as the initial guess.
 static int cg_mixed_core(const qamd_links *l64, const qamd_links *ls, qamd_field *x, const qamd_field *b,
                          double m2, double rsq, int maxits, double delta, qamd_solve_info *info) {
   double t0 = now_s();
   qamd_lattice *lat = l64->lat;
-  int ps = ls->prec;
-  int px = ps == QAMD_F16 ? QAMD_F32 : ps;
-  Work w64, ws;
-  if (work_init(w64, lat, QAMD_F64)) return -1;
-  if (work_init(ws, lat, ps)) {
-    work_free(w64);
-    return -1;
-  }
-  qamd_field *xs = qamd_field_create(lat, px, 3, 1, 0);
-  qamd_field *Ax = w64.Ap;
+  const int ps = ls->prec;
+  const bool half = (ps == QAMD_F16);
+  State st;
+  memset(&st, 0, sizeof st);
+  qamd_field *r64 = qamd_field_create(lat, QAMD_F64, 3, 1, 0);
+  qamd_field *Ax = qamd_field_create(lat, QAMD_F64, 3, 1, 0);
+  qamd_field *t64 = qamd_field_create(lat, QAMD_F64, 3, 1, 1);
+  qamd_field *r = qamd_field_create(lat, QAMD_F32, 3, 1, 0);
+  qamd_field *p = qamd_field_create(lat, QAMD_F32, 3, 1, 0);
+  qamd_field *Ap = qamd_field_create(lat, QAMD_F32, 3, 1, 0);
+  qamd_field *xs = qamd_field_create(lat, QAMD_F32, 3, 1, 0);
+  qamd_field *t = qamd_field_create(lat, ps, 3, 1, 1);
+  qamd_field *p16 = half ? qamd_field_create(lat, QAMD_F16, 3, 1, 0) : nullptr;
+  qamd_field *pin = half ? p16 : p;
   int rc = -1, nupd = 0, its = 0;
+  const bool verbose = getenv("QAMD_VERBOSE") != nullptr;
   double bsq = 0, r2 = 0;
   do {
-    if (!xs) break;
+    if (!r64 || !Ax || !t64 || !r || !p || !Ap || !xs || !t || (half && !p16) || state_init(st, lat)) break;
     if (qamd_field_norm2(b, &bsq)) break;
     if (bsq == 0) {
       if (qamd_field_zero(x)) break;
       rc = 0;
       break;
     }
-    double stop = rsq * bsq;
-    memset(ws.Sh, 0, sizeof(SolverDev));
-    ws.Sh->m2 = m2;
-    ws.Sh->itnlim = 1 << 30;
-    ws.Sh->mode = 1;
-    ws.Sh->auto_stop = 0;
-    if (push_state(ws)) break;
-    // initial true residual r = b - A x (x may be a guess)
+    const double stop = rsq * bsq;
+    SolverDev *Sh = st.Sh;
+    Sh->m2 = m2;
+    Sh->itnlim = 1 << 30;
+    Sh->mode = 1;
+    Sh->rsq_old = 1.0;
+    if (push_state(st)) break;
+    // true residual r64 = b - A x in FP64, r = r64 in FP32, |r64|^2
     auto reliable = [&](void) -> int {
-      if (apply_normal(l64, w64, Ax, x, w64.t, m2, nullptr)) return -1;
-      switch (ps) {
-        case QAMD_F32:
-          hipLaunchKernelGGL(k_ru_resid<QAMD_F32>, grid1(lat), dim3(256), 0, stream(), w64.r->f, b->f, Ax->f,
-                             ws.r->f, ws.S, red_scratch());
-          break;
-        default:
-          hipLaunchKernelGGL(k_ru_resid<QAMD_F16>, grid1(lat), dim3(256), 0, stream(), w64.r->f, b->f, Ax->f,
-                             ws.r->f, ws.S, red_scratch());
-          break;
-      }
+      if (apply_normal(l64, st.S, Ax, x, x, t64, m2, nullptr)) return -1;
+      hipLaunchKernelGGL(k_ru_resid<QAMD_F32>, grid1(lat), dim3(256), 0, stream(), r64->f, b->f, Ax->f, r->f, st.S,
+                         red_set(0));
+      launch_finish<1>(red_set(0), (int)grid1(lat).x, FinTrue{st.S}, stream());
       QAMD_HIP(hipGetLastError());
-      return pull_state(ws);
+      return pull_state(st);
     };
-    ws.Sh->rsq_old = 1.0;
-    if (push_state(ws) || reliable()) break;
-    r2 = ws.Sh->rsq;
+    if (reliable()) break;
+    r2 = Sh->rsq;
     if (r2 < stop) {
       rc = 0;
       break;
     }
-    // p = r
-    if (qamd_field_copy(ws.p, ws.r) || qamd_field_zero(xs)) break;
-    double maxrr = sqrt(r2);
+    if (qamd_field_copy(p, r) || qamd_field_zero(xs) || (half && qamd_field_copy(p16, p))) break;
     int err = 0;
     while (its < maxits) {
-      err |= apply_normal(ls, ws, ws.Ap, ws.p, ws.t, m2, nullptr);
-      err |= cg_xr(ws, xs, ws.p, ws.r, ws.Ap);
-      if (err || pull_state(ws)) {
+      for (int k = 0; k < BATCH && !err; k++) {
+        err |= apply_normal(ls, st.S, Ap, pin, p, t, m2, nullptr);
+        err |= cg_xr(lat, xs, p, r, Ap, st.S);
+        err |= cg_p(lat, p, r, p16, st.S);
+      }
+      if (err || pull_state(st)) {
         err = 1;
         break;
       }
-      its++;
-      double r2s = ws.Sh->rsq;
-      double rn = sqrt(r2s);
-      if (rn > maxrr) maxrr = rn;
-      if (rn < delta * maxrr || r2s < stop || its >= maxits) {
-        // reliable update: x += xs, r = b - A x in FP64, r_sloppy = r
-        if (px == QAMD_F32)
-          hipLaunchKernelGGL(k_ru_accum<QAMD_F32>, grid1(lat), dim3(256), 0, stream(), x->f, xs->f);
-        else
-          hipLaunchKernelGGL(k_ru_accum<QAMD_F64>, grid1(lat), dim3(256), 0, stream(), x->f, xs->f);
+      its += BATCH;
+      double rn = sqrt(Sh->rsq);
+      if (verbose)
+        fprintf(stderr, "cg_mixed prec %d its %d rel|r| %.3e max %.3e pAp %.3e alpha %.3e beta %.3e\n", ps, its,
+                rn / sqrt(bsq), Sh->maxrr / sqrt(bsq), Sh->m2 * Sh->nrm[3] + Sh->nrm[0], Sh->alpha, Sh->beta);
+      if (rn < delta * Sh->maxrr || Sh->rsq < stop || its >= maxits) {
+        // reliable update: x += xs (xs = 0), r = b - A x in FP64
+        hipLaunchKernelGGL(k_ru_accum<QAMD_F32>, grid1(lat), dim3(256), 0, stream(), x->f, xs->f);
         if (hipGetLastError() != hipSuccess || reliable()) {
           err = 1;
           break;
         }
         nupd++;
-        r2 = ws.Sh->rsq;
+        r2 = Sh->rsq;
+        if (verbose) fprintf(stderr, "cg_mixed reliable update %d: true rel|r| %.3e\n", nupd, sqrt(r2 / bsq));
         if (r2 < stop) break;
-        maxrr = sqrt(r2);
-      }
-      if (cg_p(ws, ws.p, ws.r)) {
-        err = 1;
-        break;
       }
     }
     if (err) break;
@@ -599,9 +601,16 @@ static int cg_mixed_core(const qamd_links *l64, const qamd_links *ls, qamd_field
     info->converged = bsq == 0 || r2 < rsq * bsq;
     info->seconds = now_s() - t0;
   }
+  qamd_field_destroy(r64);
+  qamd_field_destroy(Ax);
+  qamd_field_destroy(t64);
+  qamd_field_destroy(r);
+  qamd_field_destroy(p);
+  qamd_field_destroy(Ap);
   qamd_field_destroy(xs);
-  work_free(ws);
-  work_free(w64);
+  qamd_field_destroy(t);
+  qamd_field_destroy(p16);
+  state_free(st);
   return rc;
 }
 
@@ -626,17 +635,17 @@ int qamd_stag_cgms(const qamd_links *l, qamd_field *const x[], const qamd_field 
   for (int i = 0; i < n; i++)
     if (check_vec(x[i], l->prec, 0, "stag_cgms x")) return -1;
   double t0 = now_s();
-  Work w;
-  if (work_init(w, l->lat, l->prec)) return -1;
+  qamd_lattice *lat = l->lat;
+  State st;
+  memset(&st, 0, sizeof st);
+  qamd_field *t = qamd_field_create(lat, l->prec, 3, 1, 1);
+  qamd_field *Ap = qamd_field_create(lat, l->prec, 3, 1, 0);
+  qamd_field *r = qamd_field_create(lat, l->prec, 3, 1, 0);
   qamd_field *ps[MAXSHIFT] = {nullptr};
   int rc = -1;
   do {
-    ps[0] = w.p;
-    int ok = 1;
-    for (int i = 1; i < n; i++) {
-      ps[i] = qamd_field_create(l->lat, l->prec, 3, 1, 0);
-      if (!ps[i]) ok = 0;
-    }
+    int ok = t && Ap && r && !state_init(st, lat);
+    for (int i = 0; i < n; i++) ok &= (ps[i] = qamd_field_create(lat, l->prec, 3, 1, 0)) != nullptr;
     if (!ok) break;
     double insq;
     if (qamd_field_norm2(b, &insq)) break;
@@ -648,23 +657,23 @@ int qamd_stag_cgms(const qamd_links *l, qamd_field *const x[], const qamd_field 
       rc = 0;
       break;
     }
-    if (qamd_field_copy(w.r, b)) break;
-    for (int i = 0; i < n; i++) err |= qamd_field_copy(ps[i], w.r);
+    if (qamd_field_copy(r, b)) break;
+    for (int i = 0; i < n; i++) err |= qamd_field_copy(ps[i], r);
     if (err) break;
-    memset(w.Sh, 0, sizeof(SolverDev));
-    w.Sh->nshift = n;
+    SolverDev *Sh = st.Sh;
+    Sh->nshift = n;
     for (int i = 0; i < n; i++) {
-      w.Sh->shifts[i] = shifts[i];
-      w.Sh->zo[i] = w.Sh->z[i] = 1;
-      w.Sh->bo[i] = -1;
-      w.Sh->a[i] = 0;
-      w.Sh->s[i] = 1;
+      Sh->shifts[i] = shifts[i];
+      Sh->zo[i] = Sh->z[i] = 1;
+      Sh->bo[i] = -1;
+      Sh->a[i] = 0;
+      Sh->s[i] = 1;
     }
-    w.Sh->rsq = insq;
-    w.Sh->rsqstop = rsq * insq;
-    w.Sh->m2 = mass * mass;
-    w.Sh->itnlim = maxits > 0 ? maxits : 1;
-    if (push_state(w)) break;
+    Sh->rsq = insq;
+    Sh->rsqstop = rsq * insq;
+    Sh->m2 = mass * mass;
+    Sh->itnlim = maxits > 0 ? maxits : 1;
+    if (push_state(st)) break;
     MSPtrs V;
     memset(&V, 0, sizeof V);
     for (int i = 0; i < n; i++) {
@@ -673,41 +682,48 @@ int qamd_stag_cgms(const qamd_links *l, qamd_field *const x[], const qamd_field 
       V.p[i] = ps[i]->f.data;
       V.pn[i] = ps[i]->f.norm;
     }
-    const int batch = 8;
-    while (!w.Sh->done && !err) {
-      for (int k = 0; k < batch && !err; k++) {
-        err |= apply_normal(l, w, w.Ap, ps[0], w.t, mass * mass, &w.S->done);
+    while (!Sh->done && !err) {
+      for (int k = 0; k < BATCH && !err; k++) {
+        err |= apply_normal(l, st.S, Ap, ps[0], ps[0], t, mass * mass, &st.S->done);
+        RedScratch rs0 = red_set(0);
+        int nb = (int)grid1(lat).x;
         switch (l->prec) {
           case QAMD_F64:
-            hipLaunchKernelGGL(k_ms_x<QAMD_F64>, grid1(l->lat), dim3(256), 0, stream(), V, w.r->f, w.Ap->f, w.r->f, w.S, red_scratch());
-            hipLaunchKernelGGL(k_ms_p<QAMD_F64>, grid1(l->lat), dim3(256), 0, stream(), V, w.r->f, w.S);
+            hipLaunchKernelGGL(k_ms_x<QAMD_F64>, grid1(lat), dim3(256), 0, stream(), V, r->f, Ap->f, st.S, rs0);
+            launch_finish<1>(rs0, nb, FinMS{st.S}, stream(), &st.S->done);
+            hipLaunchKernelGGL(k_ms_p<QAMD_F64>, grid1(lat), dim3(256), 0, stream(), V, r->f, st.S);
             break;
           case QAMD_F32:
-            hipLaunchKernelGGL(k_ms_x<QAMD_F32>, grid1(l->lat), dim3(256), 0, stream(), V, w.r->f, w.Ap->f, w.r->f, w.S, red_scratch());
-            hipLaunchKernelGGL(k_ms_p<QAMD_F32>, grid1(l->lat), dim3(256), 0, stream(), V, w.r->f, w.S);
+            hipLaunchKernelGGL(k_ms_x<QAMD_F32>, grid1(lat), dim3(256), 0, stream(), V, r->f, Ap->f, st.S, rs0);
+            launch_finish<1>(rs0, nb, FinMS{st.S}, stream(), &st.S->done);
+            hipLaunchKernelGGL(k_ms_p<QAMD_F32>, grid1(lat), dim3(256), 0, stream(), V, r->f, st.S);
             break;
           default:
-            hipLaunchKernelGGL(k_ms_x<QAMD_F16>, grid1(l->lat), dim3(256), 0, stream(), V, w.r->f, w.Ap->f, w.r->f, w.S, red_scratch());
-            hipLaunchKernelGGL(k_ms_p<QAMD_F16>, grid1(l->lat), dim3(256), 0, stream(), V, w.r->f, w.S);
+            hipLaunchKernelGGL(k_ms_x<QAMD_F16>, grid1(lat), dim3(256), 0, stream(), V, r->f, Ap->f, st.S, rs0);
+            launch_finish<1>(rs0, nb, FinMS{st.S}, stream(), &st.S->done);
+            hipLaunchKernelGGL(k_ms_p<QAMD_F16>, grid1(lat), dim3(256), 0, stream(), V, r->f, st.S);
             break;
         }
         if (hipGetLastError() != hipSuccess) err = 1;
       }
-      if (err || pull_state(w)) err = 1;
+      if (err || pull_state(st)) err = 1;
     }
     if (err) break;
     if (info) {
       memset(info, 0, sizeof *info);
-      info->iterations = w.Sh->its;
-      info->iter_rsq = w.Sh->rsq / insq;
+      info->iterations = Sh->its;
+      info->iter_rsq = Sh->rsq / insq;
       info->true_rsq = -1;
-      info->converged = w.Sh->rsq < w.Sh->rsqstop;
+      info->converged = Sh->rsq < Sh->rsqstop;
       info->seconds = now_s() - t0;
     }
     rc = 0;
   } while (0);
-  for (int i = 1; i < n; i++) qamd_field_destroy(ps[i]);
-  work_free(w);
+  for (int i = 0; i < n; i++) qamd_field_destroy(ps[i]);
+  qamd_field_destroy(t);
+  qamd_field_destroy(Ap);
+  qamd_field_destroy(r);
+  state_free(st);
   return rc;
 }
 
@@ -722,9 +738,8 @@ int qamd_stag_cgms_mixed(const qamd_links *l64, const qamd_links *ls, qamd_field
     if (check_vec(x[i], QAMD_F64, 0, "cgms_mixed x")) return -1;
   if (check_vec(b, QAMD_F64, 0, "cgms_mixed b")) return -1;
   qamd_lattice *lat = l64->lat;
-  // multishift in FP32 (FP16 sloppy links are promoted: the shifted
-  // recurrences need the FP32 vectors)
-  qamd_links *l32 = (qamd_links *)ls;
+  // the multishift recurrences run in FP32 (FP16 links are promoted for them)
+  const qamd_links *l32 = ls;
   qamd_links *own = nullptr;
   if (ls->prec != QAMD_F32) {
     own = qamd_links_create(lat, 0, QAMD_F32, l64->nterms);
@@ -743,7 +758,7 @@ int qamd_stag_cgms_mixed(const qamd_links *l64, const qamd_links *ls, qamd_field
     for (int i = 0; i < n; i++) ok &= (xs[i] = qamd_field_create(lat, QAMD_F32, 3, 1, 0)) != nullptr;
     if (!ok) break;
     qamd_solve_info msinfo;
-    // the FP32 recurrences cannot resolve residuals far below 1e-6; refine
+    // FP32 recurrences do not resolve residuals far below 1e-6; refine
     double ms_rsq = rsq > 1e-12 ? rsq : 1e-12;
     if (qamd_stag_cgms(l32, xs, bs, mass, shifts, n, ms_rsq, maxits, &msinfo)) break;
     int err = 0;

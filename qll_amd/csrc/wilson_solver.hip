@@ -34,7 +34,7 @@ struct FinRhs {
 };
 
 // per-rhs |f|^2 (mode 0) or Re <f, h> (mode 1)
-__global__ void __launch_bounds__(256) k_rhs_dot(Field f, Field h, int mode, RedScratch rs, FinRhs fin) {
+__global__ void __launch_bounds__(256) k_rhs_dot(Field f, Field h, int mode, RedScratch rs) {
   double acc[MAXRHS];
 #pragma unroll
   for (int i = 0; i < MAXRHS; i++) acc[i] = 0.0;
@@ -50,7 +50,7 @@ __global__ void __launch_bounds__(256) k_rhs_dot(Field f, Field h, int mode, Red
     for (int i = 0; i < MAXRHS; i++)
       if (i == r) acc[i] += s;
   }
-  grid_sum<MAXRHS>(acc, rs, fin);
+  grid_partial<MAXRHS>(acc, rs);
 }
 
 // y_r = a_r x_r + b_r y_r  (x, y may differ in precision)
@@ -115,8 +115,9 @@ static inline unsigned nblk(const qamd_field *f) {
 static int rhs_dot(WS &w, const qamd_field *f, const qamd_field *h, double *out) {
   int nb = 1024;
   if (red_reserve(nb)) return -1;
-  hipLaunchKernelGGL(k_rhs_dot, dim3(nb), dim3(256), 0, stream(), f->f, h ? h->f : f->f, h ? 1 : 0, red_scratch(),
-                     FinRhs{w.coef + 32, f->f.nrhs});
+  RedScratch rs = red_set(0);
+  hipLaunchKernelGGL(k_rhs_dot, dim3(nb), dim3(256), 0, stream(), f->f, h ? h->f : f->f, h ? 1 : 0, rs);
+  launch_finish<MAXRHS>(rs, nb, FinRhs{w.coef + 32, f->f.nrhs}, stream());
   QAMD_HIP(hipGetLastError());
   QAMD_HIP(hipMemcpyAsync(w.coefh + 32, w.coef + 32, f->f.nrhs * sizeof(double), hipMemcpyDeviceToHost, stream()));
   QAMD_HIP(hipStreamSynchronize(stream()));

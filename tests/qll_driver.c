@@ -192,7 +192,8 @@ DEFINE_DIAG(float, F)
     orc_gauss_vector(dims, 9, 100, -1, 6, yn);                                                 \
     nat2qll_d(yn, tmpq, 6);                                                                    \
     for (long i = 0; i < V * 6; i++) yq[i] = (R)tmpq[i];                                       \
-    for (long i = 0; i < V * 6; i++) yn[i] = (double)yq[i];                                    \
+    for (long i = 0; i < V * 6; i++) tmpq[i] = yq[i];                                          \
+    qll2nat_d(tmpq, yn, 6);                                                                    \
     const char *subs[3] = {"all", "even", "odd"};                                              \
     for (int fb = 0; fb < 2; fb++)                                                             \
       for (int si = 0; si < 3; si++) {                                                         \
@@ -268,7 +269,7 @@ DEFINE_DIAG(float, F)
       double e = 0;                                                                            \
       for (long i = 0; i < n; i++) e = fmax(e, fabs(x[i] - b[i] / da.w[i]));                   \
       check("cgSolve host Linop", e, sizeof(R) == 8 ? 1e-12 : 1e-5);                           \
-      check("cgSolve its <= 8 (7 distinct eigenvalues)", it > 8 ? it : 0, 8);                  \
+      check("cgSolve its (7 distinct eigenvalues)", it > (sizeof(R) == 8 ? 8 : 10) ? it : 0, 0);                  \
       double sh[3] = {0, 0.5, 2.0}, rs[3] = {1e-20, 1e-20, 1e-20};                             \
       R *xm[3] = {myalloc(n * sizeof(R)), myalloc(n * sizeof(R)), myalloc(n * sizeof(R))};    \
       P(cgmsSolve)(&cg, xm, b, sh, 3, DIAG, &da, NULL, NULL, rs, 200);                          \
