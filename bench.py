@@ -25,7 +25,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FLOPS_ASQTAD = 1146          # per EO output site
-BYTES_SITE = {0: 2400, 1: 1200, 2: 608}   # algorithmic bytes per EO output site
+BYTES_SITE = {0: 2400, 1: 1200, 2: 608, 3: 608}   # algorithmic bytes per EO output site
+PREC_NAME = {0: "f64", 1: "f32", 2: "f16", 3: "q16"}
 HBM_PEAK_GBS = 8000.0
 DIMS = (32, 32, 32, 32)
 QLL_MASS = 0.02
@@ -39,8 +40,10 @@ def parse():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cg", action="store_true", help="skip the solver comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--block", type=int, default=8, help="sources in the batched mixed-precision solve")
     ap.add_argument("--dims", type=str, default=None, help="override lattice, e.g. 16,16,16,16")
-    ap.add_argument("--precs", type=str, default="0,1,2", help="Dslash precisions to time (0 f64, 1 f32, 2 f16)")
+    ap.add_argument("--precs", type=str, default="0,1,2,3",
+                    help="Dslash precisions to time (0 f64, 1 f32, 2 f16, 3 16-bit fixed point)")
     return ap.parse_args()
 
 
@@ -86,6 +89,7 @@ def main():
     L64 = Q.asqtad_links_device(lat, seed=1)
     L32 = L64.converted(Q.F32)
     L16 = L64.converted(Q.F16)
+    LQ = L64.converted(Q.Q16)
     Q.sync()
     setup_s = time.time() - t0
     print("setup (links: SU(3)+phases+fat7+Naik+doublestore, 3 precisions): %.2f s" % setup_s,
@@ -100,7 +104,7 @@ def main():
     precs = [int(p) for p in args.precs.split(",")]
     if 0 not in precs:
         precs.insert(0, 0)
-    for prec, L in ((0, L64), (1, L32), (2, L16)):
+    for prec, L in ((0, L64), (1, L32), (2, L16), (3, LQ)):
         if prec not in precs:
             continue
         yin = Q.Field(lat, prec, 3, 1, Q.ODD).gaussian(5, 100)
@@ -143,7 +147,7 @@ def main():
         solver["fp64_cg"] = {"seconds": round(dt, 4), "iterations": info["iterations"],
                              "true_rel_resid": true_rsq() ** 0.5, "solves_per_s": round(1 / dt, 4)}
         print("fp64 cg: %s" % solver["fp64_cg"], file=sys.stderr, flush=True)
-        for name, Ls in (("mixed_fp32", L32), ("mixed_fp16", L16)):
+        for name, Ls in (("mixed_fp32", L32), ("mixed_q16", LQ), ("mixed_fp16", L16)):
             x.zero()
             Q.sync()
             t0 = time.time()
@@ -152,8 +156,38 @@ def main():
             dt = time.time() - t0
             solver[name] = {"seconds": round(dt, 4), "iterations": info["iterations"],
                             "reliable_updates": info["reliable_updates"],
+                            "restarts": info["restarts"], "escalated_to_fp32": info["escalations"],
                             "true_rel_resid": true_rsq() ** 0.5, "solves_per_s": round(1 / dt, 4),
                             "speedup_vs_fp64": round(solver["fp64_cg"]["seconds"] / dt, 3)}
+            print("%s: %s" % (name, solver[name]), file=sys.stderr, flush=True)
+
+        # batched: a block of sources solved together (one block-Dslash pair per
+        # iteration for the whole block), solves/s = block size / wall time
+        nb = args.block
+        bB = Q.Field(lat, Q.F64, 3, nb, Q.EVEN).gaussian(21, 100)
+        xB = Q.Field(lat, Q.F64, 3, nb, Q.EVEN)
+        ApB = Q.Field(lat, Q.F64, 3, nb, Q.EVEN)
+        tB = Q.Field(lat, Q.F64, 3, nb, Q.ODD)
+        bBsq = bB.norm2() if nb > 1 else [bB.norm2()]
+
+        def true_rsq_block():
+            Q.stag_normal(L64, ApB, xB, tB, QLL_MASS)
+            ApB.axpby(1.0, bB, -1.0)
+            r2 = ApB.norm2() if nb > 1 else [ApB.norm2()]
+            return max(r2[i] / bBsq[i] for i in range(nb))
+
+        for name, Ls in (("block%d_mixed_q16" % nb, LQ), ("block%d_mixed_fp32" % nb, L32)):
+            Q.sync()
+            t0 = time.time()
+            infos = Q.stag_cg_mixed_block(L64, Ls, xB, bB, QLL_MASS, RSQ, 50000, 0.1)
+            Q.sync()
+            dt = time.time() - t0
+            sps = nb / dt
+            solver[name] = {"seconds": round(dt, 4), "rhs": nb, "iterations": infos[0]["iterations"],
+                            "reliable_updates": infos[0]["reliable_updates"], "restarts": infos[0]["restarts"],
+                            "converged": all(i["converged"] for i in infos),
+                            "max_true_rel_resid": true_rsq_block() ** 0.5, "solves_per_s": round(sps, 4),
+                            "speedup_vs_fp64": round(sps / solver["fp64_cg"]["solves_per_s"], 3)}
             print("%s: %s" % (name, solver[name]), file=sys.stderr, flush=True)
 
     cpu = None
@@ -182,7 +216,7 @@ def main():
                      "measured_copy_gbs": round(copy_gbs, 1),
                      "frac_of_copy": round(achieved / copy_gbs, 4)},
         "cpu_baseline": cpu,
-        "dslash": {{0: "f64", 1: "f32", 2: "f16"}[p]: prof[p] for p in prof},
+        "dslash": {PREC_NAME[p]: prof[p] for p in prof},
         "solvers": solver,
         "device": Q.device_name(),
     }

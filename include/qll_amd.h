@@ -25,7 +25,10 @@
 extern "C" {
 #endif
 
-enum { QAMD_F64 = 0, QAMD_F32 = 1, QAMD_F16 = 2 };
+/* storage precisions: IEEE double, single, half (FP16), and 16-bit fixed
+ * point (Q16: int16 mantissas, vectors with an FP32 max-norm per site, links
+ * with an FP32 scale per term; 15-bit resolution at FP16's bandwidth) */
+enum { QAMD_F64 = 0, QAMD_F32 = 1, QAMD_F16 = 2, QAMD_Q16 = 3 };
 enum { QAMD_EVEN = 0, QAMD_ODD = 1 };
 
 typedef struct qamd_lattice qamd_lattice;   /* 4-d periodic lattice       */
@@ -143,6 +146,8 @@ typedef struct {
   double iter_rsq;         /* the recurrence's own relative residual^2    */
   double seconds;
   int converged;
+  int restarts;            /* search-direction restarts at reliable updates */
+  int escalations;         /* 1 if stalled 16-bit sloppy links were replaced by FP32 */
 } qamd_solve_info;
 
 /* QLL CG semantics (cgSolveT lib/cg.c:43-122 with dslashp2): solves
@@ -157,6 +162,17 @@ int qamd_stag_cg(const qamd_links *l, qamd_field *x, const qamd_field *b, double
 int qamd_stag_cg_mixed(const qamd_links *l64, const qamd_links *ls, qamd_field *x,
                        const qamd_field *b, double mass, double rsq, int maxits,
                        double delta, qamd_solve_info *info);
+/* Batched mixed-precision CG (no reference counterpart; the batched
+ * counterpart of calling solve2, lib/stagDslash.c:594-612, once per source):
+ * the nrhs (1..8) sources of the even FP64 colour-vector block b are solved
+ * together.  Every inner iteration applies the sloppy normal operator to the
+ * whole block in one pair of block-Dslash launches (each link loaded once for
+ * all rhs), with per-rhs CG scalars; FP64 reliable updates on the block; each
+ * rhs stops on its FP64 true residual |b_r - A x_r|^2 <= rsq |b_r|^2 and
+ * leaves the iteration.  info: nrhs entries. */
+int qamd_stag_cg_mixed_block(const qamd_links *l64, const qamd_links *ls, qamd_field *x,
+                             const qamd_field *b, double mass, double rsq, int maxits,
+                             double delta, qamd_solve_info *info);
 /* QLL multishift CG semantics (cgmsSolveT lib/cgms.c:6-179): shifts[0] is
  * taken as 0, stop on the base residual, vectors at the links' precision. */
 int qamd_stag_cgms(const qamd_links *l, qamd_field *const x[], const qamd_field *b,

@@ -14,7 +14,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libqll_amd.so")
 
-F64, F32, F16 = 0, 1, 2
+F64, F32, F16, Q16 = 0, 1, 2, 3
 EVEN, ODD = 0, 1
 STAG, WILSON = 0, 1
 
@@ -30,7 +30,8 @@ class QllAmdError(RuntimeError):
 class SolveInfo(C.Structure):
     _fields_ = [("iterations", C.c_int), ("reliable_updates", C.c_int),
                 ("true_rsq", C.c_double), ("iter_rsq", C.c_double),
-                ("seconds", C.c_double), ("converged", C.c_int)]
+                ("seconds", C.c_double), ("converged", C.c_int),
+                ("restarts", C.c_int), ("escalations", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -93,6 +94,8 @@ def lib():
             "qamd_stag_normal": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.POINTER(C.c_double)]),
             "qamd_stag_cg": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_double, C.c_int, C.POINTER(SolveInfo)]),
             "qamd_stag_cg_mixed": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_double, C.c_int, C.c_double,
+                                             C.POINTER(SolveInfo)]),
+            "qamd_stag_cg_mixed_block": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_double, C.c_int, C.c_double,
                                              C.POINTER(SolveInfo)]),
             "qamd_stag_cgms": (C.c_int, [_vp, C.POINTER(_vp), _vp, C.c_double, _d, C.c_int, C.c_double, C.c_int,
                                          C.POINTER(SolveInfo)]),
@@ -292,6 +295,13 @@ def stag_cg_mixed(l64, ls, x, b, mass, rsq, maxits, delta=0.1):
     info = SolveInfo()
     _check(lib().qamd_stag_cg_mixed(l64.h, ls.h, x.h, b.h, mass, rsq, maxits, delta, C.byref(info)))
     return info.as_dict()
+
+
+def stag_cg_mixed_block(l64, ls, x, b, mass, rsq, maxits, delta=0.1):
+    """Solve every rhs of the even block b together; one info dict per rhs."""
+    infos = (SolveInfo * b.nrhs)()
+    _check(lib().qamd_stag_cg_mixed_block(l64.h, ls.h, x.h, b.h, mass, rsq, maxits, delta, infos))
+    return [i.as_dict() for i in infos]
 
 
 def stag_cgms(links, xs, b, mass, shifts, rsq, maxits):

@@ -73,7 +73,7 @@ __global__ void __launch_bounds__(256) k_stag_dslash(StagArgs<P, PO> A) {
       const int h = (k & 1) ? -hop : hop;
       int nb = nbr_cb(A.g, s, mu, h);
       C v[3];
-      VecLd<P>::ld(A.in, A.in_norm, vp, nb, v, (P == QAMD_F16) ? A.lscale[t] : (R)1);
+      VecLd<P>::ld(A.in, A.in_norm, vp, nb, v, (P == QAMD_F16 || P == QAMD_Q16) ? A.lscale[t] : (R)1);
       C m[9];
 #pragma unroll
       for (int e = 0; e < 9; e++) m[e] = to_c(ntload(lp + (t * 9 + e) * 64), (R)0);
@@ -139,7 +139,8 @@ static int launch_p(const qamd_links *l, Field &out, const Field &in, double a, 
   A.a = a;
   A.b = b;
   A.par = out.parity;
-  for (int t = 0; t < 16; t++) A.lscale[t] = l->scale[t];
+  // Q16 links are read as raw integers: fold 1/32767 into the term scale
+  for (int t = 0; t < 16; t++) A.lscale[t] = (P == QAMD_Q16) ? l->scale[t] / Q16_ONE : l->scale[t];
   const bool red = red_out != nullptr || !finish;
   const int nb = (int)((A.g.vh + 255) / 256);
   if (red) {
@@ -164,7 +165,7 @@ int stag_dslash_launch(const qamd_links *l, Field &out, const Field &in, double 
     set_error("stag_dslash: bad partial-sum set %d", pset);
     return -1;
   }
-  bool half_to_single = (l->prec == QAMD_F16 && out.prec == QAMD_F32);
+  bool half_to_single = ((l->prec == QAMD_F16 || l->prec == QAMD_Q16) && out.prec == QAMD_F32);
   if (in.prec != l->prec || (out.prec != l->prec && !half_to_single) || (w && w->prec != out.prec)) {
     set_error("stag_dslash: field precisions in %d / out %d do not fit links %d", in.prec, out.prec, l->prec);
     return -1;
@@ -184,10 +185,14 @@ int stag_dslash_launch(const qamd_links *l, Field &out, const Field &in, double 
   switch (l->prec) {
     case QAMD_F64: return launch_p<QAMD_F64>(l, out, in, a, w, b, red_out, st, skip, pset, finish);
     case QAMD_F32: return launch_p<QAMD_F32>(l, out, in, a, w, b, red_out, st, skip, pset, finish);
-    default:
+    case QAMD_F16:
       if (half_to_single)
         return launch_p<QAMD_F16, QAMD_F32>(l, out, in, a, w, b, red_out, st, skip, pset, finish);
       return launch_p<QAMD_F16>(l, out, in, a, w, b, red_out, st, skip, pset, finish);
+    default:
+      if (half_to_single)
+        return launch_p<QAMD_Q16, QAMD_F32>(l, out, in, a, w, b, red_out, st, skip, pset, finish);
+      return launch_p<QAMD_Q16>(l, out, in, a, w, b, red_out, st, skip, pset, finish);
   }
 }
 
@@ -208,6 +213,16 @@ int qamd_dslash(const qamd_links *l, qamd_field *out, const qamd_field *in, doub
     slot = scalar_slot();
     if (!slot) return -1;
   }
+  if (l->kind == 0 && in->f.nrhs > 1) {   // block of rhs: |out|^2 summed over the block
+    if (stag_block_launch(l, out->f, in->f, a, w ? &w->f : nullptr, b, false, RedScratch{}, stream())) return -1;
+    if (norm2_out) {
+      double t[16];
+      if (qamd_field_norm2(out, t)) return -1;
+      *norm2_out = 0;
+      for (int r = 0; r < out->f.nrhs; r++) *norm2_out += t[r];
+    }
+    return 0;
+  }
   int rc = (l->kind == 1)
                ? wilson_dslash_launch(l, out->f, in->f, a, w ? &w->f : nullptr, b, slot, stream(), nullptr)
                : stag_dslash_launch(l, out->f, in->f, a, w ? &w->f : nullptr, b, slot, stream(), nullptr);
@@ -223,6 +238,17 @@ int qamd_dslash(const qamd_links *l, qamd_field *out, const qamd_field *in, doub
 
 int qamd_stag_normal(const qamd_links *l, qamd_field *Ap, const qamd_field *p, qamd_field *tmp,
                      double mass, double *pAp) {
+  if (p->f.nrhs > 1) {   // a block: pAp summed over the rhs
+    if (stag_block_launch(l, tmp->f, p->f, 0.0, nullptr, 1.0, false, RedScratch{}, stream())) return -1;
+    if (stag_block_launch(l, Ap->f, tmp->f, mass * mass, &p->f, -1.0, false, RedScratch{}, stream())) return -1;
+    if (pAp) {
+      double d[16];
+      if (qamd_field_redot(p, Ap, d)) return -1;
+      *pAp = 0;
+      for (int r = 0; r < p->f.nrhs; r++) *pAp += d[r];
+    }
+    return 0;
+  }
   double *slot = scalar_slot();
   if (!slot) return -1;
   // tmp = D_oe p  (|tmp|^2 -> slot[0])
@@ -245,8 +271,9 @@ int qamd_time_dslash(const qamd_links *l, qamd_field *out, const qamd_field *in,
   QAMD_HIP(hipEventCreate(&e1));
   QAMD_HIP(hipEventRecord(e0, stream()));
   for (int i = 0; i < nrep; i++) {
-    int rc = (l->kind == 1) ? wilson_dslash_launch(l, out->f, in->f, 0.0, nullptr, 1.0, nullptr, stream())
-                            : stag_dslash_launch(l, out->f, in->f, 0.0, nullptr, 1.0, nullptr, stream(), nullptr);
+    int rc = (l->kind == 1)   ? wilson_dslash_launch(l, out->f, in->f, 0.0, nullptr, 1.0, nullptr, stream())
+             : in->f.nrhs > 1 ? stag_block_launch(l, out->f, in->f, 0.0, nullptr, 1.0, false, RedScratch{}, stream())
+                              : stag_dslash_launch(l, out->f, in->f, 0.0, nullptr, 1.0, nullptr, stream(), nullptr);
     if (rc) return -1;
   }
   QAMD_HIP(hipEventRecord(e1, stream()));

@@ -58,6 +58,14 @@ template <> struct SpIO<QAMD_F16> {
   }
   __device__ static void st(Field &f, int64_t i, float2 v) { ((__half2 *)f.data)[i] = __floats2half2_rn(v.x, v.y); }
 };
+template <> struct SpIO<QAMD_Q16> {
+  __device__ static float2 ld(const Field &f, int64_t i, float n) {
+    short2 s = ((const short2 *)f.data)[i];
+    n *= 1.0f / Q16_ONE;
+    return make_float2(s.x * n, s.y * n);
+  }
+  __device__ static void st(Field &f, int64_t i, float2 v) { ((short2 *)f.data)[i] = q16_pack(v.x, v.y); }
+};
 
 struct FinStore1 {
   double *dst;
@@ -87,7 +95,7 @@ __global__ void __launch_bounds__(256) k_wilson(WArgs<P> A) {
       const int mu = t >> 1;
       const bool bwd = t & 1;
       int nb = nbr_cb(A.g, s, mu, bwd ? -1 : 1);
-      float nrm = (P == QAMD_F16) ? A.in.norm[A.in.nidx(r, nb)] * A.lscale[t] : 1.0f;
+      float nrm = (P == QAMD_F16 || P == QAMD_Q16) ? A.in.norm[A.in.nidx(r, nb)] * A.lscale[t] : 1.0f;
       C psi[4][3];
 #pragma unroll
       for (int a = 0; a < 4; a++)
@@ -140,7 +148,7 @@ __global__ void __launch_bounds__(256) k_wilson(WArgs<P> A) {
     }
     const R b = (R)A.b, a = (R)A.a;
     C o[4][3];
-    float wn = (P == QAMD_F16 && A.has_w) ? A.w.norm[A.w.nidx(r, cb)] : 1.0f;
+    float wn = ((P == QAMD_F16 || P == QAMD_Q16) && A.has_w) ? A.w.norm[A.w.nidx(r, cb)] : 1.0f;
 #pragma unroll
     for (int sp = 0; sp < 4; sp++)
 #pragma unroll
@@ -156,7 +164,8 @@ __global__ void __launch_bounds__(256) k_wilson(WArgs<P> A) {
         o[sp][c] = v;
       }
     float on = 1.0f;
-    if (P == QAMD_F16) {
+    constexpr bool NORMED = (P == QAMD_F16 || P == QAMD_Q16);
+    if (NORMED) {
       float mx = 0.f;
 #pragma unroll
       for (int sp = 0; sp < 4; sp++)
@@ -171,7 +180,7 @@ __global__ void __launch_bounds__(256) k_wilson(WArgs<P> A) {
       for (int c = 0; c < 3; c++) {
         C v = o[sp][c];
         if (RED) red[0] += (double)v.x * v.x + (double)v.y * v.y;
-        if (P == QAMD_F16) {
+        if (NORMED) {
           v.x *= on;
           v.y *= on;
         }
@@ -199,7 +208,7 @@ static int wlaunch(const qamd_links *l, Field &out, const Field &in, double a, c
   A.b = b;
   A.par = out.parity;
   A.skip = skip;
-  for (int t = 0; t < 8; t++) A.lscale[t] = l->scale[t];
+  for (int t = 0; t < 8; t++) A.lscale[t] = (P == QAMD_Q16) ? l->scale[t] / Q16_ONE : l->scale[t];
   int64_t nthreads = A.g.vh * in.nrhs;
   int nb = (int)((nthreads + 255) / 256);
   if (red_out) {
@@ -236,7 +245,8 @@ int wilson_dslash_launch(const qamd_links *l, Field &out, const Field &in, doubl
   switch (l->prec) {
     case QAMD_F64: return wlaunch<QAMD_F64>(l, out, in, a, w, b, red_out, st, skip);
     case QAMD_F32: return wlaunch<QAMD_F32>(l, out, in, a, w, b, red_out, st, skip);
-    default: return wlaunch<QAMD_F16>(l, out, in, a, w, b, red_out, st, skip);
+    case QAMD_F16: return wlaunch<QAMD_F16>(l, out, in, a, w, b, red_out, st, skip);
+    default: return wlaunch<QAMD_Q16>(l, out, in, a, w, b, red_out, st, skip);
   }
 }
 

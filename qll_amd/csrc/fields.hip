@@ -12,7 +12,7 @@
 namespace qamd {
 
 size_t prec_bytes(int prec) {
-  return prec == QAMD_F64 ? 16 : prec == QAMD_F32 ? 8 : 4;
+  return prec == QAMD_F64 ? 16 : prec == QAMD_F32 ? 8 : 4;   // FP16 and Q16: 4 bytes
 }
 
 template <class H>
@@ -150,7 +150,7 @@ extern "C" {
 
 qamd_field *qamd_field_create(qamd_lattice *lat, int prec, int ncomp, int nrhs, int parity) {
   if (!lat) { set_error("field_create: NULL lattice"); return nullptr; }
-  if (prec < QAMD_F64 || prec > QAMD_F16) { set_error("field_create: bad precision %d", prec); return nullptr; }
+  if (prec < QAMD_F64 || prec > QAMD_Q16) { set_error("field_create: bad precision %d", prec); return nullptr; }
   if (ncomp != 3 && ncomp != 12) { set_error("field_create: ncomp must be 3 or 12"); return nullptr; }
   if (nrhs < 1 || nrhs > 64) { set_error("field_create: nrhs must be 1..64"); return nullptr; }
   if (parity != 0 && parity != 1) { set_error("field_create: parity must be 0 or 1"); return nullptr; }
@@ -167,8 +167,9 @@ qamd_field *qamd_field_create(qamd_lattice *lat, int prec, int ncomp, int nrhs, 
   f->f.rhs_inner = (ncomp == 12 && nrhs > 1);
   int64_t n = (int64_t)nrhs * ncomp * lat->g.vh_pad;
   f->f.data = dev_alloc(n * prec_bytes(prec));
-  if (prec == QAMD_F16) f->f.norm = (float *)dev_alloc(sizeof(float) * nrhs * lat->g.vh_pad);
-  if (!f->f.data || (prec == QAMD_F16 && !f->f.norm)) {
+  const bool normed = (prec == QAMD_F16 || prec == QAMD_Q16);
+  if (normed) f->f.norm = (float *)dev_alloc(sizeof(float) * nrhs * lat->g.vh_pad);
+  if (!f->f.data || (normed && !f->f.norm)) {
     dev_free(f->f.data);
     dev_free(f->f.norm);
     free(f);

@@ -81,6 +81,9 @@ template <> __device__ inline double2 ld2<__half2>(const __half2 *p, int64_t i) 
   float2 f = __half22float2(p[i]);
   return make_double2(f.x, f.y);
 }
+template <> __device__ inline double2 ld2<short2>(const short2 *p, int64_t i) {   // stored / scale
+  return make_double2(p[i].x / (double)Q16_ONE, p[i].y / (double)Q16_ONE);
+}
 
 // per-term max |component| as float bits (non-negative floats order as
 // uints).  Each wavefront reduces one 64-site block (all terms) in registers
@@ -116,7 +119,8 @@ __global__ void k_link_convert(SO *dst, const SI *src, int64_t n, int nterm, con
     double f = (double)iscale[t] / (double)oscale[t];
     v.x *= f;
     v.y *= f;
-    if constexpr (sizeof(SO) == 4) dst[i] = __floats2half2_rn((float)v.x, (float)v.y);
+    if constexpr (std::is_same<SO, __half2>::value) dst[i] = __floats2half2_rn((float)v.x, (float)v.y);
+    else if constexpr (std::is_same<SO, short2>::value) dst[i] = q16_pack((float)v.x, (float)v.y);
     else dst[i] = mk<SO>(v);
   }
 }
@@ -132,7 +136,7 @@ qamd_links *qamd_links_create(qamd_lattice *lat, int kind, int prec, int nterms)
   if (kind == 0 && nterms != 8 && nterms != 16) { set_error("staggered links need 8 or 16 terms"); return nullptr; }
   if (kind == 1 && nterms != 8) { set_error("Wilson links need 8 terms"); return nullptr; }
   if (kind != 0 && kind != 1) { set_error("links_create: bad kind"); return nullptr; }
-  if (prec < QAMD_F64 || prec > QAMD_F16) { set_error("links_create: bad precision"); return nullptr; }
+  if (prec < QAMD_F64 || prec > QAMD_Q16) { set_error("links_create: bad precision"); return nullptr; }
   qamd_links *l = (qamd_links *)calloc(1, sizeof(qamd_links));
   l->lat = lat;
   l->kind = kind;
@@ -163,10 +167,10 @@ void qamd_links_destroy(qamd_links *l) {
   free(l);
 }
 
-// import with an FP64/FP32 destination; FP16 goes through an FP64 copy
+// import with an FP64/FP32 destination; 16-bit formats go through FP64
 static int import_terms(qamd_links *l, const void *const terms[], int hprec, SrcMap m, size_t term_bytes) {
   qamd_links *dst = l;
-  if (l->prec == QAMD_F16) {
+  if (l->prec == QAMD_F16 || l->prec == QAMD_Q16) {
     dst = qamd_links_create(l->lat, l->kind, QAMD_F64, l->nterms);
     if (!dst) return -1;
   }
@@ -251,7 +255,7 @@ int qamd_links_from_gauge(qamd_links *l, const qamd_gauge *fat, const qamd_gauge
     return -1;
   }
   qamd_links *dst = l;
-  if (l->prec == QAMD_F16) {
+  if (l->prec == QAMD_F16 || l->prec == QAMD_Q16) {
     dst = qamd_links_create(l->lat, l->kind, QAMD_F64, l->nterms);
     if (!dst) return -1;
   }
@@ -287,15 +291,17 @@ int qamd_links_convert(qamd_links *dst, const qamd_links *src) {
   if (!sc || !mx) return -1;
   float oscale[16];
   for (int t = 0; t < 16; t++) oscale[t] = 1.0f;
-  if (dst->prec == QAMD_F16) {
+  if (dst->prec == QAMD_F16 || dst->prec == QAMD_Q16) {
     QAMD_HIP(hipMemsetAsync(mx, 0, 16 * sizeof(unsigned), stream()));
     for (int p = 0; p < 2; p++) {
       if (src->prec == QAMD_F64)
         hipLaunchKernelGGL(k_link_absmax<double2>, dim3(2048), dim3(256), 0, stream(), (const double2 *)src->data[p], nblk, src->nterms, mx);
       else if (src->prec == QAMD_F32)
         hipLaunchKernelGGL(k_link_absmax<float2>, dim3(2048), dim3(256), 0, stream(), (const float2 *)src->data[p], nblk, src->nterms, mx);
-      else
+      else if (src->prec == QAMD_F16)
         hipLaunchKernelGGL(k_link_absmax<__half2>, dim3(2048), dim3(256), 0, stream(), (const __half2 *)src->data[p], nblk, src->nterms, mx);
+      else
+        hipLaunchKernelGGL(k_link_absmax<short2>, dim3(2048), dim3(256), 0, stream(), (const short2 *)src->data[p], nblk, src->nterms, mx);
     }
     unsigned hm[16];
     QAMD_HIP(hipMemcpyAsync(hm, mx, sizeof hm, hipMemcpyDeviceToHost, stream()));
@@ -322,7 +328,16 @@ int qamd_links_convert(qamd_links *dst, const qamd_links *src) {
     else if (a == QAMD_F32 && b == QAMD_F16) CONV(float2, __half2);
     else if (a == QAMD_F16 && b == QAMD_F64) CONV(__half2, double2);
     else if (a == QAMD_F16 && b == QAMD_F32) CONV(__half2, float2);
-    else CONV(__half2, __half2);
+    else if (a == QAMD_F16 && b == QAMD_F16) CONV(__half2, __half2);
+    else if (a == QAMD_F64 && b == QAMD_Q16) CONV(double2, short2);
+    else if (a == QAMD_F32 && b == QAMD_Q16) CONV(float2, short2);
+    else if (a == QAMD_Q16 && b == QAMD_F64) CONV(short2, double2);
+    else if (a == QAMD_Q16 && b == QAMD_F32) CONV(short2, float2);
+    else if (a == QAMD_Q16 && b == QAMD_Q16) CONV(short2, short2);
+    else {
+      set_error("links_convert: unsupported precision pair %d -> %d", a, b);
+      return -1;
+    }
 #undef CONV
   }
   QAMD_HIP(hipGetLastError());

@@ -139,10 +139,23 @@ template <> struct Prec<QAMD_F16> {
   typedef float R;
   typedef float2 C;
 };
+// Q16: the stored integers are returned unscaled by to_c; the 1/32767 and
+// the norm / link scale are applied by the loaders
+template <> struct Prec<QAMD_Q16> {
+  typedef short2 S;
+  typedef float R;
+  typedef float2 C;
+};
+constexpr float Q16_ONE = 32767.0f;
+__device__ __forceinline__ bool is_16bit(int p) { return p == QAMD_F16 || p == QAMD_Q16; }
 
 __device__ __forceinline__ double2 to_c(double2 v, double) { return v; }
 __device__ __forceinline__ float2 to_c(float2 v, float) { return v; }
 __device__ __forceinline__ float2 to_c(__half2 v, float) { return __half22float2(v); }
+__device__ __forceinline__ float2 to_c(short2 v, float) { return make_float2((float)v.x, (float)v.y); }
+__device__ __forceinline__ short2 q16_pack(float a, float b) {   // a, b in [-1, 1]
+  return make_short2((short)__float2int_rn(a * Q16_ONE), (short)__float2int_rn(b * Q16_ONE));
+}
 
 // non-temporal (streaming) loads of data read exactly once
 typedef double dvec2_t __attribute__((ext_vector_type(2)));
@@ -154,6 +167,10 @@ __device__ __forceinline__ double2 ntload(const double2 *p) {
 __device__ __forceinline__ float2 ntload(const float2 *p) {
   fvec2_t v = __builtin_nontemporal_load(reinterpret_cast<const fvec2_t *>(p));
   return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ short2 ntload(const short2 *p) {
+  unsigned u = __builtin_nontemporal_load(reinterpret_cast<const unsigned *>(p));
+  return make_short2((short)(u & 0xffff), (short)(u >> 16));
 }
 __device__ __forceinline__ __half2 ntload(const __half2 *p) {
   unsigned u = __builtin_nontemporal_load(reinterpret_cast<const unsigned *>(p));
@@ -187,7 +204,7 @@ struct Field {             // one parity of a colour-vector / spinor field
 // one-block finisher kernel adds them in block order and runs the scalar
 // epilogue.  The kernel boundary makes the partial sums visible, which on a
 // multi-die GPU is far cheaper than a device-scope fence per block.
-constexpr int MAXRED = 16;   // values reduced by one kernel
+constexpr int MAXRED = 32;   // values reduced by one kernel
 constexpr int NSETS = 4;     // independent partial-sum buffers
 struct RedScratch {
   double *part;              // [MAXRED][max_blocks] of one set

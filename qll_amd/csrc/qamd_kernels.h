@@ -16,6 +16,11 @@ double *scalar_slot();
 int stag_dslash_launch(const qamd_links *l, Field &out, const Field &in, double a, const Field *w,
                        double b, double *red_out, hipStream_t st, const int *skip = nullptr, int pset = 0,
                        bool finish = true);
+// stag_block.hip: staggered Dslash on a block of up to 16 rhs (8 in FP64),
+// links loaded once per site; with red, {|out_r|^2, |w_r|^2} for each rhs r
+// go to partial-sum set rs (2*nrhs values)
+int stag_block_launch(const qamd_links *l, Field &out, const Field &in, double a, const Field *w, double b,
+                      bool red, RedScratch rs, hipStream_t st);
 int wilson_dslash_launch(const qamd_links *l, Field &out, const Field &in, double a, const Field *w,
                          double b, double *red_out, hipStream_t st, const int *skip = nullptr);
 

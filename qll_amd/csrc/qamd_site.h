@@ -29,11 +29,17 @@ __device__ inline void site_load(const Field &f, int rhs, int cb, double2 *v) {
     for (int j = 0; j < f.ncomp; j++) v[j] = IO<QAMD_F64>::ld(f, f.cidx(rhs, j, cb));
   } else if (f.prec == QAMD_F32) {
     for (int j = 0; j < f.ncomp; j++) v[j] = IO<QAMD_F32>::ld(f, f.cidx(rhs, j, cb));
-  } else {
+  } else if (f.prec == QAMD_F16) {
     float nrm = f.norm[f.nidx(rhs, cb)];
     for (int j = 0; j < f.ncomp; j++) {
       float2 h = __half22float2(((const __half2 *)f.data)[f.cidx(rhs, j, cb)]);
       v[j] = make_double2((double)h.x * nrm, (double)h.y * nrm);
+    }
+  } else {
+    double nrm = (double)f.norm[f.nidx(rhs, cb)] / Q16_ONE;
+    for (int j = 0; j < f.ncomp; j++) {
+      short2 h = ((const short2 *)f.data)[f.cidx(rhs, j, cb)];
+      v[j] = make_double2(h.x * nrm, h.y * nrm);
     }
   }
 }
@@ -48,8 +54,12 @@ __device__ inline void site_store(Field &f, int rhs, int cb, const double2 *v) {
     for (int j = 0; j < f.ncomp; j++) m = fmax(m, fmax(fabs(v[j].x), fabs(v[j].y)));
     float nrm = (float)m;
     float inv = nrm > 0 ? 1.0f / nrm : 0.0f;
-    for (int j = 0; j < f.ncomp; j++)
-      ((__half2 *)f.data)[f.cidx(rhs, j, cb)] = __floats2half2_rn((float)v[j].x * inv, (float)v[j].y * inv);
+    for (int j = 0; j < f.ncomp; j++) {
+      if (f.prec == QAMD_F16)
+        ((__half2 *)f.data)[f.cidx(rhs, j, cb)] = __floats2half2_rn((float)v[j].x * inv, (float)v[j].y * inv);
+      else
+        ((short2 *)f.data)[f.cidx(rhs, j, cb)] = q16_pack((float)v[j].x * inv, (float)v[j].y * inv);
+    }
     f.norm[f.nidx(rhs, cb)] = nrm;
   }
 }

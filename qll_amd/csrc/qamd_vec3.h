@@ -39,6 +39,15 @@ template <> struct VecLd<QAMD_F16> {
     v[2] = make_float2(c.x * n, c.y * n);
   }
 };
+template <> struct VecLd<QAMD_Q16> {
+  __device__ static void ld(const short2 *p, const float *nrm, int64_t vp, int i, float2 (&v)[3], float sc) {
+    float n = nrm[i] * sc * (1.0f / Q16_ONE);
+    short2 a = p[i], b = p[i + vp], c = p[i + 2 * vp];
+    v[0] = make_float2(a.x * n, a.y * n);
+    v[1] = make_float2(b.x * n, b.y * n);
+    v[2] = make_float2(c.x * n, c.y * n);
+  }
+};
 
 template <int P> struct VecSt;
 template <> struct VecSt<QAMD_F64> {
@@ -64,6 +73,18 @@ template <> struct VecSt<QAMD_F16> {
     p[i] = __floats2half2_rn(v[0].x * inv, v[0].y * inv);
     p[i + vp] = __floats2half2_rn(v[1].x * inv, v[1].y * inv);
     p[i + 2 * vp] = __floats2half2_rn(v[2].x * inv, v[2].y * inv);
+    nrm[i] = m;
+  }
+};
+template <> struct VecSt<QAMD_Q16> {
+  __device__ static void st(short2 *p, float *nrm, int64_t vp, int i, const float2 (&v)[3]) {
+    float m = 0.f;
+#pragma unroll
+    for (int j = 0; j < 3; j++) m = fmaxf(m, fmaxf(fabsf(v[j].x), fabsf(v[j].y)));
+    float inv = m > 0.f ? 1.0f / m : 0.f;
+    p[i] = q16_pack(v[0].x * inv, v[0].y * inv);
+    p[i + vp] = q16_pack(v[1].x * inv, v[1].y * inv);
+    p[i + 2 * vp] = q16_pack(v[2].x * inv, v[2].y * inv);
     nrm[i] = m;
   }
 };

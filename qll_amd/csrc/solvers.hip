@@ -81,11 +81,11 @@ __global__ void __launch_bounds__(256) k_cg_p(Field p, Field r, Field shadow, So
     }
   }
   VT<P>::st(p, i, pv);
-  if (SH) {
+  if constexpr (SH != 0) {   // 16-bit copy of p (precision SH) for the Dslash
     float2 h[3];
 #pragma unroll
     for (int j = 0; j < 3; j++) h[j] = make_float2((float)pv[j].x, (float)pv[j].y);
-    VT<QAMD_F16>::st(shadow, i, h);
+    VT<SH>::st(shadow, i, h);
   }
 }
 
@@ -403,17 +403,19 @@ static void launch_cg_p(const qamd_lattice *lat, qamd_field *p, qamd_field *r, q
   hipLaunchKernelGGL((k_cg_p<P, SH>), grid1(lat), dim3(256), 0, stream(), p->f, r->f, sh ? sh->f : p->f, S);
 }
 static int cg_p(const qamd_lattice *lat, qamd_field *p, qamd_field *r, qamd_field *sh, SolverDev *S) {
-  if (sh && p->f.prec != QAMD_F32) {
-    set_error("cg_p: the FP16 shadow needs an FP32 p");
+  if (sh && (p->f.prec != QAMD_F32 || (sh->f.prec != QAMD_F16 && sh->f.prec != QAMD_Q16))) {
+    set_error("cg_p: a 16-bit shadow needs an FP32 p");
     return -1;
   }
   switch (p->f.prec) {
     case QAMD_F64: launch_cg_p<QAMD_F64, 0>(lat, p, r, nullptr, S); break;
     case QAMD_F32:
-      if (sh) launch_cg_p<QAMD_F32, 1>(lat, p, r, sh, S);
-      else launch_cg_p<QAMD_F32, 0>(lat, p, r, nullptr, S);
+      if (!sh) launch_cg_p<QAMD_F32, 0>(lat, p, r, nullptr, S);
+      else if (sh->f.prec == QAMD_F16) launch_cg_p<QAMD_F32, QAMD_F16>(lat, p, r, sh, S);
+      else launch_cg_p<QAMD_F32, QAMD_Q16>(lat, p, r, sh, S);
       break;
-    default: launch_cg_p<QAMD_F16, 0>(lat, p, r, nullptr, S); break;
+    case QAMD_F16: launch_cg_p<QAMD_F16, 0>(lat, p, r, nullptr, S); break;
+    default: launch_cg_p<QAMD_Q16, 0>(lat, p, r, nullptr, S); break;
   }
   QAMD_HIP(hipGetLastError());
   return 0;
@@ -513,7 +515,10 @@ static int cg_mixed_core(const qamd_links *l64, const qamd_links *ls, qamd_field
   double t0 = now_s();
   qamd_lattice *lat = l64->lat;
   const int ps = ls->prec;
-  const bool half = (ps == QAMD_F16);
+  bool half = (ps == QAMD_F16 || ps == QAMD_Q16);   // 16-bit Dslash input copy of p
+  const qamd_links *lsc = ls;        // current sloppy links
+  qamd_links *l32own = nullptr;      // FP32 links made after an FP16 stall
+  int nesc = 0, nrestart = 0;
   State st;
   memset(&st, 0, sizeof st);
   qamd_field *r64 = qamd_field_create(lat, QAMD_F64, 3, 1, 0);
@@ -524,7 +529,7 @@ static int cg_mixed_core(const qamd_links *l64, const qamd_links *ls, qamd_field
   qamd_field *Ap = qamd_field_create(lat, QAMD_F32, 3, 1, 0);
   qamd_field *xs = qamd_field_create(lat, QAMD_F32, 3, 1, 0);
   qamd_field *t = qamd_field_create(lat, ps, 3, 1, 1);
-  qamd_field *p16 = half ? qamd_field_create(lat, QAMD_F16, 3, 1, 0) : nullptr;
+  qamd_field *p16 = half ? qamd_field_create(lat, ps, 3, 1, 0) : nullptr;
   qamd_field *pin = half ? p16 : p;
   int rc = -1, nupd = 0, its = 0;
   const bool verbose = getenv("QAMD_VERBOSE") != nullptr;
@@ -563,9 +568,9 @@ static int cg_mixed_core(const qamd_links *l64, const qamd_links *ls, qamd_field
     int err = 0;
     while (its < maxits) {
       for (int k = 0; k < BATCH && !err; k++) {
-        err |= apply_normal(ls, st.S, Ap, pin, p, t, m2, nullptr);
+        err |= apply_normal(lsc, st.S, Ap, pin, p, t, m2, nullptr);
         err |= cg_xr(lat, xs, p, r, Ap, st.S);
-        err |= cg_p(lat, p, r, p16, st.S);
+        err |= cg_p(lat, p, r, half ? p16 : nullptr, st.S);
       }
       if (err || pull_state(st)) {
         err = 1;
@@ -576,17 +581,60 @@ static int cg_mixed_core(const qamd_links *l64, const qamd_links *ls, qamd_field
       if (verbose)
         fprintf(stderr, "cg_mixed prec %d its %d rel|r| %.3e max %.3e pAp %.3e alpha %.3e beta %.3e\n", ps, its,
                 rn / sqrt(bsq), Sh->maxrr / sqrt(bsq), Sh->m2 * Sh->nrm[3] + Sh->nrm[0], Sh->alpha, Sh->beta);
-      if (rn < delta * Sh->maxrr || Sh->rsq < stop || its >= maxits) {
+      // update on a delta drop, on convergence, at the end, and when the
+      // iterated residual grew well above the last true one (divergence)
+      if (!isfinite(Sh->rsq) || rn < delta * Sh->maxrr || Sh->rsq < stop || its >= maxits ||
+          Sh->rsq > 4.0 * r2) {
         // reliable update: x += xs (xs = 0), r = b - A x in FP64
+        const double r2s = Sh->rsq;
         hipLaunchKernelGGL(k_ru_accum<QAMD_F32>, grid1(lat), dim3(256), 0, stream(), x->f, xs->f);
         if (hipGetLastError() != hipSuccess || reliable()) {
           err = 1;
           break;
         }
         nupd++;
+        const double r2prev = r2;
         r2 = Sh->rsq;
         if (verbose) fprintf(stderr, "cg_mixed reliable update %d: true rel|r| %.3e\n", nupd, sqrt(r2 / bsq));
+        if (!isfinite(r2)) {
+          set_error("cg_mixed: the solution is not finite after %d iterations", its);
+          err = 1;
+          break;
+        }
         if (r2 < stop) break;
+        // The sloppy operator's error shows up as drift between the iterated
+        // and the true residual.  A large drift leaves p inconsistent with the
+        // new r: restart the search direction.  When a whole cycle did not
+        // reduce the true residual, the sloppy links are too coarse for this
+        // operator: continue with FP32 links.
+        const bool stalled = r2 > 0.25 * r2prev;
+        if (stalled && lsc->prec != QAMD_F32) {
+          if (!l32own) {
+            l32own = qamd_links_create(lat, 0, QAMD_F32, l64->nterms);
+            if (!l32own || qamd_links_convert(l32own, l64)) {
+              err = 1;
+              break;
+            }
+          }
+          lsc = l32own;
+          qamd_field_destroy(t);
+          t = qamd_field_create(lat, QAMD_F32, 3, 1, 1);
+          if (!t) {
+            err = 1;
+            break;
+          }
+          pin = p;
+          half = false;
+          nesc++;
+          if (verbose) fprintf(stderr, "cg_mixed: FP16 links stalled; continuing with FP32 links\n");
+        }
+        if (stalled || r2 > 2.0 * r2s) {
+          if (qamd_field_copy(p, r) || (half && qamd_field_copy(p16, p))) {
+            err = 1;
+            break;
+          }
+          nrestart++;
+        }
       }
     }
     if (err) break;
@@ -600,7 +648,10 @@ static int cg_mixed_core(const qamd_links *l64, const qamd_links *ls, qamd_field
     info->iter_rsq = info->true_rsq;
     info->converged = bsq == 0 || r2 < rsq * bsq;
     info->seconds = now_s() - t0;
+    info->restarts = nrestart;
+    info->escalations = nesc;
   }
+  qamd_links_destroy(l32own);
   qamd_field_destroy(r64);
   qamd_field_destroy(Ax);
   qamd_field_destroy(t64);
@@ -630,6 +681,7 @@ int qamd_stag_cg_mixed(const qamd_links *l64, const qamd_links *ls, qamd_field *
 int qamd_stag_cgms(const qamd_links *l, qamd_field *const x[], const qamd_field *b, double mass,
                    const double shifts[], int n, double rsq, int maxits, qamd_solve_info *info) {
   if (!l || l->kind != 0) { set_error("stag_cgms: staggered links required"); return -1; }
+  if (l->prec == QAMD_Q16) { set_error("stag_cgms: Q16 links are for the mixed-precision solvers only"); return -1; }
   if (n < 1 || n > MAXSHIFT) { set_error("stag_cgms: 1..%d shifts", MAXSHIFT); return -1; }
   if (check_vec(b, l->prec, 0, "stag_cgms b")) return -1;
   for (int i = 0; i < n; i++)

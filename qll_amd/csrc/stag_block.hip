@@ -1,0 +1,569 @@
+// Blocks of right-hand sides for the staggered operator: a Dslash that loads
+// every link once for up to 16 right-hand sides (thread per site, the rhs
+// loop innermost, so the link stream is divided by the block size), and a
+// mixed-precision block CG with per-rhs scalars and reliable updates.
+#include "qamd_internal.h"
+#include "qamd_objects.h"
+#include "qamd_kernels.h"
+#include "qamd_vec3.h"
+
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+#include <chrono>
+
+namespace qamd {
+
+constexpr int MAXB = 16;
+
+template <int P, int PO> struct MArgs {
+  typedef typename Prec<P>::S S;
+  typedef typename Prec<PO>::S SO;
+  Geom g;
+  const S *links;
+  const S *in;
+  const float *in_norm;
+  SO *out;
+  float *out_norm;
+  const SO *w;
+  const float *w_norm;
+  double a, b;
+  int par, nr;
+  float lscale[16];
+  RedScratch rs;
+};
+
+// out_r = a*w_r + b * sum_t L_t in_r(x+d_t) for r < nr; fields rhs-outer
+template <int P, int PO, int NT, int NR, bool RED>
+__global__ void __launch_bounds__(256) k_stag_block(MArgs<P, PO> A) {
+  typedef typename Prec<P>::R R;
+  typedef typename Prec<P>::C C;
+  typedef typename Prec<P>::S S;
+  const int64_t vp = A.g.vh_pad;
+  const int nr = A.nr;
+  int blk = xcd_block(blockIdx.x, gridDim.x);
+  int cb = blk * 256 + threadIdx.x;
+  double red[2 * NR];
+#pragma unroll
+  for (int i = 0; i < 2 * NR; i++) red[i] = 0.0;
+  if (cb < A.g.vh) {
+    Site s = site_of(A.g, cb, A.par);
+    const S *lp = A.links + ((int64_t)(cb >> 6) * NT * 9 << 6) + (cb & 63);
+    C acc[NR][3];
+#pragma unroll
+    for (int r = 0; r < NR; r++)
+#pragma unroll
+      for (int j = 0; j < 3; j++) acc[r][j].x = acc[r][j].y = (R)0;
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+      const int k = (NT == 16) ? t / 2 : t;
+      const int hop = (NT == 16 && (t & 1)) ? 3 : 1;
+      const int mu = k / 2;
+      const int h = (k & 1) ? -hop : hop;
+      int nb = nbr_cb(A.g, s, mu, h);
+      C m[9];
+#pragma unroll
+      for (int e = 0; e < 9; e++) m[e] = to_c(ntload(lp + (t * 9 + e) * 64), (R)0);
+#pragma unroll
+      for (int r = 0; r < NR; r++) {
+        if (r < nr) {
+          C v[3];
+          VecLd<P>::ld(A.in + (int64_t)r * 3 * vp, A.in_norm ? A.in_norm + (int64_t)r * vp : nullptr, vp, nb, v,
+                       (P == QAMD_F16 || P == QAMD_Q16) ? A.lscale[t] : (R)1);
+#pragma unroll
+          for (int rr = 0; rr < 3; rr++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) cmac<C, R>(acc[r][rr], m[3 * c + rr], v[c]);
+        }
+      }
+    }
+    const R b = (R)A.b, a = (R)A.a;
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+      if (r < nr) {
+        C o[3];
+        if (A.w) {
+          C w[3];
+          VecLd<PO>::ld(A.w + (int64_t)r * 3 * vp, A.w_norm ? A.w_norm + (int64_t)r * vp : nullptr, vp, cb, w, (R)1);
+#pragma unroll
+          for (int j = 0; j < 3; j++) {
+            o[j].x = fma(a, w[j].x, b * acc[r][j].x);
+            o[j].y = fma(a, w[j].y, b * acc[r][j].y);
+            if (RED) red[2 * r + 1] += (double)w[j].x * w[j].x + (double)w[j].y * w[j].y;
+          }
+        } else {
+#pragma unroll
+          for (int j = 0; j < 3; j++) {
+            o[j].x = b * acc[r][j].x;
+            o[j].y = b * acc[r][j].y;
+          }
+        }
+        if (RED)
+#pragma unroll
+          for (int j = 0; j < 3; j++) red[2 * r] += (double)o[j].x * o[j].x + (double)o[j].y * o[j].y;
+        VecSt<PO>::st(A.out + (int64_t)r * 3 * vp, A.out_norm ? A.out_norm + (int64_t)r * vp : nullptr, vp, cb, o);
+      }
+    }
+  }
+  if (RED) grid_partial<2 * NR>(red, A.rs);
+}
+
+template <int P, int PO, int NT, int NR>
+static void blaunch_nr(const MArgs<P, PO> &A, bool red, hipStream_t st) {
+  int nb = (int)((A.g.vh + 255) / 256);
+  if (red) hipLaunchKernelGGL((k_stag_block<P, PO, NT, NR, true>), dim3(nb), dim3(256), 0, st, A);
+  else hipLaunchKernelGGL((k_stag_block<P, PO, NT, NR, false>), dim3(nb), dim3(256), 0, st, A);
+}
+
+template <int P, int PO, int NT>
+static int blaunch_nt(const MArgs<P, PO> &A, bool red, hipStream_t st) {
+  if (A.nr <= 1) blaunch_nr<P, PO, NT, 1>(A, red, st);
+  else if (A.nr <= 2) blaunch_nr<P, PO, NT, 2>(A, red, st);
+  else if (A.nr <= 4) blaunch_nr<P, PO, NT, 4>(A, red, st);
+  else if (A.nr <= 8) blaunch_nr<P, PO, NT, 8>(A, red, st);
+  else if constexpr (P != QAMD_F64) blaunch_nr<P, PO, NT, 16>(A, red, st);
+  else {
+    set_error("block Dslash: FP64 blocks hold at most 8 rhs");
+    return -1;
+  }
+  return 0;
+}
+
+template <int P, int PO = P>
+static int blaunch(const qamd_links *l, Field &out, const Field &in, double a, const Field *w, double b, bool red,
+                   RedScratch rs, hipStream_t st) {
+  typedef typename Prec<P>::S S;
+  typedef typename Prec<PO>::S SO;
+  MArgs<P, PO> A;
+  memset(&A, 0, sizeof A);
+  A.g = l->lat->g;
+  A.links = (const S *)l->data[out.parity];
+  A.in = (const S *)in.data;
+  A.in_norm = in.norm;
+  A.out = (SO *)out.data;
+  A.out_norm = out.norm;
+  A.w = w ? (const SO *)w->data : nullptr;
+  A.w_norm = w ? w->norm : nullptr;
+  A.a = a;
+  A.b = b;
+  A.par = out.parity;
+  A.nr = in.nrhs;
+  A.rs = rs;
+  for (int t = 0; t < 16; t++) A.lscale[t] = (P == QAMD_Q16) ? l->scale[t] / Q16_ONE : l->scale[t];
+  int rc = (l->nterms == 16) ? blaunch_nt<P, PO, 16>(A, red, st) : blaunch_nt<P, PO, 8>(A, red, st);
+  if (rc) return -1;
+  QAMD_HIP(hipGetLastError());
+  return 0;
+}
+
+// block Dslash; with red, 2*nrhs partial sums {|out_r|^2, |w_r|^2} go to rs
+int stag_block_launch(const qamd_links *l, Field &out, const Field &in, double a, const Field *w, double b,
+                      bool red, RedScratch rs, hipStream_t st) {
+  bool h2s = ((l->prec == QAMD_F16 || l->prec == QAMD_Q16) && out.prec == QAMD_F32);
+  if (l->kind != 0 || in.prec != l->prec || (out.prec != l->prec && !h2s) || (w && w->prec != out.prec)) {
+    set_error("block Dslash: precision/kind mismatch");
+    return -1;
+  }
+  if (in.nrhs != out.nrhs || (w && w->nrhs != in.nrhs) || in.nrhs > MAXB || in.ncomp != 3 || out.ncomp != 3 ||
+      in.rhs_inner || out.rhs_inner || out.parity == in.parity || (w && w->parity != out.parity)) {
+    set_error("block Dslash: field shapes do not match");
+    return -1;
+  }
+  if (red && red_reserve((int)((l->lat->g.vh + 255) / 256))) return -1;
+  switch (l->prec) {
+    case QAMD_F64: return blaunch<QAMD_F64>(l, out, in, a, w, b, red, rs, st);
+    case QAMD_F32: return blaunch<QAMD_F32>(l, out, in, a, w, b, red, rs, st);
+    case QAMD_F16:
+      return h2s ? blaunch<QAMD_F16, QAMD_F32>(l, out, in, a, w, b, red, rs, st)
+                 : blaunch<QAMD_F16>(l, out, in, a, w, b, red, rs, st);
+    default:
+      return h2s ? blaunch<QAMD_Q16, QAMD_F32>(l, out, in, a, w, b, red, rs, st)
+                 : blaunch<QAMD_Q16>(l, out, in, a, w, b, red, rs, st);
+  }
+}
+
+// ------------------------------------------------------------ block CG
+struct BatchDev {
+  double rsq[MAXB], rsq_old[MAXB], t2[MAXB], p2[MAXB], beta[MAXB], maxrr[MAXB];
+  int active[MAXB], reset[MAXB];
+  double m2;
+  int nrhs;
+};
+
+template <int N, class Fin>
+__global__ void __launch_bounds__(256) k_finish_b2(RedScratch a, RedScratch b, int nb, Fin fin) {
+  double w0[N], w1[N], t[2 * N];
+  finish_sum(a, nb, w0);
+  finish_sum(b, nb, w1);
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < N; i++) {
+      t[i] = w0[i];
+      t[N + i] = w1[i];
+    }
+    fin(t);
+  }
+}
+
+struct FinBNormal {   // set a: {|t_r|^2, 0}, set b: {|Ap_r|^2, |p_r|^2}, N values each
+  BatchDev *B;
+  int N;
+  __device__ void operator()(const double *t) const {
+    for (int r = 0; r < B->nrhs; r++) {
+      B->t2[r] = t[2 * r];
+      B->p2[r] = t[N + 2 * r + 1];
+    }
+  }
+};
+
+// finisher of the normal operator's two partial-sum sets, sized to the block
+static void finish_normal(int n, int nb, BatchDev *B, hipStream_t st) {
+  RedScratch a = red_set(1), b = red_set(2);
+  if (n <= 1) hipLaunchKernelGGL((k_finish_b2<2, FinBNormal>), dim3(1), dim3(256), 0, st, a, b, nb, FinBNormal{B, 2});
+  else if (n <= 2) hipLaunchKernelGGL((k_finish_b2<4, FinBNormal>), dim3(1), dim3(256), 0, st, a, b, nb, FinBNormal{B, 4});
+  else if (n <= 4) hipLaunchKernelGGL((k_finish_b2<8, FinBNormal>), dim3(1), dim3(256), 0, st, a, b, nb, FinBNormal{B, 8});
+  else if (n <= 8) hipLaunchKernelGGL((k_finish_b2<16, FinBNormal>), dim3(1), dim3(256), 0, st, a, b, nb, FinBNormal{B, 16});
+  else hipLaunchKernelGGL((k_finish_b2<32, FinBNormal>), dim3(1), dim3(256), 0, st, a, b, nb, FinBNormal{B, 32});
+}
+
+// finisher of one per-rhs partial-sum set (rhs r in slot r)
+template <class Fin>
+static void finish_rhs(int n, int nb, Fin fin, hipStream_t st) {
+  if (n <= 1) launch_finish<1>(red_set(0), nb, fin, st);
+  else if (n <= 2) launch_finish<2>(red_set(0), nb, fin, st);
+  else if (n <= 4) launch_finish<4>(red_set(0), nb, fin, st);
+  else if (n <= 8) launch_finish<8>(red_set(0), nb, fin, st);
+  else launch_finish<16>(red_set(0), nb, fin, st);
+}
+
+struct FinBCG {
+  BatchDev *B;
+  __device__ void operator()(const double *t) const {
+    for (int r = 0; r < B->nrhs; r++) {
+      if (!B->active[r]) continue;
+      B->rsq_old[r] = B->rsq[r];
+      B->rsq[r] = t[r];
+      B->beta[r] = B->rsq[r] / B->rsq_old[r];
+      B->maxrr[r] = fmax(B->maxrr[r], sqrt(B->rsq[r]));
+    }
+  }
+};
+
+struct FinBTrue {
+  BatchDev *B;
+  __device__ void operator()(const double *t) const {
+    for (int r = 0; r < B->nrhs; r++) {
+      B->rsq[r] = t[r];
+      B->maxrr[r] = sqrt(t[r]);
+    }
+  }
+};
+
+// per-rhs slot of the block sums: grid.y = rhs
+__device__ __forceinline__ void rhs_partial(double v, RedScratch rs) {
+  double a[1] = {v};
+  block_sum(a);
+  if (threadIdx.x == 0) rs.part[blockIdx.y * rs.max_blocks + blockIdx.x] = a[0];
+}
+
+template <int P> struct VB {
+  typedef typename Prec<P>::S S;
+  typedef typename Prec<P>::C C;
+  typedef typename Prec<P>::R R;
+  __device__ static void ld(const Field &f, int r, int i, C (&v)[3]) {
+    VecLd<P>::ld((const S *)f.data + (int64_t)r * 3 * f.vp, f.norm ? f.norm + (int64_t)r * f.vp : nullptr, f.vp, i,
+                 v, (R)1);
+  }
+  __device__ static void st(Field &f, int r, int i, const C (&v)[3]) {
+    VecSt<P>::st((S *)f.data + (int64_t)r * 3 * f.vp, f.norm ? f.norm + (int64_t)r * f.vp : nullptr, f.vp, i, v);
+  }
+};
+
+// x_r += alpha_r p_r; r_r -= alpha_r Ap_r; |r_r|^2 (FP32 vectors)
+__global__ void __launch_bounds__(256) k_bcg_xr(Field x, Field p, Field r, Field Ap, BatchDev *B, RedScratch rs) {
+  const int rr = blockIdx.y;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double red = 0.0;
+  if (B->active[rr] && i < p.g->vh) {
+    double pAp = B->m2 * B->p2[rr] + B->t2[rr];
+    float al = (float)(B->rsq[rr] / pAp);
+    float2 pv[3], rv[3], av[3], xv[3];
+    VB<QAMD_F32>::ld(p, rr, i, pv);
+    VB<QAMD_F32>::ld(r, rr, i, rv);
+    VB<QAMD_F32>::ld(Ap, rr, i, av);
+    VB<QAMD_F32>::ld(x, rr, i, xv);
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      xv[j].x = fmaf(al, pv[j].x, xv[j].x);
+      xv[j].y = fmaf(al, pv[j].y, xv[j].y);
+      rv[j].x = fmaf(-al, av[j].x, rv[j].x);
+      rv[j].y = fmaf(-al, av[j].y, rv[j].y);
+      red += (double)rv[j].x * rv[j].x + (double)rv[j].y * rv[j].y;
+    }
+    VB<QAMD_F32>::st(x, rr, i, xv);
+    VB<QAMD_F32>::st(r, rr, i, rv);
+  }
+  rhs_partial(red, rs);
+}
+
+// p_r = r_r + beta_r p_r, and the 16-bit Dslash copy
+template <int SH>
+__global__ void __launch_bounds__(256) k_bcg_p(Field p, Field r, Field sh, BatchDev *B) {
+  const int rr = blockIdx.y;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (!B->active[rr] || i >= p.g->vh) return;
+  float be = (float)B->beta[rr];
+  float2 pv[3], rv[3];
+  VB<QAMD_F32>::ld(p, rr, i, pv);
+  VB<QAMD_F32>::ld(r, rr, i, rv);
+#pragma unroll
+  for (int j = 0; j < 3; j++) {
+    pv[j].x = fmaf(be, pv[j].x, rv[j].x);
+    pv[j].y = fmaf(be, pv[j].y, rv[j].y);
+  }
+  VB<QAMD_F32>::st(p, rr, i, pv);
+  if constexpr (SH != 0) VB<SH>::st(sh, rr, i, pv);
+}
+
+// restart: p_r = r_r (and its 16-bit copy) for the rhs flagged in reset
+template <int SH>
+__global__ void __launch_bounds__(256) k_bcg_reset(Field p, Field r, Field sh, BatchDev *B) {
+  const int rr = blockIdx.y;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (!B->reset[rr] || i >= p.g->vh) return;
+  float2 rv[3];
+  VB<QAMD_F32>::ld(r, rr, i, rv);
+  VB<QAMD_F32>::st(p, rr, i, rv);
+  if constexpr (SH != 0) VB<SH>::st(sh, rr, i, rv);
+}
+
+// reliable update: x64 += xs, xs = 0 (active rhs)
+__global__ void k_bru_accum(Field x, Field xs, BatchDev *B) {
+  const int rr = blockIdx.y;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (!B->active[rr] || i >= x.g->vh) return;
+  double2 h[3];
+  float2 s[3];
+  VB<QAMD_F64>::ld(x, rr, i, h);
+  VB<QAMD_F32>::ld(xs, rr, i, s);
+#pragma unroll
+  for (int j = 0; j < 3; j++) {
+    h[j].x += s[j].x;
+    h[j].y += s[j].y;
+    s[j].x = s[j].y = 0.f;
+  }
+  VB<QAMD_F64>::st(x, rr, i, h);
+  VB<QAMD_F32>::st(xs, rr, i, s);
+}
+
+// r64 = b - Ax; r32 = r64; |r64|^2 per rhs
+__global__ void k_bru_resid(Field r64, Field b, Field Ax, Field r32, RedScratch rs) {
+  const int rr = blockIdx.y;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double red = 0.0;
+  if (i < r64.g->vh) {
+    double2 bv[3], av[3];
+    float2 sv[3];
+    VB<QAMD_F64>::ld(b, rr, i, bv);
+    VB<QAMD_F64>::ld(Ax, rr, i, av);
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      bv[j].x -= av[j].x;
+      bv[j].y -= av[j].y;
+      red += bv[j].x * bv[j].x + bv[j].y * bv[j].y;
+      sv[j] = make_float2((float)bv[j].x, (float)bv[j].y);
+    }
+    VB<QAMD_F64>::st(r64, rr, i, bv);
+    VB<QAMD_F32>::st(r32, rr, i, sv);
+  }
+  rhs_partial(red, rs);
+}
+
+static double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct BState {
+  BatchDev *B, *Bh;
+};
+
+}  // namespace qamd
+
+using namespace qamd;
+
+extern "C" {
+
+int qamd_stag_cg_mixed_block(const qamd_links *l64, const qamd_links *ls, qamd_field *x, const qamd_field *b,
+                             double mass, double rsq, int maxits, double delta, qamd_solve_info *info) {
+  if (!l64 || !ls || l64->kind != 0 || ls->kind != 0 || l64->prec != QAMD_F64 || ls->prec == QAMD_F64 ||
+      l64->lat->g.V != ls->lat->g.V) {
+    set_error("cg_mixed_block: need FP64 links and FP32/FP16/Q16 sloppy links on one lattice");
+    return -1;
+  }
+  const int n = b->f.nrhs;
+  if (n < 1 || n > 8 || x->f.nrhs != n || x->f.ncomp != 3 || b->f.ncomp != 3 || x->f.prec != QAMD_F64 ||
+      b->f.prec != QAMD_F64 || x->f.parity != 0 || b->f.parity != 0) {
+    set_error("cg_mixed_block: x, b must be even FP64 colour-vector blocks of 1..8 rhs");
+    return -1;
+  }
+  if (delta <= 0 || delta >= 1) delta = 0.1;
+  double t0 = now_s();
+  qamd_lattice *lat = l64->lat;
+  const int ps = ls->prec;
+  const bool half = (ps == QAMD_F16 || ps == QAMD_Q16);
+  const bool verbose = getenv("QAMD_VERBOSE") != nullptr;
+  const double m2 = mass * mass;
+  const int nb = (int)((lat->g.vh + 255) / 256);
+  const dim3 gb(nb, n);
+  BState st = {nullptr, nullptr};
+  qamd_field *r64 = qamd_field_create(lat, QAMD_F64, 3, n, 0);
+  qamd_field *Ax = qamd_field_create(lat, QAMD_F64, 3, n, 0);
+  qamd_field *t64 = qamd_field_create(lat, QAMD_F64, 3, n, 1);
+  qamd_field *r = qamd_field_create(lat, QAMD_F32, 3, n, 0);
+  qamd_field *p = qamd_field_create(lat, QAMD_F32, 3, n, 0);
+  qamd_field *Ap = qamd_field_create(lat, QAMD_F32, 3, n, 0);
+  qamd_field *xs = qamd_field_create(lat, QAMD_F32, 3, n, 0);
+  qamd_field *t = qamd_field_create(lat, ps, 3, n, 1);
+  qamd_field *ph = half ? qamd_field_create(lat, ps, 3, n, 0) : nullptr;
+  qamd_field *pin = half ? ph : p;
+  st.B = (BatchDev *)dev_alloc(sizeof(BatchDev));
+  if (hipHostMalloc((void **)&st.Bh, sizeof(BatchDev), hipHostMallocDefault) != hipSuccess) st.Bh = nullptr;
+  int rc = -1, nupd = 0, its = 0, nrestart = 0;
+  double bb[MAXB], r2[MAXB];
+  auto push = [&]() -> int {
+    QAMD_HIP(hipMemcpyAsync(st.B, st.Bh, sizeof(BatchDev), hipMemcpyHostToDevice, stream()));
+    return 0;
+  };
+  auto pull = [&]() -> int {
+    QAMD_HIP(hipMemcpyAsync(st.Bh, st.B, sizeof(BatchDev), hipMemcpyDeviceToHost, stream()));
+    QAMD_HIP(hipStreamSynchronize(stream()));
+    return 0;
+  };
+  // A v on blocks: t = D v (set 1), out = m2 w - D t (set 2), finisher
+  auto normal = [&](const qamd_links *L, qamd_field *out, qamd_field *vin, qamd_field *vw, qamd_field *tt) -> int {
+    if (stag_block_launch(L, tt->f, vin->f, 0.0, nullptr, 1.0, true, red_set(1), stream())) return -1;
+    if (stag_block_launch(L, out->f, tt->f, m2, &vw->f, -1.0, true, red_set(2), stream())) return -1;
+    finish_normal(n, nb, st.B, stream());
+    QAMD_HIP(hipGetLastError());
+    return 0;
+  };
+  auto reliable = [&]() -> int {
+    if (normal(l64, Ax, x, x, t64)) return -1;
+    hipLaunchKernelGGL(k_bru_resid, gb, dim3(256), 0, stream(), r64->f, b->f, Ax->f, r->f, red_set(0));
+    finish_rhs(n, nb, FinBTrue{st.B}, stream());
+    QAMD_HIP(hipGetLastError());
+    return pull();
+  };
+  do {
+    if (!r64 || !Ax || !t64 || !r || !p || !Ap || !xs || !t || (half && !ph) || !st.B || !st.Bh) break;
+    if (red_reserve(nb)) break;
+    if (qamd_field_norm2(b, bb)) break;
+    if (qamd_field_zero(x)) break;
+    memset(st.Bh, 0, sizeof(BatchDev));
+    st.Bh->m2 = m2;
+    st.Bh->nrhs = n;
+    for (int i = 0; i < n; i++) st.Bh->active[i] = 1;
+    if (push() || reliable()) break;
+    int nact = 0;
+    for (int i = 0; i < n; i++) {
+      r2[i] = st.Bh->rsq[i];
+      st.Bh->active[i] = bb[i] > 0 && r2[i] >= rsq * bb[i];
+      nact += st.Bh->active[i];
+    }
+    if (push()) break;
+    if (qamd_field_copy(p, r) || qamd_field_zero(xs) || (half && qamd_field_copy(ph, p))) break;
+    int err = 0;
+    while (nact > 0 && its < maxits && !err) {
+      for (int k = 0; k < 16 && !err; k++) {
+        err |= normal(ls, Ap, pin, p, t);
+        hipLaunchKernelGGL(k_bcg_xr, gb, dim3(256), 0, stream(), xs->f, p->f, r->f, Ap->f, st.B, red_set(0));
+        finish_rhs(n, nb, FinBCG{st.B}, stream());
+        if (!half) hipLaunchKernelGGL(k_bcg_p<0>, gb, dim3(256), 0, stream(), p->f, r->f, p->f, st.B);
+        else if (ps == QAMD_F16) hipLaunchKernelGGL(k_bcg_p<QAMD_F16>, gb, dim3(256), 0, stream(), p->f, r->f, ph->f, st.B);
+        else hipLaunchKernelGGL(k_bcg_p<QAMD_Q16>, gb, dim3(256), 0, stream(), p->f, r->f, ph->f, st.B);
+        if (hipGetLastError() != hipSuccess) err = 1;
+      }
+      if (err || pull()) {
+        err = 1;
+        break;
+      }
+      its += 16;
+      // an update when any active rhs dropped by delta, converged, or grew
+      // well above its last true residual; all active rhs take it together
+      bool upd = its >= maxits;
+      bool trig[MAXB];
+      for (int i = 0; i < n; i++) {
+        double q = st.Bh->rsq[i];
+        trig[i] = st.Bh->active[i] &&
+                  (!isfinite(q) || sqrt(q) < delta * st.Bh->maxrr[i] || q < rsq * bb[i] || q > 4.0 * r2[i]);
+        upd |= trig[i];
+      }
+      if (verbose)
+        fprintf(stderr, "cg_block its %d rel|r0| %.3e\n", its, sqrt(st.Bh->rsq[0] / bb[0]));
+      if (!upd) continue;
+      double r2s[MAXB];
+      for (int i = 0; i < n; i++) r2s[i] = st.Bh->rsq[i];
+      hipLaunchKernelGGL(k_bru_accum, gb, dim3(256), 0, stream(), x->f, xs->f, st.B);
+      if (hipGetLastError() != hipSuccess || reliable()) {
+        err = 1;
+        break;
+      }
+      nupd++;
+      // restart p_r = r_r where the true residual left the iterated one
+      // behind, or where a delta drop did not carry over to the true residual
+      int nreset = 0;
+      nact = 0;
+      for (int i = 0; i < n; i++) {
+        double q = st.Bh->rsq[i];
+        if (!isfinite(q)) {
+          set_error("cg_mixed_block: rhs %d is not finite after %d iterations", i, its);
+          err = 1;
+        }
+        bool was = st.Bh->active[i];
+        if (q < rsq * bb[i]) st.Bh->active[i] = 0;
+        st.Bh->reset[i] = st.Bh->active[i] && (q > 2.0 * r2s[i] || (trig[i] && q > 0.25 * r2[i]));
+        nreset += st.Bh->reset[i];
+        if (was) r2[i] = q;
+        nact += st.Bh->active[i];
+      }
+      if (err) break;
+      if (push()) {
+        err = 1;
+        break;
+      }
+      if (nreset) {
+        if (!half) hipLaunchKernelGGL(k_bcg_reset<0>, gb, dim3(256), 0, stream(), p->f, r->f, p->f, st.B);
+        else if (ps == QAMD_F16) hipLaunchKernelGGL(k_bcg_reset<QAMD_F16>, gb, dim3(256), 0, stream(), p->f, r->f, ph->f, st.B);
+        else hipLaunchKernelGGL(k_bcg_reset<QAMD_Q16>, gb, dim3(256), 0, stream(), p->f, r->f, ph->f, st.B);
+        if (hipGetLastError() != hipSuccess) err = 1;
+        nrestart += nreset;
+      }
+    }
+    if (err) break;
+    if (info)
+      for (int i = 0; i < n; i++) {
+        memset(&info[i], 0, sizeof info[i]);
+        info[i].iterations = its;
+        info[i].reliable_updates = nupd;
+        info[i].true_rsq = bb[i] > 0 ? st.Bh->rsq[i] / bb[i] : 0;
+        info[i].iter_rsq = info[i].true_rsq;
+        info[i].converged = bb[i] == 0 || st.Bh->rsq[i] < rsq * bb[i];
+        info[i].restarts = nrestart;
+        info[i].seconds = now_s() - t0;
+      }
+    rc = 0;
+  } while (0);
+  qamd_field_destroy(r64);
+  qamd_field_destroy(Ax);
+  qamd_field_destroy(t64);
+  qamd_field_destroy(r);
+  qamd_field_destroy(p);
+  qamd_field_destroy(Ap);
+  qamd_field_destroy(xs);
+  qamd_field_destroy(t);
+  qamd_field_destroy(ph);
+  dev_free(st.B);
+  if (st.Bh) (void)hipHostFree(st.Bh);
+  return rc;
+}
+
+}  // extern "C"

@@ -12,7 +12,7 @@ from helpers import parity_mask, rel_err
 
 pytestmark = pytest.mark.gpu
 
-TOL = {0: 1e-13, 1: 1e-5, 2: 5e-3}
+TOL = {0: 1e-13, 1: 1e-5, 2: 5e-3, 3: 2e-4}   # 3: 16-bit fixed point (15-bit mantissas)
 DIMS = [(4, 4, 4, 4), (8, 8, 8, 8), (4, 6, 8, 10), (6, 4, 4, 12)]
 
 
@@ -28,7 +28,7 @@ def _stag_case(dims, seed, nterms=16):
 
 
 @pytest.mark.parametrize("dims", DIMS)
-@pytest.mark.parametrize("prec", [0, 1, 2])
+@pytest.mark.parametrize("prec", [0, 1, 2, 3])
 def test_asqtad_eo_dslash(gpu, dims, prec):
     print("case: asqtad EO dims=%s prec=%d" % (dims, prec), flush=True)
     links, y = _stag_case(dims, 11)
@@ -85,7 +85,7 @@ def test_normal_operator(gpu):
     assert abs(pf.redot(Ap) - an_ref) <= 1e-12 * an_ref
 
 
-@pytest.mark.parametrize("prec", [0, 1, 2])
+@pytest.mark.parametrize("prec", [0, 1, 2, 3])
 def test_wilson_eo_dslash(gpu, prec):
     dims = (4, 4, 6, 8)
     u = O.random_su3_field(dims, 21)
@@ -124,6 +124,32 @@ def test_wilson_multi_rhs(gpu, nrhs):
         ref = O.wilson_dslash(dims, links, 0.0, ys[r], sub="even")
         err = rel_err(got[r], ref, m)
         assert err < 1e-5, (r, err)
+
+
+@pytest.mark.parametrize("prec,nrhs", [(0, 3), (0, 8), (1, 5), (1, 16), (2, 16), (3, 7)])
+def test_asqtad_block_dslash(gpu, prec, nrhs):
+    """The block Dslash (links loaded once for all rhs) gives the oracle's
+    Dslash on every rhs, with the a*w + b*D form and both parities."""
+    print("case: asqtad block prec=%d nrhs=%d" % (prec, nrhs), flush=True)
+    dims = (4, 6, 4, 8)
+    links, _ = _stag_case(dims, 21)
+    ys = np.stack([O.gauss_vector(dims, 60 + r, 100, parity=-1) for r in range(nrhs)])
+    lat = gpu.Lattice(dims)
+    L = gpu.Links(lat, gpu.STAG, 0, 16).set_lex(links)
+    if prec:
+        L = L.converted(prec)
+    a, b = 0.3, -0.7
+    for p in (0, 1):
+        yo = gpu.Field(lat, prec, 3, nrhs, 1 - p).set_lex(ys)
+        yp = gpu.Field(lat, prec, 3, nrhs, p).set_lex(ys)
+        out = gpu.Field(lat, prec, 3, nrhs, p)
+        gpu.dslash(L, out, yo, a=a, w=yp, b=b)
+        got = out.get_lex()
+        m = parity_mask(dims, p)
+        for r in range(nrhs):
+            ref = a * ys[r] + b * O.dslash(dims, links, 0.0, ys[r], sub="even" if p == 0 else "odd")
+            err = rel_err(got[r], ref, m)
+            assert err < TOL[prec], (p, r, err)
 
 
 @pytest.mark.parametrize("inner", [(1, 1, 1, 1), (2, 1, 1, 1), (1, 2, 2, 1), (2, 2, 2, 1)])

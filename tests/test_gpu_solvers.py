@@ -140,7 +140,7 @@ def test_cgms_matches_oracle(gpu):
         assert _true_rel_resid(dims, links, mi, b, xs[i].get_lex()) < 1e-14
 
 
-@pytest.mark.parametrize("sloppy", [1, 2])
+@pytest.mark.parametrize("sloppy", [1, 2, 3])
 def test_mixed_cg_true_residual(gpu, sloppy):
     """FP32/FP16 inner Dslash with FP64 reliable updates reaches the FP64
     true residual 1e-10 and the oracle's FP64 solution."""
@@ -158,6 +158,48 @@ def test_mixed_cg_true_residual(gpu, sloppy):
     assert _true_rel_resid(dims, links, mass, b, xs) <= 1e-20 * 1.01
     xr, _, _ = O.cg(dims, links, mass, b, 1e-24, 20000, noscale=True)
     assert rel_err(xs, xr, parity_mask(dims, 0)) < 1e-7
+
+
+@pytest.mark.parametrize("sloppy,nrhs", [(1, 1), (1, 5), (3, 8)])
+def test_mixed_cg_block(gpu, sloppy, nrhs):
+    """Batched mixed-precision CG: every rhs of the block reaches the FP64
+    true residual 1e-10 (checked with the oracle's operator) and the oracle's
+    FP64 solution; sources of very different size leave the block at their
+    own iteration."""
+    print("case: block cg sloppy=%d nrhs=%d" % (sloppy, nrhs), flush=True)
+    dims = (8, 8, 8, 8)
+    mass = 0.05
+    links = O.asqtad_links(dims, 14)
+    bs = np.stack([O.gauss_vector(dims, 70 + r, 100, parity=0) * (10.0 ** (r % 3)) for r in range(nrhs)])
+    lat = gpu.Lattice(dims)
+    L64 = gpu.Links(lat, gpu.STAG, 0, 16).set_lex(links)
+    Ls = L64.converted(sloppy)
+    b = gpu.Field(lat, 0, 3, nrhs, 0).set_lex(bs)
+    x = gpu.Field(lat, 0, 3, nrhs, 0)
+    infos = gpu.stag_cg_mixed_block(L64, Ls, x, b, mass, 1e-20, 20000)
+    xs = x.get_lex().reshape(nrhs, -1, 6)
+    m = parity_mask(dims, 0)
+    for r in range(nrhs):
+        assert infos[r]["converged"], (r, infos[r])
+        assert _true_rel_resid(dims, links, mass, bs[r], xs[r]) <= 1e-20 * 1.01, r
+    xr, _, _ = O.cg(dims, links, mass, bs[0], 1e-24, 20000, noscale=True)
+    assert rel_err(xs[0], xr, m) < 1e-7
+
+
+def test_mixed_cg_block_zero_rhs(gpu):
+    """A zero source in the block gives x = 0 and does not hold up the rest."""
+    dims = (4, 4, 4, 8)
+    links = O.asqtad_links(dims, 15)
+    bs = np.stack([O.gauss_vector(dims, 80, 100, parity=0), np.zeros((O.volume(dims), 6))])
+    lat = gpu.Lattice(dims)
+    L64 = gpu.Links(lat, gpu.STAG, 0, 16).set_lex(links)
+    b = gpu.Field(lat, 0, 3, 2, 0).set_lex(bs)
+    x = gpu.Field(lat, 0, 3, 2, 0)
+    infos = gpu.stag_cg_mixed_block(L64, L64.converted(1), x, b, 0.1, 1e-20, 5000)
+    xs = x.get_lex()
+    assert infos[0]["converged"] and infos[1]["converged"]
+    assert np.all(xs[1] == 0)
+    assert _true_rel_resid(dims, links, 0.1, bs[0], xs[0]) <= 1.01e-20
 
 
 @pytest.mark.parametrize("sloppy,nrhs", [(None, 1), (1, 4), (2, 4)])
