@@ -25,9 +25,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FLOPS_ASQTAD = 1146          # per EO output site
-BYTES_SITE = {0: 2400, 1: 1200, 2: 608, 3: 608}   # algorithmic bytes per EO output site
+# algorithmic bytes per EO output site: 16 link matrices + one input and one
+# output colour vector per rhs (16-bit vectors carry a 4-byte site norm)
+LINK_BYTES = {0: 2304, 1: 1152, 2: 576, 3: 576}
+VEC_BYTES = {0: 48, 1: 24, 2: 16, 3: 16}
+BYTES_SITE = {p: LINK_BYTES[p] + 2 * VEC_BYTES[p] for p in LINK_BYTES}
 PREC_NAME = {0: "f64", 1: "f32", 2: "f16", 3: "q16"}
 HBM_PEAK_GBS = 8000.0
+COPY_BYTES = 1 << 28          # the PMC calibration copy: reads and writes 256 MiB
 DIMS = (32, 32, 32, 32)
 QLL_MASS = 0.02
 RSQ = 1e-20
@@ -40,11 +45,73 @@ def parse():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cg", action="store_true", help="skip the solver comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-pmc", action="store_true", help="skip the rocprofv3 HBM-traffic passes")
+    ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--block", type=int, default=8, help="sources in the batched mixed-precision solve")
     ap.add_argument("--dims", type=str, default=None, help="override lattice, e.g. 16,16,16,16")
     ap.add_argument("--precs", type=str, default="0,1,2,3",
                     help="Dslash precisions to time (0 f64, 1 f32, 2 f16, 3 16-bit fixed point)")
     return ap.parse_args()
+
+
+def pmc_child(dims):
+    """Run under rocprofv3 --pmc: a copy kernel of known size, then the FP64
+    Dslash, a few dispatches each."""
+    import qll_amd as Q
+    Q.init()
+    Q.stream_copy(COPY_BYTES, 3)
+    lat = Q.Lattice(dims)
+    L = Q.asqtad_links_device(lat, seed=1)
+    yin = Q.Field(lat, Q.F64, 3, 1, Q.ODD).gaussian(5, 100)
+    out = Q.Field(lat, Q.F64, 3, 1, Q.EVEN)
+    Q.time_dslash(L, out, yin, 5)
+    Q.sync()
+
+
+def _pmc_pass(counter, dims, tmp):
+    """One rocprofv3 pass of one counter over pmc_child; returns the mean KiB
+    per dispatch of the copy kernel and of the FP64 Dslash kernel."""
+    import csv
+    import subprocess
+    d = os.path.join(tmp, counter.lower())
+    cmd = ["timeout", "-s", "KILL", "90", "rocprofv3", "--pmc", counter, "--output-format", "csv",
+           "-d", d, "-o", "pmc", "--", sys.executable, os.path.abspath(__file__), "--pmc-child",
+           "--dims", ",".join(map(str, dims))]
+    env = dict(os.environ, TMPDIR=tmp)
+    r = subprocess.run(cmd, cwd=tmp, env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, timeout=120)
+    if r.returncode != 0:
+        raise RuntimeError("rocprofv3 --pmc %s exited %d" % (counter, r.returncode))
+    vals = {"copy": [], "dslash": []}
+    for root, _, files in os.walk(d):
+        for f in files:
+            if not f.endswith("counter_collection.csv"):
+                continue
+            for row in csv.DictReader(open(os.path.join(root, f))):
+                name = row["Kernel_Name"]
+                if row["Counter_Name"] != counter:
+                    continue
+                if "k_stream_copy" in name:
+                    vals["copy"].append(float(row["Counter_Value"]))
+                elif "k_stag_dslash<0," in name.replace(" ", ""):
+                    vals["dslash"].append(float(row["Counter_Value"]))
+    if not vals["copy"] or not vals["dslash"]:
+        raise RuntimeError("rocprofv3 --pmc %s: kernels not found in the counter output" % counter)
+    return {k: sum(v) / len(v) for k, v in vals.items()}
+
+
+def pmc_traffic(dims):
+    """HBM bytes per FP64 Dslash launch from FETCH_SIZE and WRITE_SIZE, each
+    scaled by bytes/KiB-count read off the copy kernel (which reads and
+    writes COPY_BYTES)."""
+    import tempfile
+    with tempfile.TemporaryDirectory(dir="/tmp") as tmp:
+        f = _pmc_pass("FETCH_SIZE", dims, tmp)
+        w = _pmc_pass("WRITE_SIZE", dims, tmp)
+    ff = COPY_BYTES / (f["copy"] * 1024.0)
+    fw = COPY_BYTES / (w["copy"] * 1024.0)
+    return {"bytes": f["dslash"] * 1024.0 * ff + w["dslash"] * 1024.0 * fw,
+            "fetch_kib": f["dslash"], "write_kib": w["dslash"],
+            "fetch_scale": ff, "write_scale": fw}
 
 
 def cpu_baseline(dims, nsec_target=10.0):
@@ -80,8 +147,18 @@ def main():
         print("bench.py: this build measures one GPU only (every box here has one GPU); "
               "--gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
-    import qll_amd as Q
     dims = tuple(int(x) for x in args.dims.split(",")) if args.dims else DIMS
+    if args.pmc_child:
+        pmc_child(dims)
+        return
+    traffic = None
+    if not args.no_pmc:
+        try:
+            traffic = pmc_traffic(dims)
+            print("pmc traffic: %s" % traffic, file=sys.stderr, flush=True)
+        except Exception as e:   # no profiler: traffic stays null
+            print("pmc traffic unavailable: %s" % e, file=sys.stderr, flush=True)
+    import qll_amd as Q
     Q.init()
     lat = Q.Lattice(dims)
     vh = lat.V // 2
@@ -120,6 +197,23 @@ def main():
                       "GB/s": round(gbs, 1),
                       "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
         print("dslash prec=%d: %s" % (prec, prof[prec]), file=sys.stderr, flush=True)
+
+    # block Dslash: links loaded once for the whole block of rhs
+    nbk = args.block
+    for prec, L in ((1, L32), (3, LQ)):
+        if prec not in precs:
+            continue
+        yin = Q.Field(lat, prec, 3, nbk, Q.ODD).gaussian(5, 100)
+        out = Q.Field(lat, prec, 3, nbk, Q.EVEN)
+        Q.time_dslash(L, out, yin, args.warmup)
+        ms = Q.time_dslash(L, out, yin, max(1, args.steps // 4))
+        byts = (LINK_BYTES[prec] + nbk * 2 * VEC_BYTES[prec]) * vh
+        key = "%s_block%d" % (PREC_NAME[prec], nbk)
+        prof[key] = {"ms_per_call": round(ms, 5), "ms_per_rhs": round(ms / nbk, 5),
+                     "GFLOP/s": round(nbk * FLOPS_ASQTAD * vh / (ms * 1e-3) / 1e9, 1),
+                     "GB/s": round(byts / (ms * 1e-3) / 1e9, 1),
+                     "hbm_frac": round(byts / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        print("dslash %s: %s" % (key, prof[key]), file=sys.stderr, flush=True)
 
     ms64 = rates[0]
     gflops = FLOPS_ASQTAD * vh / (ms64 * 1e-3) / 1e9
@@ -212,11 +306,14 @@ def main():
                                % ("x".join(map(str, dims)), vh),
                    "lattice": list(dims), "mass_qll": QLL_MASS, "flops_per_site": FLOPS_ASQTAD},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": round(traffic["bytes"]) if traffic else None,
+                     "traffic_per_site": round(traffic["bytes"] / vh, 1) if traffic else None,
+                     "algorithmic_per_site": BYTES_SITE[0],
                      "measured_copy_gbs": round(copy_gbs, 1),
                      "frac_of_copy": round(achieved / copy_gbs, 4)},
         "cpu_baseline": cpu,
-        "dslash": {PREC_NAME[p]: prof[p] for p in prof},
+        "dslash": {(PREC_NAME[p] if p in PREC_NAME else p): prof[p] for p in prof},
         "solvers": solver,
         "device": Q.device_name(),
     }

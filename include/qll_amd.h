@@ -1,9 +1,9 @@
 /*
  * qll_amd.h -- device-resident C ABI of libqll_amd (MI355X / gfx950).
  *
- * The QLL drop-in (qll.h, solvers.h and the qop* entry points) is built on
- * this layer; it is also the interface for callers that keep their fields in
- * HBM between calls (benchmarks, batched solvers, mixed precision).  Plain C
+ * The QLL drop-in (qll.h, solvers.h) is built on this layer; it is also the
+ * interface for callers that keep their fields in HBM between calls
+ * (benchmarks, batched solvers, mixed precision).  Plain C
  * types only: opaque handles, pointers and sizes.
  *
  * Which reference interface each group replaces is noted per group; entry
@@ -129,7 +129,10 @@ int qamd_links_convert(qamd_links *dst, const qamd_links *src);
 /* ---- Dslash (replaces dslash2 lib/stagDslash.c:531-544 and wilsonDslash
  *      lib/wilsonDslash.c:272-285 for one output parity) ----
  * out = a*w + b*D in, on out's parity; w may be NULL (then a is ignored).
- * norm2_out (may be NULL): receives |out|^2 per rhs, FP64 accumulated. */
+ * norm2_out (may be NULL): receives |out|^2 summed over all rhs, FP64
+ * accumulated.  Staggered fields of nrhs > 1 (<= 16, FP64 <= 8) run the
+ * block kernel, which loads every link once for all rhs (no reference
+ * counterpart: QLL runs one dslash2 per source). */
 int qamd_dslash(const qamd_links *l, qamd_field *out, const qamd_field *in,
                 double a, const qamd_field *w, double b, double *norm2_out);
 
@@ -156,7 +159,8 @@ typedef struct {
 int qamd_stag_cg(const qamd_links *l, qamd_field *x, const qamd_field *b, double mass,
                  double rsq, int maxits, qamd_solve_info *info);
 /* Mixed precision CG with reliable updates (no reference counterpart):
- * outer FP64 (links l64, x, b), inner sloppy links ls (FP32 or FP16); stops on
+ * outer FP64 (links l64, x, b), inner sloppy links ls (FP32, FP16 or Q16,
+ * escalating to FP32 when 16-bit links stall); stops on
  * the FP64 true residual |b - A x|^2 <= rsq |b|^2.  delta: reliable-update
  * threshold on |r| relative to its maximum since the last update. */
 int qamd_stag_cg_mixed(const qamd_links *l64, const qamd_links *ls, qamd_field *x,

@@ -253,21 +253,34 @@ __device__ __forceinline__ void grid_partial(double (&v)[N], RedScratch rs) {
     for (int i = 0; i < N; i++) rs.part[i * rs.max_blocks + blockIdx.x] = v[i];
 }
 
-// fixed-order total of N slots over nb blocks (thread k adds blocks k,
-// k+256, ... in order; the block sum adds the threads in order)
+// fixed-order total of N slots over nb blocks: thread k adds blocks k,
+// k+FT, k+2FT, ... in order (all of a chunk's loads issued before its adds,
+// so the round trips overlap), then the block sum adds the threads in order
+constexpr int FT = 512;   // finisher threads
 template <int N>
 __device__ __forceinline__ void finish_sum(RedScratch rs, int nb, double (&w)[N]) {
+  constexpr int U = N >= 16 ? 1 : 16 / N;
 #pragma unroll
-  for (int i = 0; i < N; i++) {
-    double s = 0;
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) s += rs.part[i * rs.max_blocks + b];
-    w[i] = s;
+  for (int i = 0; i < N; i++) w[i] = 0.0;
+  for (int b0 = threadIdx.x; b0 < nb; b0 += U * FT) {
+    double v[U][N];
+#pragma unroll
+    for (int u = 0; u < U; u++)
+#pragma unroll
+      for (int i = 0; i < N; i++) {
+        int b = b0 + u * FT;
+        v[u][i] = b < nb ? rs.part[i * rs.max_blocks + b] : 0.0;
+      }
+#pragma unroll
+    for (int u = 0; u < U; u++)
+#pragma unroll
+      for (int i = 0; i < N; i++) w[i] += v[u][i];
   }
   block_sum(w);
 }
 
 template <int N, class Fin>
-__global__ void __launch_bounds__(256) k_finish(RedScratch rs, int nb, Fin fin, const int *skip) {
+__global__ void __launch_bounds__(FT) k_finish(RedScratch rs, int nb, Fin fin, const int *skip) {
   if (skip && *skip) return;
   double w[N];
   finish_sum(rs, nb, w);
@@ -276,7 +289,7 @@ __global__ void __launch_bounds__(256) k_finish(RedScratch rs, int nb, Fin fin, 
 
 // totals of N0 slots of set a and N1 slots of set b, passed as one array
 template <int N0, int N1, class Fin>
-__global__ void __launch_bounds__(256) k_finish2(RedScratch a, RedScratch b, int nb, Fin fin, const int *skip) {
+__global__ void __launch_bounds__(FT) k_finish2(RedScratch a, RedScratch b, int nb, Fin fin, const int *skip) {
   if (skip && *skip) return;
   double w0[N0], w1[N1], t[N0 + N1];
   finish_sum(a, nb, w0);
@@ -290,7 +303,7 @@ __global__ void __launch_bounds__(256) k_finish2(RedScratch a, RedScratch b, int
 
 template <int N, class Fin>
 inline void launch_finish(RedScratch rs, int nb, Fin fin, hipStream_t st, const int *skip = nullptr) {
-  hipLaunchKernelGGL((k_finish<N, Fin>), dim3(1), dim3(256), 0, st, rs, nb, fin, skip);
+  hipLaunchKernelGGL((k_finish<N, Fin>), dim3(1), dim3(FT), 0, st, rs, nb, fin, skip);
 }
 
 }  // namespace qamd

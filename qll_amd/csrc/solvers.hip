@@ -392,7 +392,7 @@ static int apply_normal(const qamd_links *l, SolverDev *S, qamd_field *Ap, qamd_
   if (stag_dslash_launch(l, t->f, pin->f, 0.0, nullptr, 1.0, nullptr, stream(), skip, 1, false)) return -1;
   if (stag_dslash_launch(l, Ap->f, t->f, m2, &pw->f, -1.0, nullptr, stream(), skip, 2, false)) return -1;
   int nb = (int)((l->lat->g.vh + 255) / 256);
-  hipLaunchKernelGGL((k_finish2<2, 2, FinNormal>), dim3(1), dim3(256), 0, stream(), red_set(1), red_set(2), nb,
+  hipLaunchKernelGGL((k_finish2<2, 2, FinNormal>), dim3(1), dim3(FT), 0, stream(), red_set(1), red_set(2), nb,
                      FinNormal{S}, skip);
   QAMD_HIP(hipGetLastError());
   return 0;

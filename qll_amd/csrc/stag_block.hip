@@ -191,11 +191,12 @@ struct BatchDev {
 };
 
 template <int N, class Fin>
-__global__ void __launch_bounds__(256) k_finish_b2(RedScratch a, RedScratch b, int nb, Fin fin) {
+__global__ void __launch_bounds__(FT) k_finish_b2(RedScratch a, RedScratch b, int nb, Fin fin) {
   double w0[N], w1[N], t[2 * N];
   finish_sum(a, nb, w0);
   finish_sum(b, nb, w1);
   if (threadIdx.x == 0) {
+#pragma unroll
     for (int i = 0; i < N; i++) {
       t[i] = w0[i];
       t[N + i] = w1[i];
@@ -204,59 +205,70 @@ __global__ void __launch_bounds__(256) k_finish_b2(RedScratch a, RedScratch b, i
   }
 }
 
-struct FinBNormal {   // set a: {|t_r|^2, 0}, set b: {|Ap_r|^2, |p_r|^2}, N values each
+// The epilogues loop over a compile-time block size NR (slots of rhs >= nrhs
+// are skipped), so the totals stay in registers.
+template <int NR> struct FinBNormal {   // set a: {|t_r|^2, 0}, set b: {|Ap_r|^2, |p_r|^2}
   BatchDev *B;
-  int N;
   __device__ void operator()(const double *t) const {
-    for (int r = 0; r < B->nrhs; r++) {
-      B->t2[r] = t[2 * r];
-      B->p2[r] = t[N + 2 * r + 1];
-    }
+#pragma unroll
+    for (int r = 0; r < NR; r++)
+      if (r < B->nrhs) {
+        B->t2[r] = t[2 * r];
+        B->p2[r] = t[2 * NR + 2 * r + 1];
+      }
   }
 };
 
-// finisher of the normal operator's two partial-sum sets, sized to the block
-static void finish_normal(int n, int nb, BatchDev *B, hipStream_t st) {
-  RedScratch a = red_set(1), b = red_set(2);
-  if (n <= 1) hipLaunchKernelGGL((k_finish_b2<2, FinBNormal>), dim3(1), dim3(256), 0, st, a, b, nb, FinBNormal{B, 2});
-  else if (n <= 2) hipLaunchKernelGGL((k_finish_b2<4, FinBNormal>), dim3(1), dim3(256), 0, st, a, b, nb, FinBNormal{B, 4});
-  else if (n <= 4) hipLaunchKernelGGL((k_finish_b2<8, FinBNormal>), dim3(1), dim3(256), 0, st, a, b, nb, FinBNormal{B, 8});
-  else if (n <= 8) hipLaunchKernelGGL((k_finish_b2<16, FinBNormal>), dim3(1), dim3(256), 0, st, a, b, nb, FinBNormal{B, 16});
-  else hipLaunchKernelGGL((k_finish_b2<32, FinBNormal>), dim3(1), dim3(256), 0, st, a, b, nb, FinBNormal{B, 32});
+template <int NR> struct FinBCG {
+  BatchDev *B;
+  __device__ void operator()(const double *t) const {
+#pragma unroll
+    for (int r = 0; r < NR; r++)
+      if (r < B->nrhs && B->active[r]) {
+        B->rsq_old[r] = B->rsq[r];
+        B->rsq[r] = t[r];
+        B->beta[r] = B->rsq[r] / B->rsq_old[r];
+        B->maxrr[r] = fmax(B->maxrr[r], sqrt(B->rsq[r]));
+      }
+  }
+};
+
+template <int NR> struct FinBTrue {
+  BatchDev *B;
+  __device__ void operator()(const double *t) const {
+#pragma unroll
+    for (int r = 0; r < NR; r++)
+      if (r < B->nrhs) {
+        B->rsq[r] = t[r];
+        B->maxrr[r] = sqrt(t[r]);
+      }
+  }
+};
+
+// finishers sized to the block: NR = block size rounded up to 1, 2, 4, 8, 16
+template <int NR> static void fin_normal(int nb, BatchDev *B, hipStream_t st) {
+  hipLaunchKernelGGL((k_finish_b2<2 * NR, FinBNormal<NR>>), dim3(1), dim3(FT), 0, st, red_set(1), red_set(2), nb,
+                     FinBNormal<NR>{B});
 }
-
-// finisher of one per-rhs partial-sum set (rhs r in slot r)
-template <class Fin>
-static void finish_rhs(int n, int nb, Fin fin, hipStream_t st) {
-  if (n <= 1) launch_finish<1>(red_set(0), nb, fin, st);
-  else if (n <= 2) launch_finish<2>(red_set(0), nb, fin, st);
-  else if (n <= 4) launch_finish<4>(red_set(0), nb, fin, st);
-  else if (n <= 8) launch_finish<8>(red_set(0), nb, fin, st);
-  else launch_finish<16>(red_set(0), nb, fin, st);
+template <int NR> static void fin_cg(int nb, BatchDev *B, hipStream_t st) {
+  launch_finish<NR>(red_set(0), nb, FinBCG<NR>{B}, st);
 }
-
-struct FinBCG {
-  BatchDev *B;
-  __device__ void operator()(const double *t) const {
-    for (int r = 0; r < B->nrhs; r++) {
-      if (!B->active[r]) continue;
-      B->rsq_old[r] = B->rsq[r];
-      B->rsq[r] = t[r];
-      B->beta[r] = B->rsq[r] / B->rsq_old[r];
-      B->maxrr[r] = fmax(B->maxrr[r], sqrt(B->rsq[r]));
-    }
-  }
+template <int NR> static void fin_true(int nb, BatchDev *B, hipStream_t st) {
+  launch_finish<NR>(red_set(0), nb, FinBTrue<NR>{B}, st);
+}
+struct FinOps {
+  void (*normal)(int, BatchDev *, hipStream_t);
+  void (*cg)(int, BatchDev *, hipStream_t);
+  void (*truer)(int, BatchDev *, hipStream_t);
 };
-
-struct FinBTrue {
-  BatchDev *B;
-  __device__ void operator()(const double *t) const {
-    for (int r = 0; r < B->nrhs; r++) {
-      B->rsq[r] = t[r];
-      B->maxrr[r] = sqrt(t[r]);
-    }
-  }
-};
+template <int NR> static FinOps fin_ops_nr() { return FinOps{fin_normal<NR>, fin_cg<NR>, fin_true<NR>}; }
+static FinOps fin_ops(int n) {
+  if (n <= 1) return fin_ops_nr<1>();
+  if (n <= 2) return fin_ops_nr<2>();
+  if (n <= 4) return fin_ops_nr<4>();
+  if (n <= 8) return fin_ops_nr<8>();
+  return fin_ops_nr<16>();
+}
 
 // per-rhs slot of the block sums: grid.y = rhs
 __device__ __forceinline__ void rhs_partial(double v, RedScratch rs) {
@@ -414,6 +426,7 @@ int qamd_stag_cg_mixed_block(const qamd_links *l64, const qamd_links *ls, qamd_f
   const double m2 = mass * mass;
   const int nb = (int)((lat->g.vh + 255) / 256);
   const dim3 gb(nb, n);
+  const FinOps fops = fin_ops(n);
   BState st = {nullptr, nullptr};
   qamd_field *r64 = qamd_field_create(lat, QAMD_F64, 3, n, 0);
   qamd_field *Ax = qamd_field_create(lat, QAMD_F64, 3, n, 0);
@@ -442,14 +455,14 @@ int qamd_stag_cg_mixed_block(const qamd_links *l64, const qamd_links *ls, qamd_f
   auto normal = [&](const qamd_links *L, qamd_field *out, qamd_field *vin, qamd_field *vw, qamd_field *tt) -> int {
     if (stag_block_launch(L, tt->f, vin->f, 0.0, nullptr, 1.0, true, red_set(1), stream())) return -1;
     if (stag_block_launch(L, out->f, tt->f, m2, &vw->f, -1.0, true, red_set(2), stream())) return -1;
-    finish_normal(n, nb, st.B, stream());
+    fops.normal(nb, st.B, stream());
     QAMD_HIP(hipGetLastError());
     return 0;
   };
   auto reliable = [&]() -> int {
     if (normal(l64, Ax, x, x, t64)) return -1;
     hipLaunchKernelGGL(k_bru_resid, gb, dim3(256), 0, stream(), r64->f, b->f, Ax->f, r->f, red_set(0));
-    finish_rhs(n, nb, FinBTrue{st.B}, stream());
+    fops.truer(nb, st.B, stream());
     QAMD_HIP(hipGetLastError());
     return pull();
   };
@@ -476,7 +489,7 @@ int qamd_stag_cg_mixed_block(const qamd_links *l64, const qamd_links *ls, qamd_f
       for (int k = 0; k < 16 && !err; k++) {
         err |= normal(ls, Ap, pin, p, t);
         hipLaunchKernelGGL(k_bcg_xr, gb, dim3(256), 0, stream(), xs->f, p->f, r->f, Ap->f, st.B, red_set(0));
-        finish_rhs(n, nb, FinBCG{st.B}, stream());
+        fops.cg(nb, st.B, stream());
         if (!half) hipLaunchKernelGGL(k_bcg_p<0>, gb, dim3(256), 0, stream(), p->f, r->f, p->f, st.B);
         else if (ps == QAMD_F16) hipLaunchKernelGGL(k_bcg_p<QAMD_F16>, gb, dim3(256), 0, stream(), p->f, r->f, ph->f, st.B);
         else hipLaunchKernelGGL(k_bcg_p<QAMD_Q16>, gb, dim3(256), 0, stream(), p->f, r->f, ph->f, st.B);
