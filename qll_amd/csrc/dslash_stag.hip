@@ -61,27 +61,35 @@ __global__ void __launch_bounds__(256) k_stag_dslash(StagArgs<P, PO> A) {
   double red[2] = {0.0, 0.0};
   if (cb < A.g.vh) {
     Site s = site_of(A.g, cb, A.par);
-    const S *lp = A.links + ((int64_t)(cb >> 6) * NT * 9 << 6) + (cb & 63);
+    constexpr int G = LinkG<S>::G;
+    const S *lp = link_base<S, NT>(A.links, cb);
     C acc[3];
 #pragma unroll
     for (int j = 0; j < 3; j++) acc[j].x = acc[j].y = (R)0;
 #pragma unroll
-    for (int t = 0; t < NT; t++) {
-      const int k = (NT == 16) ? t / 2 : t;
-      const int hop = (NT == 16 && (t & 1)) ? 3 : 1;
-      const int mu = k / 2;
-      const int h = (k & 1) ? -hop : hop;
-      int nb = nbr_cb(A.g, s, mu, h);
-      C v[3];
-      VecLd<P>::ld(A.in, A.in_norm, vp, nb, v, (P == QAMD_F16 || P == QAMD_Q16) ? A.lscale[t] : (R)1);
-      C m[9];
+    for (int tg = 0; tg < NT / G; tg++) {
+      S q[9][G];   // element e of terms tg*G .. tg*G+G-1, one 16-byte load each
 #pragma unroll
-      for (int e = 0; e < 9; e++) m[e] = to_c(ntload(lp + (t * 9 + e) * 64), (R)0);
-      // column major: element (r,c) = m[3c+r]
+      for (int e = 0; e < 9; e++) ntload_group(lp + link_off<S>(tg * G, e), q[e]);
 #pragma unroll
-      for (int r = 0; r < 3; r++)
+      for (int gg = 0; gg < G; gg++) {
+        const int t = tg * G + gg;
+        const int k = (NT == 16) ? t / 2 : t;
+        const int hop = (NT == 16 && (t & 1)) ? 3 : 1;
+        const int mu = k / 2;
+        const int h = (k & 1) ? -hop : hop;
+        int nb = nbr_cb(A.g, s, mu, h);
+        C v[3];
+        VecLd<P>::ld(A.in, A.in_norm, vp, nb, v, (P == QAMD_F16 || P == QAMD_Q16) ? A.lscale[t] : (R)1);
+        C m[9];
 #pragma unroll
-        for (int c = 0; c < 3; c++) cmac<C, R>(acc[r], m[3 * c + r], v[c]);
+        for (int e = 0; e < 9; e++) m[e] = to_c(q[e][gg], (R)0);
+        // column major: element (r,c) = m[3c+r]
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+#pragma unroll
+          for (int c = 0; c < 3; c++) cmac<C, R>(acc[r], m[3 * c + r], v[c]);
+      }
     }
     C o[3];
     const R b = (R)A.b;

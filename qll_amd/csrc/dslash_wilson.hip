@@ -84,7 +84,9 @@ __global__ void __launch_bounds__(256) k_wilson(WArgs<P> A) {
   double red[1] = {0.0};
   if (cb < A.g.vh) {
     Site s = site_of(A.g, cb, A.par);
-    const S *lp = A.links + ((int64_t)(cb >> 6) * 8 * 9 << 6) + (cb & 63);
+    constexpr int G = LinkG<S>::G;
+    const S *lp = link_base<S, 8>(A.links, cb);
+    S q[9][G];
     C acc[4][3];
 #pragma unroll
     for (int a = 0; a < 4; a++)
@@ -113,9 +115,14 @@ __global__ void __launch_bounds__(256) k_wilson(WArgs<P> A) {
         h[0][c] = bwd ? cadd(psi[0][c], d0) : csub(psi[0][c], d0);
         h[1][c] = bwd ? cadd(psi[1][c], d1) : csub(psi[1][c], d1);
       }
+      // one 16-byte group load per element serves G consecutive terms
+      if (t % G == 0) {
+#pragma unroll
+        for (int e = 0; e < 9; e++) ntload_group(lp + link_off<S>(t, e), q[e]);
+      }
       C m[9];
 #pragma unroll
-      for (int e = 0; e < 9; e++) m[e] = to_c(ntload(lp + (t * 9 + e) * 64), (R)0);
+      for (int e = 0; e < 9; e++) m[e] = to_c(q[e][t % G], (R)0);
       C uh[2][3];
 #pragma unroll
       for (int a = 0; a < 2; a++)

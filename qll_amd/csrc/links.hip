@@ -36,7 +36,7 @@ __global__ void k_link_import(S *d0, S *d1, const H *src, Geom g, int t, int nte
   }
   S *d = par ? d1 : d0;
   for (int e = 0; e < 9; e++)
-    d[link_idx(cb, t, e, nterm)] =
+    d[link_idx<S>(cb, t, e, nterm)] =
         mk<S>(make_double2((double)src[base + 2 * e * stride], (double)src[base + (2 * e + 1) * stride]));
 }
 
@@ -57,7 +57,7 @@ __global__ void k_link_from_gauge(S *d0, S *d1, const double2 *fat, const double
       int t = (nterm == 16) ? 2 * k + n : k;
       if (!bwd) {
         const double2 *p = src + ((int64_t)(mu * 2 + par) * 9) * vp + cb;
-        for (int e = 0; e < 9; e++) d[link_idx(cb, t, e, nterm)] = mk<S>(p[e * vp]);
+        for (int e = 0; e < 9; e++) d[link_idx<S>(cb, t, e, nterm)] = mk<S>(p[e * vp]);
       } else {
         int hop = n ? 3 : 1;
         int nb = nbr_cb(g, s, mu, -hop);
@@ -65,7 +65,7 @@ __global__ void k_link_from_gauge(S *d0, S *d1, const double2 *fat, const double
         for (int r = 0; r < 3; r++)
           for (int c = 0; c < 3; c++) {
             double2 v = p[(3 * r + c) * vp];   // element (c, r) of the source
-            d[link_idx(cb, t, 3 * c + r, nterm)] = mk<S>(make_double2(sgn * v.x, -sgn * v.y));
+            d[link_idx<S>(cb, t, 3 * c + r, nterm)] = mk<S>(make_double2(sgn * v.x, -sgn * v.y));
           }
       }
     }
@@ -95,14 +95,12 @@ __global__ void k_link_absmax(const S *src, int64_t nblk, int nterm, unsigned *m
   int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
   float m[16];
   for (int t = 0; t < 16; t++) m[t] = 0.f;
-  for (int64_t b = wave; b < nblk; b += nwaves) {
-    const S *p = src + b * nterm * 9 * 64 + lane;
+  for (int64_t b = wave; b < nblk; b += nwaves)
     for (int t = 0; t < nterm; t++)
       for (int e = 0; e < 9; e++) {
-        double2 v = ld2<S>(p, (t * 9 + e) * 64);
+        double2 v = ld2<S>(src, link_idx<S>(b * 64 + lane, t, e, nterm));
         m[t] = fmaxf(m[t], (float)fmax(fabs(v.x), fabs(v.y)));
       }
-  }
   for (int t = 0; t < nterm; t++) {
     float v = m[t];
     for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
@@ -113,9 +111,17 @@ __global__ void k_link_absmax(const S *src, int64_t nblk, int nterm, unsigned *m
 template <class SI, class SO>
 __global__ void k_link_convert(SO *dst, const SI *src, int64_t n, int nterm, const float *iscale,
                                const float *oscale) {
+  constexpr int GO = LinkG<SO>::G;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    int t = (int)((i >> 6) / 9 % nterm);
-    double2 v = ld2<SI>(src, i);
+    // (site, term, element) of destination slot i, then the source slot
+    int64_t q = i / GO;
+    int t = (int)(i % GO), lane = (int)(q & 63), k;
+    q >>= 6;
+    k = (int)(q % 9);
+    q /= 9;
+    t += (int)(q % (nterm / GO)) * GO;
+    int64_t cb = (q / (nterm / GO)) * 64 + lane;
+    double2 v = ld2<SI>(src, link_idx<SI>(cb, t, k, nterm));
     double f = (double)iscale[t] / (double)oscale[t];
     v.x *= f;
     v.y *= f;

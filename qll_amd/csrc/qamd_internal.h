@@ -8,9 +8,12 @@
 //   * colour vectors / spinors are structure-of-arrays: complex component j
 //     of rhs r at site cb lives at [(r*ncomp + j) * vh_pad + cb];  FP16 fields
 //     keep one FP32 max-norm per (rhs, site) beside them;
-//   * link sets are blocked by 64 sites (one wavefront): complex element k of
-//     term t for site cb lives at [((cb>>6)*nterm + t)*9 + k)*64 + (cb&63)],
-//     so a wavefront streams one contiguous block per site group.
+//   * link sets are blocked by 64 sites (one wavefront) and packed so that
+//     one 16-byte load returns element k of G = 16/sizeof(complex) consecutive
+//     terms (G = 1 FP64, 2 FP32, 4 FP16/Q16): element k of term t for site cb
+//     lives at [((((cb>>6)*(nterm/G) + t/G)*9 + k)*64 + (cb&63))*G + t%G],
+//     so a wavefront streams one contiguous block per site group with
+//     full-width loads.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <hip/hip_fp16.h>
@@ -104,12 +107,24 @@ __device__ __forceinline__ int nbr_cb(const Geom &g, const Site &s, int mu, int 
   return (s.lex + (c - s.c[mu]) * g.stride[mu]) >> 1;
 }
 
+// link packing: terms per 16-byte load for stored complex type S
+template <class S> struct LinkG { static constexpr int G = (int)(16 / sizeof(S)); };
+
 // blocked link index: element k (0..8) of term t for site cb
-__device__ __forceinline__ int64_t link_idx(int cb, int t, int k, int nterm) {
-  return ((((int64_t)(cb >> 6) * nterm + t) * 9 + k) << 6) + (cb & 63);
+template <class S>
+__host__ __device__ __forceinline__ int64_t link_idx(int64_t cb, int t, int k, int nterm) {
+  constexpr int G = LinkG<S>::G;
+  return (((((cb >> 6) * (nterm / G) + t / G) * 9 + k) << 6) + (cb & 63)) * G + t % G;
 }
-__host__ inline int64_t link_idx_h(int64_t cb, int t, int k, int nterm) {
-  return ((((cb >> 6) * nterm + t) * 9 + k) << 6) + (cb & 63);
+// first element of site cb's link stream, and the offset of (t, k) from it
+template <class S, int NT>
+__device__ __forceinline__ const S *link_base(const S *links, int cb) {
+  return links + (int64_t)(cb >> 6) * NT * 9 * 64 + (int64_t)(cb & 63) * LinkG<S>::G;
+}
+template <class S>
+__device__ __forceinline__ int link_off(int t, int k) {
+  constexpr int G = LinkG<S>::G;
+  return ((t / G) * 9 + k) * 64 * G + t % G;
 }
 
 // XCD-aware block order: block b runs on die b % 8; give each die one
@@ -175,6 +190,29 @@ __device__ __forceinline__ short2 ntload(const short2 *p) {
 __device__ __forceinline__ __half2 ntload(const __half2 *p) {
   unsigned u = __builtin_nontemporal_load(reinterpret_cast<const unsigned *>(p));
   return *reinterpret_cast<__half2 *>(&u);
+}
+
+// one 16-byte non-temporal load: element k of the G terms of a link group
+typedef float fvec4_t __attribute__((ext_vector_type(4)));
+typedef unsigned uvec4_t __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void ntload_group(const double2 *p, double2 (&o)[1]) { o[0] = ntload(p); }
+__device__ __forceinline__ void ntload_group(const float2 *p, float2 (&o)[2]) {
+  fvec4_t v = __builtin_nontemporal_load(reinterpret_cast<const fvec4_t *>(p));
+  o[0] = make_float2(v.x, v.y);
+  o[1] = make_float2(v.z, v.w);
+}
+__device__ __forceinline__ void ntload_group(const short2 *p, short2 (&o)[4]) {
+  uvec4_t v = __builtin_nontemporal_load(reinterpret_cast<const uvec4_t *>(p));
+#pragma unroll
+  for (int i = 0; i < 4; i++) o[i] = make_short2((short)(v[i] & 0xffff), (short)(v[i] >> 16));
+}
+__device__ __forceinline__ void ntload_group(const __half2 *p, __half2 (&o)[4]) {
+  uvec4_t v = __builtin_nontemporal_load(reinterpret_cast<const uvec4_t *>(p));
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    unsigned u = v[i];
+    o[i] = *reinterpret_cast<__half2 *>(&u);
+  }
 }
 
 // ---------------------------------------------------------------- fields

@@ -48,8 +48,10 @@ __global__ void __launch_bounds__(256) k_stag_block(MArgs<P, PO> A) {
   for (int i = 0; i < 2 * NR; i++) red[i] = 0.0;
   if (cb < A.g.vh) {
     Site s = site_of(A.g, cb, A.par);
-    const S *lp = A.links + ((int64_t)(cb >> 6) * NT * 9 << 6) + (cb & 63);
+    constexpr int G = LinkG<S>::G;
+    const S *lp = link_base<S, NT>(A.links, cb);
     C acc[NR][3];
+    S q[9][G];
 #pragma unroll
     for (int r = 0; r < NR; r++)
 #pragma unroll
@@ -61,9 +63,14 @@ __global__ void __launch_bounds__(256) k_stag_block(MArgs<P, PO> A) {
       const int mu = k / 2;
       const int h = (k & 1) ? -hop : hop;
       int nb = nbr_cb(A.g, s, mu, h);
+      // one 16-byte group load per element serves G consecutive terms
+      if (t % G == 0) {
+#pragma unroll
+        for (int e = 0; e < 9; e++) ntload_group(lp + link_off<S>(t, e), q[e]);
+      }
       C m[9];
 #pragma unroll
-      for (int e = 0; e < 9; e++) m[e] = to_c(ntload(lp + (t * 9 + e) * 64), (R)0);
+      for (int e = 0; e < 9; e++) m[e] = to_c(q[e][t % G], (R)0);
 #pragma unroll
       for (int r = 0; r < NR; r++) {
         if (r < nr) {
