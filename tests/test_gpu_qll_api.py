@@ -39,3 +39,39 @@ def test_qll_api_driver(prec, vlen, dims):
     print(r.stderr[-2000:])
     assert r.returncode == 0, r.stdout[-3000:]
     assert "PASSED" in r.stdout
+
+
+TD_SRC = os.path.join(ROOT, "tests", "qll_testdslash.c")
+
+
+def _testdslash(prec):
+    """qll_testdslash.c built against libqll_amd and the oracle's qop* library
+    of one precision (-DDOUBLE for D), as QLL builds its tests."""
+    b = os.path.join(ROOT, "tests", "qll_testdslash_%s.bin" % prec)
+    if not os.path.exists(b) or os.path.getmtime(b) < os.path.getmtime(TD_SRC):
+        subprocess.run(["gcc", "-O1", "-fopenmp", "-std=gnu11"] + (["-DDOUBLE"] if prec == "D" else []) +
+                       ["-I", os.path.join(ROOT, "include"), "-I", os.path.join(ROOT, "oracle"), "-o", b, TD_SRC,
+                        "-L", os.path.join(ROOT, "qll_amd", "lib"), "-lqll_amd",
+                        "-L", os.path.join(ROOT, "oracle"), "-lqop" + prec, "-loracle",
+                        "-Wl,-rpath," + os.path.join(ROOT, "qll_amd", "lib"),
+                        "-Wl,-rpath," + os.path.join(ROOT, "oracle"), "-lm"], check=True)
+    return b
+
+
+@pytest.mark.parametrize("prec,vlen,dims", [
+    ("D", 1, (4, 4, 4, 8)),
+    ("D", 4, (4, 4, 8, 8)),
+    ("F", 1, (4, 6, 4, 8)),
+    ("F", 8, (4, 4, 8, 8)),
+])
+def test_qll_testdslash_vs_qop(prec, vlen, dims):
+    """QLL's testdslash/testwilson flow: every API result against the qop*
+    entry point the reference's tests compare with (here the FP64 oracle)."""
+    print("case: qll_testdslash %s vlen=%d dims=%s" % (prec, vlen, dims), flush=True)
+    env = dict(os.environ, OMP_NUM_THREADS="4")
+    r = subprocess.run([_testdslash(prec), str(vlen)] + [str(d) for d in dims], capture_output=True, text=True,
+                       timeout=300, env=env)
+    print(r.stdout[-8000:])
+    print(r.stderr[-2000:])
+    assert r.returncode == 0, r.stdout[-4000:]
+    assert "PASSED" in r.stdout

@@ -6,6 +6,7 @@ invariants and dense brute force on small lattices, each chosen so a plausible
 mistake (wrong hop, sign, adjoint, index order, missing term) fails.
 """
 import itertools
+import os
 
 import numpy as np
 import pytest
@@ -424,3 +425,21 @@ def test_layout_vlen1_is_checkerboarded_lexicographic():
         lx = lex(dims, x)
         want = lx // 2 + (O.volume(dims) // 2 if sum(x) % 2 else 0)
         assert O.layout_index(lo, list(x)) == want
+
+
+# ------------------------------------------------------------ qop* layer
+def test_qop_libraries_export_every_declared_entry_point():
+    """oracle/qop.h declares QLL's qop* test entry points (include/common.h:66-86);
+    both per-precision builds export all of them."""
+    import re
+    import subprocess
+    here = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle")
+    O.build()
+    txt = re.sub(r"/\*.*?\*/", "", open(os.path.join(here, "qop.h")).read(), flags=re.S)
+    declared = set(re.findall(r"\b(qop\w+)\s*\(", txt))
+    assert len(declared) == 17   # 9 staggered + 8 Wilson
+    for prec in "DF":
+        out = subprocess.run(["nm", "-D", "--defined-only", os.path.join(here, "libqop%s.so" % prec)],
+                             capture_output=True, text=True, check=True).stdout
+        exported = set(re.findall(r"\bT (qop\w+)", out))
+        assert declared <= exported, declared - exported
