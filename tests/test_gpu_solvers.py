@@ -243,3 +243,143 @@ def test_mixed_cgms_all_shifts(gpu):
         meff = np.sqrt(mass ** 2 + s)
         assert infos[i]["converged"]
         assert _true_rel_resid(dims, links, meff, b, xs[i].get_lex()) <= 1.01e-20
+
+
+# ---------------------------------------------------------------------------
+# The reference configurations.  Oracle-sized volumes are checked against the
+# oracle; the full sizes (no oracle in reach) by FP64 true residuals computed
+# with the device's FP64 operator, itself oracle-checked above.
+
+def test_config0_fp64_cg_8x8x8x8(gpu):
+    """8^4, seeded SU(3) + fat7/Naik, Gaussian even source, QLL mass 0.2
+    (m = 0.1), |r|/|b| = 1e-8: FP64 CG with QLL's recurrences gives the
+    oracle's iteration count and solution."""
+    dims = (8, 8, 8, 8)
+    links = O.asqtad_links(dims, 100)
+    b = O.gauss_vector(dims, 101, 100, parity=0)
+    xr, its_r, _ = O.cg(dims, links, 0.2, b, 1e-16, 5000, noscale=True)
+    lat = gpu.Lattice(dims)
+    L = gpu.asqtad_links_device(lat, 100)
+    bf = gpu.Field(lat, 0, 3, 1, 0).set_lex(b)
+    x = gpu.Field(lat, 0, 3, 1, 0)
+    info = gpu.stag_cg(L, x, bf, 0.2, 1e-16, 5000)
+    assert info["converged"] and abs(info["iterations"] - its_r) <= 1, (info, its_r)
+    xs = x.get_lex()
+    assert rel_err(xs, xr, parity_mask(dims, 0)) < 1e-9
+    assert _true_rel_resid(dims, links, 0.2, b, xs) < 1.1e-16
+
+
+def _c2_shifts():
+    """12 shifts log-spaced in [1e-4, 1] on top of m^2 = 0.005^2 (QLL mass
+    0.01); sigma_0 is folded into the base mass, QLL's cgms taking shifts[0]
+    as 0 (lib/cgms.c:6-179)."""
+    sig = np.logspace(-4, 0, 12)
+    m0 = np.sqrt(0.01 ** 2 + sig[0])
+    return sig, m0, list(sig - sig[0])
+
+
+def test_config2_cgms_12_shifts_oracle_volume(gpu):
+    """Config 2 at 8^4: FP32 multishift + FP64 refinement; every shift reaches
+    FP64 true |r|/|b| = 1e-10 with the oracle's operator and the oracle's
+    FP64 multishift solution."""
+    dims = (8, 8, 8, 8)
+    links, b = _solver_case(dims, 20)
+    sig, m0, shifts = _c2_shifts()
+    lat = gpu.Lattice(dims)
+    L64 = gpu.Links(lat, gpu.STAG, 0, 16).set_lex(links)
+    bf = gpu.Field(lat, 0, 3, 1, 0).set_lex(b)
+    xs = [gpu.Field(lat, 0, 3, 1, 0) for _ in shifts]
+    infos = gpu.stag_cgms_mixed(L64, L64.converted(1), xs, bf, m0, shifts, 1e-20, 50000)
+    masses = [np.sqrt(m0 ** 2 + s) for s in shifts]
+    xr, _ = O.cgms(dims, links, masses, b, 1e-24, 50000, noscale=True)
+    m = parity_mask(dims, 0)
+    for i in range(len(shifts)):
+        assert infos[i]["converged"], (i, infos[i])
+        x = xs[i].get_lex()
+        assert _true_rel_resid(dims, links, masses[i], b, x) <= 1.01e-20, i
+        assert rel_err(x, xr[i], m) < 1e-8, i
+
+
+@pytest.mark.parametrize("sloppy", [1, 2, 3])
+def test_config3_wilson_16rhs_oracle_volume(gpu, sloppy):
+    """Config 3 at 4^3x8: 16 Gaussian spinor sources, kappa = 0.124, sloppy
+    FP32/FP16/Q16 Dslash sharing link loads across the 16 rhs, FP64 defect
+    correction; every rhs at FP64 true |r|/|b| = 1e-10 (oracle's Mhat)."""
+    dims = (4, 4, 4, 8)
+    links = O.wilson_doublestore(dims, O.random_su3_field(dims, 43))
+    bs = np.stack([O.gauss_vector(dims, 200 + r, 400, parity=0, nreal=24) for r in range(16)])
+    lat = gpu.Lattice(dims)
+    L64 = gpu.Links(lat, gpu.WILSON, 0, 8).set_lex(links)
+    b = gpu.Field(lat, 0, 12, 16, 0).set_lex(bs)
+    x = gpu.Field(lat, 0, 12, 16, 0)
+    infos = gpu.wilson_cgne(L64, x, b, 0.124, 1e-20, 5000, ls=L64.converted(sloppy))
+    xs = x.get_lex().reshape(16, -1, 24)
+    m = parity_mask(dims, 0)
+    for r in range(16):
+        assert infos[r]["converged"], (r, infos[r])
+        res = np.sum((bs[r] - O.wilson_meo(dims, links, 0.124, xs[r]))[m] ** 2) / np.sum(bs[r][m] ** 2)
+        assert res <= 1.01e-20, (r, res)
+
+
+def _device_true_rsq(gpu, L64, x, b, mass):
+    lat = L64.lat
+    Ap = gpu.Field(lat, 0, 3, x.nrhs, 0)
+    t = gpu.Field(lat, 0, 3, x.nrhs, 1)
+    gpu.stag_normal(L64, Ap, x, t, mass)
+    Ap.axpby(1.0, b, -1.0)
+    r2, b2 = Ap.norm2(), b.norm2()
+    if x.nrhs == 1:
+        return r2 / b2
+    return max(r2[i] / b2[i] for i in range(x.nrhs))
+
+
+def test_config1_full_32x32x32x32_mixed_vs_fp64(gpu):
+    """Config 1 at full size: FP64 CG and the 16-bit mixed CG both reach FP64
+    true |r|/|b| <= 1e-10 and agree on the solution."""
+    dims = (32, 32, 32, 32)
+    print("case: config1 full 32^4", flush=True)
+    lat = gpu.Lattice(dims)
+    L64 = gpu.asqtad_links_device(lat, 1)
+    LQ = L64.converted(gpu.Q16)
+    b = gpu.Field(lat, 0, 3, 1, 0).gaussian(11, 100)
+    x64 = gpu.Field(lat, 0, 3, 1, 0)
+    xm = gpu.Field(lat, 0, 3, 1, 0)
+    assert gpu.stag_cg(L64, x64, b, 0.02, 1e-20, 50000)["converged"]
+    assert gpu.stag_cg_mixed(L64, LQ, xm, b, 0.02, 1e-20, 50000)["converged"]
+    assert _device_true_rsq(gpu, L64, xm, b, 0.02) <= 1.01e-20
+    assert _device_true_rsq(gpu, L64, x64, b, 0.02) <= 1.01e-20
+    d = gpu.Field(lat, 0, 3, 1, 0).copy_from(xm)
+    d.axpby(-1.0, x64, 1.0)
+    assert (d.norm2() / x64.norm2()) ** 0.5 < 1e-6
+
+
+def test_config2_full_32x32x32x64_cgms(gpu):
+    """Config 2 at full size: 12 shifts, FP32 inner multishift + FP64
+    refinement; every shift at FP64 true |r|/|b| <= 1e-10."""
+    dims = (32, 32, 32, 64)
+    print("case: config2 full 32^3x64", flush=True)
+    sig, m0, shifts = _c2_shifts()
+    lat = gpu.Lattice(dims)
+    L64 = gpu.asqtad_links_device(lat, 2)
+    b = gpu.Field(lat, 0, 3, 1, 0).gaussian(12, 100)
+    xs = [gpu.Field(lat, 0, 3, 1, 0) for _ in shifts]
+    infos = gpu.stag_cgms_mixed(L64, L64.converted(gpu.F32), xs, b, m0, shifts, 1e-20, 50000)
+    for i, s in enumerate(shifts):
+        assert infos[i]["converged"], (i, infos[i])
+        assert _device_true_rsq(gpu, L64, xs[i], b, float(np.sqrt(m0 ** 2 + s))) <= 1.01e-20, i
+
+
+def test_config3_full_32x32x32x64_wilson_16rhs(gpu):
+    """Config 3 at full size: 16 rhs, kappa 0.124, FP32 inner; every rhs at
+    FP64 true |r|/|b| <= 1e-10 (the solver's own FP64 check, whose operator
+    the oracle pins at small volume)."""
+    dims = (32, 32, 32, 64)
+    print("case: config3 full 32^3x64 x 16 rhs", flush=True)
+    lat = gpu.Lattice(dims)
+    g = gpu.Gauge(lat).random_su3(3)
+    L64 = gpu.Links(lat, gpu.WILSON, 0, 8).from_gauge(g)
+    b = gpu.Field(lat, 0, 12, 16, 0).gaussian(13, 100)
+    x = gpu.Field(lat, 0, 12, 16, 0)
+    infos = gpu.wilson_cgne(L64, x, b, 0.124, 1e-20, 5000, ls=L64.converted(gpu.F32))
+    for r in range(16):
+        assert infos[r]["converged"] and infos[r]["true_rsq"] <= 1.01e-20, (r, infos[r])
