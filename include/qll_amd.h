@@ -177,6 +177,26 @@ int qamd_stag_cg_mixed(const qamd_links *l64, const qamd_links *ls, qamd_field *
 int qamd_stag_cg_mixed_block(const qamd_links *l64, const qamd_links *ls, qamd_field *x,
                              const qamd_field *b, double mass, double rsq, int maxits,
                              double delta, qamd_solve_info *info);
+/* One problem of a heterogeneous batch: its own lattice, gauge field
+ * (links), mass and block of 1..8 sources. */
+typedef struct {
+  const qamd_links *l64;   /* FP64 16-term asqtad links of the problem's lattice */
+  const qamd_links *ls;    /* sloppy links (FP32, FP16 or Q16; one precision for all) */
+  qamd_field *x;           /* even FP64 colour-vector block (solutions)          */
+  const qamd_field *b;     /* even FP64 sources, same nrhs as x                  */
+  double mass;             /* solves (mass^2 - D_eo D_oe) x = b                 */
+} qamd_stag_problem;
+/* Heterogeneous batch of independent mixed-precision CG solves (no reference
+ * counterpart; the batched form of calling solve2, lib/stagDslash.c:594-612,
+ * once per problem and source): up to 64 problems on lattices of any size
+ * advance together, each iteration one fused launch per kernel over all
+ * problems (problems grouped by rhs count), FP64 reliable updates per
+ * problem, each rhs leaving the iteration when its FP64 true residual
+ * |b - A x|^2 <= rsq |b|^2.  use_graph != 0 captures the 16 iterations
+ * between host checks in a hipGraph once and replays it.  info: one entry per
+ * rhs, problems in order. */
+int qamd_stag_cg_mixed_batch(const qamd_stag_problem *probs, int nprob, double rsq, int maxits,
+                             double delta, int use_graph, qamd_solve_info *info);
 /* QLL multishift CG semantics (cgmsSolveT lib/cgms.c:6-179): shifts[0] is
  * taken as 0, stop on the base residual, vectors at the links' precision. */
 int qamd_stag_cgms(const qamd_links *l, qamd_field *const x[], const qamd_field *b,

@@ -95,6 +95,8 @@ def lib():
             "qamd_stag_cg": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_double, C.c_int, C.POINTER(SolveInfo)]),
             "qamd_stag_cg_mixed": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_double, C.c_int, C.c_double,
                                              C.POINTER(SolveInfo)]),
+            "qamd_stag_cg_mixed_batch": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_double, C.c_int,
+                                                   C.POINTER(SolveInfo)]),
             "qamd_stag_cg_mixed_block": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_double, C.c_int, C.c_double,
                                              C.POINTER(SolveInfo)]),
             "qamd_stag_cgms": (C.c_int, [_vp, C.POINTER(_vp), _vp, C.c_double, _d, C.c_int, C.c_double, C.c_int,
@@ -301,6 +303,21 @@ def stag_cg_mixed_block(l64, ls, x, b, mass, rsq, maxits, delta=0.1):
     """Solve every rhs of the even block b together; one info dict per rhs."""
     infos = (SolveInfo * b.nrhs)()
     _check(lib().qamd_stag_cg_mixed_block(l64.h, ls.h, x.h, b.h, mass, rsq, maxits, delta, infos))
+    return [i.as_dict() for i in infos]
+
+
+class _Problem(C.Structure):
+    _fields_ = [("l64", _vp), ("ls", _vp), ("x", _vp), ("b", _vp), ("mass", C.c_double)]
+
+
+def stag_cg_mixed_batch(problems, rsq, maxits, delta=0.1, use_graph=True):
+    """problems: list of (l64, ls, x, b, mass).  Solves them together; returns
+    one info dict per rhs, problems in order."""
+    arr = (_Problem * len(problems))(*[_Problem(p[0].h, p[1].h, p[2].h, p[3].h, p[4]) for p in problems])
+    n = sum(p[3].nrhs for p in problems)
+    infos = (SolveInfo * n)()
+    _check(lib().qamd_stag_cg_mixed_batch(C.cast(arr, C.c_void_p), len(problems), rsq, maxits, delta,
+                                          1 if use_graph else 0, infos))
     return [i.as_dict() for i in infos]
 
 

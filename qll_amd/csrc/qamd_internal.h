@@ -282,13 +282,18 @@ __device__ __forceinline__ void block_sum(double (&v)[N]) {
   __syncthreads();
 }
 
-// block sums of v into the partial-sum set; every block calls it once
+// block sums of v into slot idx of the partial-sum set
 template <int N>
-__device__ __forceinline__ void grid_partial(double (&v)[N], RedScratch rs) {
+__device__ __forceinline__ void grid_partial_at(double (&v)[N], RedScratch rs, int idx) {
   block_sum(v);
   if (threadIdx.x == 0)
 #pragma unroll
-    for (int i = 0; i < N; i++) rs.part[i * rs.max_blocks + blockIdx.x] = v[i];
+    for (int i = 0; i < N; i++) rs.part[i * rs.max_blocks + idx] = v[i];
+}
+// block sums of v into the partial-sum set; every block calls it once
+template <int N>
+__device__ __forceinline__ void grid_partial(double (&v)[N], RedScratch rs) {
+  grid_partial_at(v, rs, blockIdx.x);
 }
 
 // fixed-order total of N slots over nb blocks: thread k adds blocks k,

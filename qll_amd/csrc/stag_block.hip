@@ -6,6 +6,7 @@
 #include "qamd_objects.h"
 #include "qamd_kernels.h"
 #include "qamd_vec3.h"
+#include "qamd_block.h"
 
 #include <stdlib.h>
 #include <string.h>
@@ -16,102 +17,15 @@ namespace qamd {
 
 constexpr int MAXB = 16;
 
-template <int P, int PO> struct MArgs {
-  typedef typename Prec<P>::S S;
-  typedef typename Prec<PO>::S SO;
-  Geom g;
-  const S *links;
-  const S *in;
-  const float *in_norm;
-  SO *out;
-  float *out_norm;
-  const SO *w;
-  const float *w_norm;
-  double a, b;
-  int par, nr;
-  float lscale[16];
-  RedScratch rs;
-};
-
 // out_r = a*w_r + b * sum_t L_t in_r(x+d_t) for r < nr; fields rhs-outer
 template <int P, int PO, int NT, int NR, bool RED>
 __global__ void __launch_bounds__(256) k_stag_block(MArgs<P, PO> A) {
-  typedef typename Prec<P>::R R;
-  typedef typename Prec<P>::C C;
-  typedef typename Prec<P>::S S;
-  const int64_t vp = A.g.vh_pad;
-  const int nr = A.nr;
   int blk = xcd_block(blockIdx.x, gridDim.x);
   int cb = blk * 256 + threadIdx.x;
   double red[2 * NR];
 #pragma unroll
   for (int i = 0; i < 2 * NR; i++) red[i] = 0.0;
-  if (cb < A.g.vh) {
-    Site s = site_of(A.g, cb, A.par);
-    constexpr int G = LinkG<S>::G;
-    const S *lp = link_base<S, NT>(A.links, cb);
-    C acc[NR][3];
-    S q[9][G];
-#pragma unroll
-    for (int r = 0; r < NR; r++)
-#pragma unroll
-      for (int j = 0; j < 3; j++) acc[r][j].x = acc[r][j].y = (R)0;
-#pragma unroll
-    for (int t = 0; t < NT; t++) {
-      const int k = (NT == 16) ? t / 2 : t;
-      const int hop = (NT == 16 && (t & 1)) ? 3 : 1;
-      const int mu = k / 2;
-      const int h = (k & 1) ? -hop : hop;
-      int nb = nbr_cb(A.g, s, mu, h);
-      // one 16-byte group load per element serves G consecutive terms
-      if (t % G == 0) {
-#pragma unroll
-        for (int e = 0; e < 9; e++) ntload_group(lp + link_off<S>(t, e), q[e]);
-      }
-      C m[9];
-#pragma unroll
-      for (int e = 0; e < 9; e++) m[e] = to_c(q[e][t % G], (R)0);
-#pragma unroll
-      for (int r = 0; r < NR; r++) {
-        if (r < nr) {
-          C v[3];
-          VecLd<P>::ld(A.in + (int64_t)r * 3 * vp, A.in_norm ? A.in_norm + (int64_t)r * vp : nullptr, vp, nb, v,
-                       (P == QAMD_F16 || P == QAMD_Q16) ? A.lscale[t] : (R)1);
-#pragma unroll
-          for (int rr = 0; rr < 3; rr++)
-#pragma unroll
-            for (int c = 0; c < 3; c++) cmac<C, R>(acc[r][rr], m[3 * c + rr], v[c]);
-        }
-      }
-    }
-    const R b = (R)A.b, a = (R)A.a;
-#pragma unroll
-    for (int r = 0; r < NR; r++) {
-      if (r < nr) {
-        C o[3];
-        if (A.w) {
-          C w[3];
-          VecLd<PO>::ld(A.w + (int64_t)r * 3 * vp, A.w_norm ? A.w_norm + (int64_t)r * vp : nullptr, vp, cb, w, (R)1);
-#pragma unroll
-          for (int j = 0; j < 3; j++) {
-            o[j].x = fma(a, w[j].x, b * acc[r][j].x);
-            o[j].y = fma(a, w[j].y, b * acc[r][j].y);
-            if (RED) red[2 * r + 1] += (double)w[j].x * w[j].x + (double)w[j].y * w[j].y;
-          }
-        } else {
-#pragma unroll
-          for (int j = 0; j < 3; j++) {
-            o[j].x = b * acc[r][j].x;
-            o[j].y = b * acc[r][j].y;
-          }
-        }
-        if (RED)
-#pragma unroll
-          for (int j = 0; j < 3; j++) red[2 * r] += (double)o[j].x * o[j].x + (double)o[j].y * o[j].y;
-        VecSt<PO>::st(A.out + (int64_t)r * 3 * vp, A.out_norm ? A.out_norm + (int64_t)r * vp : nullptr, vp, cb, o);
-      }
-    }
-  }
+  if (cb < A.g.vh) block_site<P, PO, NT, NR, RED>(A, cb, red);
   if (RED) grid_partial<2 * NR>(red, A.rs);
 }
 
