@@ -46,6 +46,8 @@ def parse():
     ap.add_argument("--no-cg", action="store_true", help="skip the solver comparison")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-pmc", action="store_true", help="skip the rocprofv3 HBM-traffic passes")
+    ap.add_argument("--config4", action="store_true",
+                    help="also measure the 48-solve heterogeneous batch against looping FP64 CG (~1 min)")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--block", type=int, default=8, help="sources in the batched mixed-precision solve")
     ap.add_argument("--dims", type=str, default=None, help="override lattice, e.g. 16,16,16,16")
@@ -112,6 +114,61 @@ def pmc_traffic(dims):
     return {"bytes": f["dslash"] * 1024.0 * ff + w["dslash"] * 1024.0 * fw,
             "fetch_kib": f["dslash"], "write_kib": w["dslash"],
             "fetch_scale": ff, "write_scale": fw}
+
+
+C4_VOLUMES = [(12, 12, 12, 12), (16, 16, 16, 16), (16, 16, 16, 32), (20, 20, 20, 20), (24, 24, 24, 24),
+              (24, 24, 24, 48)]
+
+
+def config4(Q):
+    """48 independent asqtad solves, 8 on each of six volumes, each with its
+    own seeded links, 1-8 Gaussian rhs and m log-uniform in [0.005, 0.1]
+    (QLL mass 2m): the fused batch (Q16 Dslash, FP64 reliable updates,
+    hipGraph-replayed and eager) against looping plain FP64 CG over every rhs
+    on the same GPU.  A solve is one rhs reaching FP64 |r|/|b| <= 1e-10."""
+    import numpy as np
+    rng = np.random.default_rng(2024)
+    probs, meta = [], []
+    t0 = time.time()
+    seed = 500
+    for dims in C4_VOLUMES:
+        lat = Q.Lattice(dims)
+        for _ in range(8):
+            seed += 1
+            nrhs = int(rng.integers(1, 9))
+            m = float(np.exp(rng.uniform(np.log(0.005), np.log(0.1))))
+            L64 = Q.asqtad_links_device(lat, seed)
+            b = Q.Field(lat, Q.F64, 3, nrhs, Q.EVEN).gaussian(seed, 100)
+            probs.append((L64, L64.converted(Q.Q16), Q.Field(lat, Q.F64, 3, nrhs, Q.EVEN), b, 2 * m))
+            meta.append((lat, seed, nrhs))
+    Q.sync()
+    setup = time.time() - t0
+    nsolve = sum(p[3].nrhs for p in probs)
+    out = {"problems": len(probs), "solves": nsolve, "setup_s": round(setup, 2)}
+    for name, graph in (("batch_graph", True), ("batch_eager", False)):
+        Q.sync()
+        t0 = time.time()
+        infos = Q.stag_cg_mixed_batch(probs, RSQ, 50000, use_graph=graph)
+        Q.sync()
+        dt = time.time() - t0
+        out[name] = {"seconds": round(dt, 3), "solves_per_s": round(nsolve / dt, 2),
+                     "all_converged": all(i["converged"] for i in infos),
+                     "max_true_rel_resid": max(i["true_rsq"] for i in infos) ** 0.5}
+        print("config4 %s: %s" % (name, out[name]), file=sys.stderr, flush=True)
+    loop = 0.0
+    for (L64, _, _, _, mass), (lat, sd, nrhs) in zip(probs, meta):
+        for r in range(nrhs):
+            b1 = Q.Field(lat, Q.F64, 3, 1, Q.EVEN).gaussian(sd, 100 + r)   # rhs r of the block
+            x1 = Q.Field(lat, Q.F64, 3, 1, Q.EVEN)
+            Q.sync()
+            t0 = time.time()
+            Q.stag_cg(L64, x1, b1, mass, RSQ, 50000)
+            Q.sync()
+            loop += time.time() - t0
+    out["fp64_cg_loop"] = {"seconds": round(loop, 3), "solves_per_s": round(nsolve / loop, 2)}
+    out["speedup_graph_vs_fp64_loop"] = round(loop / out["batch_graph"]["seconds"], 2)
+    print("config4 fp64 loop: %s" % out["fp64_cg_loop"], file=sys.stderr, flush=True)
+    return out
 
 
 def cpu_baseline(dims, nsec_target=10.0):
@@ -284,6 +341,8 @@ def main():
                             "speedup_vs_fp64": round(sps / solver["fp64_cg"]["solves_per_s"], 3)}
             print("%s: %s" % (name, solver[name]), file=sys.stderr, flush=True)
 
+    c4 = config4(Q) if args.config4 else None
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(dims)
@@ -315,6 +374,7 @@ def main():
         "cpu_baseline": cpu,
         "dslash": {(PREC_NAME[p] if p in PREC_NAME else p): prof[p] for p in prof},
         "solvers": solver,
+        "config4_batch": c4,
         "device": Q.device_name(),
     }
     print(json.dumps(line), flush=True)

@@ -166,6 +166,12 @@ int qamd_stag_cg(const qamd_links *l, qamd_field *x, const qamd_field *b, double
 int qamd_stag_cg_mixed(const qamd_links *l64, const qamd_links *ls, qamd_field *x,
                        const qamd_field *b, double mass, double rsq, int maxits,
                        double delta, qamd_solve_info *info);
+/* Absolute error of the sloppy normal operator the mixed solvers apply
+ * (sloppy input copy, sloppy odd temporary, FP32 output), |(A_s - A) v|/|v|
+ * for a Gaussian v against A = m^2 - D_eo D_oe in FP64 (no reference
+ * counterpart).  The mixed solvers measure it at set-up and use FP32 links
+ * in place of 16-bit ones when it is not small against m^2. */
+int qamd_stag_sloppy_error(const qamd_links *l64, const qamd_links *ls, double mass, double *eps);
 /* Batched mixed-precision CG (no reference counterpart; the batched
  * counterpart of calling solve2, lib/stagDslash.c:594-612, once per source):
  * the nrhs (1..8) sources of the even FP64 colour-vector block b are solved

@@ -95,6 +95,7 @@ def lib():
             "qamd_stag_cg": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_double, C.c_int, C.POINTER(SolveInfo)]),
             "qamd_stag_cg_mixed": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_double, C.c_int, C.c_double,
                                              C.POINTER(SolveInfo)]),
+            "qamd_stag_sloppy_error": (C.c_int, [_vp, _vp, C.c_double, C.POINTER(C.c_double)]),
             "qamd_stag_cg_mixed_batch": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_double, C.c_int,
                                                    C.POINTER(SolveInfo)]),
             "qamd_stag_cg_mixed_block": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_double, C.c_int, C.c_double,
@@ -304,6 +305,13 @@ def stag_cg_mixed_block(l64, ls, x, b, mass, rsq, maxits, delta=0.1):
     infos = (SolveInfo * b.nrhs)()
     _check(lib().qamd_stag_cg_mixed_block(l64.h, ls.h, x.h, b.h, mass, rsq, maxits, delta, infos))
     return [i.as_dict() for i in infos]
+
+
+def stag_sloppy_error(l64, ls, mass):
+    """|(A_sloppy - A) v| / |v| of the mixed solvers' sloppy normal operator."""
+    e = C.c_double()
+    _check(lib().qamd_stag_sloppy_error(l64.h, ls.h, mass, C.byref(e)))
+    return e.value
 
 
 class _Problem(C.Structure):

@@ -182,6 +182,7 @@ __global__ void __launch_bounds__(256) k_bat_xr(ProbDev *pd, Plan all, RedScratc
 template <int SH>
 __device__ __forceinline__ void store_shadow(const ProbDev &D, int r, int i, const float2 (&v)[3]) {
   if constexpr (SH != 0) {
+    if (!D.ph) return;   // a problem running on FP32 links reads p itself
     typedef typename Prec<SH>::S S;
     VecSt<SH>::st((S *)D.ph + (int64_t)r * 3 * D.g.vh_pad, D.ph_n + (int64_t)r * D.g.vh_pad, D.g.vh_pad, i, v);
   }
@@ -324,6 +325,7 @@ static MArgs<P, PO> margs(const qamd_links *l, int par, const void *in, const fl
 
 struct Work {   // per problem, library-owned work fields
   qamd_field *r64, *Ax, *t64, *r, *p, *Ap, *xs, *t, *ph;
+  qamd_links *l32;   // FP32 links when the 16-bit ones are too coarse for its mass
 };
 
 // one batch solve with sloppy links of precision PS (16 terms)
@@ -331,10 +333,11 @@ template <int PS>
 struct Runner {
   static constexpr bool HALF = (PS == QAMD_F16 || PS == QAMD_Q16);
   static constexpr int SH = HALF ? PS : 0;
-  MArgs<PS, PS> *s1 = nullptr;
+  MArgs<PS, PS> *s1 = nullptr;          // group 0: the common sloppy links
   MArgs<PS, QAMD_F32> *s2 = nullptr;
+  MArgs<QAMD_F32, QAMD_F32> *f1 = nullptr, *f2 = nullptr;   // group 1: FP32 fallback
   MArgs<QAMD_F64, QAMD_F64> *r1 = nullptr, *r2 = nullptr;
-  Plan cls[NCLS], all;
+  Plan cls[2][NCLS], all;                // [group][rhs class]
   ProbDev *pd = nullptr;
   int nprob = 0;
 
@@ -344,15 +347,17 @@ struct Runner {
       hipLaunchKernelGGL((k_bat_block<P, PO, 16, NR, RED>), dim3(pl.nblocks), dim3(256), 0, st, a, pl, pd, gate);
   }
   template <int P, int PO, bool RED>
-  void bl_all(const MArgs<P, PO> *a, int gate, hipStream_t st) {
-    bl<P, PO, 1, RED>(a, cls[0], pd, gate, st);
-    bl<P, PO, 2, RED>(a, cls[1], pd, gate, st);
-    bl<P, PO, 4, RED>(a, cls[2], pd, gate, st);
-    bl<P, PO, 8, RED>(a, cls[3], pd, gate, st);
+  void bl_all(const MArgs<P, PO> *a, int grp, int gate, hipStream_t st) {
+    bl<P, PO, 1, RED>(a, cls[grp][0], pd, gate, st);
+    bl<P, PO, 2, RED>(a, cls[grp][1], pd, gate, st);
+    bl<P, PO, 4, RED>(a, cls[grp][2], pd, gate, st);
+    bl<P, PO, 8, RED>(a, cls[grp][3], pd, gate, st);
   }
   void iterate(hipStream_t st) {
-    bl_all<PS, PS, true>(s1, 0, st);
-    bl_all<PS, QAMD_F32, true>(s2, 0, st);
+    bl_all<PS, PS, true>(s1, 0, 0, st);
+    if (PS != QAMD_F32) bl_all<QAMD_F32, QAMD_F32, true>(f1, 1, 0, st);
+    bl_all<PS, QAMD_F32, true>(s2, 0, 0, st);
+    if (PS != QAMD_F32) bl_all<QAMD_F32, QAMD_F32, true>(f2, 1, 0, st);
     hipLaunchKernelGGL(k_bat_fin_normal, dim3(nprob), dim3(256), 0, st, red_set(1), red_set(2), pd);
     hipLaunchKernelGGL(k_bat_xr, dim3(all.nblocks), dim3(256), 0, st, pd, all, red_set(0));
     hipLaunchKernelGGL(k_bat_fin_cg, dim3(nprob), dim3(256), 0, st, red_set(0), pd);
@@ -360,8 +365,8 @@ struct Runner {
   }
   void reliable(hipStream_t st) {
     hipLaunchKernelGGL(k_bat_accum, dim3(all.nblocks), dim3(256), 0, st, pd, all);
-    bl_all<QAMD_F64, QAMD_F64, false>(r1, 1, st);
-    bl_all<QAMD_F64, QAMD_F64, false>(r2, 1, st);
+    for (int g = 0; g < 2; g++) bl_all<QAMD_F64, QAMD_F64, false>(r1, g, 1, st);
+    for (int g = 0; g < 2; g++) bl_all<QAMD_F64, QAMD_F64, false>(r2, g, 1, st);
     hipLaunchKernelGGL(k_bat_resid, dim3(all.nblocks), dim3(256), 0, st, pd, all, red_set(0));
     hipLaunchKernelGGL(k_bat_fin_true, dim3(nprob), dim3(256), 0, st, red_set(0), pd);
   }
@@ -369,6 +374,8 @@ struct Runner {
   void release() {
     dev_free(s1);
     dev_free(s2);
+    dev_free(f1);
+    dev_free(f2);
     dev_free(r1);
     dev_free(r2);
   }
@@ -381,20 +388,29 @@ template <int PS>
 int Runner<PS>::run(const qamd_stag_problem *pr, int n, std::vector<ProbDev> &H, std::vector<Work> &wk, double rsq,
                     int maxits, double delta, int use_graph, qamd_solve_info *info) {
   const double t0 = bnow();
+  const bool verbose = getenv("QAMD_VERBOSE") != nullptr;
   nprob = n;
   // kernel arguments (the scratch was reserved before, so its sets are final)
   std::vector<MArgs<PS, PS>> a1(n);
   std::vector<MArgs<PS, QAMD_F32>> a2(n);
+  std::vector<MArgs<QAMD_F32, QAMD_F32>> c1(n), c2(n);
   std::vector<MArgs<QAMD_F64, QAMD_F64>> b1(n), b2(n);
   for (int i = 0; i < n; i++) {
     const Work &w = wk[i];
     const int nr = H[i].nrhs;
     const double m2 = H[i].m2;
-    const qamd_field *pin = HALF ? w.ph : w.p;
-    a1[i] = margs<PS, PS>(pr[i].ls, 1, pin->f.data, pin->f.norm, w.t->f.data, w.t->f.norm, nullptr, nullptr, 0.0, 1.0,
-                          nr, red_set(1));
-    a2[i] = margs<PS, QAMD_F32>(pr[i].ls, 0, w.t->f.data, w.t->f.norm, w.Ap->f.data, nullptr, w.p->f.data, nullptr,
-                                m2, -1.0, nr, red_set(2));
+    if (!w.l32) {
+      const qamd_field *pin = HALF ? w.ph : w.p;
+      a1[i] = margs<PS, PS>(pr[i].ls, 1, pin->f.data, pin->f.norm, w.t->f.data, w.t->f.norm, nullptr, nullptr, 0.0,
+                            1.0, nr, red_set(1));
+      a2[i] = margs<PS, QAMD_F32>(pr[i].ls, 0, w.t->f.data, w.t->f.norm, w.Ap->f.data, nullptr, w.p->f.data,
+                                  nullptr, m2, -1.0, nr, red_set(2));
+    } else {
+      c1[i] = margs<QAMD_F32, QAMD_F32>(w.l32, 1, w.p->f.data, nullptr, w.t->f.data, nullptr, nullptr, nullptr, 0.0,
+                                        1.0, nr, red_set(1));
+      c2[i] = margs<QAMD_F32, QAMD_F32>(w.l32, 0, w.t->f.data, nullptr, w.Ap->f.data, nullptr, w.p->f.data, nullptr,
+                                        m2, -1.0, nr, red_set(2));
+    }
     b1[i] = margs<QAMD_F64, QAMD_F64>(pr[i].l64, 1, pr[i].x->f.data, nullptr, w.t64->f.data, nullptr, nullptr,
                                       nullptr, 0.0, 1.0, nr, red_set(3));
     b2[i] = margs<QAMD_F64, QAMD_F64>(pr[i].l64, 0, w.t64->f.data, nullptr, w.Ax->f.data, nullptr,
@@ -402,9 +418,11 @@ int Runner<PS>::run(const qamd_stag_problem *pr, int n, std::vector<ProbDev> &H,
   }
   s1 = upload(a1);
   s2 = upload(a2);
+  f1 = upload(c1);
+  f2 = upload(c2);
   r1 = upload(b1);
   r2 = upload(b2);
-  if (!s1 || !s2 || !r1 || !r2) return -1;
+  if (!s1 || !s2 || !f1 || !f2 || !r1 || !r2) return -1;
 
   ProbDev *hp = nullptr;   // pinned mirror of the device state
   if (hipHostMalloc((void **)&hp, n * sizeof(ProbDev), hipHostMallocDefault) != hipSuccess) {
@@ -472,6 +490,17 @@ int Runner<PS>::run(const qamd_stag_problem *pr, int n, std::vector<ProbDev> &H,
       if (err || pull()) {
         err = 1;
         break;
+      }
+      if (verbose) {
+        int na = 0, nr = 0;
+        long sites = 0;
+        for (int i = 0; i < n; i++)
+          if (hp[i].any_active) {
+            na++;
+            sites += (long)hp[i].g.vh;
+            for (int r = 0; r < hp[i].nrhs; r++) nr += hp[i].active[r];
+          }
+        fprintf(stderr, "cg_batch %.3f s: %d problems, %d rhs, %ld even sites active\n", bnow() - t0, na, nr, sites);
       }
       // which problems take a reliable update
       int nru = 0;
@@ -557,6 +586,7 @@ int Runner<PS>::run(const qamd_stag_problem *pr, int n, std::vector<ProbDev> &H,
           info[k].true_rsq = b2 > 0 ? hp[i].rsq[r] / b2 : 0.0;
           info[k].iter_rsq = info[k].true_rsq;
           info[k].converged = b2 == 0 || hp[i].rsq[r] < rsq * b2;
+          info[k].escalations = wk[i].l32 != nullptr;
           info[k].seconds = secs;
         }
     }
@@ -598,13 +628,36 @@ int qamd_stag_cg_mixed_batch(const qamd_stag_problem *pr, int n, double rsq, int
     }
   }
   if (delta <= 0 || delta >= 1) delta = 0.1;
-  // blocks: global numbering (problem order) and one launch plan per rhs class
   std::vector<ProbDev> H(n);
   std::vector<Work> wk(n);
   memset(H.data(), 0, n * sizeof(ProbDev));
   memset(wk.data(), 0, n * sizeof(Work));
+  auto cleanup = [&]() {
+    for (int i = 0; i < n; i++) {
+      Work &w = wk[i];
+      qamd_field_destroy(w.r64);
+      qamd_field_destroy(w.Ax);
+      qamd_field_destroy(w.t64);
+      qamd_field_destroy(w.r);
+      qamd_field_destroy(w.p);
+      qamd_field_destroy(w.Ap);
+      qamd_field_destroy(w.xs);
+      qamd_field_destroy(w.t);
+      qamd_field_destroy(w.ph);
+      qamd_links_destroy(w.l32);
+    }
+  };
+  // group 1: problems whose mass is too light for the 16-bit links
+  for (int i = 0; i < n; i++)
+    if (sloppy_too_coarse(pr[i].l64, pr[i].ls, pr[i].mass, &wk[i].l32)) {
+      cleanup();
+      return -1;
+    }
+  // blocks: global numbering (problem order) and one launch plan per
+  // (precision group, rhs class)
+  constexpr int NPL = 2 * NCLS;
   int total = 0;
-  std::vector<int> cls_of(n), cstart(NCLS * BMAXP, 0), gstart(BMAXP, 0), cblocks(NCLS, 0);
+  std::vector<int> pl_of(n), cstart(NPL * BMAXP, 0), gstart(BMAXP, 0), cblocks(NPL, 0);
   for (int i = 0; i < n; i++) {
     const Geom &g = pr[i].l64->lat->g;
     H[i].g = g;
@@ -614,23 +667,27 @@ int qamd_stag_cg_mixed_batch(const qamd_stag_problem *pr, int n, double rsq, int
     H[i].m2 = pr[i].mass * pr[i].mass;
     total += H[i].nb;
     int c = H[i].nrhs <= 1 ? 0 : H[i].nrhs <= 2 ? 1 : H[i].nrhs <= 4 ? 2 : 3;
-    cls_of[i] = c;
-    cstart[c * BMAXP + i] = cblocks[c];
-    cblocks[c] += H[i].nb;
+    int k = (wk[i].l32 ? NCLS : 0) + c;
+    pl_of[i] = k;
+    cstart[k * BMAXP + i] = cblocks[k];
+    cblocks[k] += H[i].nb;
   }
   std::vector<unsigned char> b2p_all(total);
-  std::vector<std::vector<unsigned char>> b2p(NCLS);
-  for (int c = 0; c < NCLS; c++) b2p[c].resize(cblocks[c] > 0 ? cblocks[c] : 1);
+  std::vector<std::vector<unsigned char>> b2p(NPL);
+  for (int k = 0; k < NPL; k++) b2p[k].resize(cblocks[k] > 0 ? cblocks[k] : 1);
   for (int i = 0; i < n; i++) {
     for (int k = 0; k < H[i].nb; k++) {
       b2p_all[H[i].gstart + k] = (unsigned char)i;
-      b2p[cls_of[i]][cstart[cls_of[i] * BMAXP + i] + k] = (unsigned char)i;
+      b2p[pl_of[i]][cstart[pl_of[i] * BMAXP + i] + k] = (unsigned char)i;
     }
   }
-  if (red_reserve(total)) return -1;
+  if (red_reserve(total)) {
+    cleanup();
+    return -1;
+  }
   int rc = -1;
   ProbDev *pd = nullptr;
-  unsigned char *d_all = nullptr, *d_cls[NCLS] = {nullptr};
+  unsigned char *d_all = nullptr, *d_cls[NPL] = {nullptr};
   int *d_gstart = nullptr, *d_cstart = nullptr;
   do {
     bool ok = true;
@@ -638,6 +695,7 @@ int qamd_stag_cg_mixed_batch(const qamd_stag_problem *pr, int n, double rsq, int
       qamd_lattice *lat = pr[i].l64->lat;
       int nr = H[i].nrhs;
       Work &w = wk[i];
+      const int pt = w.l32 ? QAMD_F32 : ps;   // precision of the odd temporary
       w.r64 = qamd_field_create(lat, QAMD_F64, 3, nr, 0);
       w.Ax = qamd_field_create(lat, QAMD_F64, 3, nr, 0);
       w.t64 = qamd_field_create(lat, QAMD_F64, 3, nr, 1);
@@ -645,9 +703,9 @@ int qamd_stag_cg_mixed_batch(const qamd_stag_problem *pr, int n, double rsq, int
       w.p = qamd_field_create(lat, QAMD_F32, 3, nr, 0);
       w.Ap = qamd_field_create(lat, QAMD_F32, 3, nr, 0);
       w.xs = qamd_field_create(lat, QAMD_F32, 3, nr, 0);
-      w.t = qamd_field_create(lat, ps, 3, nr, 1);
-      w.ph = (ps == QAMD_F32) ? nullptr : qamd_field_create(lat, ps, 3, nr, 0);
-      ok = w.r64 && w.Ax && w.t64 && w.r && w.p && w.Ap && w.xs && w.t && (ps == QAMD_F32 || w.ph);
+      w.t = qamd_field_create(lat, pt, 3, nr, 1);
+      w.ph = (pt == QAMD_F32) ? nullptr : qamd_field_create(lat, ps, 3, nr, 0);
+      ok = w.r64 && w.Ax && w.t64 && w.r && w.p && w.Ap && w.xs && w.t && (pt == QAMD_F32 || w.ph);
       ok = ok && !qamd_field_zero(pr[i].x) && !qamd_field_zero(w.xs) && !qamd_field_zero(w.p);
       if (!ok) break;
       ProbDev &D = H[i];
@@ -667,12 +725,12 @@ int qamd_stag_cg_mixed_batch(const qamd_stag_problem *pr, int n, double rsq, int
     d_all = upload(b2p_all);
     d_gstart = upload(gstart);
     d_cstart = upload(cstart);
-    for (int c = 0; c < NCLS; c++) d_cls[c] = upload(b2p[c]);
-    if (!pd || !d_all || !d_gstart || !d_cstart || !d_cls[0] || !d_cls[1] || !d_cls[2] || !d_cls[3]) break;
+    for (int k = 0; k < NPL; k++) ok = ok && (d_cls[k] = upload(b2p[k])) != nullptr;
+    if (!pd || !d_all || !d_gstart || !d_cstart || !ok) break;
     auto go = [&](auto &R) -> int {
       R.pd = pd;
       R.all = Plan{d_all, d_gstart, total};
-      for (int c = 0; c < NCLS; c++) R.cls[c] = Plan{d_cls[c], d_cstart + c * BMAXP, cblocks[c]};
+      for (int k = 0; k < NPL; k++) R.cls[k / NCLS][k % NCLS] = Plan{d_cls[k], d_cstart + k * BMAXP, cblocks[k]};
       int r = R.run(pr, n, H, wk, rsq, maxits, delta, use_graph, info);
       R.release();
       return r;
@@ -696,19 +754,8 @@ int qamd_stag_cg_mixed_batch(const qamd_stag_problem *pr, int n, double rsq, int
   dev_free(d_all);
   dev_free(d_gstart);
   dev_free(d_cstart);
-  for (int c = 0; c < NCLS; c++) dev_free(d_cls[c]);
-  for (int i = 0; i < n; i++) {
-    Work &w = wk[i];
-    qamd_field_destroy(w.r64);
-    qamd_field_destroy(w.Ax);
-    qamd_field_destroy(w.t64);
-    qamd_field_destroy(w.r);
-    qamd_field_destroy(w.p);
-    qamd_field_destroy(w.Ap);
-    qamd_field_destroy(w.xs);
-    qamd_field_destroy(w.t);
-    qamd_field_destroy(w.ph);
-  }
+  for (int k = 0; k < NPL; k++) dev_free(d_cls[k]);
+  cleanup();
   return rc;
 }
 

@@ -21,6 +21,9 @@ int stag_dslash_launch(const qamd_links *l, Field &out, const Field &in, double 
 // go to partial-sum set rs (2*nrhs values)
 int stag_block_launch(const qamd_links *l, Field &out, const Field &in, double a, const Field *w, double b,
                       bool red, RedScratch rs, hipStream_t st);
+// solvers.hip: with 16-bit sloppy links too coarse for this mass (operator
+// error not small against m^2), *l32 receives FP32 links made from l64
+int sloppy_too_coarse(const qamd_links *l64, const qamd_links *ls, double mass, qamd_links **l32);
 int wilson_dslash_launch(const qamd_links *l, Field &out, const Field &in, double a, const Field *w,
                          double b, double *red_out, hipStream_t st, const int *skip = nullptr);
 

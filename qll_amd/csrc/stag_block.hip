@@ -325,8 +325,9 @@ using namespace qamd;
 
 extern "C" {
 
-int qamd_stag_cg_mixed_block(const qamd_links *l64, const qamd_links *ls, qamd_field *x, const qamd_field *b,
+int qamd_stag_cg_mixed_block(const qamd_links *l64, const qamd_links *ls_in, qamd_field *x, const qamd_field *b,
                              double mass, double rsq, int maxits, double delta, qamd_solve_info *info) {
+  const qamd_links *ls = ls_in;
   if (!l64 || !ls || l64->kind != 0 || ls->kind != 0 || l64->prec != QAMD_F64 || ls->prec == QAMD_F64 ||
       l64->lat->g.V != ls->lat->g.V) {
     set_error("cg_mixed_block: need FP64 links and FP32/FP16/Q16 sloppy links on one lattice");
@@ -340,6 +341,10 @@ int qamd_stag_cg_mixed_block(const qamd_links *l64, const qamd_links *ls, qamd_f
   }
   if (delta <= 0 || delta >= 1) delta = 0.1;
   double t0 = now_s();
+  // 16-bit links too coarse for this mass: FP32 links for the whole block
+  qamd_links *l32 = nullptr;
+  if (sloppy_too_coarse(l64, ls, mass, &l32)) return -1;
+  if (l32) ls = l32;
   qamd_lattice *lat = l64->lat;
   const int ps = ls->prec;
   const bool half = (ps == QAMD_F16 || ps == QAMD_Q16);
@@ -482,6 +487,7 @@ int qamd_stag_cg_mixed_block(const qamd_links *l64, const qamd_links *ls, qamd_f
         info[i].iter_rsq = info[i].true_rsq;
         info[i].converged = bb[i] == 0 || st.Bh->rsq[i] < rsq * bb[i];
         info[i].restarts = nrestart;
+        info[i].escalations = l32 != nullptr;
         info[i].seconds = now_s() - t0;
       }
     rc = 0;
@@ -497,6 +503,7 @@ int qamd_stag_cg_mixed_block(const qamd_links *l64, const qamd_links *ls, qamd_f
   qamd_field_destroy(ph);
   dev_free(st.B);
   if (st.Bh) (void)hipHostFree(st.Bh);
+  qamd_links_destroy(l32);
   return rc;
 }
 

@@ -113,3 +113,31 @@ def test_config4_full_heterogeneous_batch(gpu):
         r2 = r2 if x.nrhs > 1 else [r2]
         b2 = b2 if x.nrhs > 1 else [b2]
         assert max(r2[i] / b2[i] for i in range(x.nrhs)) <= 1.01e-20
+
+
+def test_sloppy_error_and_light_mass_fallback(gpu):
+    """The sloppy operator's error is ~1e-4 with 16-bit links and ~1e-6 with
+    FP32 ones; a mass light enough that the 16-bit error is not small against
+    m^2 runs on FP32 links (single solve and batch) and still converges."""
+    dims = (8, 8, 8, 8)
+    links = O.asqtad_links(dims, 51)
+    lat = gpu.Lattice(dims)
+    L64 = gpu.Links(lat, gpu.STAG, 0, 16).set_lex(links)
+    LQ, L32 = L64.converted(gpu.Q16), L64.converted(gpu.F32)
+    eq, e32 = gpu.stag_sloppy_error(L64, LQ, 0.05), gpu.stag_sloppy_error(L64, L32, 0.05)
+    assert 1e-5 < eq < 1e-3 and 1e-8 < e32 < 1e-5, (eq, e32)
+    light, heavy = 0.008, 0.1   # 16-bit error > 0.6 m^2 / << 0.6 m^2
+    b = O.gauss_vector(dims, 52, 100, parity=0)
+    bf = gpu.Field(lat, 0, 3, 1, 0).set_lex(b)
+    x = gpu.Field(lat, 0, 3, 1, 0)
+    info = gpu.stag_cg_mixed(L64, LQ, x, bf, light, 1e-20, 50000)
+    assert info["converged"] and info["escalations"] == 1
+    Ax, _ = O.normal_op(dims, links, light, x.get_lex())
+    m = parity_mask(dims, 0)
+    assert np.sum((b - Ax)[m] ** 2) / np.sum(b[m] ** 2) <= 1.01e-20
+    xs = [gpu.Field(lat, 0, 3, 1, 0) for _ in range(2)]
+    infos = gpu.stag_cg_mixed_batch([(L64, LQ, xs[0], bf, light), (L64, LQ, xs[1], bf, heavy)], 1e-20, 50000)
+    assert [i["escalations"] for i in infos] == [1, 0]
+    for xi, mass in zip(xs, (light, heavy)):
+        Ax, _ = O.normal_op(dims, links, mass, xi.get_lex())
+        assert np.sum((b - Ax)[m] ** 2) / np.sum(b[m] ** 2) <= 1.01e-20
