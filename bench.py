@@ -328,6 +328,7 @@ def main():
             return max(r2[i] / bBsq[i] for i in range(nb))
 
         for name, Ls in (("block%d_mixed_q16" % nb, LQ), ("block%d_mixed_fp32" % nb, L32)):
+            Q.stag_cg_mixed_block(L64, Ls, xB, bB, QLL_MASS, RSQ, 16, 0.1)   # untimed: launch-shape tuning
             Q.sync()
             t0 = time.time()
             infos = Q.stag_cg_mixed_block(L64, Ls, xB, bB, QLL_MASS, RSQ, 50000, 0.1)

@@ -250,6 +250,15 @@ int qamd_host_sum(const void *host, int64_t n, int host_prec, int squares, doubl
 int qamd_flat_axpby(int64_t n, int prec, double a, const void *x, double b, void *y); /* y = a x + b y */
 int qamd_flat_dot(int64_t n, int prec, const void *x, const void *y, double *out);   /* FP64 sum x.y */
 
+/* ---- launch-shape autotuning (no reference counterpart) ----
+ * Kernels with several launch shapes (the block Dslash: rhs per launch and
+ * XCD block order) time each candidate on first use for a given lattice,
+ * precision and block size and keep the fastest.  mode 1: on (the
+ * default); 0 or QAMD_TUNE=0 in the environment: off, the default shape is
+ * used and the cache cleared; mode >= 2 forces candidate mode - 2 (the last
+ * one if fewer) everywhere, for tests.  QAMD_VERBOSE=1 prints each choice. */
+int qamd_set_tuning(int mode);
+
 /* ---- timing helpers (hipEvents on the library stream) ---- */
 int qamd_time_dslash(const qamd_links *l, qamd_field *out, const qamd_field *in,
                      int nrep, double *ms_per_call);

@@ -95,6 +95,7 @@ def lib():
             "qamd_stag_cg": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_double, C.c_int, C.POINTER(SolveInfo)]),
             "qamd_stag_cg_mixed": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_double, C.c_int, C.c_double,
                                              C.POINTER(SolveInfo)]),
+            "qamd_set_tuning": (C.c_int, [C.c_int]),
             "qamd_stag_sloppy_error": (C.c_int, [_vp, _vp, C.c_double, C.POINTER(C.c_double)]),
             "qamd_stag_cg_mixed_batch": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_double, C.c_int,
                                                    C.POINTER(SolveInfo)]),
@@ -305,6 +306,12 @@ def stag_cg_mixed_block(l64, ls, x, b, mass, rsq, maxits, delta=0.1):
     infos = (SolveInfo * b.nrhs)()
     _check(lib().qamd_stag_cg_mixed_block(l64.h, ls.h, x.h, b.h, mass, rsq, maxits, delta, infos))
     return [i.as_dict() for i in infos]
+
+
+def set_tuning(mode):
+    """Launch-shape autotuning: True/1 on (default), False/0 off, k >= 2 forces
+    candidate k - 2 (tests)."""
+    _check(lib().qamd_set_tuning(int(mode)))
 
 
 def stag_sloppy_error(l64, ls, mass):

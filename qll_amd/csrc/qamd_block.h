@@ -20,7 +20,8 @@ template <int P, int PO> struct MArgs {
   double a, b;
   int par, nr;
   float lscale[16];
-  RedScratch rs;
+  RedScratch rs;   // partial sums of rhs r in slots 2r, 2r+1 (offset by the caller)
+  int xcd;         // 1: XCD-aware block order
 };
 
 // one output site cb: out_r = a*w_r + b * sum_t L_t in_r(x+d_t) for r < A.nr

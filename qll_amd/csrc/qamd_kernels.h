@@ -5,6 +5,10 @@
 
 namespace qamd {
 
+// runtime.hip: index of the fastest of ncand launch candidates for key,
+// measured on first use (run(c, ctx) launches candidate c, 0 on success)
+int tune_pick(const char *key, int ncand, int def, int (*run)(int, void *), void *ctx);
+
 // fields.hip
 const Geom *dev_geom(qamd_lattice *lat);
 double *scalar_slot();

@@ -7,6 +7,8 @@
 #include <stdio.h>
 #include <string.h>
 #include <stdlib.h>
+#include <map>
+#include <string>
 
 namespace qamd {
 
@@ -162,3 +164,70 @@ void qamd_lattice_destroy(qamd_lattice *lat) { free(lat); }
 int64_t qamd_lattice_sites(const qamd_lattice *lat) { return lat ? lat->g.V : 0; }
 
 }  // extern "C"
+
+// ------------------------------------------------------------- autotuner
+// Launch-shape choices measured on first use and cached per key (lattice,
+// precisions, block size, ...): every candidate is launched once to warm up,
+// then timed over three launches with events on the library stream, and the
+// fastest is kept for the rest of the process.  The kernels tuned this way
+// are out of place, so repeating a launch leaves the same result.  No tuning
+// while the stream is being captured into a graph (the cached or default
+// choice is used) or when QAMD_TUNE=0.
+namespace qamd {
+static std::map<std::string, int> g_tuned;
+static int g_tune_on = -1;
+static int g_tune_force = -1;   // >= 0: every tuned launch uses this candidate
+
+static bool tuning_enabled() {
+  if (g_tune_on < 0) {
+    const char *e = getenv("QAMD_TUNE");
+    g_tune_on = !(e && e[0] == '0');
+  }
+  return g_tune_on;
+}
+
+int tune_pick(const char *key, int ncand, int def, int (*run)(int, void *), void *ctx) {
+  if (g_tune_force >= 0) return g_tune_force < ncand ? g_tune_force : ncand - 1;
+  auto it = g_tuned.find(key);
+  if (it != g_tuned.end()) return it->second;
+  if (ncand <= 1 || !tuning_enabled()) return def;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(g_stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return def;
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess) return def;
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return def;
+  }
+  int best = def;
+  float tbest = 1e30f;
+  for (int c = 0; c < ncand; c++) {
+    if (run(c, ctx)) continue;
+    (void)hipEventRecord(e0, g_stream);
+    int bad = 0;
+    for (int k = 0; k < 3; k++) bad |= run(c, ctx);
+    (void)hipEventRecord(e1, g_stream);
+    if (bad || hipEventSynchronize(e1) != hipSuccess) continue;
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (ms < tbest) {
+      tbest = ms;
+      best = c;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  g_tuned[key] = best;
+  if (getenv("QAMD_VERBOSE")) fprintf(stderr, "qll_amd tune %s: candidate %d (%.4f ms)\n", key, best, tbest / 3);
+  return best;
+}
+}  // namespace qamd
+
+extern "C" {
+int qamd_set_tuning(int mode) {
+  qamd::g_tune_on = mode ? 1 : 0;
+  qamd::g_tune_force = mode >= 2 ? mode - 2 : -1;
+  if (mode != 1) qamd::g_tuned.clear();
+  return 0;
+}
+}

@@ -12,6 +12,7 @@
 #include <string.h>
 #include <math.h>
 #include <chrono>
+#include <stdio.h>
 
 namespace qamd {
 
@@ -20,13 +21,14 @@ constexpr int MAXB = 16;
 // out_r = a*w_r + b * sum_t L_t in_r(x+d_t) for r < nr; fields rhs-outer
 template <int P, int PO, int NT, int NR, bool RED>
 __global__ void __launch_bounds__(256) k_stag_block(MArgs<P, PO> A) {
-  int blk = xcd_block(blockIdx.x, gridDim.x);
+  int blk = A.xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
   int cb = blk * 256 + threadIdx.x;
   double red[2 * NR];
 #pragma unroll
   for (int i = 0; i < 2 * NR; i++) red[i] = 0.0;
   if (cb < A.g.vh) block_site<P, PO, NT, NR, RED>(A, cb, red);
-  if (RED) grid_partial<2 * NR>(red, A.rs);
+  // indexed by site block, so the sums do not depend on the block order
+  if (RED) grid_partial_at<2 * NR>(red, A.rs, blk);
 }
 
 template <int P, int PO, int NT, int NR>
@@ -50,13 +52,43 @@ static int blaunch_nt(const MArgs<P, PO> &A, bool red, hipStream_t st) {
   return 0;
 }
 
+// One block-Dslash application, in launches of k rhs with or without the
+// XCD block order; (k, xcd) is chosen by the autotuner per lattice,
+// precision and block size.
+template <int P, int PO>
+struct BlockRun {
+  MArgs<P, PO> A;
+  int n, nt, red, ncand, k[8], xcd[8];
+  hipStream_t st;
+  static int run(int c, void *ctx) {
+    BlockRun &R = *(BlockRun *)ctx;
+    const int64_t vp = R.A.g.vh_pad;
+    for (int r0 = 0; r0 < R.n; r0 += R.k[c]) {
+      MArgs<P, PO> B = R.A;
+      B.nr = R.n - r0 < R.k[c] ? R.n - r0 : R.k[c];
+      B.xcd = R.xcd[c];
+      B.in = R.A.in + (int64_t)r0 * 3 * vp;
+      B.out = R.A.out + (int64_t)r0 * 3 * vp;
+      if (R.A.w) B.w = R.A.w + (int64_t)r0 * 3 * vp;
+      if (R.A.in_norm) B.in_norm = R.A.in_norm + (int64_t)r0 * vp;
+      if (R.A.out_norm) B.out_norm = R.A.out_norm + (int64_t)r0 * vp;
+      if (R.A.w_norm) B.w_norm = R.A.w_norm + (int64_t)r0 * vp;
+      if (R.A.rs.part) B.rs.part = R.A.rs.part + (int64_t)2 * r0 * R.A.rs.max_blocks;
+      int rc = (R.nt == 16) ? blaunch_nt<P, PO, 16>(B, R.red, R.st) : blaunch_nt<P, PO, 8>(B, R.red, R.st);
+      if (rc) return -1;
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+  }
+};
+
 template <int P, int PO = P>
 static int blaunch(const qamd_links *l, Field &out, const Field &in, double a, const Field *w, double b, bool red,
                    RedScratch rs, hipStream_t st) {
   typedef typename Prec<P>::S S;
   typedef typename Prec<PO>::S SO;
-  MArgs<P, PO> A;
-  memset(&A, 0, sizeof A);
+  BlockRun<P, PO> R;
+  memset(&R, 0, sizeof R);
+  MArgs<P, PO> &A = R.A;
   A.g = l->lat->g;
   A.links = (const S *)l->data[out.parity];
   A.in = (const S *)in.data;
@@ -70,10 +102,31 @@ static int blaunch(const qamd_links *l, Field &out, const Field &in, double a, c
   A.par = out.parity;
   A.nr = in.nrhs;
   A.rs = rs;
+  A.xcd = 1;
   for (int t = 0; t < 16; t++) A.lscale[t] = (P == QAMD_Q16) ? l->scale[t] / Q16_ONE : l->scale[t];
-  int rc = (l->nterms == 16) ? blaunch_nt<P, PO, 16>(A, red, st) : blaunch_nt<P, PO, 8>(A, red, st);
-  if (rc) return -1;
-  QAMD_HIP(hipGetLastError());
+  R.n = in.nrhs;
+  R.nt = l->nterms;
+  R.red = red;
+  R.st = st;
+  // candidates: all rhs in one launch (default), then halves, quarters, ...,
+  // each with and without the XCD block order
+  for (int kk = R.n; R.ncand < 8; kk = (kk + 1) / 2) {
+    R.k[R.ncand] = kk;
+    R.xcd[R.ncand++] = 1;
+    R.k[R.ncand] = kk;
+    R.xcd[R.ncand++] = 0;
+    if (kk == 1) break;
+  }
+  // out = a*out + ... would compound over the timed repeats: no tuning then
+  const bool inplace = w && w->data == out.data;
+  char key[128];
+  snprintf(key, sizeof key, "stag_block p%d/%d nt%d n%d red%d %dx%dx%dx%d", P, PO, R.nt, R.n, (int)red, A.g.L[0],
+           A.g.L[1], A.g.L[2], A.g.L[3]);
+  int c = inplace ? 0 : tune_pick(key, R.ncand, 0, &BlockRun<P, PO>::run, &R);
+  if (BlockRun<P, PO>::run(c, &R)) {
+    set_error("block Dslash: launch failed");
+    return -1;
+  }
   return 0;
 }
 

@@ -176,3 +176,30 @@ def test_qll_layout_transfer(gpu, inner):
         assert rel_err(f.get_lex(), v, parity_mask(dims, p)) == 0.0
         qll_amd._check(qll_amd.lib().qamd_field_to_qll(f.h, C.byref(ql), back.ctypes.data, 0))
     assert np.array_equal(back, q)
+
+
+def test_block_dslash_every_launch_shape(gpu):
+    """Every launch shape the autotuner can pick (rhs per launch, block order)
+    gives the block Dslash and the block CG of the default shape bit for bit
+    (partial sums are indexed by site block, not by launch order)."""
+    dims = (8, 8, 8, 8)
+    links, _ = _stag_case(dims, 23)
+    ys = np.stack([O.gauss_vector(dims, 90 + r, 100, parity=1) for r in range(6)])
+    bs = np.stack([O.gauss_vector(dims, 95 + r, 100, parity=0) for r in range(6)])
+    lat = gpu.Lattice(dims)
+    L64 = gpu.Links(lat, gpu.STAG, 0, 16).set_lex(links)
+    L = L64.converted(1)
+    yin = gpu.Field(lat, 1, 3, 6, 1).set_lex(ys)
+    b = gpu.Field(lat, 0, 3, 6, 0).set_lex(bs)
+    res = []
+    for mode in [0] + list(range(2, 10)) + [1]:   # default, each candidate, tuned
+        gpu.set_tuning(mode)
+        out = gpu.Field(lat, 1, 3, 6, 0)
+        gpu.dslash(L, out, yin)
+        x = gpu.Field(lat, 0, 3, 6, 0)
+        infos = gpu.stag_cg_mixed_block(L64, L, x, b, 0.1, 1e-20, 5000)
+        res.append((out.get_lex(), x.get_lex(), [i["iterations"] for i in infos]))
+    gpu.set_tuning(1)
+    for o, x, its in res[1:]:
+        assert np.array_equal(o, res[0][0])
+        assert np.array_equal(x, res[0][1]) and its == res[0][2]
