@@ -280,6 +280,29 @@ def main():
                      "hbm_frac": round(byts / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
         print("dslash %s: %s" % (key, prof[key]), file=sys.stderr, flush=True)
 
+    # Wilson EO Dslash: FP64 one rhs, and 16 rhs sharing every link load
+    # (algorithmic bytes per site: 8 links + nrhs x (input + output spinor))
+    WLINK = {0: 8 * 144, 1: 8 * 72, 2: 8 * 36, 3: 8 * 36}
+    WSPIN = {0: 192, 1: 96, 2: 52, 3: 52}
+    gw = Q.Gauge(lat).random_su3(3)
+    W64 = Q.Links(lat, Q.WILSON, Q.F64, 8).from_gauge(gw)
+    del gw
+    for prec, nrw in ((0, 1), (1, 16), (3, 16)):
+        Lw = W64 if prec == 0 else W64.converted(prec)
+        yin = Q.Field(lat, prec, 12, nrw, Q.ODD).gaussian(7, 100)
+        out = Q.Field(lat, prec, 12, nrw, Q.EVEN)
+        Q.time_dslash(Lw, out, yin, 10)
+        ms = Q.time_dslash(Lw, out, yin, max(1, args.steps // 4))
+        byts = (WLINK[prec] + nrw * 2 * WSPIN[prec]) * vh
+        key = "wilson_%s_%drhs" % (PREC_NAME[prec], nrw)
+        prof[key] = {"ms_per_call": round(ms, 5), "ms_per_rhs": round(ms / nrw, 5),
+                     "GFLOP/s": round(nrw * 1320 * vh / (ms * 1e-3) / 1e9, 1),
+                     "GB/s": round(byts / (ms * 1e-3) / 1e9, 1),
+                     "hbm_frac": round(byts / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        print("dslash %s: %s" % (key, prof[key]), file=sys.stderr, flush=True)
+        del Lw, yin, out
+    del W64
+
     ms64 = rates[0]
     gflops = FLOPS_ASQTAD * vh / (ms64 * 1e-3) / 1e9
     achieved = BYTES_SITE[0] * vh / (ms64 * 1e-3) / 1e9

@@ -43,11 +43,11 @@ template <class C> __device__ __forceinline__ C cneg(C a) { a.x = -a.x; a.y = -a
 template <int P> struct SpIO;
 template <> struct SpIO<QAMD_F64> {
   __device__ static double2 ld(const Field &f, int64_t i, float) { return ((const double2 *)f.data)[i]; }
-  __device__ static void st(Field &f, int64_t i, double2 v) { ((double2 *)f.data)[i] = v; }
+  __device__ static void st(const Field &f, int64_t i, double2 v) { ((double2 *)f.data)[i] = v; }
 };
 template <> struct SpIO<QAMD_F32> {
   __device__ static float2 ld(const Field &f, int64_t i, float) { return ((const float2 *)f.data)[i]; }
-  __device__ static void st(Field &f, int64_t i, float2 v) { ((float2 *)f.data)[i] = v; }
+  __device__ static void st(const Field &f, int64_t i, float2 v) { ((float2 *)f.data)[i] = v; }
 };
 template <> struct SpIO<QAMD_F16> {
   __device__ static float2 ld(const Field &f, int64_t i, float n) {
@@ -56,7 +56,7 @@ template <> struct SpIO<QAMD_F16> {
     v.y *= n;
     return v;
   }
-  __device__ static void st(Field &f, int64_t i, float2 v) { ((__half2 *)f.data)[i] = __floats2half2_rn(v.x, v.y); }
+  __device__ static void st(const Field &f, int64_t i, float2 v) { ((__half2 *)f.data)[i] = __floats2half2_rn(v.x, v.y); }
 };
 template <> struct SpIO<QAMD_Q16> {
   __device__ static float2 ld(const Field &f, int64_t i, float n) {
@@ -64,7 +64,7 @@ template <> struct SpIO<QAMD_Q16> {
     n *= 1.0f / Q16_ONE;
     return make_float2(s.x * n, s.y * n);
   }
-  __device__ static void st(Field &f, int64_t i, float2 v) { ((short2 *)f.data)[i] = q16_pack(v.x, v.y); }
+  __device__ static void st(const Field &f, int64_t i, float2 v) { ((short2 *)f.data)[i] = q16_pack(v.x, v.y); }
 };
 
 struct FinStore1 {
@@ -78,31 +78,45 @@ __global__ void __launch_bounds__(256) k_wilson(WArgs<P> A) {
   typedef typename Prec<P>::C C;
   typedef typename Prec<P>::S S;
   if (A.skip && *A.skip) return;
-  const int nr = A.in.nrhs;
+  // local copies: the kernel reads its argument structs only here
+  const Field fin = A.in, fw = A.w;
+  const Field fout = A.out;
+  const Geom g = A.g;
+  const int nr = fin.nrhs;
   int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int cb = (int)(gid / nr), r = (int)(gid - (int64_t)cb * nr);
   double red[1] = {0.0};
-  if (cb < A.g.vh) {
-    Site s = site_of(A.g, cb, A.par);
+  if (cb < g.vh) {
+    Site s = site_of(g, cb, A.par);
     constexpr int G = LinkG<S>::G;
     const S *lp = link_base<S, 8>(A.links, cb);
-    S q[9][G];
     C acc[4][3];
 #pragma unroll
     for (int a = 0; a < 4; a++)
 #pragma unroll
       for (int c = 0; c < 3; c++) acc[a][c].x = acc[a][c].y = (R)0;
+    // one link group (G terms, one 16-byte load per element) per iteration;
+    // the iterations are not unrolled, which keeps the spinor working set of
+    // one group in registers (unrolled, the hoisted loads of all eight terms
+    // spill)
+#pragma unroll 1
+    for (int tg = 0; tg < 8 / G; tg++) {
+    S q[9][G];
 #pragma unroll
-    for (int t = 0; t < 8; t++) {
+    for (int e = 0; e < 9; e++) ntload_group(lp + link_off<S>(tg * G, e), q[e]);
+#pragma unroll
+    for (int gg = 0; gg < G; gg++) {
+      const int t = tg * G + gg;
       const int mu = t >> 1;
       const bool bwd = t & 1;
-      int nb = nbr_cb(A.g, s, mu, bwd ? -1 : 1);
-      float nrm = (P == QAMD_F16 || P == QAMD_Q16) ? A.in.norm[A.in.nidx(r, nb)] * A.lscale[t] : 1.0f;
+      int nb = nbr_cb_rt(g, s, mu, bwd ? -1 : 1);
+      float nrm = 1.0f;
+      if constexpr (P == QAMD_F16 || P == QAMD_Q16) nrm = fin.norm[fin.nidx(r, nb)] * A.lscale[t];
       C psi[4][3];
 #pragma unroll
       for (int a = 0; a < 4; a++)
 #pragma unroll
-        for (int c = 0; c < 3; c++) psi[a][c] = SpIO<P>::ld(A.in, A.in.cidx(r, 3 * a + c, nb), nrm);
+        for (int c = 0; c < 3; c++) psi[a][c] = SpIO<P>::ld(fin, fin.cidx(r, 3 * a + c, nb), nrm);
       // h = psi_up + s A psi_dn, s = +1 for backward terms
       C h[2][3];
 #pragma unroll
@@ -115,14 +129,9 @@ __global__ void __launch_bounds__(256) k_wilson(WArgs<P> A) {
         h[0][c] = bwd ? cadd(psi[0][c], d0) : csub(psi[0][c], d0);
         h[1][c] = bwd ? cadd(psi[1][c], d1) : csub(psi[1][c], d1);
       }
-      // one 16-byte group load per element serves G consecutive terms
-      if (t % G == 0) {
-#pragma unroll
-        for (int e = 0; e < 9; e++) ntload_group(lp + link_off<S>(t, e), q[e]);
-      }
       C m[9];
 #pragma unroll
-      for (int e = 0; e < 9; e++) m[e] = to_c(q[e][t % G], (R)0);
+      for (int e = 0; e < 9; e++) m[e] = to_c(q[e][gg], (R)0);
       C uh[2][3];
 #pragma unroll
       for (int a = 0; a < 2; a++)
@@ -153,9 +162,12 @@ __global__ void __launch_bounds__(256) k_wilson(WArgs<P> A) {
         acc[3][c] = bwd ? cadd(acc[3][c], l1) : csub(acc[3][c], l1);
       }
     }
+    }
     const R b = (R)A.b, a = (R)A.a;
     C o[4][3];
-    float wn = ((P == QAMD_F16 || P == QAMD_Q16) && A.has_w) ? A.w.norm[A.w.nidx(r, cb)] : 1.0f;
+    float wn = 1.0f;
+    if constexpr (P == QAMD_F16 || P == QAMD_Q16)
+      if (A.has_w) wn = fw.norm[fw.nidx(r, cb)];
 #pragma unroll
     for (int sp = 0; sp < 4; sp++)
 #pragma unroll
@@ -164,7 +176,7 @@ __global__ void __launch_bounds__(256) k_wilson(WArgs<P> A) {
         v.x = b * acc[sp][c].x;
         v.y = b * acc[sp][c].y;
         if (A.has_w) {
-          C wv = SpIO<P>::ld(A.w, A.w.cidx(r, 3 * sp + c, cb), wn);
+          C wv = SpIO<P>::ld(fw, fw.cidx(r, 3 * sp + c, cb), wn);
           v.x = fma(a, wv.x, v.x);
           v.y = fma(a, wv.y, v.y);
         }
@@ -178,7 +190,7 @@ __global__ void __launch_bounds__(256) k_wilson(WArgs<P> A) {
       for (int sp = 0; sp < 4; sp++)
 #pragma unroll
         for (int c = 0; c < 3; c++) mx = fmaxf(mx, fmaxf(fabsf((float)o[sp][c].x), fabsf((float)o[sp][c].y)));
-      A.out.norm[A.out.nidx(r, cb)] = mx;
+      fout.norm[fout.nidx(r, cb)] = mx;
       on = mx > 0.f ? 1.0f / mx : 0.f;
     }
 #pragma unroll
@@ -191,7 +203,7 @@ __global__ void __launch_bounds__(256) k_wilson(WArgs<P> A) {
           v.x *= on;
           v.y *= on;
         }
-        SpIO<P>::st(A.out, A.out.cidx(r, 3 * sp + c, cb), v);
+        SpIO<P>::st(fout, fout.cidx(r, 3 * sp + c, cb), v);
       }
   }
   if (RED) grid_partial<1>(red, A.rs);

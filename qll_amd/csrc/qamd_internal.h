@@ -110,6 +110,21 @@ __device__ __forceinline__ int nbr_cb(const Geom &g, const Site &s, int mu, int 
 // link packing: terms per 16-byte load for stored complex type S
 template <class S> struct LinkG { static constexpr int G = (int)(16 / sizeof(S)); };
 
+// the same for a direction known only at run time (selects, no indexed
+// access to the coordinate array)
+__device__ __forceinline__ int sel4(int mu, int a, int b, int c, int d) {   // values, not lvalues
+  return mu == 0 ? a : mu == 1 ? b : mu == 2 ? c : d;
+}
+__device__ __forceinline__ int nbr_cb_rt(const Geom &g, const Site &s, int mu, int h) {
+  const int c = sel4(mu, s.c[0], s.c[1], s.c[2], s.c[3]);
+  const int L = sel4(mu, g.L[0], g.L[1], g.L[2], g.L[3]);
+  const int st = sel4(mu, g.stride[0], g.stride[1], g.stride[2], g.stride[3]);
+  int n = c + h;
+  n += (n < 0) ? L : 0;
+  n -= (n >= L) ? L : 0;
+  return (s.lex + (n - c) * st) >> 1;
+}
+
 // blocked link index: element k (0..8) of term t for site cb
 template <class S>
 __host__ __device__ __forceinline__ int64_t link_idx(int64_t cb, int t, int k, int nterm) {
