@@ -15,56 +15,62 @@ size_t prec_bytes(int prec) {
   return prec == QAMD_F64 ? 16 : prec == QAMD_F32 ? 8 : 4;   // FP16 and Q16: 4 bytes
 }
 
-template <class H>
+template <class H, int NC>
 __global__ void k_import(Field f, const H *src, SrcMap m) {
   int cb = blockIdx.x * blockDim.x + threadIdx.x;
   int rhs = blockIdx.y;
   if (cb >= f.g->vh) return;
   int64_t base, stride;
   m.map(*f.g, rhs, f.parity, cb, base, stride);
-  double2 v[12];
-  for (int j = 0; j < f.ncomp; j++)
+  double2 v[NC];
+#pragma unroll
+  for (int j = 0; j < NC; j++)
     v[j] = make_double2((double)src[base + (2 * j) * stride], (double)src[base + (2 * j + 1) * stride]);
-  site_store(f, rhs, cb, v);
+  site_store<NC>(f, rhs, cb, v);
 }
 
-template <class H>
+template <class H, int NC>
 __global__ void k_export(Field f, H *dst, SrcMap m) {
   int cb = blockIdx.x * blockDim.x + threadIdx.x;
   int rhs = blockIdx.y;
   if (cb >= f.g->vh) return;
   int64_t base, stride;
   m.map(*f.g, rhs, f.parity, cb, base, stride);
-  double2 v[12];
-  site_load(f, rhs, cb, v);
-  for (int j = 0; j < f.ncomp; j++) {
+  double2 v[NC];
+  site_load<NC>(f, rhs, cb, v);
+#pragma unroll
+  for (int j = 0; j < NC; j++) {
     dst[base + (2 * j) * stride] = (H)v[j].x;
     dst[base + (2 * j + 1) * stride] = (H)v[j].y;
   }
 }
 
+template <int NC>
 __global__ void k_gaussian(Field f, uint64_t seed, uint64_t stream) {
   int cb = blockIdx.x * blockDim.x + threadIdx.x;
   int rhs = blockIdx.y;
   if (cb >= f.g->vh) return;
   Site s = site_of(*f.g, cb, f.parity);
-  double2 v[12];
-  for (int j = 0; j < f.ncomp; j++) {
+  double2 v[NC];
+#pragma unroll
+  for (int j = 0; j < NC; j++) {
     double a, b;
     gauss2(seed, stream + rhs, (uint64_t)s.lex, j, a, b);
     v[j] = make_double2(a * 0.70710678118654752440, b * 0.70710678118654752440);
   }
-  site_store(f, rhs, cb, v);
+  site_store<NC>(f, rhs, cb, v);
 }
 
 // --------------------------------------------------------------- BLAS-1
+template <int NC>
 __global__ void k_norm_dot(Field a, Field b, int rhs, int dot, RedScratch rs) {
   double acc[1] = {0.0};
   for (int cb = blockIdx.x * blockDim.x + threadIdx.x; cb < a.g->vh; cb += gridDim.x * blockDim.x) {
-    double2 va[12], vb[12];
-    site_load(a, rhs, cb, va);
-    if (dot) site_load(b, rhs, cb, vb);
-    for (int j = 0; j < a.ncomp; j++) {
+    double2 va[NC], vb[NC];
+    site_load<NC>(a, rhs, cb, va);
+    if (dot) site_load<NC>(b, rhs, cb, vb);
+#pragma unroll
+    for (int j = 0; j < NC; j++) {
       if (dot) acc[0] += va[j].x * vb[j].x + va[j].y * vb[j].y;
       else acc[0] += va[j].x * va[j].x + va[j].y * va[j].y;
     }
@@ -77,17 +83,19 @@ struct StoreTo {
   __device__ void operator()(const double *t) const { *dst = t[0]; }
 };
 
+template <int NC>
 __global__ void k_axpby(double a, Field x, double b, Field y) {
   int cb = blockIdx.x * blockDim.x + threadIdx.x;
   int rhs = blockIdx.y;
   if (cb >= y.g->vh) return;
-  double2 vx[12], vy[12];
-  site_load(x, rhs, cb, vx);
-  if (b != 0.0) site_load(y, rhs, cb, vy);
-  for (int j = 0; j < y.ncomp; j++)
+  double2 vx[NC], vy[NC];
+  site_load<NC>(x, rhs, cb, vx);
+  if (b != 0.0) site_load<NC>(y, rhs, cb, vy);
+#pragma unroll
+  for (int j = 0; j < NC; j++)
     vy[j] = (b != 0.0) ? make_double2(a * vx[j].x + b * vy[j].x, a * vx[j].y + b * vy[j].y)
                        : make_double2(a * vx[j].x, a * vx[j].y);
-  site_store(y, rhs, cb, vy);
+  site_store<NC>(y, rhs, cb, vy);
 }
 
 // ------------------------------------------------------------- helpers
@@ -109,7 +117,8 @@ int reduce_field(const Field &a, const Field *b, int rhs, double *out) {
   StoreTo fin{slot};
   Field bb = b ? *b : a;
   RedScratch rs = red_set(0);
-  hipLaunchKernelGGL(k_norm_dot, dim3(nb), dim3(256), 0, stream(), a, bb, rhs, b ? 1 : 0, rs);
+  hipLaunchKernelGGL((a.ncomp == 3 ? k_norm_dot<3> : k_norm_dot<12>), dim3(nb), dim3(256), 0, stream(), a, bb, rhs,
+                     b ? 1 : 0, rs);
   launch_finish<1>(rs, nb, fin, stream());
   QAMD_HIP(hipGetLastError());
   QAMD_HIP(hipMemcpyAsync(out, slot, sizeof(double), hipMemcpyDeviceToHost, stream()));
@@ -225,10 +234,13 @@ static int import_common(qamd_field *f, const void *host, size_t bytes, int hpre
     return -1;
   }
   dim3 grid = site_grid(f->lat->g, f->f.nrhs);
+  const bool c3 = f->f.ncomp == 3;
   if (hprec == QAMD_F64)
-    hipLaunchKernelGGL(k_import<double>, grid, dim3(256), 0, stream(), f->f, (const double *)stage, m);
+    hipLaunchKernelGGL((c3 ? k_import<double, 3> : k_import<double, 12>), grid, dim3(256), 0, stream(), f->f,
+                       (const double *)stage, m);
   else
-    hipLaunchKernelGGL(k_import<float>, grid, dim3(256), 0, stream(), f->f, (const float *)stage, m);
+    hipLaunchKernelGGL((c3 ? k_import<float, 3> : k_import<float, 12>), grid, dim3(256), 0, stream(), f->f,
+                       (const float *)stage, m);
   e = hipGetLastError();
   if (e == hipSuccess) e = hipStreamSynchronize(stream());
   dev_free(stage);
@@ -246,10 +258,13 @@ static int export_common(const qamd_field *f, void *host, size_t bytes, int hpre
   if (prefill) e = hipMemcpyAsync(stage, host, bytes, hipMemcpyHostToDevice, stream());
   dim3 grid = site_grid(f->lat->g, f->f.nrhs);
   if (e == hipSuccess) {
+    const bool c3 = f->f.ncomp == 3;
     if (hprec == QAMD_F64)
-      hipLaunchKernelGGL(k_export<double>, grid, dim3(256), 0, stream(), f->f, (double *)stage, m);
+      hipLaunchKernelGGL((c3 ? k_export<double, 3> : k_export<double, 12>), grid, dim3(256), 0, stream(), f->f,
+                         (double *)stage, m);
     else
-      hipLaunchKernelGGL(k_export<float>, grid, dim3(256), 0, stream(), f->f, (float *)stage, m);
+      hipLaunchKernelGGL((c3 ? k_export<float, 3> : k_export<float, 12>), grid, dim3(256), 0, stream(), f->f,
+                         (float *)stage, m);
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipStreamSynchronize(stream());
@@ -326,7 +341,8 @@ int qamd_field_to_qll(const qamd_field *f, const qamd_qll_layout *l, void *host,
 }
 
 int qamd_field_gaussian(qamd_field *f, uint64_t seed, uint64_t strm) {
-  hipLaunchKernelGGL(k_gaussian, site_grid(f->lat->g, f->f.nrhs), dim3(256), 0, stream(), f->f, seed, strm);
+  hipLaunchKernelGGL((f->f.ncomp == 3 ? k_gaussian<3> : k_gaussian<12>), site_grid(f->lat->g, f->f.nrhs), dim3(256),
+                     0, stream(), f->f, seed, strm);
   QAMD_HIP(hipGetLastError());
   return 0;
 }
@@ -352,7 +368,8 @@ int qamd_field_axpby(double a, const qamd_field *x, double b, qamd_field *y) {
     set_error("axpby: shape mismatch");
     return -1;
   }
-  hipLaunchKernelGGL(k_axpby, site_grid(y->lat->g, y->f.nrhs), dim3(256), 0, stream(), a, x->f, b, y->f);
+  hipLaunchKernelGGL((y->f.ncomp == 3 ? k_axpby<3> : k_axpby<12>), site_grid(y->lat->g, y->f.nrhs), dim3(256), 0,
+                     stream(), a, x->f, b, y->f);
   QAMD_HIP(hipGetLastError());
   return 0;
 }

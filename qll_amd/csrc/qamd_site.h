@@ -23,38 +23,50 @@ template <> struct IO<QAMD_F32> {
   }
 };
 
-// generic (slow path) access to a whole site: ncomp complex numbers in double
+// generic (slow path) access to a whole site: NC complex numbers in double
+// (NC = the field's ncomp, 3 or 12, fixed at compile time so the site stays
+// in registers)
+template <int NC>
 __device__ inline void site_load(const Field &f, int rhs, int cb, double2 *v) {
   if (f.prec == QAMD_F64) {
-    for (int j = 0; j < f.ncomp; j++) v[j] = IO<QAMD_F64>::ld(f, f.cidx(rhs, j, cb));
+#pragma unroll
+    for (int j = 0; j < NC; j++) v[j] = IO<QAMD_F64>::ld(f, f.cidx(rhs, j, cb));
   } else if (f.prec == QAMD_F32) {
-    for (int j = 0; j < f.ncomp; j++) v[j] = IO<QAMD_F32>::ld(f, f.cidx(rhs, j, cb));
+#pragma unroll
+    for (int j = 0; j < NC; j++) v[j] = IO<QAMD_F32>::ld(f, f.cidx(rhs, j, cb));
   } else if (f.prec == QAMD_F16) {
     float nrm = f.norm[f.nidx(rhs, cb)];
-    for (int j = 0; j < f.ncomp; j++) {
+#pragma unroll
+    for (int j = 0; j < NC; j++) {
       float2 h = __half22float2(((const __half2 *)f.data)[f.cidx(rhs, j, cb)]);
       v[j] = make_double2((double)h.x * nrm, (double)h.y * nrm);
     }
   } else {
     double nrm = (double)f.norm[f.nidx(rhs, cb)] / Q16_ONE;
-    for (int j = 0; j < f.ncomp; j++) {
+#pragma unroll
+    for (int j = 0; j < NC; j++) {
       short2 h = ((const short2 *)f.data)[f.cidx(rhs, j, cb)];
       v[j] = make_double2(h.x * nrm, h.y * nrm);
     }
   }
 }
 
+template <int NC>
 __device__ inline void site_store(Field &f, int rhs, int cb, const double2 *v) {
   if (f.prec == QAMD_F64) {
-    for (int j = 0; j < f.ncomp; j++) IO<QAMD_F64>::st(f, f.cidx(rhs, j, cb), v[j]);
+#pragma unroll
+    for (int j = 0; j < NC; j++) IO<QAMD_F64>::st(f, f.cidx(rhs, j, cb), v[j]);
   } else if (f.prec == QAMD_F32) {
-    for (int j = 0; j < f.ncomp; j++) IO<QAMD_F32>::st(f, f.cidx(rhs, j, cb), v[j]);
+#pragma unroll
+    for (int j = 0; j < NC; j++) IO<QAMD_F32>::st(f, f.cidx(rhs, j, cb), v[j]);
   } else {
     double m = 0;
-    for (int j = 0; j < f.ncomp; j++) m = fmax(m, fmax(fabs(v[j].x), fabs(v[j].y)));
+#pragma unroll
+    for (int j = 0; j < NC; j++) m = fmax(m, fmax(fabs(v[j].x), fabs(v[j].y)));
     float nrm = (float)m;
     float inv = nrm > 0 ? 1.0f / nrm : 0.0f;
-    for (int j = 0; j < f.ncomp; j++) {
+#pragma unroll
+    for (int j = 0; j < NC; j++) {
       if (f.prec == QAMD_F16)
         ((__half2 *)f.data)[f.cidx(rhs, j, cb)] = __floats2half2_rn((float)v[j].x * inv, (float)v[j].y * inv);
       else

@@ -199,22 +199,19 @@ __global__ void k_ru_resid(Field rhi, Field b, Field Ax, Field rs, SolverDev *S,
 
 // ------------------------------------------------------- multishift kernels
 // coefficients of cgmsSolveT (lib/cgms.c:78-89) from the current state
-__device__ inline void ms_coefs(const SolverDev *S, double pkp, double *b, double *zn) {
-  b[0] = S->rsq / pkp;
-  zn[0] = 1;
-  for (int i = 1; i < S->nshift; i++) {
-    double z = S->z[i], zo = S->zo[i], bo0 = S->bo[0];
-    double zni = z * zo * bo0;
-    double c1 = b[0] * S->a[0] * (zo - z);
-    c1 += zo * bo0 * (1 + S->shifts[i] * b[0]);
-    if (c1 != 0.0) zni /= c1;
-    else zni = 0;
-    double bi;
-    if (z != 0.0) bi = b[0] * zni / z;
-    else zni = bi = 0;
-    zn[i] = zni;
-    b[i] = bi;
-  }
+// shift i >= 1 of the multishift recurrences (lib/cgms.c:79-88): the new
+// zeta_i and beta_i from beta_0 of this iteration and the previous iteration's
+// alpha_0, beta_0 (a00, bo0), computed per shift so nothing is indexed at run time
+__device__ inline void ms_coef(const SolverDev *S, int i, double b0, double a00, double bo0, double &bi,
+                               double &zni) {
+  double z = S->z[i], zo = S->zo[i];
+  zni = z * zo * bo0;
+  double c1 = b0 * a00 * (zo - z);
+  c1 += zo * bo0 * (1 + S->shifts[i] * b0);
+  if (c1 != 0.0) zni /= c1;
+  else zni = 0;
+  if (z != 0.0) bi = b0 * zni / z;
+  else zni = bi = 0;
 }
 
 struct MSPtrs {
@@ -228,24 +225,25 @@ struct FinMS {
   SolverDev *S;
   __device__ void operator()(const double *t) const {
     double pkp = (S->m2 * S->nrm[3] + S->nrm[0]) * S->s[0] * S->s[0];
-    double b[MAXSHIFT], zn[MAXSHIFT];
-    ms_coefs(S, pkp, b, zn);
+    const double b0 = S->rsq / pkp, a00 = S->a[0], bo0 = S->bo[0];
     double oldrsq = S->rsq;
     S->rsq = t[0];
     S->its += 1;
     if (S->its >= S->itnlim || S->rsqstop <= 0 || S->rsq < S->rsqstop) S->done = 1;
     double a0 = S->rsq / oldrsq;
-    S->pa[0] = a0;
-    for (int i = 1; i < S->nshift; i++) {
-      double c2 = S->z[i] * b[0];
-      S->pa[i] = (c2 != 0.0) ? a0 * zn[i] * b[i] / c2 : 0;
-    }
     for (int i = 0; i < S->nshift; i++) {
-      S->pzn[i] = zn[i];
-      S->a[i] = S->pa[i];
-      S->bo[i] = b[i];
+      double bi = b0, zni = 1, pai = a0;
+      if (i > 0) {
+        ms_coef(S, i, b0, a00, bo0, bi, zni);
+        double c2 = S->z[i] * b0;
+        pai = (c2 != 0.0) ? a0 * zni * bi / c2 : 0;
+      }
+      S->pa[i] = pai;
+      S->pzn[i] = zni;
+      S->a[i] = pai;
+      S->bo[i] = bi;
       S->zo[i] = S->z[i];
-      S->z[i] = zn[i];
+      S->z[i] = zni;
     }
   }
 };
@@ -265,8 +263,7 @@ __global__ void __launch_bounds__(256) k_ms_x(MSPtrs V, Field r, Field Ap, Solve
     }
     return;
   }
-  double b[MAXSHIFT], zn[MAXSHIFT];
-  ms_coefs(S, pkp, b, zn);
+  const double b0 = S->rsq / pkp, a00 = S->a[0], bo0 = S->bo[0];
   const int64_t vp = r.vp;
   double red[1] = {0.0};
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -275,7 +272,9 @@ __global__ void __launch_bounds__(256) k_ms_x(MSPtrs V, Field r, Field Ap, Solve
       C xv[3], pv[3];
       VecLd<P>::ld((const Sx *)V.x[k], V.xn[k], vp, i, xv, (R)1);
       VecLd<P>::ld((const Sx *)V.p[k], V.pn[k], vp, i, pv, (R)1);
-      R t = (R)(b[k] * S->s[k]);
+      double bk = b0, znk;
+      if (k > 0) ms_coef(S, k, b0, a00, bo0, bk, znk);
+      R t = (R)(bk * S->s[k]);
 #pragma unroll
       for (int j = 0; j < 3; j++) {
         xv[j].x = fma(t, pv[j].x, xv[j].x);
@@ -286,7 +285,7 @@ __global__ void __launch_bounds__(256) k_ms_x(MSPtrs V, Field r, Field Ap, Solve
     C rv[3], av[3];
     VT<P>::ld(r, i, rv);
     VT<P>::ld(Ap, i, av);
-    R t0 = (R)(b[0] * S->s[0]);
+    R t0 = (R)(b0 * S->s[0]);
 #pragma unroll
     for (int j = 0; j < 3; j++) {
       rv[j].x = fma(-t0, av[j].x, rv[j].x);

@@ -15,14 +15,15 @@ namespace qamd {
 
 constexpr int MAXRHS = 16;
 
-__global__ void k_gamma5(Field f) {
+__global__ void __launch_bounds__(256) k_gamma5(Field f) {
   int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int cb = (int)(g / f.nrhs), r = (int)(g % f.nrhs);
   if (cb >= f.g->vh) return;
   double2 v[12];
-  site_load(f, r, cb, v);
+  site_load<12>(f, r, cb, v);
+#pragma unroll
   for (int j = 6; j < 12; j++) v[j] = make_double2(-v[j].x, -v[j].y);
-  site_store(f, r, cb, v);
+  site_store<12>(f, r, cb, v);
 }
 
 struct FinRhs {
@@ -42,10 +43,11 @@ __global__ void __launch_bounds__(256) k_rhs_dot(Field f, Field h, int mode, Red
   for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (int64_t)gridDim.x * blockDim.x) {
     int cb = (int)(g / f.nrhs), r = (int)(g % f.nrhs);
     double2 a[12], b[12];
-    site_load(f, r, cb, a);
-    if (mode) site_load(h, r, cb, b);
+    site_load<12>(f, r, cb, a);
+    if (mode) site_load<12>(h, r, cb, b);
     double s = 0;
-    for (int j = 0; j < f.ncomp; j++) s += mode ? a[j].x * b[j].x + a[j].y * b[j].y : a[j].x * a[j].x + a[j].y * a[j].y;
+#pragma unroll
+    for (int j = 0; j < 12; j++) s += mode ? a[j].x * b[j].x + a[j].y * b[j].y : a[j].x * a[j].x + a[j].y * a[j].y;
 #pragma unroll
     for (int i = 0; i < MAXRHS; i++)
       if (i == r) acc[i] += s;
@@ -54,18 +56,19 @@ __global__ void __launch_bounds__(256) k_rhs_dot(Field f, Field h, int mode, Red
 }
 
 // y_r = a_r x_r + b_r y_r  (x, y may differ in precision)
-__global__ void k_rhs_axpby(Field x, Field y, const double *a, const double *b) {
+__global__ void __launch_bounds__(256) k_rhs_axpby(Field x, Field y, const double *a, const double *b) {
   int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int cb = (int)(g / y.nrhs), r = (int)(g % y.nrhs);
   if (cb >= y.g->vh) return;
   double2 vx[12], vy[12];
   double ar = a[r], br = b[r];
-  site_load(x, r, cb, vx);
-  if (br != 0.0) site_load(y, r, cb, vy);
-  for (int j = 0; j < y.ncomp; j++)
+  site_load<12>(x, r, cb, vx);
+  if (br != 0.0) site_load<12>(y, r, cb, vy);
+#pragma unroll
+  for (int j = 0; j < 12; j++)
     vy[j] = br != 0.0 ? make_double2(ar * vx[j].x + br * vy[j].x, ar * vx[j].y + br * vy[j].y)
                       : make_double2(ar * vx[j].x, ar * vx[j].y);
-  site_store(y, r, cb, vy);
+  site_store<12>(y, r, cb, vy);
 }
 
 struct WS {   // work space of one precision
