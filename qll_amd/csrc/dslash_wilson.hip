@@ -9,13 +9,15 @@
 // s A^+ h.  Each term therefore multiplies only the 2-spin half spinor h by
 // the link (2 mat-vecs instead of 4) and reconstructs the lower half.
 //
-// One thread per (site, rhs); with several right-hand sides (rhs fastest in
-// memory) the lanes of a wavefront that share a site load the same link
-// elements, so each link is fetched once for all of them.
+// One thread per (site, rhs); spinors are stored in 16-byte chunks with the
+// rhs next to each other, so with several right-hand sides the lanes of a
+// wavefront that share a site load the same link elements (each link is
+// fetched once for all of them) and read consecutive chunks.
 #include "qamd_internal.h"
 #include "qamd_objects.h"
 #include "qamd_kernels.h"
 
+#include <stdio.h>
 #include <string.h>
 
 namespace qamd {
@@ -32,39 +34,75 @@ template <int P> struct WArgs {
   RedScratch rs;
   double *red_out;
   const int *skip;
+  int xcd;     // XCD-aware block order (each die takes one contiguous range of sites)
+  int tiled;   // sites in the tiled order of tl (else checkerboard order)
+  Tiling tl;
 };
 
 template <class C> __device__ __forceinline__ C cadd(C a, C b) { a.x += b.x; a.y += b.y; return a; }
 template <class C> __device__ __forceinline__ C csub(C a, C b) { a.x -= b.x; a.y -= b.y; return a; }
-template <class C> __device__ __forceinline__ C cmi(C a) { C r; r.x = a.y; r.y = -a.x; return r; }   // -i a
-template <class C> __device__ __forceinline__ C cpi(C a) { C r; r.x = -a.y; r.y = a.x; return r; }   // +i a
-template <class C> __device__ __forceinline__ C cneg(C a) { a.x = -a.x; a.y = -a.y; return a; }
 
-template <int P> struct SpIO;
-template <> struct SpIO<QAMD_F64> {
-  __device__ static double2 ld(const Field &f, int64_t i, float) { return ((const double2 *)f.data)[i]; }
-  __device__ static void st(const Field &f, int64_t i, double2 v) { ((double2 *)f.data)[i] = v; }
+// One 16-byte chunk of a spinor: K complex components of one (site, rhs)
+// (K = 1 FP64, 2 FP32, 4 FP16/Q16), scaled by the site norm n for 16-bit
+// storage.  A wavefront's vector-memory instructions are issued at a fixed
+// rate whatever their width, so full 16-byte lanes carry the most data.
+template <int P> struct Chunk;
+template <> struct Chunk<QAMD_F64> {
+  static constexpr int K = 1;
+  __device__ static void ld(const void *d, int64_t i, float, double2 (&v)[1]) { v[0] = ((const double2 *)d)[i]; }
+  __device__ static void st(void *d, int64_t i, const double2 (&v)[1]) { ((double2 *)d)[i] = v[0]; }
 };
-template <> struct SpIO<QAMD_F32> {
-  __device__ static float2 ld(const Field &f, int64_t i, float) { return ((const float2 *)f.data)[i]; }
-  __device__ static void st(const Field &f, int64_t i, float2 v) { ((float2 *)f.data)[i] = v; }
-};
-template <> struct SpIO<QAMD_F16> {
-  __device__ static float2 ld(const Field &f, int64_t i, float n) {
-    float2 v = __half22float2(((const __half2 *)f.data)[i]);
-    v.x *= n;
-    v.y *= n;
-    return v;
+template <> struct Chunk<QAMD_F32> {
+  static constexpr int K = 2;
+  __device__ static void ld(const void *d, int64_t i, float, float2 (&v)[2]) {
+    fvec4_t u = ((const fvec4_t *)d)[i];
+    v[0] = make_float2(u.x, u.y);
+    v[1] = make_float2(u.z, u.w);
   }
-  __device__ static void st(const Field &f, int64_t i, float2 v) { ((__half2 *)f.data)[i] = __floats2half2_rn(v.x, v.y); }
+  __device__ static void st(void *d, int64_t i, const float2 (&v)[2]) {
+    fvec4_t u = {v[0].x, v[0].y, v[1].x, v[1].y};
+    ((fvec4_t *)d)[i] = u;
+  }
 };
-template <> struct SpIO<QAMD_Q16> {
-  __device__ static float2 ld(const Field &f, int64_t i, float n) {
-    short2 s = ((const short2 *)f.data)[i];
+template <> struct Chunk<QAMD_F16> {
+  static constexpr int K = 4;
+  __device__ static void ld(const void *d, int64_t i, float n, float2 (&v)[4]) {
+    uvec4_t u = ((const uvec4_t *)d)[i];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      unsigned w = u[k];
+      float2 h = __half22float2(*reinterpret_cast<__half2 *>(&w));
+      v[k] = make_float2(h.x * n, h.y * n);
+    }
+  }
+  __device__ static void st(void *d, int64_t i, const float2 (&v)[4]) {
+    uvec4_t u;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      __half2 h = __floats2half2_rn(v[k].x, v[k].y);
+      u[k] = *reinterpret_cast<unsigned *>(&h);
+    }
+    ((uvec4_t *)d)[i] = u;
+  }
+};
+template <> struct Chunk<QAMD_Q16> {
+  static constexpr int K = 4;
+  __device__ static void ld(const void *d, int64_t i, float n, float2 (&v)[4]) {
+    uvec4_t u = ((const uvec4_t *)d)[i];
     n *= 1.0f / Q16_ONE;
-    return make_float2(s.x * n, s.y * n);
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+      v[k] = make_float2((float)(short)(u[k] & 0xffff) * n, (float)(short)(u[k] >> 16) * n);
   }
-  __device__ static void st(const Field &f, int64_t i, float2 v) { ((short2 *)f.data)[i] = q16_pack(v.x, v.y); }
+  __device__ static void st(void *d, int64_t i, const float2 (&v)[4]) {
+    uvec4_t u;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      short2 q = q16_pack(v[k].x, v[k].y);
+      u[k] = (unsigned)(unsigned short)q.x | ((unsigned)(unsigned short)q.y << 16);
+    }
+    ((uvec4_t *)d)[i] = u;
+  }
 };
 
 struct FinStore1 {
@@ -83,12 +121,15 @@ __global__ void __launch_bounds__(256) k_wilson(WArgs<P> A) {
   const Field fout = A.out;
   const Geom g = A.g;
   const int nr = fin.nrhs;
-  int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int cb = (int)(gid / nr), r = (int)(gid - (int64_t)cb * nr);
+  const int blk = A.xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+  int64_t gid = (int64_t)blk * blockDim.x + threadIdx.x;
+  const int li = (int)(gid / nr), r = (int)(gid - (int64_t)li * nr);
   double red[1] = {0.0};
-  if (cb < g.vh) {
-    Site s = site_of(g, cb, A.par);
+  if (li < g.vh) {
+    const Site s = A.tiled ? site_of_tiled(g, A.tl, li, A.par) : site_of(g, li, A.par);
+    const int cb = s.lex >> 1;
     constexpr int G = LinkG<S>::G;
+    constexpr int K = Chunk<P>::K;
     const S *lp = link_base<S, 8>(A.links, cb);
     C acc[4][3];
 #pragma unroll
@@ -114,18 +155,24 @@ __global__ void __launch_bounds__(256) k_wilson(WArgs<P> A) {
       if constexpr (P == QAMD_F16 || P == QAMD_Q16) nrm = fin.norm[fin.nidx(r, nb)] * A.lscale[t];
       C psi[4][3];
 #pragma unroll
-      for (int a = 0; a < 4; a++)
+      for (int q = 0; q < 12 / K; q++) {
+        C v[K];
+        Chunk<P>::ld(fin.data, fin.chunk(r, q, nb), nrm, v);
 #pragma unroll
-        for (int c = 0; c < 3; c++) psi[a][c] = SpIO<P>::ld(fin, fin.cidx(r, 3 * a + c, nb), nrm);
-      // h = psi_up + s A psi_dn, s = +1 for backward terms
+        for (int k = 0; k < K; k++) psi[(q * K + k) / 3][(q * K + k) % 3] = v[k];
+      }
+      // h = psi_up + s A psi_dn, s = +1 for backward terms.  A psi_dn per mu:
+      //   mu 0: (i p3, i p2)  mu 1: (-p3, p2)  mu 2: (i p2, -i p3)  mu 3: (p2, p3)
+      // written as component selects (no branches over mu)
       C h[2][3];
 #pragma unroll
       for (int c = 0; c < 3; c++) {
-        C d0, d1;   // A psi_dn
-        if (mu == 0) { d0 = cpi(psi[3][c]); d1 = cpi(psi[2][c]); }
-        else if (mu == 1) { d0 = cneg(psi[3][c]); d1 = psi[2][c]; }
-        else if (mu == 2) { d0 = cpi(psi[2][c]); d1 = cmi(psi[3][c]); }
-        else { d0 = psi[2][c]; d1 = psi[3][c]; }
+        const C p2 = psi[2][c], p3 = psi[3][c];
+        C d0, d1;
+        d0.x = sel4(mu, -p3.y, -p3.x, -p2.y, p2.x);
+        d0.y = sel4(mu, p3.x, -p3.y, p2.x, p2.y);
+        d1.x = sel4(mu, -p2.y, p2.x, p3.y, p3.x);
+        d1.y = sel4(mu, p2.x, p2.y, -p3.x, p3.y);
         h[0][c] = bwd ? cadd(psi[0][c], d0) : csub(psi[0][c], d0);
         h[1][c] = bwd ? cadd(psi[1][c], d1) : csub(psi[1][c], d1);
       }
@@ -153,11 +200,14 @@ __global__ void __launch_bounds__(256) k_wilson(WArgs<P> A) {
       for (int c = 0; c < 3; c++) {
         acc[0][c] = cadd(acc[0][c], uh[0][c]);
         acc[1][c] = cadd(acc[1][c], uh[1][c]);
-        C l0, l1;   // A^+ chi
-        if (mu == 0) { l0 = cmi(uh[1][c]); l1 = cmi(uh[0][c]); }
-        else if (mu == 1) { l0 = uh[1][c]; l1 = cneg(uh[0][c]); }
-        else if (mu == 2) { l0 = cmi(uh[0][c]); l1 = cpi(uh[1][c]); }
-        else { l0 = uh[0][c]; l1 = uh[1][c]; }
+        // A^+ chi per mu:  mu 0: (-i u1, -i u0)  mu 1: (u1, -u0)
+        //                  mu 2: (-i u0, i u1)   mu 3: (u0, u1)
+        const C u0 = uh[0][c], u1 = uh[1][c];
+        C l0, l1;
+        l0.x = sel4(mu, u1.y, u1.x, u0.y, u0.x);
+        l0.y = sel4(mu, -u1.x, u1.y, -u0.x, u0.y);
+        l1.x = sel4(mu, u0.y, -u0.x, -u1.y, u1.x);
+        l1.y = sel4(mu, -u0.x, -u0.y, u1.x, u1.y);
         acc[2][c] = bwd ? cadd(acc[2][c], l0) : csub(acc[2][c], l0);
         acc[3][c] = bwd ? cadd(acc[3][c], l1) : csub(acc[3][c], l1);
       }
@@ -165,23 +215,28 @@ __global__ void __launch_bounds__(256) k_wilson(WArgs<P> A) {
     }
     const R b = (R)A.b, a = (R)A.a;
     C o[4][3];
-    float wn = 1.0f;
-    if constexpr (P == QAMD_F16 || P == QAMD_Q16)
-      if (A.has_w) wn = fw.norm[fw.nidx(r, cb)];
 #pragma unroll
     for (int sp = 0; sp < 4; sp++)
 #pragma unroll
       for (int c = 0; c < 3; c++) {
-        C v;
-        v.x = b * acc[sp][c].x;
-        v.y = b * acc[sp][c].y;
-        if (A.has_w) {
-          C wv = SpIO<P>::ld(fw, fw.cidx(r, 3 * sp + c, cb), wn);
-          v.x = fma(a, wv.x, v.x);
-          v.y = fma(a, wv.y, v.y);
-        }
-        o[sp][c] = v;
+        o[sp][c].x = b * acc[sp][c].x;
+        o[sp][c].y = b * acc[sp][c].y;
       }
+    if (A.has_w) {
+      float wn = 1.0f;
+      if constexpr (P == QAMD_F16 || P == QAMD_Q16) wn = fw.norm[fw.nidx(r, cb)];
+#pragma unroll
+      for (int q = 0; q < 12 / K; q++) {
+        C wv[K];
+        Chunk<P>::ld(fw.data, fw.chunk(r, q, cb), wn, wv);
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+          C &v = o[(q * K + k) / 3][(q * K + k) % 3];
+          v.x = fma(a, wv[k].x, v.x);
+          v.y = fma(a, wv[k].y, v.y);
+        }
+      }
+    }
     float on = 1.0f;
     constexpr bool NORMED = (P == QAMD_F16 || P == QAMD_Q16);
     if (NORMED) {
@@ -194,20 +249,55 @@ __global__ void __launch_bounds__(256) k_wilson(WArgs<P> A) {
       on = mx > 0.f ? 1.0f / mx : 0.f;
     }
 #pragma unroll
-    for (int sp = 0; sp < 4; sp++)
+    for (int q = 0; q < 12 / K; q++) {
+      C v[K];
 #pragma unroll
-      for (int c = 0; c < 3; c++) {
-        C v = o[sp][c];
-        if (RED) red[0] += (double)v.x * v.x + (double)v.y * v.y;
+      for (int k = 0; k < K; k++) {
+        v[k] = o[(q * K + k) / 3][(q * K + k) % 3];
+        if (RED) red[0] += (double)v[k].x * v[k].x + (double)v[k].y * v[k].y;
         if (NORMED) {
-          v.x *= on;
-          v.y *= on;
+          v[k].x *= on;
+          v[k].y *= on;
         }
-        SpIO<P>::st(fout, fout.cidx(r, 3 * sp + c, cb), v);
       }
+      Chunk<P>::st(fout.data, fout.chunk(r, q, cb), v);
+    }
   }
-  if (RED) grid_partial<1>(red, A.rs);
+  if (RED) grid_partial_at(red, A.rs, blk);   // indexed by site block: same sums in either order
 }
+
+// launch shapes the tuner tries: block order and site tiling (log2 extents)
+struct WShape {
+  int xcd, tile[4];
+};
+constexpr WShape WSHAPES[] = {
+    {0, {0, 0, 0, 0}}, {1, {0, 0, 0, 0}}, {1, {3, 3, 3, 3}}, {1, {2, 2, 2, 2}},
+    {1, {4, 3, 3, 3}}, {1, {3, 3, 2, 2}}, {1, {2, 2, 2, 4}}, {1, {5, 2, 2, 2}},
+};
+constexpr int NWSHAPE = sizeof(WSHAPES) / sizeof(WSHAPES[0]);
+
+template <int P> struct WRun {
+  WArgs<P> A;
+  int nb;
+  hipStream_t st;
+  int ncand, shape[NWSHAPE];
+  static int run(int c, void *ctx) {
+    WRun &R = *(WRun *)ctx;
+    WArgs<P> B = R.A;
+    const WShape &sh = WSHAPES[R.shape[c]];
+    B.xcd = sh.xcd;
+    B.tiled = sh.tile[0] ? 1 : 0;
+    if (B.tiled && !B.tl.init(B.g.L, sh.tile)) return -1;
+    if (B.red_out) {
+      hipLaunchKernelGGL((k_wilson<P, true>), dim3(R.nb), dim3(256), 0, R.st, B);
+      launch_finish<1>(B.rs, R.nb, FinStore1{B.red_out}, R.st, B.skip);
+    } else {
+      hipLaunchKernelGGL((k_wilson<P, false>), dim3(R.nb), dim3(256), 0, R.st, B);
+    }
+    QAMD_HIP(hipGetLastError());
+    return 0;
+  }
+};
 
 template <int P>
 static int wlaunch(const qamd_links *l, Field &out, const Field &in, double a, const Field *w, double b,
@@ -229,18 +319,29 @@ static int wlaunch(const qamd_links *l, Field &out, const Field &in, double a, c
   A.skip = skip;
   for (int t = 0; t < 8; t++) A.lscale[t] = (P == QAMD_Q16) ? l->scale[t] / Q16_ONE : l->scale[t];
   int64_t nthreads = A.g.vh * in.nrhs;
-  int nb = (int)((nthreads + 255) / 256);
+  WRun<P> R;
+  R.A = A;
+  R.nb = (int)((nthreads + 255) / 256);
+  R.st = st;
   if (red_out) {
-    if (red_reserve(nb)) return -1;
-    A.rs = red_set(0);
-    A.red_out = red_out;
-    hipLaunchKernelGGL((k_wilson<P, true>), dim3(nb), dim3(256), 0, st, A);
-    launch_finish<1>(A.rs, nb, FinStore1{red_out}, st, skip);
-  } else {
-    hipLaunchKernelGGL((k_wilson<P, false>), dim3(nb), dim3(256), 0, st, A);
+    if (red_reserve(R.nb)) return -1;
+    R.A.rs = red_set(0);
+    R.A.red_out = red_out;
   }
-  QAMD_HIP(hipGetLastError());
-  return 0;
+  // with a reduction only the block order may change (a tiled order would
+  // regroup the per-block partial sums and change the rounding)
+  R.ncand = 0;
+  for (int k = 0; k < (red_out ? 2 : NWSHAPE); k++) {
+    Tiling t;
+    if (!WSHAPES[k].tile[0] || t.init(A.g.L, WSHAPES[k].tile)) R.shape[R.ncand++] = k;
+  }
+  // the tuner's trial launches must not feed the output back into an input
+  const bool alias = out.data == in.data || (w && out.data == w->data);
+  char key[96];
+  snprintf(key, sizeof key, "wilson p%d n%d red%d %dx%dx%dx%d", P, in.nrhs, red_out ? 1 : 0, A.g.L[0], A.g.L[1],
+           A.g.L[2], A.g.L[3]);
+  int c = alias ? 1 : tune_pick(key, R.ncand, 1, &WRun<P>::run, &R);
+  return WRun<P>::run(c, &R);
 }
 
 int wilson_dslash_launch(const qamd_links *l, Field &out, const Field &in, double a, const Field *w, double b,

@@ -173,7 +173,8 @@ qamd_field *qamd_field_create(qamd_lattice *lat, int prec, int ncomp, int nrhs, 
   f->f.parity = parity;
   f->f.g = dg;
   f->f.vp = lat->g.vh_pad;
-  f->f.rhs_inner = (ncomp == 12 && nrhs > 1);
+  f->f.rhs_inner = (ncomp == 12);
+  f->f.kc = prec == QAMD_F64 ? 1 : prec == QAMD_F32 ? 2 : 4;
   int64_t n = (int64_t)nrhs * ncomp * lat->g.vh_pad;
   f->f.data = dev_alloc(n * prec_bytes(prec));
   const bool normed = (prec == QAMD_F16 || prec == QAMD_Q16);

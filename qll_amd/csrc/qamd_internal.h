@@ -98,6 +98,47 @@ __device__ __forceinline__ Site site_of(const Geom &g, int cb, int par) {
   return s;
 }
 
+// Tiled site order: logical site i of parity par walks tiles of
+// 2^s[0] x 2^s[1] x 2^s[2] x 2^s[3] sites (s[0] >= 1), half of them of each
+// parity, x fastest inside a tile and tiles x fastest.  Keeps the neighbours
+// of a sweep inside a small window of recently read sites.
+struct Tiling {
+  int s[4];            // log2 tile extents
+  FastDiv nx, ny, nz;  // tiles per direction
+  __host__ bool init(const int L[4], const int sh[4]) {
+    if (sh[0] < 1) return false;
+    for (int i = 0; i < 4; i++) {
+      s[i] = sh[i];
+      if ((1 << sh[i]) > L[i] || L[i] % (1 << sh[i])) return false;
+    }
+    nx.init(L[0] >> s[0]);
+    ny.init(L[1] >> s[1]);
+    nz.init(L[2] >> s[2]);
+    return true;
+  }
+};
+
+__host__ __device__ __forceinline__ Site site_of_tiled(const Geom &g, const Tiling &T, int i, int par) {
+  const int bx = T.s[0] - 1, per = bx + T.s[1] + T.s[2] + T.s[3];   // log2 sites per parity per tile
+  const uint32_t tile = (uint32_t)i >> per;
+  int j = i & ((1 << per) - 1);
+  const int xh = j & ((1 << bx) - 1);
+  j >>= bx;
+  const int y = j & ((1 << T.s[1]) - 1);
+  j >>= T.s[1];
+  const int z = j & ((1 << T.s[2]) - 1);
+  const int t = j >> T.s[2];
+  const uint32_t q1 = T.nx.div(tile), q2 = T.ny.div(q1), q3 = T.nz.div(q2);
+  Site s;
+  s.c[1] = (((int)q1 - (int)q2 * (int)T.ny.d) << T.s[1]) + y;
+  s.c[2] = (((int)q2 - (int)q3 * (int)T.nz.d) << T.s[2]) + z;
+  s.c[3] = ((int)q3 << T.s[3]) + t;
+  const int xb = (s.c[1] + s.c[2] + s.c[3] + par) & 1;
+  s.c[0] = (((int)tile - (int)q1 * (int)T.nx.d) << T.s[0]) + 2 * xh + xb;
+  s.lex = s.c[0] + s.c[1] * g.stride[1] + s.c[2] * g.stride[2] + s.c[3] * g.stride[3];
+  return s;
+}
+
 // checkerboard index of the neighbour at hop h (|h| <= 3 < L) along mu
 __device__ __forceinline__ int nbr_cb(const Geom &g, const Site &s, int mu, int h) {
   int c = s.c[mu] + h;
@@ -112,7 +153,8 @@ template <class S> struct LinkG { static constexpr int G = (int)(16 / sizeof(S))
 
 // the same for a direction known only at run time (selects, no indexed
 // access to the coordinate array)
-__device__ __forceinline__ int sel4(int mu, int a, int b, int c, int d) {   // values, not lvalues
+template <class T>
+__device__ __forceinline__ T sel4(int mu, T a, T b, T c, T d) {   // values, not lvalues
   return mu == 0 ? a : mu == 1 ? b : mu == 2 ? c : d;
 }
 __device__ __forceinline__ int nbr_cb_rt(const Geom &g, const Site &s, int mu, int h) {
@@ -236,16 +278,23 @@ struct Field {             // one parity of a colour-vector / spinor field
   int ncomp;               // complex components per site and rhs (3 or 12)
   int nrhs;
   int parity;              // parity the field lives on (0/1)
-  int rhs_inner;           // 1: rhs fastest (spinor blocks sharing links)
+  int rhs_inner;           // 1: spinor layout, 16-byte chunks of kc components, rhs next
+  int kc;                  // components per 16-byte chunk: 1 (F64), 2 (F32), 4 (F16/Q16)
   int64_t vp;              // sites per parity, padded to 64
   const Geom *g;           // device copy of the geometry
   void *data;              // complex numbers, see cidx()
   float *norm;             // FP16 only, see nidx()
   __host__ __device__ int64_t elems() const { return (int64_t)nrhs * ncomp * vp; }
-  // complex component j of rhs r at checkerboard site cb
+  // complex component j of rhs r at checkerboard site cb.  Spinors
+  // (rhs_inner): [j / kc][cb][r][j % kc], so that one thread's components
+  // come in 16-byte loads and the threads of one site (consecutive r) read
+  // consecutive chunks; colour vectors: [r][j][cb].
   __host__ __device__ int64_t cidx(int r, int j, int64_t cb) const {
-    return rhs_inner ? ((int64_t)j * vp + cb) * nrhs + r : ((int64_t)r * ncomp + j) * vp + cb;
+    return rhs_inner ? ((((int64_t)(j / kc) * vp + cb) * nrhs + r) * kc + j % kc)
+                     : ((int64_t)r * ncomp + j) * vp + cb;
   }
+  // 16-byte chunk q (components q*kc ... q*kc+kc-1) of rhs r at site cb
+  __host__ __device__ int64_t chunk(int r, int q, int64_t cb) const { return ((int64_t)q * vp + cb) * nrhs + r; }
   __host__ __device__ int64_t nidx(int r, int64_t cb) const {
     return rhs_inner ? cb * nrhs + r : (int64_t)r * vp + cb;
   }
