@@ -110,7 +110,11 @@ struct FinStore1 {
   __device__ void operator()(const double *t) const { dst[0] = t[0]; }
 };
 
-template <int P, bool RED>
+// most sites one 256-thread block covers when links are staged in LDS
+// (8 or more rhs per site)
+constexpr int WMAXS = 256 / 8 + 1;
+
+template <int P, bool RED, bool LDSL>
 __global__ void __launch_bounds__(256) k_wilson(WArgs<P> A) {
   typedef typename Prec<P>::R R;
   typedef typename Prec<P>::C C;
@@ -124,11 +128,30 @@ __global__ void __launch_bounds__(256) k_wilson(WArgs<P> A) {
   const int blk = A.xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
   int64_t gid = (int64_t)blk * blockDim.x + threadIdx.x;
   const int li = (int)(gid / nr), r = (int)(gid - (int64_t)li * nr);
+  constexpr int G = LinkG<S>::G;
+  constexpr int NCH = 72 / G;   // 16-byte link words per site (8 terms x 9 elements)
+  [[maybe_unused]] const uvec4_t *ls = nullptr;
+  if constexpr (LDSL) {
+    // links of the block's sites, loaded once into LDS: the nr threads of a
+    // site then read them there instead of each issuing the same loads
+    __shared__ uvec4_t lds[WMAXS * NCH];
+    const int li0 = (int)(((int64_t)blk * blockDim.x) / nr);
+    int li1 = (int)(((int64_t)blk * blockDim.x + blockDim.x - 1) / nr);
+    li1 = li1 < g.vh - 1 ? li1 : (int)g.vh - 1;
+    const int ns = li1 - li0 + 1;
+    for (int k = threadIdx.x; k < ns * NCH; k += blockDim.x) {
+      const int sl = k % ns, ch = k / ns;   // neighbouring threads: neighbouring sites
+      const Site ss = A.tiled ? site_of_tiled(g, A.tl, li0 + sl, A.par) : site_of(g, li0 + sl, A.par);
+      const S *lps = link_base<S, 8>(A.links, ss.lex >> 1) + link_off<S>((ch / 9) * G, ch % 9);
+      lds[sl * NCH + ch] = __builtin_nontemporal_load(reinterpret_cast<const uvec4_t *>(lps));
+    }
+    __syncthreads();
+    ls = lds + (li - li0) * NCH;
+  }
   double red[1] = {0.0};
   if (li < g.vh) {
     const Site s = A.tiled ? site_of_tiled(g, A.tl, li, A.par) : site_of(g, li, A.par);
     const int cb = s.lex >> 1;
-    constexpr int G = LinkG<S>::G;
     constexpr int K = Chunk<P>::K;
     const S *lp = link_base<S, 8>(A.links, cb);
     C acc[4][3];
@@ -143,8 +166,13 @@ __global__ void __launch_bounds__(256) k_wilson(WArgs<P> A) {
 #pragma unroll 1
     for (int tg = 0; tg < 8 / G; tg++) {
     S q[9][G];
+    if constexpr (LDSL) {
 #pragma unroll
-    for (int e = 0; e < 9; e++) ntload_group(lp + link_off<S>(tg * G, e), q[e]);
+      for (int e = 0; e < 9; e++) unpack_group(ls[tg * 9 + e], q[e]);
+    } else {
+#pragma unroll
+      for (int e = 0; e < 9; e++) ntload_group(lp + link_off<S>(tg * G, e), q[e]);
+    }
 #pragma unroll
     for (int gg = 0; gg < G; gg++) {
       const int t = tg * G + gg;
@@ -266,13 +294,15 @@ __global__ void __launch_bounds__(256) k_wilson(WArgs<P> A) {
   if (RED) grid_partial_at(red, A.rs, blk);   // indexed by site block: same sums in either order
 }
 
-// launch shapes the tuner tries: block order and site tiling (log2 extents)
+// launch shapes the tuner tries: block order, links staged in LDS (8 or more
+// rhs) and site tiling (log2 extents)
 struct WShape {
-  int xcd, tile[4];
+  int xcd, lds, tile[4];
 };
 constexpr WShape WSHAPES[] = {
-    {0, {0, 0, 0, 0}}, {1, {0, 0, 0, 0}}, {1, {3, 3, 3, 3}}, {1, {2, 2, 2, 2}},
-    {1, {4, 3, 3, 3}}, {1, {3, 3, 2, 2}}, {1, {2, 2, 2, 4}}, {1, {5, 2, 2, 2}},
+    {0, 0, {0, 0, 0, 0}}, {1, 0, {0, 0, 0, 0}}, {1, 0, {3, 3, 3, 3}}, {1, 0, {2, 2, 2, 2}},
+    {1, 0, {4, 3, 3, 3}}, {1, 0, {3, 3, 2, 2}}, {1, 0, {2, 2, 2, 4}}, {1, 0, {5, 2, 2, 2}},
+    {0, 1, {0, 0, 0, 0}}, {1, 1, {0, 0, 0, 0}}, {1, 1, {3, 3, 2, 2}}, {1, 1, {5, 2, 2, 2}},
 };
 constexpr int NWSHAPE = sizeof(WSHAPES) / sizeof(WSHAPES[0]);
 
@@ -289,10 +319,12 @@ template <int P> struct WRun {
     B.tiled = sh.tile[0] ? 1 : 0;
     if (B.tiled && !B.tl.init(B.g.L, sh.tile)) return -1;
     if (B.red_out) {
-      hipLaunchKernelGGL((k_wilson<P, true>), dim3(R.nb), dim3(256), 0, R.st, B);
+      if (sh.lds) hipLaunchKernelGGL((k_wilson<P, true, true>), dim3(R.nb), dim3(256), 0, R.st, B);
+      else hipLaunchKernelGGL((k_wilson<P, true, false>), dim3(R.nb), dim3(256), 0, R.st, B);
       launch_finish<1>(B.rs, R.nb, FinStore1{B.red_out}, R.st, B.skip);
     } else {
-      hipLaunchKernelGGL((k_wilson<P, false>), dim3(R.nb), dim3(256), 0, R.st, B);
+      if (sh.lds) hipLaunchKernelGGL((k_wilson<P, false, true>), dim3(R.nb), dim3(256), 0, R.st, B);
+      else hipLaunchKernelGGL((k_wilson<P, false, false>), dim3(R.nb), dim3(256), 0, R.st, B);
     }
     QAMD_HIP(hipGetLastError());
     return 0;
@@ -328,12 +360,16 @@ static int wlaunch(const qamd_links *l, Field &out, const Field &in, double a, c
     R.A.rs = red_set(0);
     R.A.red_out = red_out;
   }
-  // with a reduction only the block order may change (a tiled order would
-  // regroup the per-block partial sums and change the rounding)
+  // with a reduction the sites of a block stay fixed (a tiled order would
+  // regroup the per-block partial sums and change the rounding); LDS staging
+  // needs at least 8 rhs per site (at most WMAXS sites per block)
   R.ncand = 0;
-  for (int k = 0; k < (red_out ? 2 : NWSHAPE); k++) {
+  for (int k = 0; k < NWSHAPE; k++) {
+    const WShape &sh = WSHAPES[k];
     Tiling t;
-    if (!WSHAPES[k].tile[0] || t.init(A.g.L, WSHAPES[k].tile)) R.shape[R.ncand++] = k;
+    if (sh.tile[0] && (red_out || !t.init(A.g.L, sh.tile))) continue;
+    if (sh.lds && in.nrhs < 8) continue;
+    R.shape[R.ncand++] = k;
   }
   // the tuner's trial launches must not feed the output back into an input
   const bool alias = out.data == in.data || (w && out.data == w->data);

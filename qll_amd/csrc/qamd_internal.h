@@ -252,6 +252,13 @@ __device__ __forceinline__ __half2 ntload(const __half2 *p) {
 // one 16-byte non-temporal load: element k of the G terms of a link group
 typedef float fvec4_t __attribute__((ext_vector_type(4)));
 typedef unsigned uvec4_t __attribute__((ext_vector_type(4)));
+// the G terms of one link group from a 16-byte word (same byte order as the
+// grouped link layout)
+template <class S>
+__device__ __forceinline__ void unpack_group(const uvec4_t &v, S (&o)[LinkG<S>::G]) {
+  static_assert(sizeof(o) == 16, "a link group is 16 bytes");
+  __builtin_memcpy(o, &v, 16);
+}
 __device__ __forceinline__ void ntload_group(const double2 *p, double2 (&o)[1]) { o[0] = ntload(p); }
 __device__ __forceinline__ void ntload_group(const float2 *p, float2 (&o)[2]) {
   fvec4_t v = __builtin_nontemporal_load(reinterpret_cast<const fvec4_t *>(p));

@@ -203,3 +203,35 @@ def test_block_dslash_every_launch_shape(gpu):
     for o, x, its in res[1:]:
         assert np.array_equal(o, res[0][0])
         assert np.array_equal(x, res[0][1]) and its == res[0][2]
+
+
+@pytest.mark.parametrize("prec,nrhs", [(0, 1), (1, 16), (3, 8), (0, 8)])
+def test_wilson_every_launch_shape(gpu, prec, nrhs):
+    """Every Wilson launch shape the autotuner can pick (block order, links
+    staged in LDS, tiled site orders) gives the same output bit for bit, and
+    the oracle's Dslash; with the |out|^2 reduction too."""
+    print("case: wilson shapes prec=%d nrhs=%d" % (prec, nrhs), flush=True)
+    dims = (8, 8, 8, 8)
+    links = O.wilson_doublestore(dims, O.random_su3_field(dims, 33))
+    ys = np.stack([O.gauss_vector(dims, 70 + r, 300, parity=-1, nreal=24) for r in range(nrhs)])
+    lat = gpu.Lattice(dims)
+    L = gpu.Links(lat, gpu.WILSON, 0, 8).set_lex(links)
+    if prec:
+        L = L.converted(prec)
+    yin = gpu.Field(lat, prec, 12, nrhs, 1).set_lex(ys)
+    res, norms = [], []
+    for mode in [0] + list(range(2, 14)) + [1]:   # default, each candidate, tuned
+        gpu.set_tuning(mode)
+        out = gpu.Field(lat, prec, 12, nrhs, 0)
+        gpu.dslash(L, out, yin, b=-1.0)
+        res.append(out.get_lex())
+        norms.append(gpu.dslash(L, out, yin, b=-1.0, want_norm=True))
+    gpu.set_tuning(1)
+    for o in res[1:]:
+        assert np.array_equal(o, res[0])
+    assert all(n == norms[0] for n in norms)
+    got = res[0] if nrhs > 1 else res[0][None]
+    m = parity_mask(dims, 0)
+    for r in range(nrhs):
+        ref = O.wilson_dslash(dims, links, 0.0, ys[r], sub="even")
+        assert rel_err(got[r], ref, m) < TOL[prec], r
