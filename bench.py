@@ -49,6 +49,7 @@ def parse():
     ap.add_argument("--no-graph", action="store_true", help="config4: eager launches only")
     ap.add_argument("--config4", action="store_true",
                     help="also measure the 48-solve heterogeneous batch against looping FP64 CG (~1 min)")
+    ap.add_argument("--no-wilson", action="store_true", help="skip the 16-rhs Wilson solves")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--block", type=int, default=8, help="sources in the batched mixed-precision solve")
     ap.add_argument("--dims", type=str, default=None, help="override lattice, e.g. 16,16,16,16")
@@ -204,6 +205,37 @@ def cpu_baseline(dims, nsec_target=10.0):
         "sample": "%d EO Dslash calls at %s (%.2f s each; links generated in %.1f s)" % (
             nrep, "x".join(map(str, dims)), t, gen_s),
     }
+
+
+def wilson_solves(Q, dims=(32, 32, 32, 64), nrhs=16, kappa=0.124):
+    """Wilson even-odd CGNE on 1 - kappa^2 H_eo H_oe for a block of 16
+    Gaussian sources sharing every link load, to FP64 true |r|/|b| <= 1e-10
+    per rhs: plain FP64 against FP64 defect correction around an FP32 and a
+    half-precision (FP16 storage, per-site norm) inner solve."""
+    lat = Q.Lattice(dims)
+    g = Q.Gauge(lat).random_su3(3)
+    L64 = Q.Links(lat, Q.WILSON, Q.F64, 8).from_gauge(g)
+    del g
+    b = Q.Field(lat, Q.F64, 12, nrhs, Q.EVEN).gaussian(13, 100)
+    x = Q.Field(lat, Q.F64, 12, nrhs, Q.EVEN)
+    res = {"lattice": list(dims), "rhs": nrhs, "kappa": kappa}
+    for name, ls in (("fp64", None), ("mixed_fp32", Q.F32), ("mixed_fp16", Q.F16)):
+        Ls = None if ls is None else L64.converted(ls)
+        Q.wilson_cgne(L64, x, b, kappa, RSQ, 4, ls=Ls)   # untimed: launch-shape tuning
+        Q.sync()
+        t0 = time.time()
+        infos = Q.wilson_cgne(L64, x, b, kappa, RSQ, 20000, ls=Ls)
+        Q.sync()
+        dt = time.time() - t0
+        res[name] = {"seconds": round(dt, 4), "iterations": max(i["iterations"] for i in infos),
+                     "converged": all(i["converged"] for i in infos),
+                     "max_true_rel_resid": max(i["true_rsq"] for i in infos) ** 0.5,
+                     "solves_per_s": round(nrhs / dt, 3)}
+        if name != "fp64":
+            res[name]["speedup_vs_fp64"] = round(res["fp64"]["seconds"] / dt, 3)
+        print("wilson %s: %s" % (name, res[name]), file=sys.stderr, flush=True)
+        del Ls
+    return res
 
 
 def main():
@@ -373,6 +405,7 @@ def main():
                             "speedup_vs_fp64": round(sps / solver["fp64_cg"]["solves_per_s"], 3)}
             print("%s: %s" % (name, solver[name]), file=sys.stderr, flush=True)
 
+    wil = None if args.no_wilson else wilson_solves(Q)
     c4 = config4(Q, graph=not args.no_graph) if args.config4 else None
 
     cpu = None
@@ -406,6 +439,7 @@ def main():
         "cpu_baseline": cpu,
         "dslash": {(PREC_NAME[p] if p in PREC_NAME else p): prof[p] for p in prof},
         "solvers": solver,
+        "wilson_cgne_16rhs": wil,
         "config4_batch": c4,
         "device": Q.device_name(),
     }
