@@ -51,6 +51,9 @@ def parse():
                     help="also measure the 48-solve heterogeneous batch against looping FP64 CG (~1 min)")
     ap.add_argument("--no-wilson", action="store_true", help="skip the 16-rhs Wilson solves")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--kernel-child", type=str, default=None,
+                    help="KIND,PREC,NRHS[,MODE]: 20 Dslash launches of one kind (stag or wilson) for a "
+                         "profiler; MODE is the tuning mode (1 tuned, >= 2 forces launch shape MODE-2)")
     ap.add_argument("--block", type=int, default=8, help="sources in the batched mixed-precision solve")
     ap.add_argument("--dims", type=str, default=None, help="override lattice, e.g. 16,16,16,16")
     ap.add_argument("--precs", type=str, default="0,1,2,3",
@@ -75,6 +78,35 @@ def pmc_child(dims):
     ob = Q.Field(lat, Q.Q16, 3, 8, Q.EVEN)
     Q.time_dslash(LQ, ob, yb, 5)
     Q.sync()
+
+
+def kernel_child(dims, spec):
+    """20 launches of one Dslash for a profiler run: asqtad ("stag", 16 terms,
+    fat7/Naik links) or Wilson, precision PREC, NRHS rhs; the launch shape is
+    fixed beforehand (MODE), so every profiled dispatch is the same kernel."""
+    import qll_amd as Q
+    kind, prec, nrhs = spec.split(",")[:3]
+    mode = int(spec.split(",")[3]) if spec.count(",") >= 3 else 1
+    prec, nrhs = int(prec), int(nrhs)
+    Q.init()
+    lat = Q.Lattice(dims)
+    if kind == "stag":
+        L = Q.asqtad_links_device(lat, seed=1)
+        nc = 3
+    else:
+        g = Q.Gauge(lat).random_su3(3)
+        L = Q.Links(lat, Q.WILSON, Q.F64, 8).from_gauge(g)
+        nc = 12
+    if prec:
+        L = L.converted(prec)
+    yin = Q.Field(lat, prec, nc, nrhs, Q.ODD).gaussian(5, 100)
+    out = Q.Field(lat, prec, nc, nrhs, Q.EVEN)
+    Q.set_tuning(mode)
+    if mode == 1:
+        Q.time_dslash(L, out, yin, 1)   # the tuner's trial launches happen here
+    ms = Q.time_dslash(L, out, yin, 20)
+    Q.sync()
+    print("%s prec %d nrhs %d mode %d: %.4f ms" % (kind, prec, nrhs, mode, ms), flush=True)
 
 
 def _pmc_pass(counter, dims, tmp):
@@ -247,6 +279,9 @@ def main():
     dims = tuple(int(x) for x in args.dims.split(",")) if args.dims else DIMS
     if args.pmc_child:
         pmc_child(dims)
+        return
+    if args.kernel_child:
+        kernel_child(dims, args.kernel_child)
         return
     traffic = None
     if not args.no_pmc:

@@ -159,12 +159,8 @@ __global__ void __launch_bounds__(256) k_wilson(WArgs<P> A) {
     for (int a = 0; a < 4; a++)
 #pragma unroll
       for (int c = 0; c < 3; c++) acc[a][c].x = acc[a][c].y = (R)0;
-    // one link group (G terms, one 16-byte load per element) per iteration;
-    // the iterations are not unrolled, which keeps the spinor working set of
-    // one group in registers (unrolled, the hoisted loads of all eight terms
-    // spill)
-#pragma unroll 1
-    for (int tg = 0; tg < 8 / G; tg++) {
+    // one link group: G terms sharing one 16-byte load per link element
+    auto group = [&](const int tg) __attribute__((always_inline)) {
     S q[9][G];
     if constexpr (LDSL) {
 #pragma unroll
@@ -240,6 +236,21 @@ __global__ void __launch_bounds__(256) k_wilson(WArgs<P> A) {
         acc[3][c] = bwd ? cadd(acc[3][c], l1) : csub(acc[3][c], l1);
       }
     }
+    };
+    // FP64/FP32 (G <= 2): the groups are unrolled, so every term's direction
+    // is a compile-time constant, with a scheduling barrier between groups
+    // (without it the compiler hoists every group's loads to the top and
+    // spills); 16-bit (G = 4): the two groups stay a rolled loop, which keeps
+    // the four terms' spinors of one group in registers
+    if constexpr (G <= 2) {
+#pragma unroll
+      for (int tg = 0; tg < 8 / G; tg++) {
+        __builtin_amdgcn_sched_barrier(0);
+        group(tg);
+      }
+    } else {
+#pragma unroll 1
+      for (int tg = 0; tg < 8 / G; tg++) group(tg);
     }
     const R b = (R)A.b, a = (R)A.a;
     C o[4][3];
