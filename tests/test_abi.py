@@ -56,3 +56,15 @@ def test_no_device_fails_loudly():
         assert "device" in str(e)
     else:
         pytest.skip("a GPU is present")
+
+
+def test_lattice_arguments_checked_before_any_device_call():
+    """Odd or too small extents are refused with a message (no GPU needed)."""
+    lib = ctypes.CDLL(LIB)
+    lib.qamd_lattice_create.restype = ctypes.c_void_p
+    lib.qamd_last_error.restype = ctypes.c_char_p
+    for dims in ((4, 4, 4, 5), (2, 4, 4, 4), (8, 8, 0, 8)):
+        assert not lib.qamd_lattice_create((ctypes.c_int * 4)(*dims))
+        assert b"extents must be even and >= 4" in lib.qamd_last_error()
+    assert not lib.qamd_lattice_create((ctypes.c_int * 4)(2048, 2048, 1024, 1024))
+    assert b"site limit" in lib.qamd_last_error()
