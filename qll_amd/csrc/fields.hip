@@ -15,11 +15,25 @@ size_t prec_bytes(int prec) {
   return prec == QAMD_F64 ? 16 : prec == QAMD_F32 ? 8 : 4;   // FP16 and Q16: 4 bytes
 }
 
+// the (site, rhs) of this thread in a (sites/256, nrhs) grid: rhs in
+// blockIdx.y for colour vectors; for spinors, whose rhs are adjacent in
+// memory, the flattened thread index with the rhs fastest
+__device__ __forceinline__ bool site_rhs(const Field &f, int &cb, int &rhs) {
+  if (!f.rhs_inner) {
+    cb = blockIdx.x * blockDim.x + threadIdx.x;
+    rhs = blockIdx.y;
+  } else {
+    const int64_t i = ((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * blockDim.x + threadIdx.x;
+    cb = (int)(i / f.nrhs);
+    rhs = (int)(i - (int64_t)cb * f.nrhs);
+  }
+  return cb < f.g->vh;
+}
+
 template <class H, int NC>
 __global__ void k_import(Field f, const H *src, SrcMap m) {
-  int cb = blockIdx.x * blockDim.x + threadIdx.x;
-  int rhs = blockIdx.y;
-  if (cb >= f.g->vh) return;
+  int cb, rhs;
+  if (!site_rhs(f, cb, rhs)) return;
   int64_t base, stride;
   m.map(*f.g, rhs, f.parity, cb, base, stride);
   double2 v[NC];
@@ -31,9 +45,8 @@ __global__ void k_import(Field f, const H *src, SrcMap m) {
 
 template <class H, int NC>
 __global__ void k_export(Field f, H *dst, SrcMap m) {
-  int cb = blockIdx.x * blockDim.x + threadIdx.x;
-  int rhs = blockIdx.y;
-  if (cb >= f.g->vh) return;
+  int cb, rhs;
+  if (!site_rhs(f, cb, rhs)) return;
   int64_t base, stride;
   m.map(*f.g, rhs, f.parity, cb, base, stride);
   double2 v[NC];
@@ -47,9 +60,8 @@ __global__ void k_export(Field f, H *dst, SrcMap m) {
 
 template <int NC>
 __global__ void k_gaussian(Field f, uint64_t seed, uint64_t stream) {
-  int cb = blockIdx.x * blockDim.x + threadIdx.x;
-  int rhs = blockIdx.y;
-  if (cb >= f.g->vh) return;
+  int cb, rhs;
+  if (!site_rhs(f, cb, rhs)) return;
   Site s = site_of(*f.g, cb, f.parity);
   double2 v[NC];
 #pragma unroll
@@ -85,9 +97,8 @@ struct StoreTo {
 
 template <int NC>
 __global__ void k_axpby(double a, Field x, double b, Field y) {
-  int cb = blockIdx.x * blockDim.x + threadIdx.x;
-  int rhs = blockIdx.y;
-  if (cb >= y.g->vh) return;
+  int cb, rhs;
+  if (!site_rhs(y, cb, rhs)) return;
   double2 vx[NC], vy[NC];
   site_load<NC>(x, rhs, cb, vx);
   if (b != 0.0) site_load<NC>(y, rhs, cb, vy);
