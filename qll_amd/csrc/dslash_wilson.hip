@@ -34,13 +34,13 @@ template <int P> struct WArgs {
   RedScratch rs;
   double *red_out;
   const int *skip;
+  int dag;     // 1: the hermitian conjugate g5 H g5 (every projector sign flipped)
   int xcd;     // XCD-aware block order (each die takes one contiguous range of sites)
   int tiled;   // sites in the tiled order of tl (else checkerboard order)
   Tiling tl;
 };
 
 template <class C> __device__ __forceinline__ C cadd(C a, C b) { a.x += b.x; a.y += b.y; return a; }
-template <class C> __device__ __forceinline__ C csub(C a, C b) { a.x -= b.x; a.y -= b.y; return a; }
 
 // One 16-byte chunk of a spinor: K complex components of one (site, rhs)
 // (K = 1 FP64, 2 FP32, 4 FP16/Q16), scaled by the site norm n for 16-bit
@@ -175,6 +175,10 @@ __global__ void __launch_bounds__(256) k_wilson(WArgs<P> A) {
       const int mu = t >> 1;
       const bool bwd = t & 1;
       int nb = nbr_cb_rt(g, s, mu, bwd ? -1 : 1);
+      // projector sign s = +1 for backward terms (-1 forward); the conjugate
+      // flips it, g5 (1 -+ g_mu) g5 = (1 +- g_mu).  Multiplying by +-1 and
+      // adding in one fma is exact, the same as adding or subtracting.
+      const R sg = (bwd != (A.dag != 0)) ? (R)1 : (R)-1;
       float nrm = 1.0f;
       if constexpr (P == QAMD_F16 || P == QAMD_Q16) nrm = fin.norm[fin.nidx(r, nb)] * A.lscale[t];
       C psi[4][3];
@@ -197,8 +201,10 @@ __global__ void __launch_bounds__(256) k_wilson(WArgs<P> A) {
         d0.y = sel4(mu, p3.x, -p3.y, p2.x, p2.y);
         d1.x = sel4(mu, -p2.y, p2.x, p3.y, p3.x);
         d1.y = sel4(mu, p2.x, p2.y, -p3.x, p3.y);
-        h[0][c] = bwd ? cadd(psi[0][c], d0) : csub(psi[0][c], d0);
-        h[1][c] = bwd ? cadd(psi[1][c], d1) : csub(psi[1][c], d1);
+        h[0][c].x = fma(sg, d0.x, psi[0][c].x);
+        h[0][c].y = fma(sg, d0.y, psi[0][c].y);
+        h[1][c].x = fma(sg, d1.x, psi[1][c].x);
+        h[1][c].y = fma(sg, d1.y, psi[1][c].y);
       }
       C m[9];
 #pragma unroll
@@ -232,8 +238,10 @@ __global__ void __launch_bounds__(256) k_wilson(WArgs<P> A) {
         l0.y = sel4(mu, -u1.x, u1.y, -u0.x, u0.y);
         l1.x = sel4(mu, u0.y, -u0.x, -u1.y, u1.x);
         l1.y = sel4(mu, -u0.x, -u0.y, u1.x, u1.y);
-        acc[2][c] = bwd ? cadd(acc[2][c], l0) : csub(acc[2][c], l0);
-        acc[3][c] = bwd ? cadd(acc[3][c], l1) : csub(acc[3][c], l1);
+        acc[2][c].x = fma(sg, l0.x, acc[2][c].x);
+        acc[2][c].y = fma(sg, l0.y, acc[2][c].y);
+        acc[3][c].x = fma(sg, l1.x, acc[3][c].x);
+        acc[3][c].y = fma(sg, l1.y, acc[3][c].y);
       }
     }
     };
@@ -344,7 +352,7 @@ template <int P> struct WRun {
 
 template <int P>
 static int wlaunch(const qamd_links *l, Field &out, const Field &in, double a, const Field *w, double b,
-                   double *red_out, hipStream_t st, const int *skip) {
+                   double *red_out, hipStream_t st, const int *skip, bool dag) {
   typedef typename Prec<P>::S S;
   WArgs<P> A;
   memset(&A, 0, sizeof A);
@@ -360,6 +368,7 @@ static int wlaunch(const qamd_links *l, Field &out, const Field &in, double a, c
   A.b = b;
   A.par = out.parity;
   A.skip = skip;
+  A.dag = dag ? 1 : 0;
   for (int t = 0; t < 8; t++) A.lscale[t] = (P == QAMD_Q16) ? l->scale[t] / Q16_ONE : l->scale[t];
   int64_t nthreads = A.g.vh * in.nrhs;
   WRun<P> R;
@@ -392,7 +401,7 @@ static int wlaunch(const qamd_links *l, Field &out, const Field &in, double a, c
 }
 
 int wilson_dslash_launch(const qamd_links *l, Field &out, const Field &in, double a, const Field *w, double b,
-                         double *red_out, hipStream_t st, const int *skip) {
+                         double *red_out, hipStream_t st, const int *skip, bool dag) {
   if (l->kind != 1) {
     set_error("wilson_dslash: links are not Wilson links");
     return -1;
@@ -410,10 +419,10 @@ int wilson_dslash_launch(const qamd_links *l, Field &out, const Field &in, doubl
     return -1;
   }
   switch (l->prec) {
-    case QAMD_F64: return wlaunch<QAMD_F64>(l, out, in, a, w, b, red_out, st, skip);
-    case QAMD_F32: return wlaunch<QAMD_F32>(l, out, in, a, w, b, red_out, st, skip);
-    case QAMD_F16: return wlaunch<QAMD_F16>(l, out, in, a, w, b, red_out, st, skip);
-    default: return wlaunch<QAMD_Q16>(l, out, in, a, w, b, red_out, st, skip);
+    case QAMD_F64: return wlaunch<QAMD_F64>(l, out, in, a, w, b, red_out, st, skip, dag);
+    case QAMD_F32: return wlaunch<QAMD_F32>(l, out, in, a, w, b, red_out, st, skip, dag);
+    case QAMD_F16: return wlaunch<QAMD_F16>(l, out, in, a, w, b, red_out, st, skip, dag);
+    default: return wlaunch<QAMD_Q16>(l, out, in, a, w, b, red_out, st, skip, dag);
   }
 }
 

@@ -28,7 +28,8 @@ int stag_block_launch(const qamd_links *l, Field &out, const Field &in, double a
 // solvers.hip: with 16-bit sloppy links too coarse for this mass (operator
 // error not small against m^2), *l32 receives FP32 links made from l64
 int sloppy_too_coarse(const qamd_links *l64, const qamd_links *ls, double mass, qamd_links **l32);
+// dag: the hermitian conjugate g5 D g5 of the hopping term
 int wilson_dslash_launch(const qamd_links *l, Field &out, const Field &in, double a, const Field *w,
-                         double b, double *red_out, hipStream_t st, const int *skip = nullptr);
+                         double b, double *red_out, hipStream_t st, const int *skip = nullptr, bool dag = false);
 
 }  // namespace qamd

@@ -74,7 +74,7 @@ __global__ void __launch_bounds__(256) k_rhs_axpby(Field x, Field y, const doubl
 struct WS {   // work space of one precision
   qamd_lattice *lat;
   int prec, nrhs;
-  qamd_field *r, *p, *q, *s, *g, *t;   // t odd, others even
+  qamd_field *r, *p, *q, *s, *t;   // t odd, others even
   double *coef;                        // device: a[16], b[16], norms[16]
   double *coefh;                       // pinned host mirror
 };
@@ -84,7 +84,6 @@ static void ws_free(WS &w) {
   qamd_field_destroy(w.p);
   qamd_field_destroy(w.q);
   qamd_field_destroy(w.s);
-  qamd_field_destroy(w.g);
   qamd_field_destroy(w.t);
   dev_free(w.coef);
   if (w.coefh) (void)hipHostFree(w.coefh);
@@ -100,11 +99,10 @@ static int ws_init(WS &w, qamd_lattice *lat, int prec, int nrhs) {
   w.p = qamd_field_create(lat, prec, 12, nrhs, 0);
   w.q = qamd_field_create(lat, prec, 12, nrhs, 0);
   w.s = qamd_field_create(lat, prec, 12, nrhs, 0);
-  w.g = qamd_field_create(lat, prec, 12, nrhs, 0);
   w.t = qamd_field_create(lat, prec, 12, nrhs, 1);
   w.coef = (double *)dev_alloc(64 * sizeof(double));
   if (hipHostMalloc((void **)&w.coefh, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess) w.coefh = nullptr;
-  if (!w.r || !w.p || !w.q || !w.s || !w.g || !w.t || !w.coef || !w.coefh) {
+  if (!w.r || !w.p || !w.q || !w.s || !w.t || !w.coef || !w.coefh) {
     ws_free(w);
     return -1;
   }
@@ -140,16 +138,13 @@ static int rhs_axpby(WS &w, const double *a, const qamd_field *x, const double *
   return 0;
 }
 
-// out = Mhat in  (dagger: g5 Mhat g5 in, using w.g as scratch)
+// out = Mhat in, or Mhat^+ in = g5 Mhat g5 in = in - kappa^2 (g5 H_eo g5)(g5 H_oe g5) in,
+// whose hopping terms are the Dslash with every projector sign flipped
 static int apply_mhat(const qamd_links *l, WS &w, qamd_field *out, const qamd_field *in, double kappa, int dagger) {
-  const qamd_field *src = in;
-  if (dagger) {
-    if (qamd_field_copy(w.g, in) || qamd_field_gamma5(w.g)) return -1;
-    src = w.g;
-  }
-  if (wilson_dslash_launch(l, w.t->f, src->f, 0.0, nullptr, 1.0, nullptr, stream())) return -1;
-  if (wilson_dslash_launch(l, out->f, w.t->f, 1.0, &src->f, -kappa * kappa, nullptr, stream())) return -1;
-  if (dagger && qamd_field_gamma5(out)) return -1;
+  const bool dag = dagger != 0;
+  if (wilson_dslash_launch(l, w.t->f, in->f, 0.0, nullptr, 1.0, nullptr, stream(), nullptr, dag)) return -1;
+  if (wilson_dslash_launch(l, out->f, w.t->f, 1.0, &in->f, -kappa * kappa, nullptr, stream(), nullptr, dag))
+    return -1;
   return 0;
 }
 
