@@ -25,6 +25,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FLOPS_ASQTAD = 1146          # per EO output site
+FLOPS_ASQTAD_QLL = 1152      # QLL's own count (72 per term), reported beside it
+FLOPS_WILSON, FLOPS_WILSON_QLL = 1320, 1344
+ORACLE_CFLAGS = "gcc -O3 -fopenmp -std=gnu11 (oracle/Makefile)"
 # algorithmic bytes per EO output site: 16 link matrices + one input and one
 # output colour vector per rhs (16-bit vectors carry a 4-byte site norm)
 LINK_BYTES = {0: 2304, 1: 1152, 2: 576, 3: 576}
@@ -229,6 +232,14 @@ def cpu_baseline(dims, nsec_target=10.0):
     nrep = max(1, min(5000, int(nsec_target / max(t1, 1e-6))))
     t, nthreads = O.time_dslash(dims, links, y, nrep)
     vh = O.volume(dims) // 2
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     return {
         "value": round(FLOPS_ASQTAD * vh / t / 1e9, 3),
         "unit": "GFLOP/s",
@@ -236,6 +247,9 @@ def cpu_baseline(dims, nsec_target=10.0):
         "kind": "port",
         "sample": "%d EO Dslash calls at %s (%.2f s each; links generated in %.1f s)" % (
             nrep, "x".join(map(str, dims)), t, gen_s),
+        "value_qll_flops": round(FLOPS_ASQTAD_QLL * vh / t / 1e9, 3),
+        "cpu_model": model,
+        "build": ORACLE_CFLAGS,
     }
 
 
@@ -326,6 +340,7 @@ def main():
         gbs = BYTES_SITE[prec] * vh / (ms * 1e-3) / 1e9
         prof[prec] = {"ms_per_call": round(ms, 5),
                       "GFLOP/s": round(FLOPS_ASQTAD * vh / (ms * 1e-3) / 1e9, 1),
+                      "GFLOP/s_qll_count": round(FLOPS_ASQTAD_QLL * vh / (ms * 1e-3) / 1e9, 1),
                       "GB/s": round(gbs, 1),
                       "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
         print("dslash prec=%d: %s" % (prec, prof[prec]), file=sys.stderr, flush=True)
@@ -363,7 +378,8 @@ def main():
         byts = (WLINK[prec] + nrw * 2 * WSPIN[prec]) * vh
         key = "wilson_%s_%drhs" % (PREC_NAME[prec], nrw)
         prof[key] = {"ms_per_call": round(ms, 5), "ms_per_rhs": round(ms / nrw, 5),
-                     "GFLOP/s": round(nrw * 1320 * vh / (ms * 1e-3) / 1e9, 1),
+                     "GFLOP/s": round(nrw * FLOPS_WILSON * vh / (ms * 1e-3) / 1e9, 1),
+                     "GFLOP/s_qll_count": round(nrw * FLOPS_WILSON_QLL * vh / (ms * 1e-3) / 1e9, 1),
                      "GB/s": round(byts / (ms * 1e-3) / 1e9, 1),
                      "hbm_frac": round(byts / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
         print("dslash %s: %s" % (key, prof[key]), file=sys.stderr, flush=True)
@@ -463,7 +479,8 @@ def main():
         "data": "synthetic (seeded SU(3) + fat7/Naik generated on the GPU)",
         "config": {"workload": "asqtad even-odd Dslash, %s lattice (%d output sites), FP64, 1 rhs"
                                % ("x".join(map(str, dims)), vh),
-                   "lattice": list(dims), "mass_qll": QLL_MASS, "flops_per_site": FLOPS_ASQTAD},
+                   "lattice": list(dims), "mass_qll": QLL_MASS, "flops_per_site": FLOPS_ASQTAD,
+                   "flops_per_site_qll_count": FLOPS_ASQTAD_QLL},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": round(traffic["bytes"]) if traffic else None,
