@@ -86,6 +86,18 @@ typedef struct {
  * (host_prec QAMD_F64) or float (QAMD_F32); nelem = 2*ncomp */
 int qamd_field_from_qll(qamd_field *f, const qamd_qll_layout *l, const void *host, int host_prec);
 int qamd_field_to_qll(const qamd_field *f, const qamd_qll_layout *l, void *host, int host_prec);
+/* host array in a caller's site order: site i of the host array has the
+ * coordinates coords[4*i .. 4*i+3] (x fastest convention: coords[4*i+mu] is
+ * the mu coordinate) and its nelem = 2*ncomp reals at host[i*nelem ...].
+ * The table lists every site of the field's parity (sites of the other
+ * parity may be listed too and are skipped; an export leaves them as they
+ * are).  This is the site order a QDP/QOP host code exposes: it builds the
+ * table once with QDP_get_coords_L for each of its sites, the loop of
+ * toQDP_F / fromQDP_F (embedqop.c:57-90, qopstag.c:71-103).  Errors: a
+ * coordinate out of range, two sites with the same coordinates, or a site of
+ * the parity missing.  nrhs = 1; host_prec QAMD_F64 or QAMD_F32. */
+int qamd_field_from_sites(qamd_field *f, const int *coords, int64_t nsites, const void *host, int host_prec);
+int qamd_field_to_sites(const qamd_field *f, const int *coords, int64_t nsites, void *host, int host_prec);
 
 /* ---- gauge fields (device), FP64 ---- */
 qamd_gauge *qamd_gauge_create(qamd_lattice *lat);
@@ -96,6 +108,14 @@ int qamd_gauge_to_lex(const qamd_gauge *u, double *host);
 /* from four QLL-layout arrays u[mu] (18 reals per site) */
 int qamd_gauge_from_qll(qamd_gauge *u, const qamd_qll_layout *l, const void *const host[4], int host_prec);
 int qamd_gauge_to_qll(const qamd_gauge *u, const qamd_qll_layout *l, void *const host[4], int host_prec);
+/* four arrays host[mu] in a caller's site order (as qamd_field_from_sites;
+ * the table lists every site), 18 reals per site; row_major = 1 for matrices
+ * stored row by row as QDP does (QLL's are column-major; the reference
+ * converts with QDP_M_eq_transpose_M, qopstag.c:123-135) */
+int qamd_gauge_from_sites(qamd_gauge *u, const int *coords, int64_t nsites, const void *const host[4], int host_prec,
+                          int row_major);
+int qamd_gauge_to_sites(const qamd_gauge *u, const int *coords, int64_t nsites, void *const host[4], int host_prec,
+                        int row_major);
 /* seeded SU(3): complex Gaussian entries -> Gram-Schmidt columns -> det = 1
  * (no reference counterpart; column order of makeU lib/vla.c:1635-1654) */
 int qamd_gauge_random_su3(qamd_gauge *u, uint64_t seed);

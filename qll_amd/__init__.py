@@ -110,6 +110,11 @@ def lib():
             "qamd_wilson_cgne": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_double, C.c_int,
                                            C.POINTER(SolveInfo)]),
             "qamd_field_gamma5": (C.c_int, [_vp]),
+            "qamd_field_from_sites": (C.c_int, [_vp, C.POINTER(C.c_int), C.c_int64, _vp, C.c_int]),
+            "qamd_field_to_sites": (C.c_int, [_vp, C.POINTER(C.c_int), C.c_int64, _vp, C.c_int]),
+            "qamd_gauge_from_sites": (C.c_int, [_vp, C.POINTER(C.c_int), C.c_int64, C.POINTER(_vp), C.c_int,
+                                                C.c_int]),
+            "qamd_gauge_to_sites": (C.c_int, [_vp, C.POINTER(C.c_int), C.c_int64, C.POINTER(_vp), C.c_int, C.c_int]),
         }
         for name, (res, args) in sigs.items():
             fn = getattr(L, name)
@@ -208,6 +213,24 @@ class Field:
         _check(lib().qamd_field_axpby(a, x.h, b, self.h))
         return self
 
+    def set_sites(self, coords, a):
+        """a: (nsites, 2*ncomp) in the site order of coords (nsites, 4)."""
+        c = np.ascontiguousarray(coords, dtype=np.int32)
+        a = np.ascontiguousarray(a)
+        prec = F64 if a.dtype == np.float64 else F32
+        assert a.dtype in (np.float64, np.float32) and a.size == c.shape[0] * 2 * self.ncomp
+        _check(lib().qamd_field_from_sites(self.h, c.ctypes.data_as(C.POINTER(C.c_int)), c.shape[0],
+                                           a.ctypes.data, prec))
+        return self
+
+    def get_sites(self, coords, base=None, dtype=np.float64):
+        c = np.ascontiguousarray(coords, dtype=np.int32)
+        shape = (c.shape[0], 2 * self.ncomp)
+        out = np.zeros(shape, dtype=dtype) if base is None else np.array(base, dtype=dtype).reshape(shape)
+        _check(lib().qamd_field_to_sites(self.h, c.ctypes.data_as(C.POINTER(C.c_int)), c.shape[0], out.ctypes.data,
+                                         F64 if dtype == np.float64 else F32))
+        return out
+
 
 class Gauge:
     def __init__(self, lat):
@@ -231,6 +254,24 @@ class Gauge:
     def random_su3(self, seed):
         _check(lib().qamd_gauge_random_su3(self.h, seed))
         return self
+
+    def set_sites(self, coords, u, row_major=False):
+        """u: (4, nsites, 18) in the site order of coords (nsites, 4)."""
+        c = np.ascontiguousarray(coords, dtype=np.int32)
+        arrs = [np.ascontiguousarray(u[mu]) for mu in range(4)]
+        prec = F64 if arrs[0].dtype == np.float64 else F32
+        ptrs = (_vp * 4)(*[a.ctypes.data for a in arrs])
+        _check(lib().qamd_gauge_from_sites(self.h, c.ctypes.data_as(C.POINTER(C.c_int)), c.shape[0], ptrs, prec,
+                                           1 if row_major else 0))
+        return self
+
+    def get_sites(self, coords, row_major=False):
+        c = np.ascontiguousarray(coords, dtype=np.int32)
+        out = np.zeros((4, c.shape[0], 18))
+        ptrs = (_vp * 4)(*[out[mu].ctypes.data for mu in range(4)])
+        _check(lib().qamd_gauge_to_sites(self.h, c.ctypes.data_as(C.POINTER(C.c_int)), c.shape[0], ptrs, F64,
+                                         1 if row_major else 0))
+        return out
 
     def stag_phases(self):
         _check(lib().qamd_gauge_stag_phases(self.h))

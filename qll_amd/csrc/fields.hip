@@ -137,6 +137,58 @@ int reduce_field(const Field &a, const Field *b, int rhs, double *out) {
   return 0;
 }
 
+// need_parity: 0/1 = every site of that parity must appear, 2 = every site
+int site_table_inverse(const qamd_lattice *lat, const int *coords, int64_t nsites, int need_parity, int **inv) {
+  const Geom &g = lat->g;
+  *inv = nullptr;
+  if (!coords || nsites < 1 || nsites > g.V) {
+    set_error("site table: %lld sites for a lattice of %lld", (long long)nsites, (long long)g.V);
+    return -1;
+  }
+  int *h = (int *)malloc(sizeof(int) * g.V);
+  if (!h) {
+    set_error("site table: host allocation failed");
+    return -1;
+  }
+  for (int64_t i = 0; i < g.V; i++) h[i] = -1;
+  int64_t have[2] = {0, 0};
+  for (int64_t i = 0; i < nsites; i++) {
+    const int *c = coords + 4 * i;
+    int64_t lex = 0;
+    for (int d = 3; d >= 0; d--) {
+      if (c[d] < 0 || c[d] >= g.L[d]) {
+        set_error("site table: site %lld has coordinate %d = %d outside 0..%d", (long long)i, d, c[d], g.L[d] - 1);
+        free(h);
+        return -1;
+      }
+      lex = lex * g.L[d] + c[d];
+    }
+    if (h[lex] >= 0) {
+      set_error("site table: sites %d and %lld have the same coordinates", h[lex], (long long)i);
+      free(h);
+      return -1;
+    }
+    h[lex] = (int)i;
+    have[(c[0] + c[1] + c[2] + c[3]) & 1]++;
+  }
+  if ((need_parity != 1 && have[0] != g.vh) || (need_parity != 0 && have[1] != g.vh)) {
+    set_error("site table: %lld even and %lld odd sites listed, the lattice has %lld of each",
+              (long long)have[0], (long long)have[1], (long long)g.vh);
+    free(h);
+    return -1;
+  }
+  int *d = (int *)dev_alloc(sizeof(int) * g.V);
+  hipError_t e = d ? hipMemcpy(d, h, sizeof(int) * g.V, hipMemcpyHostToDevice) : hipErrorOutOfMemory;
+  free(h);
+  if (e != hipSuccess) {
+    dev_free(d);
+    if (d) set_error("site table: copy failed: %s", hipGetErrorString(e));
+    return -1;
+  }
+  *inv = d;
+  return 0;
+}
+
 }  // namespace qamd
 
 using namespace qamd;
@@ -350,6 +402,37 @@ int qamd_field_to_qll(const qamd_field *f, const qamd_qll_layout *l, void *host,
   size_t bytes = (size_t)m.nreal * l->nSitesInner *
                  (f->f.parity ? (l->nSitesOuter - l->nEvenOuter) : l->nEvenOuter) * esz;
   return export_common(f, h, bytes, hprec, m, 0);
+}
+
+static int sites_xfer(qamd_field *f, const int *coords, int64_t nsites, void *host, int hprec, int to_host) {
+  if (f->f.nrhs != 1) {
+    set_error("site-table transfer needs nrhs = 1");
+    return -1;
+  }
+  if (hprec != QAMD_F64 && hprec != QAMD_F32) {
+    set_error("site-table transfer: host precision must be QAMD_F64 or QAMD_F32");
+    return -1;
+  }
+  SrcMap m;
+  memset(&m, 0, sizeof m);
+  m.kind = 2;
+  m.nreal = 2 * f->f.ncomp;
+  int *inv = nullptr;
+  if (site_table_inverse(f->lat, coords, nsites, f->f.parity, &inv)) return -1;
+  m.inv = inv;
+  size_t bytes = (size_t)nsites * m.nreal * (hprec == QAMD_F64 ? 8 : 4);
+  // export: sites of the other parity keep the caller's values
+  int rc = to_host ? export_common(f, host, bytes, hprec, m, 1) : import_common(f, host, bytes, hprec, m);
+  dev_free(inv);
+  return rc;
+}
+
+int qamd_field_from_sites(qamd_field *f, const int *coords, int64_t nsites, const void *host, int hprec) {
+  return sites_xfer(f, coords, nsites, (void *)host, hprec, 0);
+}
+
+int qamd_field_to_sites(const qamd_field *f, const int *coords, int64_t nsites, void *host, int hprec) {
+  return sites_xfer((qamd_field *)f, coords, nsites, host, hprec, 1);
 }
 
 int qamd_field_gaussian(qamd_field *f, uint64_t seed, uint64_t strm) {

@@ -304,9 +304,11 @@ __global__ void k_plaq(const double2 *u, Geom g, RedScratch rs) {
   grid_partial<6>(acc, rs);
 }
 
-// host transfer: [4][V][18] lexicographic or 4 QLL arrays
+// host transfer: [4][V][18] lexicographic, 4 QLL arrays, or 4 arrays in a
+// caller's site order; row_major: host matrices are stored row by row
+// (device element e = 3c + r holds entry (r, c), column-major like QLL)
 template <class H>
-__global__ void k_gauge_xfer(double2 *u, H *h, Geom g, int mu, SrcMap m, int to_host) {
+__global__ void k_gauge_xfer(double2 *u, H *h, Geom g, int mu, SrcMap m, int to_host, int row_major) {
   int cb = blockIdx.x * blockDim.x + threadIdx.x;
   int par = blockIdx.y;
   if (cb >= g.vh) return;
@@ -314,6 +316,9 @@ __global__ void k_gauge_xfer(double2 *u, H *h, Geom g, int mu, SrcMap m, int to_
   int64_t base, stride;
   if (m.kind == 0) {
     base = ((int64_t)mu * g.V + s.lex) * 18;
+    stride = 1;
+  } else if (m.kind == 2) {
+    base = (int64_t)m.inv[s.lex] * 18;
     stride = 1;
   } else {
     int oi, ii;
@@ -323,12 +328,13 @@ __global__ void k_gauge_xfer(double2 *u, H *h, Geom g, int mu, SrcMap m, int to_
   }
   double2 *p = u + ((int64_t)(mu * 2 + par) * 9) * g.vh_pad + cb;
   for (int e = 0; e < 9; e++) {
+    const int he = row_major ? 3 * (e % 3) + e / 3 : e;
     if (to_host) {
       double2 v = p[e * g.vh_pad];
-      h[base + 2 * e * stride] = (H)v.x;
-      h[base + (2 * e + 1) * stride] = (H)v.y;
+      h[base + 2 * he * stride] = (H)v.x;
+      h[base + (2 * he + 1) * stride] = (H)v.y;
     } else {
-      p[e * g.vh_pad] = make_double2((double)h[base + 2 * e * stride], (double)h[base + (2 * e + 1) * stride]);
+      p[e * g.vh_pad] = make_double2((double)h[base + 2 * he * stride], (double)h[base + (2 * he + 1) * stride]);
     }
   }
 }
@@ -372,7 +378,7 @@ static int gauge_xfer_lex(qamd_gauge *u, double *host, int to_host) {
   if (!to_host) QAMD_HIP(hipMemcpyAsync(st, host, bytes, hipMemcpyHostToDevice, stream()));
   for (int mu = 0; mu < 4; mu++)
     hipLaunchKernelGGL(k_gauge_xfer<double>, dim3((unsigned)((g.vh + 255) / 256), 2), dim3(256), 0, stream(),
-                       u->d, st, g, mu, m, to_host);
+                       u->d, st, g, mu, m, to_host, 0);
   QAMD_HIP(hipGetLastError());
   if (to_host) QAMD_HIP(hipMemcpyAsync(host, st, bytes, hipMemcpyDeviceToHost, stream()));
   QAMD_HIP(hipStreamSynchronize(stream()));
@@ -402,9 +408,9 @@ static int gauge_xfer_qll(qamd_gauge *u, const qamd_qll_layout *l, void *const h
     if (!to_host) QAMD_HIP(hipMemcpyAsync(st, host[mu], bytes, hipMemcpyHostToDevice, stream()));
     dim3 grid((unsigned)((g.vh + 255) / 256), 2);
     if (hprec == QAMD_F64)
-      hipLaunchKernelGGL(k_gauge_xfer<double>, grid, dim3(256), 0, stream(), u->d, (double *)st, g, mu, m, to_host);
+      hipLaunchKernelGGL(k_gauge_xfer<double>, grid, dim3(256), 0, stream(), u->d, (double *)st, g, mu, m, to_host, 0);
     else
-      hipLaunchKernelGGL(k_gauge_xfer<float>, grid, dim3(256), 0, stream(), u->d, (float *)st, g, mu, m, to_host);
+      hipLaunchKernelGGL(k_gauge_xfer<float>, grid, dim3(256), 0, stream(), u->d, (float *)st, g, mu, m, to_host, 0);
     QAMD_HIP(hipGetLastError());
     if (to_host) QAMD_HIP(hipMemcpyAsync(host[mu], st, bytes, hipMemcpyDeviceToHost, stream()));
     QAMD_HIP(hipStreamSynchronize(stream()));
@@ -418,6 +424,58 @@ int qamd_gauge_from_qll(qamd_gauge *u, const qamd_qll_layout *l, const void *con
 }
 int qamd_gauge_to_qll(const qamd_gauge *u, const qamd_qll_layout *l, void *const host[4], int hprec) {
   return gauge_xfer_qll((qamd_gauge *)u, l, host, hprec, 1);
+}
+
+static int gauge_xfer_sites(qamd_gauge *u, const int *coords, int64_t nsites, void *const host[4], int hprec,
+                            int row_major, int to_host) {
+  const Geom &g = u->lat->g;
+  if (hprec != QAMD_F64 && hprec != QAMD_F32) {
+    set_error("gauge site-table transfer: host precision must be QAMD_F64 or QAMD_F32");
+    return -1;
+  }
+  SrcMap m;
+  memset(&m, 0, sizeof m);
+  m.kind = 2;
+  int *inv = nullptr;
+  if (site_table_inverse(u->lat, coords, nsites, 2, &inv)) return -1;
+  m.inv = inv;
+  size_t bytes = (size_t)nsites * 18 * (hprec == QAMD_F64 ? 8 : 4);
+  void *st = dev_alloc(bytes);
+  if (!st) {
+    dev_free(inv);
+    return -1;
+  }
+  hipError_t e = hipSuccess;
+  for (int mu = 0; mu < 4 && e == hipSuccess; mu++) {
+    if (!to_host) e = hipMemcpyAsync(st, host[mu], bytes, hipMemcpyHostToDevice, stream());
+    dim3 grid((unsigned)((g.vh + 255) / 256), 2);
+    if (hprec == QAMD_F64)
+      hipLaunchKernelGGL(k_gauge_xfer<double>, grid, dim3(256), 0, stream(), u->d, (double *)st, g, mu, m, to_host,
+                         row_major);
+    else
+      hipLaunchKernelGGL(k_gauge_xfer<float>, grid, dim3(256), 0, stream(), u->d, (float *)st, g, mu, m, to_host,
+                         row_major);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (to_host && e == hipSuccess) e = hipMemcpyAsync(host[mu], st, bytes, hipMemcpyDeviceToHost, stream());
+    if (e == hipSuccess) e = hipStreamSynchronize(stream());
+  }
+  dev_free(st);
+  dev_free(inv);
+  if (e != hipSuccess) {
+    set_error("gauge site-table transfer: %s", hipGetErrorString(e));
+    return -1;
+  }
+  return 0;
+}
+
+int qamd_gauge_from_sites(qamd_gauge *u, const int *coords, int64_t nsites, const void *const host[4], int hprec,
+                          int row_major) {
+  return gauge_xfer_sites(u, coords, nsites, (void *const *)host, hprec, row_major, 0);
+}
+
+int qamd_gauge_to_sites(const qamd_gauge *u, const int *coords, int64_t nsites, void *const host[4], int hprec,
+                        int row_major) {
+  return gauge_xfer_sites((qamd_gauge *)u, coords, nsites, host, hprec, row_major, 1);
 }
 
 int qamd_gauge_random_su3(qamd_gauge *u, uint64_t seed) {

@@ -9,6 +9,10 @@ namespace qamd {
 // measured on first use (run(c, ctx) launches candidate c, 0 on success)
 int tune_pick(const char *key, int ncand, int def, int (*run)(int, void *), void *ctx);
 
+// fields.hip: a caller's site table (coordinates of each host site) checked
+// and inverted on the host; *inv (device, V ints, lexicographic site -> host
+// site index or -1) is freed by the caller with dev_free
+int site_table_inverse(const qamd_lattice *lat, const int *coords, int64_t nsites, int need_parity, int **inv);
 // fields.hip
 const Geom *dev_geom(qamd_lattice *lat);
 double *scalar_slot();

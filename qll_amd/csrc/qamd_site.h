@@ -115,13 +115,17 @@ inline QllLay to_lay(const qamd_qll_layout *l) {
 
 // host element index of element 0 of (rhs, site) and the element stride
 struct SrcMap {
-  int kind;        // 0 lexicographic [rhs][V][nreal], 1 QLL half array
+  int kind;        // 0 lexicographic [rhs][V][nreal], 1 QLL half array, 2 site table
   int nreal;
   QllLay lay;
+  const int *inv;  // site table: host site index of each lexicographic site
   __device__ void map(const Geom &g, int rhs, int par, int cb, int64_t &base, int64_t &stride) const {
     Site s = site_of(g, cb, par);
     if (kind == 0) {
       base = ((int64_t)rhs * g.V + s.lex) * nreal;
+      stride = 1;
+    } else if (kind == 2) {
+      base = (int64_t)inv[s.lex] * nreal;
       stride = 1;
     } else {
       int oi, ii;
