@@ -53,6 +53,7 @@ def parse():
     ap.add_argument("--config4", action="store_true",
                     help="also measure the 48-solve heterogeneous batch against looping FP64 CG (~1 min)")
     ap.add_argument("--no-wilson", action="store_true", help="skip the 16-rhs Wilson solves")
+    ap.add_argument("--no-cgms", action="store_true", help="skip the 12-shift multishift solves")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--kernel-child", type=str, default=None,
                     help="KIND,PREC,NRHS[,MODE]: 20 Dslash launches of one kind (stag or wilson) for a "
@@ -251,6 +252,60 @@ def cpu_baseline(dims, nsec_target=10.0):
         "cpu_model": model,
         "build": ORACLE_CFLAGS,
     }
+
+
+def cgms_solves(Q, dims=(32, 32, 32, 64)):
+    """Asqtad multishift CG: 12 shifts log-spaced in [1e-4, 1] on the base
+    m^2 = 0.005^2 (QLL mass 0.01; the smallest shift folded into the base
+    mass, QLL's shifts[0] = 0), one Gaussian source, every shift to FP64 true
+    |r|/|b| <= 1e-10: plain FP64 CGMS against the FP32 multishift with FP64
+    per-shift refinement."""
+    import numpy as np
+    sig = np.logspace(-4, 0, 12)
+    m0 = float(np.sqrt(0.01 ** 2 + sig[0]))
+    shifts = [float(v) for v in sig - sig[0]]
+    lat = Q.Lattice(dims)
+    L64 = Q.asqtad_links_device(lat, 2)
+    L32 = L64.converted(Q.F32)
+    b = Q.Field(lat, Q.F64, 3, 1, Q.EVEN).gaussian(12, 100)
+    xs = [Q.Field(lat, Q.F64, 3, 1, Q.EVEN) for _ in shifts]
+    Ap = Q.Field(lat, Q.F64, 3, 1, Q.EVEN)
+    t = Q.Field(lat, Q.F64, 3, 1, Q.ODD)
+    bsq = b.norm2()
+
+    def worst_rel_resid():
+        w = 0.0
+        for x, sh in zip(xs, shifts):
+            Q.stag_normal(L64, Ap, x, t, float(np.sqrt(m0 ** 2 + sh)))
+            Ap.axpby(1.0, b, -1.0)
+            w = max(w, Ap.norm2() / bsq)
+        return w ** 0.5
+
+    res = {"lattice": list(dims), "shifts": len(shifts), "mass_qll": 0.01}
+    Q.sync()
+    t0 = time.time()
+    info = Q.stag_cgms(L64, xs, b, m0, shifts, RSQ, 50000)
+    Q.sync()
+    dt = time.time() - t0
+    res["fp64_cgms"] = {"seconds": round(dt, 4), "iterations": info["iterations"],
+                        "max_true_rel_resid": worst_rel_resid()}
+    print("cgms fp64: %s" % res["fp64_cgms"], file=sys.stderr, flush=True)
+    # FP32 multishift, then per-shift refinement on FP32 links, or on Q16
+    # links for the shifts heavy enough for them (FP32 below that)
+    for name, Ls in (("mixed_fp32_cgms", L32), ("mixed_fp32_q16_cgms", L64.converted(Q.Q16))):
+        for x in xs:
+            x.zero()
+        Q.sync()
+        t0 = time.time()
+        infos = Q.stag_cgms_mixed(L64, Ls, xs, b, m0, shifts, RSQ, 50000)
+        Q.sync()
+        dt = time.time() - t0
+        res[name] = {"seconds": round(dt, 4), "iterations": max(i["iterations"] for i in infos),
+                     "converged": all(i["converged"] for i in infos),
+                     "max_true_rel_resid": worst_rel_resid(),
+                     "speedup_vs_fp64": round(res["fp64_cgms"]["seconds"] / dt, 3)}
+        print("cgms %s: %s" % (name, res[name]), file=sys.stderr, flush=True)
+    return res
 
 
 def wilson_solves(Q, dims=(32, 32, 32, 64), nrhs=16, kappa=0.124):
@@ -457,6 +512,7 @@ def main():
             print("%s: %s" % (name, solver[name]), file=sys.stderr, flush=True)
 
     wil = None if args.no_wilson else wilson_solves(Q)
+    cgms = None if args.no_cgms else cgms_solves(Q)
     c4 = config4(Q, graph=not args.no_graph) if args.config4 else None
 
     cpu = None
@@ -492,6 +548,7 @@ def main():
         "dslash": {(PREC_NAME[p] if p in PREC_NAME else p): prof[p] for p in prof},
         "solvers": solver,
         "wilson_cgne_16rhs": wil,
+        "cgms_12_shifts": cgms,
         "config4_batch": c4,
         "device": Q.device_name(),
     }
