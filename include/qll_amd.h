@@ -145,6 +145,15 @@ int qamd_links_from_qll(qamd_links *l, const qamd_qll_layout *lay, const void *c
 int qamd_links_from_gauge(qamd_links *l, const qamd_gauge *fat, const qamd_gauge *lng);
 /* copy with precision conversion (same kind and term count) */
 int qamd_links_convert(qamd_links *dst, const qamd_links *src);
+/* 1 if the FP64 asqtad Dslash on these links rebuilds the long (Naik) terms'
+ * third rows instead of loading them: set when every long term of a 16-term
+ * FP64 staggered link set is a scale times a unitary matrix to 1e-12 (the
+ * case for fat7/Naik links built from SU(3) gauge fields with phases), after
+ * each import, doublestore or conversion; links that are not of that form
+ * (e.g. hash-filled test fields) are read in full.  No counterpart in QLL;
+ * the reconstruction is QUDA's two-row compression, QLL's next-step list
+ * names it for the long links. */
+int qamd_links_reconstructs(const qamd_links *l);
 
 /* ---- Dslash (replaces dslash2 lib/stagDslash.c:531-544 and wilsonDslash
  *      lib/wilsonDslash.c:272-285 for one output parity) ----

@@ -90,6 +90,7 @@ def lib():
             "qamd_links_from_qll": (C.c_int, [_vp, C.POINTER(QllLayout), C.POINTER(_vp), C.POINTER(_vp), C.c_int]),
             "qamd_links_from_gauge": (C.c_int, [_vp, _vp, _vp]),
             "qamd_links_convert": (C.c_int, [_vp, _vp]),
+            "qamd_links_reconstructs": (C.c_int, [_vp]),
             "qamd_dslash": (C.c_int, [_vp, _vp, _vp, C.c_double, _vp, C.c_double, C.POINTER(C.c_double)]),
             "qamd_stag_normal": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.POINTER(C.c_double)]),
             "qamd_stag_cg": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_double, C.c_int, C.POINTER(SolveInfo)]),
@@ -310,6 +311,11 @@ class Links:
     def from_gauge(self, fat, lng=None):
         _check(lib().qamd_links_from_gauge(self.h, fat.h, lng.h if lng is not None else None))
         return self
+
+    @property
+    def reconstructs(self):
+        """FP64 Dslash rebuilds the long terms' third rows (fewer bytes)."""
+        return bool(lib().qamd_links_reconstructs(self.h))
 
     def converted(self, prec):
         out = Links(self.lat, self.kind, prec, self.nterms)

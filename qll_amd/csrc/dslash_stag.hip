@@ -39,7 +39,25 @@ template <int P, int PO = P> struct StagArgs {
   RedScratch rs;
   double *red_out;    // receives {|out|^2, |w|^2}
   const int *skip;    // optional device flag: whole grid returns when set
+  const uint8_t *lsign;   // long-term determinant signs when their third rows are rebuilt
 };
+
+// third row of a long term c*W (W unitary, det W = -1 if neg): det W *
+// conj(row0 x row1) / c, c = |row0|; element (r, c) is q[3c + r]
+__device__ __forceinline__ void long_row3(double2 (&q)[9][1], bool neg) {
+  double c2 = 0;
+#pragma unroll
+  for (int c = 0; c < 3; c++) c2 += q[3 * c][0].x * q[3 * c][0].x + q[3 * c][0].y * q[3 * c][0].y;
+  const double ic = (neg ? -1.0 : 1.0) / sqrt(c2);
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const double2 a1 = q[3 * ((c + 1) % 3)][0], a2 = q[3 * ((c + 2) % 3)][0];
+    const double2 b1 = q[3 * ((c + 1) % 3) + 1][0], b2 = q[3 * ((c + 2) % 3) + 1][0];
+    const double re = (a1.x * b2.x - a1.y * b2.y) - (a2.x * b1.x - a2.y * b1.y);
+    const double im = (a1.x * b2.y + a1.y * b2.x) - (a2.x * b1.y + a2.y * b1.x);
+    q[3 * c + 2][0] = make_double2(re * ic, -im * ic);
+  }
+}
 
 struct FinStore2 {
   double *dst;
@@ -49,7 +67,9 @@ struct FinStore2 {
   }
 };
 
-template <int P, int PO, int NT, bool RED>
+// REC (FP64, 16 terms): the long terms' third rows are not loaded but
+// rebuilt from the first two (2016 instead of 2400 bytes per site)
+template <int P, int PO, int NT, bool RED, bool REC = false>
 __global__ void __launch_bounds__(256) k_stag_dslash(StagArgs<P, PO> A) {
   typedef typename Prec<P>::R R;
   typedef typename Prec<P>::C C;
@@ -62,7 +82,10 @@ __global__ void __launch_bounds__(256) k_stag_dslash(StagArgs<P, PO> A) {
   if (cb < A.g.vh) {
     Site s = site_of(A.g, cb, A.par);
     constexpr int G = LinkG<S>::G;
+    static_assert(!REC || (P == QAMD_F64 && NT == 16), "long-row rebuild: FP64 16-term links only");
     const S *lp = link_base<S, NT>(A.links, cb);
+    [[maybe_unused]] unsigned lsb = 0;
+    if constexpr (REC) lsb = A.lsign[cb];
     C acc[3];
 #pragma unroll
     for (int j = 0; j < 3; j++) acc[j].x = acc[j].y = (R)0;
@@ -70,7 +93,15 @@ __global__ void __launch_bounds__(256) k_stag_dslash(StagArgs<P, PO> A) {
     for (int tg = 0; tg < NT / G; tg++) {
       S q[9][G];   // element e of terms tg*G .. tg*G+G-1, one 16-byte load each
 #pragma unroll
-      for (int e = 0; e < 9; e++) ntload_group(lp + link_off<S>(tg * G, e), q[e]);
+      for (int e = 0; e < 9; e++) {
+        if constexpr (REC) {
+          if ((tg & 1) && e % 3 == 2) continue;   // third row of a long term: rebuilt below
+        }
+        ntload_group(lp + link_off<S>(tg * G, e), q[e]);
+      }
+      if constexpr (REC) {
+        if (tg & 1) long_row3(q, (lsb >> (tg >> 1)) & 1);
+      }
 #pragma unroll
       for (int gg = 0; gg < G; gg++) {
         const int t = tg * G + gg;
@@ -122,6 +153,15 @@ __global__ void __launch_bounds__(256) k_stag_dslash(StagArgs<P, PO> A) {
 template <int P, int PO, int NT>
 static void launch_t(const StagArgs<P, PO> &A, bool red, hipStream_t st) {
   int nb = (int)((A.g.vh + 255) / 256);
+  if constexpr (P == QAMD_F64 && NT == 16) {
+    if (A.lsign) {
+      if (red)
+        hipLaunchKernelGGL((k_stag_dslash<P, PO, NT, true, true>), dim3(nb), dim3(256), 0, st, A);
+      else
+        hipLaunchKernelGGL((k_stag_dslash<P, PO, NT, false, true>), dim3(nb), dim3(256), 0, st, A);
+      return;
+    }
+  }
   if (red)
     hipLaunchKernelGGL((k_stag_dslash<P, PO, NT, true>), dim3(nb), dim3(256), 0, st, A);
   else
@@ -147,6 +187,7 @@ static int launch_p(const qamd_links *l, Field &out, const Field &in, double a, 
   A.a = a;
   A.b = b;
   A.par = out.parity;
+  A.lsign = (P == QAMD_F64 && l->recon) ? l->lsign[out.parity] : nullptr;
   // Q16 links are read as raw integers: fold 1/32767 into the term scale
   for (int t = 0; t < 16; t++) A.lscale[t] = (P == QAMD_Q16) ? l->scale[t] / Q16_ONE : l->scale[t];
   const bool red = red_out != nullptr || !finish;

@@ -131,6 +131,83 @@ __global__ void k_link_convert(SO *dst, const SI *src, int64_t n, int nterm, con
   }
 }
 
+// Long (Naik) terms as c * W with W unitary (c > 0, det W = +-1): then the
+// third row is det W * conj(row0 x row1) / c.  Checks every long term of
+// FP64 16-term links to 1e-12 relative, records det W per (site, term) and
+// flags any term that is not of that form.
+__global__ void k_long_check(const double2 *d, uint8_t *sign, Geom g, int64_t nblk, unsigned *bad) {
+  const int cb = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cb >= g.vh) return;
+  unsigned bits = 0, fail = 0;
+  for (int k = 0; k < 8; k++) {
+    double2 m[9];
+    for (int e = 0; e < 9; e++) m[e] = d[link_idx<double2>(cb, 2 * k + 1, e, 16)];
+    // row r holds elements 3c + r
+    double c2 = 0, n2 = 0;
+    double2 dot = make_double2(0, 0);
+    for (int c = 0; c < 3; c++) {
+      const double2 a = m[3 * c], b = m[3 * c + 1];
+      c2 += a.x * a.x + a.y * a.y;
+      n2 += b.x * b.x + b.y * b.y;
+      dot.x += a.x * b.x + a.y * b.y;   // row0 . conj(row1)
+      dot.y += a.y * b.x - a.x * b.y;
+    }
+    const double tol = 1e-12;
+    if (!(c2 > 1e-200) || fabs(n2 - c2) > tol * c2 || sqrt(dot.x * dot.x + dot.y * dot.y) > tol * c2) {
+      fail = 1;
+      continue;
+    }
+    const double ic = 1.0 / sqrt(c2);
+    double2 x[3];   // conj(row0 x row1) / c
+    double proj = 0, dev = 0;
+    for (int c = 0; c < 3; c++) {
+      const double2 a1 = m[3 * ((c + 1) % 3)], a2 = m[3 * ((c + 2) % 3)];
+      const double2 b1 = m[3 * ((c + 1) % 3) + 1], b2 = m[3 * ((c + 2) % 3) + 1];
+      const double re = (a1.x * b2.x - a1.y * b2.y) - (a2.x * b1.x - a2.y * b1.y);
+      const double im = (a1.x * b2.y + a1.y * b2.x) - (a2.x * b1.y + a2.y * b1.x);
+      x[c] = make_double2(re * ic, -im * ic);
+      proj += m[3 * c + 2].x * x[c].x + m[3 * c + 2].y * x[c].y;
+    }
+    const double sg = proj < 0 ? -1.0 : 1.0;
+    for (int c = 0; c < 3; c++) {
+      const double dx = m[3 * c + 2].x - sg * x[c].x, dy = m[3 * c + 2].y - sg * x[c].y;
+      dev += dx * dx + dy * dy;
+    }
+    if (dev > tol * tol * c2) fail = 1;
+    if (sg < 0) bits |= 1u << k;
+  }
+  sign[cb] = (uint8_t)bits;
+  if (fail) atomicOr(bad, 1u);
+}
+
+// after FP64 16-term staggered links change: decide whether the Dslash may
+// rebuild the long terms' third rows (all long terms pass the check)
+static int long_link_check(qamd_links *l) {
+  l->recon = 0;
+  if (l->kind != 0 || l->prec != QAMD_F64 || l->nterms != 16) return 0;
+  const Geom &g = l->lat->g;
+  for (int p = 0; p < 2; p++)
+    if (!l->lsign[p] && !(l->lsign[p] = (uint8_t *)dev_alloc(g.vh_pad))) return -1;
+  unsigned *bad = (unsigned *)dev_alloc(sizeof(unsigned));
+  if (!bad) return -1;
+  unsigned hb = 0;
+  hipError_t e = hipMemsetAsync(bad, 0, sizeof(unsigned), stream());
+  for (int p = 0; p < 2 && e == hipSuccess; p++) {
+    hipLaunchKernelGGL(k_long_check, dim3((unsigned)((g.vh + 255) / 256)), dim3(256), 0, stream(),
+                       (const double2 *)l->data[p], l->lsign[p], g, g.vh_pad / 64, bad);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(&hb, bad, sizeof hb, hipMemcpyDeviceToHost, stream());
+  if (e == hipSuccess) e = hipStreamSynchronize(stream());
+  dev_free(bad);
+  if (e != hipSuccess) {
+    set_error("long-link check: %s", hipGetErrorString(e));
+    return -1;
+  }
+  l->recon = hb ? 0 : 1;
+  return 0;
+}
+
 }  // namespace qamd
 
 using namespace qamd;
@@ -170,6 +247,8 @@ void qamd_links_destroy(qamd_links *l) {
   (void)hipStreamSynchronize(stream());
   dev_free(l->data[0]);
   dev_free(l->data[1]);
+  dev_free(l->lsign[0]);
+  dev_free(l->lsign[1]);
   free(l);
 }
 
@@ -214,6 +293,7 @@ static int import_terms(qamd_links *l, const void *const terms[], int hprec, Src
   dev_free(stage);
   if (!rc && dst != l) rc = qamd_links_convert(l, dst);
   if (dst != l) qamd_links_destroy(dst);
+  if (!rc) rc = long_link_check(l);
   return rc;
 }
 
@@ -282,8 +362,11 @@ int qamd_links_from_gauge(qamd_links *l, const qamd_gauge *fat, const qamd_gauge
   }
   if (!rc && dst != l) rc = qamd_links_convert(l, dst);
   if (dst != l) qamd_links_destroy(dst);
+  if (!rc) rc = long_link_check(l);
   return rc;
 }
+
+int qamd_links_reconstructs(const qamd_links *l) { return l && l->recon ? 1 : 0; }
 
 int qamd_links_convert(qamd_links *dst, const qamd_links *src) {
   if (dst->lat->g.V != src->lat->g.V || dst->nterms != src->nterms || dst->kind != src->kind) {
@@ -294,7 +377,11 @@ int qamd_links_convert(qamd_links *dst, const qamd_links *src) {
   int64_t nblk = src->lat->g.vh_pad / 64;
   float *sc = (float *)dev_alloc(64 * sizeof(float));
   unsigned *mx = (unsigned *)dev_alloc(16 * sizeof(unsigned));
-  if (!sc || !mx) return -1;
+  if (!sc || !mx) {
+    dev_free(sc);
+    dev_free(mx);
+    return -1;
+  }
   float oscale[16];
   for (int t = 0; t < 16; t++) oscale[t] = 1.0f;
   if (dst->prec == QAMD_F16 || dst->prec == QAMD_Q16) {
@@ -342,6 +429,8 @@ int qamd_links_convert(qamd_links *dst, const qamd_links *src) {
     else if (a == QAMD_Q16 && b == QAMD_Q16) CONV(short2, short2);
     else {
       set_error("links_convert: unsupported precision pair %d -> %d", a, b);
+      dev_free(sc);
+      dev_free(mx);
       return -1;
     }
 #undef CONV
@@ -351,7 +440,7 @@ int qamd_links_convert(qamd_links *dst, const qamd_links *src) {
   for (int t = 0; t < 16; t++) dst->scale[t] = oscale[t];
   dev_free(sc);
   dev_free(mx);
-  return 0;
+  return long_link_check(dst);
 }
 
 }  // extern "C"

@@ -26,6 +26,11 @@ struct qamd_links {
   int nterms;
   void *data[2];
   float scale[16];   // FP16 only: per-term scale (stored = link / scale)
+  // FP64 staggered links with long (Naik) terms that are all a scale times a
+  // unitary matrix: the Dslash loads their first two rows only and rebuilds
+  // the third; lsign[p][cb] bit k = determinant sign of long term k
+  int recon;
+  uint8_t *lsign[2];
 };
 
 namespace qamd {
