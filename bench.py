@@ -31,12 +31,16 @@ FLOPS_WILSON, FLOPS_WILSON_QLL = 1320, 1344
 ORACLE_CFLAGS = "gcc -O3 -fopenmp -std=gnu11 (oracle/Makefile)"
 # algorithmic bytes per EO output site: 16 link matrices + one input and one
 # output colour vector per rhs (16-bit vectors carry a 4-byte site norm)
-LINK_BYTES = {0: 2304, 1: 1152, 2: 576, 3: 576}
+CPLX_BYTES = {0: 16, 1: 8, 2: 4, 3: 4}
+LINK_BYTES = {p: 16 * 9 * c for p, c in CPLX_BYTES.items()}
+# links whose long terms are scaled unitary matrices: the Dslash reads two of
+# their three rows (8 terms of 9 complex numbers + 8 of 6)
+LINK_BYTES_REBUILT = {p: (8 * 9 + 8 * 6) * c for p, c in CPLX_BYTES.items()}
 VEC_BYTES = {0: 48, 1: 24, 2: 16, 3: 16}
-BYTES_SITE = {p: LINK_BYTES[p] + 2 * VEC_BYTES[p] for p in LINK_BYTES}
-# FP64 links whose long terms are scaled unitary matrices: the Dslash reads
-# two of their three rows (8 x 144 + 8 x 96 link bytes per site)
-LINK_BYTES_F64_REBUILT = 8 * 144 + 8 * 96
+
+
+def link_bytes(L, prec):
+    return LINK_BYTES_REBUILT[prec] if L.reconstructs else LINK_BYTES[prec]
 PREC_NAME = {0: "f64", 1: "f32", 2: "f16", 3: "q16"}
 HBM_PEAK_GBS = 8000.0
 COPY_BYTES = 1 << 28          # the PMC calibration copy: reads and writes 256 MiB
@@ -387,8 +391,7 @@ def main():
     precs = [int(p) for p in args.precs.split(",")]
     if 0 not in precs:
         precs.insert(0, 0)
-    if L64.reconstructs:
-        BYTES_SITE[0] = LINK_BYTES_F64_REBUILT + 2 * VEC_BYTES[0]
+    bytes_site = {}
     for prec, L in ((0, L64), (1, L32), (2, L16), (3, LQ)):
         if prec not in precs:
             continue
@@ -399,8 +402,9 @@ def main():
         ms = Q.time_dslash(L, out, yin, args.steps)
         Q.sync()
         rates[prec] = ms
-        gbs = BYTES_SITE[prec] * vh / (ms * 1e-3) / 1e9
-        prof[prec] = {"ms_per_call": round(ms, 5),
+        bytes_site[prec] = link_bytes(L, prec) + 2 * VEC_BYTES[prec]
+        gbs = bytes_site[prec] * vh / (ms * 1e-3) / 1e9
+        prof[prec] = {"ms_per_call": round(ms, 5), "bytes_per_site": bytes_site[prec],
                       "GFLOP/s": round(FLOPS_ASQTAD * vh / (ms * 1e-3) / 1e9, 1),
                       "GFLOP/s_qll_count": round(FLOPS_ASQTAD_QLL * vh / (ms * 1e-3) / 1e9, 1),
                       "GB/s": round(gbs, 1),
@@ -416,7 +420,7 @@ def main():
         out = Q.Field(lat, prec, 3, nbk, Q.EVEN)
         Q.time_dslash(L, out, yin, args.warmup)
         ms = Q.time_dslash(L, out, yin, max(1, args.steps // 4))
-        byts = (LINK_BYTES[prec] + nbk * 2 * VEC_BYTES[prec]) * vh
+        byts = (link_bytes(L, prec) + nbk * 2 * VEC_BYTES[prec]) * vh
         key = "%s_block%d" % (PREC_NAME[prec], nbk)
         prof[key] = {"ms_per_call": round(ms, 5), "ms_per_rhs": round(ms / nbk, 5),
                      "GFLOP/s": round(nbk * FLOPS_ASQTAD * vh / (ms * 1e-3) / 1e9, 1),
@@ -450,7 +454,7 @@ def main():
 
     ms64 = rates[0]
     gflops = FLOPS_ASQTAD * vh / (ms64 * 1e-3) / 1e9
-    achieved = BYTES_SITE[0] * vh / (ms64 * 1e-3) / 1e9
+    achieved = bytes_site[0] * vh / (ms64 * 1e-3) / 1e9
 
     solver = None
     if not args.no_cg:
@@ -548,7 +552,7 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": round(traffic["bytes"]) if traffic else None,
                      "traffic_per_site": round(traffic["bytes"] / vh, 1) if traffic else None,
-                     "algorithmic_per_site": BYTES_SITE[0],
+                     "algorithmic_per_site": bytes_site[0],
                      "long_link_rows_read": 2 if L64.reconstructs else 3,
                      "measured_copy_gbs": round(copy_gbs, 1),
                      "frac_of_copy": round(achieved / copy_gbs, 4)},

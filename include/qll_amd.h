@@ -116,7 +116,8 @@ int qamd_gauge_from_sites(qamd_gauge *u, const int *coords, int64_t nsites, cons
                           int row_major);
 int qamd_gauge_to_sites(const qamd_gauge *u, const int *coords, int64_t nsites, void *const host[4], int host_prec,
                         int row_major);
-/* seeded SU(3): complex Gaussian entries -> Gram-Schmidt columns -> det = 1
+/* seeded SU(3): complex Gaussian entries -> Gram-Schmidt columns (each
+ * projection removed twice, so every matrix is unitary to a few eps) -> det = 1
  * (no reference counterpart; column order of makeU lib/vla.c:1635-1654) */
 int qamd_gauge_random_su3(qamd_gauge *u, uint64_t seed);
 /* multiply U_mu(x) by eta_mu(x) = (-1)^(sum_{nu<mu} x_nu) */
@@ -145,13 +146,15 @@ int qamd_links_from_qll(qamd_links *l, const qamd_qll_layout *lay, const void *c
 int qamd_links_from_gauge(qamd_links *l, const qamd_gauge *fat, const qamd_gauge *lng);
 /* copy with precision conversion (same kind and term count) */
 int qamd_links_convert(qamd_links *dst, const qamd_links *src);
-/* 1 if the FP64 asqtad Dslash on these links rebuilds the long (Naik) terms'
+/* 1 if the asqtad Dslash on these links rebuilds the long (Naik) terms'
  * third rows instead of loading them: set when every long term of a 16-term
  * FP64 staggered link set is a scale times a unitary matrix to 1e-12 (the
  * case for fat7/Naik links built from SU(3) gauge fields with phases), after
- * each import, doublestore or conversion; links that are not of that form
- * (e.g. hash-filled test fields) are read in full.  No counterpart in QLL;
- * the reconstruction is QUDA's two-row compression, QLL's next-step list
+ * each import, doublestore or conversion.  FP32/FP16/Q16 copies converted
+ * from such a set (and 16-bit sets imported or doublestored through FP64)
+ * inherit it; FP32 sets imported directly, and links that are not of that
+ * form (e.g. hash-filled test fields), are read in full.  No counterpart in
+ * QLL; the reconstruction is QUDA's two-row compression, QLL's next-step list
  * names it for the long links. */
 int qamd_links_reconstructs(const qamd_links *l);
 

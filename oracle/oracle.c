@@ -124,7 +124,11 @@ static cplx mdet(const double *m)
 
 /* Gram-Schmidt of columns in the order of makeUsite (lib/vla.c:1635-1654):
  * normalise column i, then for each j<i remove the projection on column j and
- * renormalise.  Then divide by the principal cube root of det so det = 1. */
+ * renormalise.  Then divide by the principal cube root of det so det = 1.
+ * Difference from makeUsite: the projections of column i are removed twice
+ * ("twice is enough", Giraud, Langou and Rozloznik 2005).  One pass leaves
+ * orthogonality errors of eps * cond(G), up to ~1e-12 among the millions of
+ * matrices of a 32^4 field; two passes leave a few eps whatever cond(G). */
 void orc_su3_from_gauss(double *m, const double *g)
 {
   cplx col[3][3];
@@ -135,15 +139,16 @@ void orc_su3_from_gauss(double *m, const double *g)
     for (int r = 0; r < 3; r++) n += creal(col[i][r] * conj(col[i][r]));
     double ni = 1.0 / sqrt(n);
     for (int r = 0; r < 3; r++) col[i][r] *= ni;
-    for (int j = 0; j < i; j++) {
-      cplx d = 0;
-      for (int r = 0; r < 3; r++) d += conj(col[j][r]) * col[i][r];
-      for (int r = 0; r < 3; r++) col[i][r] -= d * col[j][r];
-      n = 0;
-      for (int r = 0; r < 3; r++) n += creal(col[i][r] * conj(col[i][r]));
-      ni = 1.0 / sqrt(n);
-      for (int r = 0; r < 3; r++) col[i][r] *= ni;
-    }
+    for (int pass = 0; pass < 2; pass++)
+      for (int j = 0; j < i; j++) {
+        cplx d = 0;
+        for (int r = 0; r < 3; r++) d += conj(col[j][r]) * col[i][r];
+        for (int r = 0; r < 3; r++) col[i][r] -= d * col[j][r];
+        n = 0;
+        for (int r = 0; r < 3; r++) n += creal(col[i][r] * conj(col[i][r]));
+        ni = 1.0 / sqrt(n);
+        for (int r = 0; r < 3; r++) col[i][r] *= ni;
+      }
   }
   for (int c = 0; c < 3; c++)
     for (int r = 0; r < 3; r++) mset(m, r, c, col[c][r]);

@@ -319,6 +319,7 @@ static MArgs<P, PO> margs(const qamd_links *l, int par, const void *in, const fl
   A.par = par;
   A.nr = nr;
   A.rs = rs;
+  A.lsign = (l->recon && l->nterms == 16) ? l->lsign[par] : nullptr;
   for (int t = 0; t < 16; t++) A.lscale[t] = (P == QAMD_Q16) ? l->scale[t] / Q16_ONE : l->scale[t];
   return A;
 }

@@ -8,7 +8,8 @@
 //
 // Term order follows QLL's StaggeredLinks (lib/stagDslash.c:438-448, 820-826):
 // with Naik, term 2k is the 1-hop link u[k] and 2k+1 the 3-hop link u3[k];
-// k = 2mu gathers from x+mu, k = 2mu+1 from x-mu.
+// k = 2mu gathers from x+mu, k = 2mu+1 from x-mu.  In storage the 1-hop terms
+// come first (slots 0-7) and the 3-hop terms after them (slots 8-15).
 #include "qamd_internal.h"
 #include "qamd_objects.h"
 #include "qamd_kernels.h"
@@ -42,23 +43,6 @@ template <int P, int PO = P> struct StagArgs {
   const uint8_t *lsign;   // long-term determinant signs when their third rows are rebuilt
 };
 
-// third row of a long term c*W (W unitary, det W = -1 if neg): det W *
-// conj(row0 x row1) / c, c = |row0|; element (r, c) is q[3c + r]
-__device__ __forceinline__ void long_row3(double2 (&q)[9][1], bool neg) {
-  double c2 = 0;
-#pragma unroll
-  for (int c = 0; c < 3; c++) c2 += q[3 * c][0].x * q[3 * c][0].x + q[3 * c][0].y * q[3 * c][0].y;
-  const double ic = (neg ? -1.0 : 1.0) / sqrt(c2);
-#pragma unroll
-  for (int c = 0; c < 3; c++) {
-    const double2 a1 = q[3 * ((c + 1) % 3)][0], a2 = q[3 * ((c + 2) % 3)][0];
-    const double2 b1 = q[3 * ((c + 1) % 3) + 1][0], b2 = q[3 * ((c + 2) % 3) + 1][0];
-    const double re = (a1.x * b2.x - a1.y * b2.y) - (a2.x * b1.x - a2.y * b1.y);
-    const double im = (a1.x * b2.y + a1.y * b2.x) - (a2.x * b1.y + a2.y * b1.x);
-    q[3 * c + 2][0] = make_double2(re * ic, -im * ic);
-  }
-}
-
 struct FinStore2 {
   double *dst;
   __device__ void operator()(const double *t) const {
@@ -67,8 +51,8 @@ struct FinStore2 {
   }
 };
 
-// REC (FP64, 16 terms): the long terms' third rows are not loaded but
-// rebuilt from the first two (2016 instead of 2400 bytes per site)
+// REC (16 terms): the long terms' third rows are not loaded but rebuilt
+// from the first two (FP64: 2016 instead of 2400 bytes per site)
 template <int P, int PO, int NT, bool RED, bool REC = false>
 __global__ void __launch_bounds__(256) k_stag_dslash(StagArgs<P, PO> A) {
   typedef typename Prec<P>::R R;
@@ -82,7 +66,7 @@ __global__ void __launch_bounds__(256) k_stag_dslash(StagArgs<P, PO> A) {
   if (cb < A.g.vh) {
     Site s = site_of(A.g, cb, A.par);
     constexpr int G = LinkG<S>::G;
-    static_assert(!REC || (P == QAMD_F64 && NT == 16), "long-row rebuild: FP64 16-term links only");
+    static_assert(!REC || NT == 16, "long-row rebuild: 16-term links only");
     const S *lp = link_base<S, NT>(A.links, cb);
     [[maybe_unused]] unsigned lsb = 0;
     if constexpr (REC) lsb = A.lsign[cb];
@@ -90,21 +74,19 @@ __global__ void __launch_bounds__(256) k_stag_dslash(StagArgs<P, PO> A) {
 #pragma unroll
     for (int j = 0; j < 3; j++) acc[j].x = acc[j].y = (R)0;
 #pragma unroll
-    for (int tg = 0; tg < NT / G; tg++) {
-      S q[9][G];   // element e of terms tg*G .. tg*G+G-1, one 16-byte load each
+    for (int sg = 0; sg < NT / G; sg++) {
+      // slots sg*G .. sg*G+G-1: all 1-hop terms (slot < 8) or all 3-hop ones
+      const bool lng = REC && sg * G >= 8;
+      S q[9][G];   // element e of the group's G terms, one 16-byte load each
 #pragma unroll
       for (int e = 0; e < 9; e++) {
-        if constexpr (REC) {
-          if ((tg & 1) && e % 3 == 2) continue;   // third row of a long term: rebuilt below
-        }
-        ntload_group(lp + link_off<S>(tg * G, e), q[e]);
-      }
-      if constexpr (REC) {
-        if (tg & 1) long_row3(q, (lsb >> (tg >> 1)) & 1);
+        if (lng && e % 3 == 2) continue;   // third row of a long term: rebuilt below
+        ntload_group(lp + link_off<S, NT>(sg * G, e), q[e]);
       }
 #pragma unroll
       for (int gg = 0; gg < G; gg++) {
-        const int t = tg * G + gg;
+        const int sl = sg * G + gg;
+        const int t = slot_term(sl, NT);
         const int k = (NT == 16) ? t / 2 : t;
         const int hop = (NT == 16 && (t & 1)) ? 3 : 1;
         const int mu = k / 2;
@@ -114,7 +96,9 @@ __global__ void __launch_bounds__(256) k_stag_dslash(StagArgs<P, PO> A) {
         VecLd<P>::ld(A.in, A.in_norm, vp, nb, v, (P == QAMD_F16 || P == QAMD_Q16) ? A.lscale[t] : (R)1);
         C m[9];
 #pragma unroll
-        for (int e = 0; e < 9; e++) m[e] = to_c(q[e][gg], (R)0);
+        for (int e = 0; e < 9; e++)
+          if (!(lng && e % 3 == 2)) m[e] = to_c(q[e][gg], (R)0);
+        if (lng) long_row3<C, R>(m, (lsb >> (sl - 8)) & 1);
         // column major: element (r,c) = m[3c+r]
 #pragma unroll
         for (int r = 0; r < 3; r++)
@@ -153,7 +137,7 @@ __global__ void __launch_bounds__(256) k_stag_dslash(StagArgs<P, PO> A) {
 template <int P, int PO, int NT>
 static void launch_t(const StagArgs<P, PO> &A, bool red, hipStream_t st) {
   int nb = (int)((A.g.vh + 255) / 256);
-  if constexpr (P == QAMD_F64 && NT == 16) {
+  if constexpr (NT == 16) {
     if (A.lsign) {
       if (red)
         hipLaunchKernelGGL((k_stag_dslash<P, PO, NT, true, true>), dim3(nb), dim3(256), 0, st, A);
@@ -187,7 +171,7 @@ static int launch_p(const qamd_links *l, Field &out, const Field &in, double a, 
   A.a = a;
   A.b = b;
   A.par = out.parity;
-  A.lsign = (P == QAMD_F64 && l->recon) ? l->lsign[out.parity] : nullptr;
+  A.lsign = (l->recon && l->nterms == 16) ? l->lsign[out.parity] : nullptr;
   // Q16 links are read as raw integers: fold 1/32767 into the term scale
   for (int t = 0; t < 16; t++) A.lscale[t] = (P == QAMD_Q16) ? l->scale[t] / Q16_ONE : l->scale[t];
   const bool red = red_out != nullptr || !finish;

@@ -142,7 +142,7 @@ __global__ void __launch_bounds__(256) k_wilson(WArgs<P> A) {
     for (int k = threadIdx.x; k < ns * NCH; k += blockDim.x) {
       const int sl = k % ns, ch = k / ns;   // neighbouring threads: neighbouring sites
       const Site ss = A.tiled ? site_of_tiled(g, A.tl, li0 + sl, A.par) : site_of(g, li0 + sl, A.par);
-      const S *lps = link_base<S, 8>(A.links, ss.lex >> 1) + link_off<S>((ch / 9) * G, ch % 9);
+      const S *lps = link_base<S, 8>(A.links, ss.lex >> 1) + link_off<S, 8>((ch / 9) * G, ch % 9);
       lds[sl * NCH + ch] = __builtin_nontemporal_load(reinterpret_cast<const uvec4_t *>(lps));
     }
     __syncthreads();
@@ -167,7 +167,7 @@ __global__ void __launch_bounds__(256) k_wilson(WArgs<P> A) {
       for (int e = 0; e < 9; e++) unpack_group(ls[tg * 9 + e], q[e]);
     } else {
 #pragma unroll
-      for (int e = 0; e < 9; e++) ntload_group(lp + link_off<S>(tg * G, e), q[e]);
+      for (int e = 0; e < 9; e++) ntload_group(lp + link_off<S, 8>(tg * G, e), q[e]);
     }
 #pragma unroll
     for (int gg = 0; gg < G; gg++) {

@@ -17,6 +17,23 @@ template <class S> __device__ inline S mk(double2 v);
 template <> __device__ inline double2 mk<double2>(double2 v) { return v; }
 template <> __device__ inline float2 mk<float2>(double2 v) { return make_float2((float)v.x, (float)v.y); }
 
+// The 16-term block layout is a permutation of the block's 144 elements
+// (slots x 9), and with the long third rows left out the rest is a prefix.
+template <int G>
+constexpr bool link_rows_ok() {
+  bool seen[16 * 9] = {};
+  const int prefix = (16 / G) * 9 - (8 / G) * 3;
+  for (int sl = 0; sl < 16; sl++)
+    for (int k = 0; k < 9; k++) {
+      const int r = link_row<G>(sl, k, 16), i = r * G + sl % G;
+      if (i < 0 || i >= 16 * 9 || seen[i]) return false;
+      seen[i] = true;
+      if ((sl >= 8 && k % 3 == 2) != (r >= prefix)) return false;
+    }
+  return true;
+}
+static_assert(link_rows_ok<1>() && link_rows_ok<2>() && link_rows_ok<4>(), "16-term link rows");
+
 // one term from a host-ordered staging array into the blocked layout
 template <class H, class S>
 __global__ void k_link_import(S *d0, S *d1, const H *src, Geom g, int t, int nterm, SrcMap m) {
@@ -111,23 +128,22 @@ __global__ void k_link_absmax(const S *src, int64_t nblk, int nterm, unsigned *m
 template <class SI, class SO>
 __global__ void k_link_convert(SO *dst, const SI *src, int64_t n, int nterm, const float *iscale,
                                const float *oscale) {
-  constexpr int GO = LinkG<SO>::G;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    // (site, term, element) of destination slot i, then the source slot
-    int64_t q = i / GO;
-    int t = (int)(i % GO), lane = (int)(q & 63), k;
-    q >>= 6;
-    k = (int)(q % 9);
+    // (site block, term, element, lane) of work item i; consecutive threads
+    // take consecutive sites
+    int64_t q = i >> 6;
+    const int lane = (int)(i & 63), k = (int)(q % 9);
     q /= 9;
-    t += (int)(q % (nterm / GO)) * GO;
-    int64_t cb = (q / (nterm / GO)) * 64 + lane;
+    const int t = (int)(q % nterm);
+    const int64_t cb = (q / nterm) * 64 + lane;
+    const int64_t o = link_idx<SO>(cb, t, k, nterm);
     double2 v = ld2<SI>(src, link_idx<SI>(cb, t, k, nterm));
     double f = (double)iscale[t] / (double)oscale[t];
     v.x *= f;
     v.y *= f;
-    if constexpr (std::is_same<SO, __half2>::value) dst[i] = __floats2half2_rn((float)v.x, (float)v.y);
-    else if constexpr (std::is_same<SO, short2>::value) dst[i] = q16_pack((float)v.x, (float)v.y);
-    else dst[i] = mk<SO>(v);
+    if constexpr (std::is_same<SO, __half2>::value) dst[o] = __floats2half2_rn((float)v.x, (float)v.y);
+    else if constexpr (std::is_same<SO, short2>::value) dst[o] = q16_pack((float)v.x, (float)v.y);
+    else dst[o] = mk<SO>(v);
   }
 }
 
@@ -205,6 +221,25 @@ static int long_link_check(qamd_links *l) {
     return -1;
   }
   l->recon = hb ? 0 : 1;
+  return 0;
+}
+
+// A lower-precision copy of links that pass the check rebuilds its long
+// terms' third rows the same way (same determinant signs): its stored rows
+// are the roundings of a scaled unitary matrix's, so the rebuilt row carries
+// the storage precision's error, as a stored one would.
+static int inherit_recon(qamd_links *dst, const qamd_links *src) {
+  dst->recon = 0;
+  if (!src->recon || dst->kind != 0 || dst->nterms != 16) return 0;
+  const Geom &g = dst->lat->g;
+  for (int p = 0; p < 2; p++) {
+    if (!dst->lsign[p] && !(dst->lsign[p] = (uint8_t *)dev_alloc(g.vh_pad))) return -1;
+    if (hipMemcpyAsync(dst->lsign[p], src->lsign[p], g.vh_pad, hipMemcpyDeviceToDevice, stream()) != hipSuccess) {
+      set_error("links_convert: sign copy failed");
+      return -1;
+    }
+  }
+  dst->recon = 1;
   return 0;
 }
 
@@ -291,9 +326,13 @@ static int import_terms(qamd_links *l, const void *const terms[], int hprec, Src
     }
   }
   dev_free(stage);
-  if (!rc && dst != l) rc = qamd_links_convert(l, dst);
-  if (dst != l) qamd_links_destroy(dst);
-  if (!rc) rc = long_link_check(l);
+  if (dst != l) {   // 16-bit: checked in FP64, then converted (the copy inherits the result)
+    if (!rc) rc = long_link_check(dst);
+    if (!rc) rc = qamd_links_convert(l, dst);
+    qamd_links_destroy(dst);
+  } else if (!rc) {
+    rc = long_link_check(l);
+  }
   return rc;
 }
 
@@ -360,9 +399,13 @@ int qamd_links_from_gauge(qamd_links *l, const qamd_gauge *fat, const qamd_gauge
     set_error("links_from_gauge kernel failed");
     rc = -1;
   }
-  if (!rc && dst != l) rc = qamd_links_convert(l, dst);
-  if (dst != l) qamd_links_destroy(dst);
-  if (!rc) rc = long_link_check(l);
+  if (dst != l) {   // 16-bit: checked in FP64, then converted (the copy inherits the result)
+    if (!rc) rc = long_link_check(dst);
+    if (!rc) rc = qamd_links_convert(l, dst);
+    qamd_links_destroy(dst);
+  } else if (!rc) {
+    rc = long_link_check(l);
+  }
   return rc;
 }
 
@@ -440,7 +483,7 @@ int qamd_links_convert(qamd_links *dst, const qamd_links *src) {
   for (int t = 0; t < 16; t++) dst->scale[t] = oscale[t];
   dev_free(sc);
   dev_free(mx);
-  return long_link_check(dst);
+  return dst->prec == QAMD_F64 ? long_link_check(dst) : inherit_recon(dst, src);
 }
 
 }  // extern "C"

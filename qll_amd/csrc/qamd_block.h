@@ -22,6 +22,7 @@ template <int P, int PO> struct MArgs {
   float lscale[16];
   RedScratch rs;   // partial sums of rhs r in slots 2r, 2r+1 (offset by the caller)
   int xcd;         // 1: XCD-aware block order
+  const uint8_t *lsign;   // set: long terms' third rows rebuilt (determinant sign bits per site)
 };
 
 // one output site cb: out_r = a*w_r + b * sum_t L_t in_r(x+d_t) for r < A.nr
@@ -42,21 +43,30 @@ __device__ __forceinline__ void block_site(const MArgs<P, PO> &A, int cb, double
   for (int r = 0; r < NR; r++)
 #pragma unroll
     for (int j = 0; j < 3; j++) acc[r][j].x = acc[r][j].y = (R)0;
+  // the long terms' third rows are rebuilt when the link set allows it (a
+  // per-problem, hence workgroup-uniform, choice)
+  const bool rec = NT == 16 && A.lsign != nullptr;
+  const unsigned lsb = rec ? A.lsign[cb] : 0u;
 #pragma unroll
-  for (int t = 0; t < NT; t++) {
+  for (int sl = 0; sl < NT; sl++) {
+    const int t = slot_term(sl, NT);
     const int k = (NT == 16) ? t / 2 : t;
     const int hop = (NT == 16 && (t & 1)) ? 3 : 1;
     const int mu = k / 2;
     const int h = (k & 1) ? -hop : hop;
+    const bool lng = NT == 16 && sl >= 8;
     int nb = nbr_cb(A.g, s, mu, h);
-    // one 16-byte group load per element serves G consecutive terms
-    if (t % G == 0) {
+    // one 16-byte group load per element serves G consecutive slots
+    if (sl % G == 0) {
 #pragma unroll
-      for (int e = 0; e < 9; e++) ntload_group(lp + link_off<S>(t, e), q[e]);
+      for (int e = 0; e < 9; e++)
+        if (!(lng && rec && e % 3 == 2)) ntload_group(lp + link_off<S, NT>(sl, e), q[e]);
     }
     C m[9];
 #pragma unroll
-    for (int e = 0; e < 9; e++) m[e] = to_c(q[e][t % G], (R)0);
+    for (int e = 0; e < 9; e++)
+      if (!(lng && rec && e % 3 == 2)) m[e] = to_c(q[e][sl % G], (R)0);
+    if (lng && rec) long_row3<C, R>(m, (lsb >> (sl - 8)) & 1);
 #pragma unroll
     for (int r = 0; r < NR; r++) {
       if (r < nr) {

@@ -10,10 +10,10 @@
 //     keep one FP32 max-norm per (rhs, site) beside them;
 //   * link sets are blocked by 64 sites (one wavefront) and packed so that
 //     one 16-byte load returns element k of G = 16/sizeof(complex) consecutive
-//     terms (G = 1 FP64, 2 FP32, 4 FP16/Q16): element k of term t for site cb
-//     lives at [((((cb>>6)*(nterm/G) + t/G)*9 + k)*64 + (cb&63))*G + t%G],
-//     so a wavefront streams one contiguous block per site group with
-//     full-width loads.
+//     storage slots (G = 1 FP64, 2 FP32, 4 FP16/Q16): element k of slot sl
+//     for site cb lives at [(((cb>>6)*(nterm/G)*9 + row)*64 + (cb&63))*G +
+//     sl%G] with row = link_row<G>(sl, k, nterm) (below), so a wavefront
+//     streams one contiguous block per site group with full-width loads.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <hip/hip_fp16.h>
@@ -167,21 +167,69 @@ __device__ __forceinline__ int nbr_cb_rt(const Geom &g, const Site &s, int mu, i
   return (s.lex + (n - c) * st) >> 1;
 }
 
+// Storage slot of term t.  In a 16-term staggered set the 1-hop terms
+// (t = 2k) fill slots 0-7 and the 3-hop terms (t = 2k+1) slots 8-15, so a
+// 16-byte group of G slots never mixes the two kinds and a long-term group can
+// skip its third-row loads as a whole.  8-term sets store term t in slot t.
+__host__ __device__ constexpr int term_slot(int t, int nterm) {
+  return nterm == 16 ? ((t & 1) << 3) | (t >> 1) : t;
+}
+__host__ __device__ constexpr int slot_term(int sl, int nterm) {
+  return nterm == 16 ? ((sl & 7) << 1) | (sl >> 3) : sl;
+}
+
+// Row (a run of 64 sites x G slots) of element k of slot sl inside one
+// 64-site block.  8-term sets: group-major, (sl/G)*9 + k.  16-term sets: the
+// 1-hop groups as 8-term sets, then the first two rows (k%3 != 2) of each
+// 3-hop group, then all 3-hop third rows, so a Dslash that rebuilds the third
+// rows reads one contiguous prefix (120 of 144 rows) of every block.
+template <int G>
+__host__ __device__ constexpr int link_row(int sl, int k, int nterm) {
+  if (nterm != 16 || sl < 8) return (sl / G) * 9 + k;
+  const int fat = (8 / G) * 9, g = (sl - 8) / G;
+  return (k % 3 != 2) ? fat + g * 6 + (k / 3) * 2 + k % 3 : fat + (8 / G) * 6 + g * 3 + k / 3;
+}
+
 // blocked link index: element k (0..8) of term t for site cb
 template <class S>
 __host__ __device__ __forceinline__ int64_t link_idx(int64_t cb, int t, int k, int nterm) {
   constexpr int G = LinkG<S>::G;
-  return (((((cb >> 6) * (nterm / G) + t / G) * 9 + k) << 6) + (cb & 63)) * G + t % G;
+  const int sl = term_slot(t, nterm);
+  return ((((cb >> 6) * (nterm / G) * 9 + link_row<G>(sl, k, nterm)) << 6) + (cb & 63)) * G + sl % G;
 }
-// first element of site cb's link stream, and the offset of (t, k) from it
+// first element of site cb's link stream, and the offset of (slot, k) from it
 template <class S, int NT>
 __device__ __forceinline__ const S *link_base(const S *links, int cb) {
   return links + (int64_t)(cb >> 6) * NT * 9 * 64 + (int64_t)(cb & 63) * LinkG<S>::G;
 }
-template <class S>
-__device__ __forceinline__ int link_off(int t, int k) {
+template <class S, int NT>
+__device__ __forceinline__ int link_off(int sl, int k) {
   constexpr int G = LinkG<S>::G;
-  return ((t / G) * 9 + k) * 64 * G + t % G;
+  return link_row<G>(sl, k, NT) * 64 * G + sl % G;
+}
+
+// Third row of a long term c*W (W unitary, det W = -1 if neg) from its first
+// two: det W * conj(row0 x row1) / c, c = |row0|.  Element (r, c) is m[3c+r].
+// Scale-free, so it also holds for 16-bit terms read as raw integers.
+// Contraction is off: which products the compiler would fuse into FMAs
+// depends on the surrounding code, and every kernel instance (rhs count,
+// reduction, launch shape) must rebuild the same bits.
+template <class C, class R>
+__device__ __forceinline__ void long_row3(C (&m)[9], bool neg) {
+#pragma clang fp contract(off)
+  R c2 = 0;
+#pragma unroll
+  for (int c = 0; c < 3; c++) c2 += m[3 * c].x * m[3 * c].x + m[3 * c].y * m[3 * c].y;
+  const R ic = (neg ? (R)-1 : (R)1) / sqrt(c2);
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const C a1 = m[3 * ((c + 1) % 3)], a2 = m[3 * ((c + 2) % 3)];
+    const C b1 = m[3 * ((c + 1) % 3) + 1], b2 = m[3 * ((c + 2) % 3) + 1];
+    const R re = (a1.x * b2.x - a1.y * b2.y) - (a2.x * b1.x - a2.y * b1.y);
+    const R im = (a1.x * b2.y + a1.y * b2.x) - (a2.x * b1.y + a2.y * b1.x);
+    m[3 * c + 2].x = re * ic;
+    m[3 * c + 2].y = -im * ic;
+  }
 }
 
 // XCD-aware block order: block b runs on die b % 8; give each die one

@@ -103,6 +103,7 @@ static int blaunch(const qamd_links *l, Field &out, const Field &in, double a, c
   A.nr = in.nrhs;
   A.rs = rs;
   A.xcd = 1;
+  A.lsign = (l->recon && l->nterms == 16) ? l->lsign[out.parity] : nullptr;
   for (int t = 0; t < 16; t++) A.lscale[t] = (P == QAMD_Q16) ? l->scale[t] / Q16_ONE : l->scale[t];
   R.n = in.nrhs;
   R.nt = l->nterms;

@@ -28,6 +28,16 @@ def test_su3_generator_unitary_det1():
     assert np.abs(np.linalg.det(M) - 1).max() < 1e-14
 
 
+def test_su3_generator_unitary_at_scale():
+    """Over the 262,144 links of a 16^4 field the worst matrix stays unitary
+    to a few eps.  One Gram-Schmidt pass leaves eps * cond(G) errors, which
+    reach 1e-13..1e-12 at this count and break the 1e-12 check that lets the
+    FP64 Dslash rebuild the long links' third rows."""
+    M = mats(O.random_su3_field((16, 16, 16, 16), 3))
+    eye = np.eye(3)
+    assert np.abs(M @ M.conj().swapaxes(-1, -2) - eye).max() < 1e-14
+
+
 def test_su3_gram_schmidt_order():
     """makeU order (lib/vla.c:1635-1654): column 0 of U is the normalised
     column 0 of the Gaussian matrix, times the det phase factor."""
