@@ -429,8 +429,11 @@ def main():
         print("dslash %s: %s" % (key, prof[key]), file=sys.stderr, flush=True)
 
     # Wilson EO Dslash: FP64 one rhs, and 16 rhs sharing every link load
-    # (algorithmic bytes per site: 8 links + nrhs x (input + output spinor))
-    WLINK = {0: 8 * 144, 1: 8 * 72, 2: 8 * 36, 3: 8 * 36}
+    # (algorithmic bytes per site: 8 links + nrhs x (input + output spinor);
+    # with one or two rhs SU(3) links are read as two rows of each term and
+    # the third is rebuilt)
+    WLINK = {p: 8 * 9 * c for p, c in CPLX_BYTES.items()}
+    WLINK_REBUILT = {p: 8 * 6 * c for p, c in CPLX_BYTES.items()}
     WSPIN = {0: 192, 1: 96, 2: 52, 3: 52}
     gw = Q.Gauge(lat).random_su3(3)
     W64 = Q.Links(lat, Q.WILSON, Q.F64, 8).from_gauge(gw)
@@ -441,9 +444,11 @@ def main():
         out = Q.Field(lat, prec, 12, nrw, Q.EVEN)
         Q.time_dslash(Lw, out, yin, 10)
         ms = Q.time_dslash(Lw, out, yin, max(1, args.steps // 4))
-        byts = (WLINK[prec] + nrw * 2 * WSPIN[prec]) * vh
+        wl = WLINK_REBUILT[prec] if Lw.reconstructs and nrw <= 2 else WLINK[prec]   # the rule in wlaunch
+        byts = (wl + nrw * 2 * WSPIN[prec]) * vh
         key = "wilson_%s_%drhs" % (PREC_NAME[prec], nrw)
         prof[key] = {"ms_per_call": round(ms, 5), "ms_per_rhs": round(ms / nrw, 5),
+                     "bytes_per_site": wl + nrw * 2 * WSPIN[prec],
                      "GFLOP/s": round(nrw * FLOPS_WILSON * vh / (ms * 1e-3) / 1e9, 1),
                      "GFLOP/s_qll_count": round(nrw * FLOPS_WILSON_QLL * vh / (ms * 1e-3) / 1e9, 1),
                      "GB/s": round(byts / (ms * 1e-3) / 1e9, 1),

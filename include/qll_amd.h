@@ -146,16 +146,18 @@ int qamd_links_from_qll(qamd_links *l, const qamd_qll_layout *lay, const void *c
 int qamd_links_from_gauge(qamd_links *l, const qamd_gauge *fat, const qamd_gauge *lng);
 /* copy with precision conversion (same kind and term count) */
 int qamd_links_convert(qamd_links *dst, const qamd_links *src);
-/* 1 if the asqtad Dslash on these links rebuilds the long (Naik) terms'
- * third rows instead of loading them: set when every long term of a 16-term
- * FP64 staggered link set is a scale times a unitary matrix to 1e-12 (the
- * case for fat7/Naik links built from SU(3) gauge fields with phases), after
- * each import, doublestore or conversion.  FP32/FP16/Q16 copies converted
- * from such a set (and 16-bit sets imported or doublestored through FP64)
- * inherit it; FP32 sets imported directly, and links that are not of that
- * form (e.g. hash-filled test fields), are read in full.  No counterpart in
- * QLL; the reconstruction is QUDA's two-row compression, QLL's next-step list
- * names it for the long links. */
+/* 1 if the Dslash on these links rebuilds third rows instead of loading
+ * them: set when every long term of a 16-term FP64 staggered link set (the
+ * case for fat7/Naik links built from SU(3) gauge fields with phases), or
+ * every term of an FP64 Wilson set, is a scale times a unitary matrix to
+ * 1e-12, after each import, doublestore or conversion.  FP32/FP16/Q16 copies
+ * converted from such a set (and 16-bit sets imported or doublestored
+ * through FP64) inherit it; FP32 sets imported directly, and links that are
+ * not of that form (e.g. hash-filled test fields), are read in full.  The
+ * Wilson Dslash rebuilds for one or two rhs only (with more, the rhs threads
+ * of a site would each repeat the arithmetic).  No counterpart in QLL; the
+ * reconstruction is QUDA's two-row compression, QLL's next-step list names
+ * it for the long links. */
 int qamd_links_reconstructs(const qamd_links *l);
 
 /* ---- Dslash (replaces dslash2 lib/stagDslash.c:531-544 and wilsonDslash

@@ -38,6 +38,7 @@ template <int P> struct WArgs {
   int xcd;     // XCD-aware block order (each die takes one contiguous range of sites)
   int tiled;   // sites in the tiled order of tl (else checkerboard order)
   Tiling tl;
+  const uint8_t *lsign;   // set: third rows rebuilt (determinant sign bit per site and term)
 };
 
 template <class C> __device__ __forceinline__ C cadd(C a, C b) { a.x += b.x; a.y += b.y; return a; }
@@ -114,8 +115,13 @@ struct FinStore1 {
 // (8 or more rhs per site)
 constexpr int WMAXS = 256 / 8 + 1;
 
-template <int P, bool RED, bool LDSL>
+// REC: the links' third rows are not read but rebuilt from the first two
+// (every term is a scale times a unitary matrix; 8·96 instead of 8·144 link
+// bytes per site in FP64).  Used for one or two rhs, LDS staging for eight or
+// more: never both.
+template <int P, bool RED, bool LDSL, bool REC>
 __global__ void __launch_bounds__(256) k_wilson(WArgs<P> A) {
+  static_assert(!(LDSL && REC), "the LDS link stage reads whole links");
   typedef typename Prec<P>::R R;
   typedef typename Prec<P>::C C;
   typedef typename Prec<P>::S S;
@@ -154,6 +160,8 @@ __global__ void __launch_bounds__(256) k_wilson(WArgs<P> A) {
     const int cb = s.lex >> 1;
     constexpr int K = Chunk<P>::K;
     const S *lp = link_base<S, 8>(A.links, cb);
+    [[maybe_unused]] unsigned lsb = 0;
+    if constexpr (REC) lsb = A.lsign[cb];
     C acc[4][3];
 #pragma unroll
     for (int a = 0; a < 4; a++)
@@ -167,7 +175,8 @@ __global__ void __launch_bounds__(256) k_wilson(WArgs<P> A) {
       for (int e = 0; e < 9; e++) unpack_group(ls[tg * 9 + e], q[e]);
     } else {
 #pragma unroll
-      for (int e = 0; e < 9; e++) ntload_group(lp + link_off<S, 8>(tg * G, e), q[e]);
+      for (int e = 0; e < 9; e++)
+        if (!(REC && e % 3 == 2)) ntload_group(lp + link_off<S, 8>(tg * G, e), q[e]);
     }
 #pragma unroll
     for (int gg = 0; gg < G; gg++) {
@@ -208,7 +217,9 @@ __global__ void __launch_bounds__(256) k_wilson(WArgs<P> A) {
       }
       C m[9];
 #pragma unroll
-      for (int e = 0; e < 9; e++) m[e] = to_c(q[e][gg], (R)0);
+      for (int e = 0; e < 9; e++)
+        if (!(REC && e % 3 == 2)) m[e] = to_c(q[e][gg], (R)0);
+      if constexpr (REC) long_row3<C, R>(m, (lsb >> t) & 1);
       C uh[2][3];
 #pragma unroll
       for (int a = 0; a < 2; a++)
@@ -330,6 +341,12 @@ template <int P> struct WRun {
   int nb;
   hipStream_t st;
   int ncand, shape[NWSHAPE];
+  template <bool RED, bool REC>
+  static void launch_k(int lds, const WRun &R, const WArgs<P> &B) {
+    if constexpr (REC) hipLaunchKernelGGL((k_wilson<P, RED, false, true>), dim3(R.nb), dim3(256), 0, R.st, B);
+    else if (lds) hipLaunchKernelGGL((k_wilson<P, RED, true, false>), dim3(R.nb), dim3(256), 0, R.st, B);
+    else hipLaunchKernelGGL((k_wilson<P, RED, false, false>), dim3(R.nb), dim3(256), 0, R.st, B);
+  }
   static int run(int c, void *ctx) {
     WRun &R = *(WRun *)ctx;
     WArgs<P> B = R.A;
@@ -338,12 +355,12 @@ template <int P> struct WRun {
     B.tiled = sh.tile[0] ? 1 : 0;
     if (B.tiled && !B.tl.init(B.g.L, sh.tile)) return -1;
     if (B.red_out) {
-      if (sh.lds) hipLaunchKernelGGL((k_wilson<P, true, true>), dim3(R.nb), dim3(256), 0, R.st, B);
-      else hipLaunchKernelGGL((k_wilson<P, true, false>), dim3(R.nb), dim3(256), 0, R.st, B);
+      if (B.lsign) launch_k<true, true>(sh.lds, R, B);
+      else launch_k<true, false>(sh.lds, R, B);
       launch_finish<1>(B.rs, R.nb, FinStore1{B.red_out}, R.st, B.skip);
     } else {
-      if (sh.lds) hipLaunchKernelGGL((k_wilson<P, false, true>), dim3(R.nb), dim3(256), 0, R.st, B);
-      else hipLaunchKernelGGL((k_wilson<P, false, false>), dim3(R.nb), dim3(256), 0, R.st, B);
+      if (B.lsign) launch_k<false, true>(sh.lds, R, B);
+      else launch_k<false, false>(sh.lds, R, B);
     }
     QAMD_HIP(hipGetLastError());
     return 0;
@@ -369,6 +386,10 @@ static int wlaunch(const qamd_links *l, Field &out, const Field &in, double a, c
   A.par = out.parity;
   A.skip = skip;
   A.dag = dag ? 1 : 0;
+  // rebuilding costs every thread of a site the same arithmetic: it pays for
+  // one or two rhs (link bytes dominate: FP64 1 rhs 0.160 -> 0.139 ms at
+  // 32^4) and not for blocks (16 rhs: FP32 +3 %, Q16 +18 % time)
+  A.lsign = (l->recon && in.nrhs <= 2) ? l->lsign[out.parity] : nullptr;
   for (int t = 0; t < 8; t++) A.lscale[t] = (P == QAMD_Q16) ? l->scale[t] / Q16_ONE : l->scale[t];
   int64_t nthreads = A.g.vh * in.nrhs;
   WRun<P> R;

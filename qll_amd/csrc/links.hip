@@ -17,22 +17,24 @@ template <class S> __device__ inline S mk(double2 v);
 template <> __device__ inline double2 mk<double2>(double2 v) { return v; }
 template <> __device__ inline float2 mk<float2>(double2 v) { return make_float2((float)v.x, (float)v.y); }
 
-// The 16-term block layout is a permutation of the block's 144 elements
-// (slots x 9), and with the long third rows left out the rest is a prefix.
+// The block layout is a permutation of the block's nterm x 9 elements, and
+// with the rebuildable terms' third rows left out the rest is a prefix.
 template <int G>
-constexpr bool link_rows_ok() {
+constexpr bool link_rows_ok(int nterm) {
   bool seen[16 * 9] = {};
-  const int prefix = (16 / G) * 9 - (8 / G) * 3;
-  for (int sl = 0; sl < 16; sl++)
+  const int prefix = (nterm / G) * 9 - (8 / G) * 3;
+  for (int sl = 0; sl < nterm; sl++)
     for (int k = 0; k < 9; k++) {
-      const int r = link_row<G>(sl, k, 16), i = r * G + sl % G;
-      if (i < 0 || i >= 16 * 9 || seen[i]) return false;
+      const int r = link_row<G>(sl, k, nterm), i = r * G + sl % G;
+      if (i < 0 || i >= nterm * 9 || seen[i]) return false;
       seen[i] = true;
-      if ((sl >= 8 && k % 3 == 2) != (r >= prefix)) return false;
+      if ((sl >= nterm - 8 && k % 3 == 2) != (r >= prefix)) return false;
     }
   return true;
 }
-static_assert(link_rows_ok<1>() && link_rows_ok<2>() && link_rows_ok<4>(), "16-term link rows");
+static_assert(link_rows_ok<1>(16) && link_rows_ok<2>(16) && link_rows_ok<4>(16) && link_rows_ok<1>(8) &&
+                  link_rows_ok<2>(8) && link_rows_ok<4>(8),
+              "link block rows");
 
 // one term from a host-ordered staging array into the blocked layout
 template <class H, class S>
@@ -147,17 +149,18 @@ __global__ void k_link_convert(SO *dst, const SI *src, int64_t n, int nterm, con
   }
 }
 
-// Long (Naik) terms as c * W with W unitary (c > 0, det W = +-1): then the
-// third row is det W * conj(row0 x row1) / c.  Checks every long term of
-// FP64 16-term links to 1e-12 relative, records det W per (site, term) and
-// flags any term that is not of that form.
-__global__ void k_long_check(const double2 *d, uint8_t *sign, Geom g, int64_t nblk, unsigned *bad) {
+// Terms of the form c * W with W unitary (c > 0, det W = +-1): then the
+// third row is det W * conj(row0 x row1) / c.  Checks the eight terms
+// t0 + dt*k of FP64 links (16-term staggered: the long terms 2k+1; Wilson:
+// all eight) to 1e-12 relative, records det W per (site, k) and flags any
+// term that is not of that form.
+__global__ void k_long_check(const double2 *d, uint8_t *sign, Geom g, int nterm, int t0, int dt, unsigned *bad) {
   const int cb = blockIdx.x * blockDim.x + threadIdx.x;
   if (cb >= g.vh) return;
   unsigned bits = 0, fail = 0;
   for (int k = 0; k < 8; k++) {
     double2 m[9];
-    for (int e = 0; e < 9; e++) m[e] = d[link_idx<double2>(cb, 2 * k + 1, e, 16)];
+    for (int e = 0; e < 9; e++) m[e] = d[link_idx<double2>(cb, t0 + dt * k, e, nterm)];
     // row r holds elements 3c + r
     double c2 = 0, n2 = 0;
     double2 dot = make_double2(0, 0);
@@ -196,11 +199,16 @@ __global__ void k_long_check(const double2 *d, uint8_t *sign, Geom g, int64_t nb
   if (fail) atomicOr(bad, 1u);
 }
 
-// after FP64 16-term staggered links change: decide whether the Dslash may
-// rebuild the long terms' third rows (all long terms pass the check)
+// link sets whose Dslash may rebuild third rows: 16-term staggered (the long
+// terms) and Wilson (every term)
+static bool rebuild_kind(const qamd_links *l) { return (l->kind == 0 && l->nterms == 16) || l->kind == 1; }
+
+// after FP64 links change: decide whether the Dslash may rebuild the third
+// rows (every checked term passes)
 static int long_link_check(qamd_links *l) {
   l->recon = 0;
-  if (l->kind != 0 || l->prec != QAMD_F64 || l->nterms != 16) return 0;
+  if (!rebuild_kind(l) || l->prec != QAMD_F64) return 0;
+  const int t0 = l->kind == 0 ? 1 : 0, dt = l->kind == 0 ? 2 : 1;
   const Geom &g = l->lat->g;
   for (int p = 0; p < 2; p++)
     if (!l->lsign[p] && !(l->lsign[p] = (uint8_t *)dev_alloc(g.vh_pad))) return -1;
@@ -210,7 +218,7 @@ static int long_link_check(qamd_links *l) {
   hipError_t e = hipMemsetAsync(bad, 0, sizeof(unsigned), stream());
   for (int p = 0; p < 2 && e == hipSuccess; p++) {
     hipLaunchKernelGGL(k_long_check, dim3((unsigned)((g.vh + 255) / 256)), dim3(256), 0, stream(),
-                       (const double2 *)l->data[p], l->lsign[p], g, g.vh_pad / 64, bad);
+                       (const double2 *)l->data[p], l->lsign[p], g, l->nterms, t0, dt, bad);
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipMemcpyAsync(&hb, bad, sizeof hb, hipMemcpyDeviceToHost, stream());
@@ -230,7 +238,7 @@ static int long_link_check(qamd_links *l) {
 // the storage precision's error, as a stored one would.
 static int inherit_recon(qamd_links *dst, const qamd_links *src) {
   dst->recon = 0;
-  if (!src->recon || dst->kind != 0 || dst->nterms != 16) return 0;
+  if (!src->recon || !rebuild_kind(dst)) return 0;
   const Geom &g = dst->lat->g;
   for (int p = 0; p < 2; p++) {
     if (!dst->lsign[p] && !(dst->lsign[p] = (uint8_t *)dev_alloc(g.vh_pad))) return -1;

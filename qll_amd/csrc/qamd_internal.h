@@ -179,15 +179,17 @@ __host__ __device__ constexpr int slot_term(int sl, int nterm) {
 }
 
 // Row (a run of 64 sites x G slots) of element k of slot sl inside one
-// 64-site block.  8-term sets: group-major, (sl/G)*9 + k.  16-term sets: the
-// 1-hop groups as 8-term sets, then the first two rows (k%3 != 2) of each
-// 3-hop group, then all 3-hop third rows, so a Dslash that rebuilds the third
-// rows reads one contiguous prefix (120 of 144 rows) of every block.
+// 64-site block.  The eight terms whose third row a Dslash may rebuild (the
+// 3-hop slots 8-15 of a 16-term staggered set; all eight terms of an 8-term
+// set) store the first two rows (k%3 != 2) of every group, then all third
+// rows; a 16-term set puts its 1-hop groups (9 rows each) in front.  A Dslash
+// that rebuilds the third rows then reads one contiguous prefix of every
+// block (16 terms: 120 of 144 rows; 8 terms: 48 of 72).
 template <int G>
 __host__ __device__ constexpr int link_row(int sl, int k, int nterm) {
-  if (nterm != 16 || sl < 8) return (sl / G) * 9 + k;
-  const int fat = (8 / G) * 9, g = (sl - 8) / G;
-  return (k % 3 != 2) ? fat + g * 6 + (k / 3) * 2 + k % 3 : fat + (8 / G) * 6 + g * 3 + k / 3;
+  if (nterm == 16 && sl < 8) return (sl / G) * 9 + k;
+  const int base = nterm == 16 ? (8 / G) * 9 : 0, g = (nterm == 16 ? sl - 8 : sl) / G;
+  return (k % 3 != 2) ? base + g * 6 + (k / 3) * 2 + k % 3 : base + (8 / G) * 6 + g * 3 + k / 3;
 }
 
 // blocked link index: element k (0..8) of term t for site cb
@@ -208,7 +210,7 @@ __device__ __forceinline__ int link_off(int sl, int k) {
   return link_row<G>(sl, k, NT) * 64 * G + sl % G;
 }
 
-// Third row of a long term c*W (W unitary, det W = -1 if neg) from its first
+// Third row of a term c*W (W unitary, det W = -1 if neg) from its first
 // two: det W * conj(row0 x row1) / c, c = |row0|.  Element (r, c) is m[3c+r].
 // Scale-free, so it also holds for 16-bit terms read as raw integers.
 // Contraction is off: which products the compiler would fuse into FMAs

@@ -26,9 +26,10 @@ struct qamd_links {
   int nterms;
   void *data[2];
   float scale[16];   // FP16 only: per-term scale (stored = link / scale)
-  // FP64 staggered links with long (Naik) terms that are all a scale times a
-  // unitary matrix: the Dslash loads their first two rows only and rebuilds
-  // the third; lsign[p][cb] bit k = determinant sign of long term k
+  // links whose rebuildable terms (the long terms of a 16-term staggered set,
+  // every Wilson term) are all a scale times a unitary matrix, checked in
+  // FP64: the Dslash loads their first two rows only and rebuilds the third;
+  // lsign[p][cb] bit k = determinant sign of rebuildable term k
   int recon;
   uint8_t *lsign[2];
 };

@@ -65,4 +65,7 @@ def test_wilson_16rhs_block_at_64_4(gpu, big):
         one = gpu.Field(lat, 1, 12, 1, 1).gaussian(4, 100 + r)
         o1 = gpu.Field(lat, 1, 12, 1, 0)
         gpu.dslash(W32, o1, one, b=-1.0)
-        assert n_blk[r] == pytest.approx(o1.norm2(), rel=1e-12), r
+        # the single-rhs launch rebuilds the links' third rows from the FP32
+        # first two, the 16-rhs block reads the stored FP32 third rows: both
+        # are FP32 roundings of the same SU(3) matrices
+        assert n_blk[r] == pytest.approx(o1.norm2(), rel=1e-6), r

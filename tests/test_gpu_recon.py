@@ -1,7 +1,7 @@
-"""GPU: asqtad links whose long (Naik) terms are a scale times a unitary
-matrix (checked in FP64) are read as two rows and the third is rebuilt in the
-Dslash, also in the FP32/FP16/Q16 copies converted from them; other links are
-read in full.  Both give the oracle's Dslash to the precision's bound (FP64
+"""GPU: asqtad links whose long (Naik) terms, and Wilson links whose terms,
+are a scale times a unitary matrix (checked in FP64) are read as two rows and
+the third is rebuilt in the Dslash, also in the FP32/FP16/Q16 copies converted
+from them; other links are read in full.  Both give the oracle's Dslash to the precision's bound (FP64
 1e-13, FP32 1e-5, FP16 5e-3, Q16 2e-4, relative L2 over the output parity)."""
 import numpy as np
 import pytest
@@ -57,7 +57,9 @@ def test_device_links_and_conversions(gpu):
     assert not L64.converted(gpu.Q16).converted(gpu.F64).reconstructs   # 16-bit round trip: not unitary to 1e-12
     assert L64.converted(gpu.F64).reconstructs
     W = gpu.Links(lat, gpu.WILSON, 0, 8).from_gauge(gpu.Gauge(lat).random_su3(2))
-    assert not W.reconstructs
+    assert W.reconstructs and W.converted(gpu.F16).reconstructs   # every Wilson term is SU(3)
+    S8 = gpu.Links(lat, gpu.STAG, 0, 8).from_gauge(gpu.Gauge(lat).random_su3(3))
+    assert not S8.reconstructs                                   # no long terms
 
 
 def test_generated_links_reconstruct_at_full_size(gpu):
@@ -142,3 +144,37 @@ def test_block_dslash_rebuilt_and_full(gpu, prec):
                 ref = O.dslash(dims, lk, 0.0, ys[r], sub="even" if p == 0 else "odd")
                 err = rel_err(outs[p][r], ref, parity_mask(dims, p))
                 assert err < TOL[prec], (rec, p, r, err)
+
+
+@pytest.mark.parametrize("prec,nrhs", [(0, 1), (1, 1), (3, 1), (0, 8), (1, 16), (2, 8)])
+def test_wilson_rebuilt_and_full(gpu, prec, nrhs):
+    """Wilson links built from SU(3) fields rebuild every term's third row (in
+    every launch shape, with and without the LDS link stage); perturbed links
+    are read in full.  Both match the oracle; all shapes agree bit for bit."""
+    print("case: wilson rebuild prec=%d nrhs=%d" % (prec, nrhs), flush=True)
+    dims = (8, 8, 8, 8)
+    links = O.wilson_doublestore(dims, O.random_su3_field(dims, 35))
+    bad = links.copy()
+    bad[3, 11, 0:2] *= 1.001
+    ys = np.stack([O.gauss_vector(dims, 80 + r, 300, parity=-1, nreal=24) for r in range(nrhs)])
+    lat = gpu.Lattice(dims)
+    m = parity_mask(dims, 0)
+    for lk, rec in ((links, True), (bad, False)):
+        L = gpu.Links(lat, gpu.WILSON, 0, 8).set_lex(lk)
+        if prec:
+            L = L.converted(prec)
+        assert L.reconstructs == rec
+        yin = gpu.Field(lat, prec, 12, nrhs, 1).set_lex(ys)
+        res = []
+        for mode in [0] + list(range(2, 14)) + [1]:
+            gpu.set_tuning(mode)
+            out = gpu.Field(lat, prec, 12, nrhs, 0)
+            gpu.dslash(L, out, yin, b=-1.0)
+            res.append(out.get_lex())
+        gpu.set_tuning(1)
+        for o in res[1:]:
+            assert np.array_equal(o, res[0])
+        got = res[0] if nrhs > 1 else res[0][None]
+        for r in range(nrhs):
+            ref = O.wilson_dslash(dims, lk, 0.0, ys[r], sub="even")
+            assert rel_err(got[r], ref, m) < TOL[prec], (rec, r)
