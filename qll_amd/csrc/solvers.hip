@@ -39,6 +39,9 @@ struct SolverDev {
   double a[MAXSHIFT], b[MAXSHIFT], s[MAXSHIFT], bo[MAXSHIFT];
   double z[MAXSHIFT], zo[MAXSHIFT], zn[MAXSHIFT];
   double pa[MAXSHIFT], pzn[MAXSHIFT];   // coefficients for the p update
+  double xb[MAXSHIFT];                  // coefficients for the x update
+  int xrun;                             // 1: this iteration's x update is due
+  int drop;                             // 1: shifts stop updating once converged
 };
 
 template <int P> struct VT {
@@ -221,6 +224,12 @@ struct MSPtrs {
   float *pn[MAXSHIFT];
 };
 
+// After the |r|^2 reduction: the iteration's x-update coefficients b_i s_i
+// (the x_i update of lib/cgms.c:91-96, applied by k_ms_xp together with the
+// p update), the stopping test, and the p-update coefficients.  With drop
+// set, a shift i > 0 whose residual |zeta_i r| is below the stopping
+// residual takes no further updates (no counterpart in QLL, which iterates
+// every shift to the end).
 struct FinMS {
   SolverDev *S;
   __device__ void operator()(const double *t) const {
@@ -238,22 +247,26 @@ struct FinMS {
         double c2 = S->z[i] * b0;
         pai = (c2 != 0.0) ? a0 * zni * bi / c2 : 0;
       }
+      S->xb[i] = bi * S->s[i];
       S->pa[i] = pai;
       S->pzn[i] = zni;
       S->a[i] = pai;
       S->bo[i] = bi;
       S->zo[i] = S->z[i];
       S->z[i] = zni;
+      if (S->drop && i > 0 && zni * zni * S->rsq < S->rsqstop) S->s[i] = 0;
     }
+    S->xrun = 1;
   }
 };
 
-// x_i += b_i s_i p_i for all shifts; r -= b_0 s_0 Ap; |r|^2
+// r -= b_0 s_0 Ap; |r|^2 (lib/cgms.c:97-102).  Clears the previous
+// iteration's x-update gate first, so k_ms_xp only runs after a FinMS.
 template <int P>
-__global__ void __launch_bounds__(256) k_ms_x(MSPtrs V, Field r, Field Ap, SolverDev *S, RedScratch rs) {
+__global__ void __launch_bounds__(256) k_ms_r(Field r, Field Ap, SolverDev *S, RedScratch rs) {
   typedef typename Prec<P>::C C;
   typedef typename Prec<P>::R R;
-  typedef typename Prec<P>::S Sx;
+  if (blockIdx.x == 0 && threadIdx.x == 0) S->xrun = 0;
   if (S->done) return;
   double pkp = (S->m2 * S->nrm[3] + S->nrm[0]) * S->s[0] * S->s[0];
   if (pkp <= 0) {   // lib/cgms.c:76: loss of precision, leave
@@ -263,25 +276,10 @@ __global__ void __launch_bounds__(256) k_ms_x(MSPtrs V, Field r, Field Ap, Solve
     }
     return;
   }
-  const double b0 = S->rsq / pkp, a00 = S->a[0], bo0 = S->bo[0];
-  const int64_t vp = r.vp;
+  const double b0 = S->rsq / pkp;
   double red[1] = {0.0};
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < r.g->vh) {
-    for (int k = 0; k < S->nshift; k++) {
-      C xv[3], pv[3];
-      VecLd<P>::ld((const Sx *)V.x[k], V.xn[k], vp, i, xv, (R)1);
-      VecLd<P>::ld((const Sx *)V.p[k], V.pn[k], vp, i, pv, (R)1);
-      double bk = b0, znk;
-      if (k > 0) ms_coef(S, k, b0, a00, bo0, bk, znk);
-      R t = (R)(bk * S->s[k]);
-#pragma unroll
-      for (int j = 0; j < 3; j++) {
-        xv[j].x = fma(t, pv[j].x, xv[j].x);
-        xv[j].y = fma(t, pv[j].y, xv[j].y);
-      }
-      VecSt<P>::st((Sx *)V.x[k], V.xn[k], vp, i, xv);
-    }
     C rv[3], av[3];
     VT<P>::ld(r, i, rv);
     VT<P>::ld(Ap, i, av);
@@ -293,36 +291,49 @@ __global__ void __launch_bounds__(256) k_ms_x(MSPtrs V, Field r, Field Ap, Solve
     }
     VT<P>::st(r, i, rv);
     C rr[3];
-    VT<P>::ld(r, i, rr);
+    VT<P>::ld(r, i, rr);   // the stored value (16-bit storage rounds)
 #pragma unroll
     for (int j = 0; j < 3; j++) red[0] += (double)rr[j].x * rr[j].x + (double)rr[j].y * rr[j].y;
   }
   grid_partial<1>(red, rs);
 }
 
-// p_0 = a_0 p_0 + r;  p_i = a_i p_i + zn_i r   (lib/cgms.c:141-155)
+// x_i += b_i s_i p_i (lib/cgms.c:91-96), then, unless the solve stopped this
+// iteration, p_0 = a_0 p_0 + r and p_i = a_i p_i + zn_i r (lib/cgms.c:141-155):
+// one pass over each shift's x and p.  Shifts with s_i = 0 are skipped.
 template <int P>
-__global__ void __launch_bounds__(256) k_ms_p(MSPtrs V, Field r, SolverDev *S) {
+__global__ void __launch_bounds__(256) k_ms_xp(MSPtrs V, Field r, SolverDev *S) {
   typedef typename Prec<P>::C C;
   typedef typename Prec<P>::R R;
   typedef typename Prec<P>::S Sx;
-  if (S->done) return;
+  if (!S->xrun) return;
+  const bool pupd = !S->done;
   const int64_t vp = r.vp;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= r.g->vh) return;
   C rv[3];
-  VT<P>::ld(r, i, rv);
+  if (pupd) VT<P>::ld(r, i, rv);
   for (int k = 0; k < S->nshift; k++) {
-    if (k > 0 && S->s[k] == 0) continue;
-    C pv[3];
+    if (k > 0 && S->s[k] == 0 && S->xb[k] == 0) continue;
+    C xv[3], pv[3];
+    VecLd<P>::ld((const Sx *)V.x[k], V.xn[k], vp, i, xv, (R)1);
     VecLd<P>::ld((const Sx *)V.p[k], V.pn[k], vp, i, pv, (R)1);
-    R a = (R)S->pa[k], z = (R)(k == 0 ? 1.0 : S->pzn[k]);
+    const R t = (R)S->xb[k];
 #pragma unroll
     for (int j = 0; j < 3; j++) {
-      pv[j].x = fma(a, pv[j].x, z * rv[j].x);
-      pv[j].y = fma(a, pv[j].y, z * rv[j].y);
+      xv[j].x = fma(t, pv[j].x, xv[j].x);
+      xv[j].y = fma(t, pv[j].y, xv[j].y);
     }
-    VecSt<P>::st((Sx *)V.p[k], V.pn[k], vp, i, pv);
+    VecSt<P>::st((Sx *)V.x[k], V.xn[k], vp, i, xv);
+    if (pupd && (k == 0 || S->s[k] != 0)) {
+      R a = (R)S->pa[k], z = (R)(k == 0 ? 1.0 : S->pzn[k]);
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        pv[j].x = fma(a, pv[j].x, z * rv[j].x);
+        pv[j].y = fma(a, pv[j].y, z * rv[j].y);
+      }
+      VecSt<P>::st((Sx *)V.p[k], V.pn[k], vp, i, pv);
+    }
   }
 }
 
@@ -745,8 +756,17 @@ int qamd_stag_cg_mixed(const qamd_links *l64, const qamd_links *ls, qamd_field *
   return rc;
 }
 
+static int cgms_core(const qamd_links *l, qamd_field *const x[], const qamd_field *b, double mass,
+                     const double shifts[], int n, double rsq, int maxits, qamd_solve_info *info, bool drop);
+
 int qamd_stag_cgms(const qamd_links *l, qamd_field *const x[], const qamd_field *b, double mass,
                    const double shifts[], int n, double rsq, int maxits, qamd_solve_info *info) {
+  return cgms_core(l, x, b, mass, shifts, n, rsq, maxits, info, false);
+}
+
+// QLL's multishift CG; with drop, converged shifts i > 0 stop updating
+static int cgms_core(const qamd_links *l, qamd_field *const x[], const qamd_field *b, double mass,
+                     const double shifts[], int n, double rsq, int maxits, qamd_solve_info *info, bool drop) {
   if (!l || l->kind != 0) { set_error("stag_cgms: staggered links required"); return -1; }
   if (l->prec == QAMD_Q16) { set_error("stag_cgms: Q16 links are for the mixed-precision solvers only"); return -1; }
   if (n < 1 || n > MAXSHIFT) { set_error("stag_cgms: 1..%d shifts", MAXSHIFT); return -1; }
@@ -792,6 +812,8 @@ int qamd_stag_cgms(const qamd_links *l, qamd_field *const x[], const qamd_field 
     Sh->rsqstop = rsq * insq;
     Sh->m2 = mass * mass;
     Sh->itnlim = maxits > 0 ? maxits : 1;
+    Sh->xrun = 0;
+    Sh->drop = drop ? 1 : 0;
     if (push_state(st)) break;
     MSPtrs V;
     memset(&V, 0, sizeof V);
@@ -808,19 +830,19 @@ int qamd_stag_cgms(const qamd_links *l, qamd_field *const x[], const qamd_field 
         int nb = (int)grid1(lat).x;
         switch (l->prec) {
           case QAMD_F64:
-            hipLaunchKernelGGL(k_ms_x<QAMD_F64>, grid1(lat), dim3(256), 0, stream(), V, r->f, Ap->f, st.S, rs0);
+            hipLaunchKernelGGL(k_ms_r<QAMD_F64>, grid1(lat), dim3(256), 0, stream(), r->f, Ap->f, st.S, rs0);
             launch_finish<1>(rs0, nb, FinMS{st.S}, stream(), &st.S->done);
-            hipLaunchKernelGGL(k_ms_p<QAMD_F64>, grid1(lat), dim3(256), 0, stream(), V, r->f, st.S);
+            hipLaunchKernelGGL(k_ms_xp<QAMD_F64>, grid1(lat), dim3(256), 0, stream(), V, r->f, st.S);
             break;
           case QAMD_F32:
-            hipLaunchKernelGGL(k_ms_x<QAMD_F32>, grid1(lat), dim3(256), 0, stream(), V, r->f, Ap->f, st.S, rs0);
+            hipLaunchKernelGGL(k_ms_r<QAMD_F32>, grid1(lat), dim3(256), 0, stream(), r->f, Ap->f, st.S, rs0);
             launch_finish<1>(rs0, nb, FinMS{st.S}, stream(), &st.S->done);
-            hipLaunchKernelGGL(k_ms_p<QAMD_F32>, grid1(lat), dim3(256), 0, stream(), V, r->f, st.S);
+            hipLaunchKernelGGL(k_ms_xp<QAMD_F32>, grid1(lat), dim3(256), 0, stream(), V, r->f, st.S);
             break;
           default:
-            hipLaunchKernelGGL(k_ms_x<QAMD_F16>, grid1(lat), dim3(256), 0, stream(), V, r->f, Ap->f, st.S, rs0);
+            hipLaunchKernelGGL(k_ms_r<QAMD_F16>, grid1(lat), dim3(256), 0, stream(), r->f, Ap->f, st.S, rs0);
             launch_finish<1>(rs0, nb, FinMS{st.S}, stream(), &st.S->done);
-            hipLaunchKernelGGL(k_ms_p<QAMD_F16>, grid1(lat), dim3(256), 0, stream(), V, r->f, st.S);
+            hipLaunchKernelGGL(k_ms_xp<QAMD_F16>, grid1(lat), dim3(256), 0, stream(), V, r->f, st.S);
             break;
         }
         if (hipGetLastError() != hipSuccess) err = 1;
@@ -879,7 +901,9 @@ int qamd_stag_cgms_mixed(const qamd_links *l64, const qamd_links *ls, qamd_field
     qamd_solve_info msinfo;
     // FP32 recurrences do not resolve residuals far below 1e-6; refine
     double ms_rsq = rsq > 1e-12 ? rsq : 1e-12;
-    if (qamd_stag_cgms(l32, xs, bs, mass, shifts, n, ms_rsq, maxits, &msinfo)) break;
+    // (a shift whose FP32 residual is there stops updating: its refinement
+    // starts from it)
+    if (cgms_core(l32, xs, bs, mass, shifts, n, ms_rsq, maxits, &msinfo, true)) break;
     int err = 0;
     // refinement with 16-bit links only for shifts where they are accurate
     // enough (see sloppy_too_coarse), FP32 links otherwise
