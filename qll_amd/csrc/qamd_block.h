@@ -17,7 +17,8 @@ template <int P, int PO> struct MArgs {
   float *out_norm;
   const SO *w;
   const float *w_norm;
-  double a, b;
+  double ar[16], b;   // out_r = ar[r]*w_r + b * D in_r
+  unsigned act;       // bit r set: rhs r is computed (others are skipped)
   int par, nr;
   float lscale[16];
   RedScratch rs;   // partial sums of rhs r in slots 2r, 2r+1 (offset by the caller)
@@ -25,8 +26,9 @@ template <int P, int PO> struct MArgs {
   const uint8_t *lsign;   // set: long terms' third rows rebuilt (determinant sign bits per site)
 };
 
-// one output site cb: out_r = a*w_r + b * sum_t L_t in_r(x+d_t) for r < A.nr
-// (fields rhs-outer); with RED, red[2r] += |out_r|^2, red[2r+1] += |w_r|^2
+// one output site cb: out_r = a_r*w_r + b * sum_t L_t in_r(x+d_t) for r < A.nr
+// with bit r of A.act set (fields rhs-outer); with RED, red[2r] += |out_r|^2,
+// red[2r+1] += |w_r|^2
 template <int P, int PO, int NT, int NR, bool RED>
 __device__ __forceinline__ void block_site(const MArgs<P, PO> &A, int cb, double (&red)[2 * NR]) {
   typedef typename Prec<P>::R R;
@@ -69,7 +71,7 @@ __device__ __forceinline__ void block_site(const MArgs<P, PO> &A, int cb, double
     if (lng && rec) long_row3<C, R>(m, (lsb >> (sl - 8)) & 1);
 #pragma unroll
     for (int r = 0; r < NR; r++) {
-      if (r < nr) {
+      if (r < nr && ((A.act >> r) & 1u)) {
         C v[3];
         VecLd<P>::ld(A.in + (int64_t)r * 3 * vp, A.in_norm ? A.in_norm + (int64_t)r * vp : nullptr, vp, nb, v,
                      (P == QAMD_F16 || P == QAMD_Q16) ? A.lscale[t] : (R)1);
@@ -80,10 +82,11 @@ __device__ __forceinline__ void block_site(const MArgs<P, PO> &A, int cb, double
       }
     }
   }
-  const R b = (R)A.b, a = (R)A.a;
+  const R b = (R)A.b;
 #pragma unroll
   for (int r = 0; r < NR; r++) {
-    if (r < nr) {
+    if (r < nr && ((A.act >> r) & 1u)) {
+      const R a = (R)A.ar[r];
       C o[3];
       if (A.w) {
         C w[3];

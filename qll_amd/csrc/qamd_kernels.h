@@ -29,6 +29,15 @@ int stag_dslash_launch(const qamd_links *l, Field &out, const Field &in, double 
 // go to partial-sum set rs (2*nrhs values)
 int stag_block_launch(const qamd_links *l, Field &out, const Field &in, double a, const Field *w, double b,
                       bool red, RedScratch rs, hipStream_t st);
+// the same with a coefficient ar[r] per rhs, computing only the rhs whose bit
+// is set in act (the others' outputs and sums are left untouched / zero)
+int stag_block_launch_m(const qamd_links *l, Field &out, const Field &in, const double *ar, const Field *w,
+                        double b, bool red, RedScratch rs, hipStream_t st, unsigned act);
+// stag_block.hip: block CG on (m2[r] - D_eo D_oe) x_r = b_r (one m2 per rhs,
+// 1..8 rhs, FP64 x and b), sloppy links ls, FP64 reliable updates on l64;
+// x0: start from the x passed in
+int stag_cg_block_core(const qamd_links *l64, const qamd_links *ls, qamd_field *x, const qamd_field *b,
+                       const double *m2, bool x0, double rsq, int maxits, double delta, qamd_solve_info *info);
 // solvers.hip: with 16-bit sloppy links too coarse for this mass (operator
 // error not small against m^2), *l32 receives FP32 links made from l64
 int sloppy_too_coarse(const qamd_links *l64, const qamd_links *ls, double mass, qamd_links **l32);

@@ -338,6 +338,21 @@ __global__ void __launch_bounds__(256) k_ms_xp(MSPtrs V, Field r, SolverDev *S) 
 }
 
 // ------------------------------------------------------------ host helpers
+// rhs rs of FP64 colour-vector field src into rhs rd of dst (same lattice and
+// parity; rhs-outer storage, so one contiguous copy)
+static int rhs_copy(qamd_field *dst, int rd, const qamd_field *src, int rs) {
+  const Field &d = dst->f, &s = src->f;
+  if (d.prec != QAMD_F64 || s.prec != QAMD_F64 || d.ncomp != 3 || s.ncomp != 3 || d.vp != s.vp ||
+      d.parity != s.parity || rd >= d.nrhs || rs >= s.nrhs) {
+    set_error("rhs copy: field mismatch");
+    return -1;
+  }
+  const size_t bytes = (size_t)3 * d.vp * sizeof(double2);
+  QAMD_HIP(hipMemcpyAsync((char *)d.data + rd * bytes, (const char *)s.data + rs * bytes, bytes,
+                          hipMemcpyDeviceToDevice, stream()));
+  return 0;
+}
+
 struct State {
   SolverDev *S;    // device
   SolverDev *Sh;   // pinned host mirror
@@ -905,21 +920,49 @@ int qamd_stag_cgms_mixed(const qamd_links *l64, const qamd_links *ls, qamd_field
     // starts from it)
     if (cgms_core(l32, xs, bs, mass, shifts, n, ms_rsq, maxits, &msinfo, true)) break;
     int err = 0;
-    // refinement with 16-bit links only for shifts where they are accurate
-    // enough (see sloppy_too_coarse), FP32 links otherwise
+    // refinement: the shifts, lightest first, in blocks of up to 8 that
+    // share every link load (each rhs with its own mass^2 + shift, starting
+    // from its FP32 solution); a block runs on the 16-bit links when they are
+    // accurate enough for every shift in it (see sloppy_too_coarse), else on
+    // FP32 links
     double eps = 0;
     if ((ls->prec == QAMD_F16 || ls->prec == QAMD_Q16) && qamd_stag_sloppy_error(l64, ls, mass, &eps)) break;
-    for (int i = 0; i < n && !err; i++) {
-      err |= qamd_field_copy(x[i], xs[i]);
-      qamd_solve_info ri;
-      const double m2i = mass * mass + shifts[i];
-      const qamd_links *lsi = (eps >= SLOPPY_MAX * m2i) ? l32 : ls;
-      err = err || cg_mixed_core(l64, lsi, x[i], b, m2i, rsq, maxits, 0.1, &ri);
-      if (info && !err) {
-        info[i] = ri;
-        info[i].iterations += msinfo.iterations;
-        info[i].seconds += (i == 0) ? msinfo.seconds : 0;
+    for (int i = 0; i < n && !err; i++) err |= qamd_field_copy(x[i], xs[i]);
+    int idx[MAXSHIFT];
+    for (int i = 0; i < n; i++) idx[i] = i;
+    for (int i = 1; i < n; i++)   // insertion sort by shift
+      for (int j = i; j > 0 && shifts[idx[j]] < shifts[idx[j - 1]]; j--) {
+        const int t = idx[j];
+        idx[j] = idx[j - 1];
+        idx[j - 1] = t;
       }
+    for (int c0 = 0; c0 < n && !err; c0 += 8) {
+      const int kb = n - c0 < 8 ? n - c0 : 8;
+      bool coarse = false;
+      for (int q = 0; q < kb; q++) coarse |= eps >= SLOPPY_MAX * (mass * mass + shifts[idx[c0 + q]]);
+      const qamd_links *lg = coarse ? l32 : ls;
+      qamd_field *X = qamd_field_create(lat, QAMD_F64, 3, kb, 0);
+      qamd_field *B = qamd_field_create(lat, QAMD_F64, 3, kb, 0);
+      double m2r[8];
+      qamd_solve_info ri[8];
+      err = !X || !B;
+      for (int q = 0; q < kb && !err; q++) {
+        m2r[q] = mass * mass + shifts[idx[c0 + q]];
+        err |= rhs_copy(X, q, x[idx[c0 + q]], 0) || rhs_copy(B, q, b, 0);
+      }
+      err = err || stag_cg_block_core(l64, lg, X, B, m2r, true, rsq, maxits, 0.1, ri);
+      for (int q = 0; q < kb && !err; q++) {
+        const int i = idx[c0 + q];
+        err |= rhs_copy(x[i], 0, X, q);
+        if (info) {
+          info[i] = ri[q];
+          info[i].iterations += msinfo.iterations;
+          info[i].escalations = (lg != ls);
+          info[i].seconds += (c0 == 0 && q == 0) ? msinfo.seconds : 0;
+        }
+      }
+      qamd_field_destroy(X);
+      qamd_field_destroy(B);
     }
     if (err) break;
     rc = 0;

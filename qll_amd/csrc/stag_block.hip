@@ -66,6 +66,8 @@ struct BlockRun {
     for (int r0 = 0; r0 < R.n; r0 += R.k[c]) {
       MArgs<P, PO> B = R.A;
       B.nr = R.n - r0 < R.k[c] ? R.n - r0 : R.k[c];
+      for (int q = 0; q < B.nr; q++) B.ar[q] = R.A.ar[r0 + q];
+      B.act = R.A.act >> r0;
       B.xcd = R.xcd[c];
       B.in = R.A.in + (int64_t)r0 * 3 * vp;
       B.out = R.A.out + (int64_t)r0 * 3 * vp;
@@ -82,8 +84,8 @@ struct BlockRun {
 };
 
 template <int P, int PO = P>
-static int blaunch(const qamd_links *l, Field &out, const Field &in, double a, const Field *w, double b, bool red,
-                   RedScratch rs, hipStream_t st) {
+static int blaunch(const qamd_links *l, Field &out, const Field &in, const double *ar, const Field *w, double b,
+                   bool red, RedScratch rs, hipStream_t st, unsigned act) {
   typedef typename Prec<P>::S S;
   typedef typename Prec<PO>::S SO;
   BlockRun<P, PO> R;
@@ -97,7 +99,8 @@ static int blaunch(const qamd_links *l, Field &out, const Field &in, double a, c
   A.out_norm = out.norm;
   A.w = w ? (const SO *)w->data : nullptr;
   A.w_norm = w ? w->norm : nullptr;
-  A.a = a;
+  for (int r = 0; r < in.nrhs; r++) A.ar[r] = ar[r];
+  A.act = act;
   A.b = b;
   A.par = out.parity;
   A.nr = in.nrhs;
@@ -131,9 +134,10 @@ static int blaunch(const qamd_links *l, Field &out, const Field &in, double a, c
   return 0;
 }
 
-// block Dslash; with red, 2*nrhs partial sums {|out_r|^2, |w_r|^2} go to rs
-int stag_block_launch(const qamd_links *l, Field &out, const Field &in, double a, const Field *w, double b,
-                      bool red, RedScratch rs, hipStream_t st) {
+// block Dslash out_r = ar[r] w_r + b D in_r for the rhs r whose bit is set
+// in act; with red, 2*nrhs partial sums {|out_r|^2, |w_r|^2} go to rs
+int stag_block_launch_m(const qamd_links *l, Field &out, const Field &in, const double *ar, const Field *w,
+                        double b, bool red, RedScratch rs, hipStream_t st, unsigned act) {
   bool h2s = ((l->prec == QAMD_F16 || l->prec == QAMD_Q16) && out.prec == QAMD_F32);
   if (l->kind != 0 || in.prec != l->prec || (out.prec != l->prec && !h2s) || (w && w->prec != out.prec)) {
     set_error("block Dslash: precision/kind mismatch");
@@ -146,22 +150,30 @@ int stag_block_launch(const qamd_links *l, Field &out, const Field &in, double a
   }
   if (red && red_reserve((int)((l->lat->g.vh + 255) / 256))) return -1;
   switch (l->prec) {
-    case QAMD_F64: return blaunch<QAMD_F64>(l, out, in, a, w, b, red, rs, st);
-    case QAMD_F32: return blaunch<QAMD_F32>(l, out, in, a, w, b, red, rs, st);
+    case QAMD_F64: return blaunch<QAMD_F64>(l, out, in, ar, w, b, red, rs, st, act);
+    case QAMD_F32: return blaunch<QAMD_F32>(l, out, in, ar, w, b, red, rs, st, act);
     case QAMD_F16:
-      return h2s ? blaunch<QAMD_F16, QAMD_F32>(l, out, in, a, w, b, red, rs, st)
-                 : blaunch<QAMD_F16>(l, out, in, a, w, b, red, rs, st);
+      return h2s ? blaunch<QAMD_F16, QAMD_F32>(l, out, in, ar, w, b, red, rs, st, act)
+                 : blaunch<QAMD_F16>(l, out, in, ar, w, b, red, rs, st, act);
     default:
-      return h2s ? blaunch<QAMD_Q16, QAMD_F32>(l, out, in, a, w, b, red, rs, st)
-                 : blaunch<QAMD_Q16>(l, out, in, a, w, b, red, rs, st);
+      return h2s ? blaunch<QAMD_Q16, QAMD_F32>(l, out, in, ar, w, b, red, rs, st, act)
+                 : blaunch<QAMD_Q16>(l, out, in, ar, w, b, red, rs, st, act);
   }
+}
+
+// every rhs, one coefficient a
+int stag_block_launch(const qamd_links *l, Field &out, const Field &in, double a, const Field *w, double b,
+                      bool red, RedScratch rs, hipStream_t st) {
+  double ar[MAXB];
+  for (int r = 0; r < MAXB; r++) ar[r] = a;
+  return stag_block_launch_m(l, out, in, ar, w, b, red, rs, st, ~0u);
 }
 
 // ------------------------------------------------------------ block CG
 struct BatchDev {
   double rsq[MAXB], rsq_old[MAXB], t2[MAXB], p2[MAXB], beta[MAXB], maxrr[MAXB];
   int active[MAXB], reset[MAXB];
-  double m2;
+  double m2[MAXB];   // mass^2 (plus shift) of rhs r's operator
   int nrhs;
 };
 
@@ -208,12 +220,12 @@ template <int NR> struct FinBCG {
   }
 };
 
-template <int NR> struct FinBTrue {
+template <int NR> struct FinBTrue {   // active rhs only: the others keep their last true |r|^2
   BatchDev *B;
   __device__ void operator()(const double *t) const {
 #pragma unroll
     for (int r = 0; r < NR; r++)
-      if (r < B->nrhs) {
+      if (r < B->nrhs && B->active[r]) {
         B->rsq[r] = t[r];
         B->maxrr[r] = sqrt(t[r]);
       }
@@ -271,7 +283,7 @@ __global__ void __launch_bounds__(256) k_bcg_xr(Field x, Field p, Field r, Field
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   double red = 0.0;
   if (B->active[rr] && i < p.g->vh) {
-    double pAp = B->m2 * B->p2[rr] + B->t2[rr];
+    double pAp = B->m2[rr] * B->p2[rr] + B->t2[rr];
     float al = (float)(B->rsq[rr] / pAp);
     float2 pv[3], rv[3], av[3], xv[3];
     VB<QAMD_F32>::ld(p, rr, i, pv);
@@ -342,12 +354,12 @@ __global__ void k_bru_accum(Field x, Field xs, BatchDev *B) {
   VB<QAMD_F32>::st(xs, rr, i, s);
 }
 
-// r64 = b - Ax; r32 = r64; |r64|^2 per rhs
-__global__ void k_bru_resid(Field r64, Field b, Field Ax, Field r32, RedScratch rs) {
+// r64 = b - Ax; r32 = r64; |r64|^2 per active rhs
+__global__ void k_bru_resid(Field r64, Field b, Field Ax, Field r32, BatchDev *B, RedScratch rs) {
   const int rr = blockIdx.y;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   double red = 0.0;
-  if (i < r64.g->vh) {
+  if (B->active[rr] && i < r64.g->vh) {
     double2 bv[3], av[3];
     float2 sv[3];
     VB<QAMD_F64>::ld(b, rr, i, bv);
@@ -393,17 +405,35 @@ int qamd_stag_cg_mixed_block(const qamd_links *l64, const qamd_links *ls_in, qam
     set_error("cg_mixed_block: x, b must be even FP64 colour-vector blocks of 1..8 rhs");
     return -1;
   }
-  if (delta <= 0 || delta >= 1) delta = 0.1;
-  double t0 = now_s();
   // 16-bit links too coarse for this mass: FP32 links for the whole block
   qamd_links *l32 = nullptr;
   if (sloppy_too_coarse(l64, ls, mass, &l32)) return -1;
-  if (l32) ls = l32;
+  double m2r[MAXB];
+  for (int i = 0; i < n; i++) m2r[i] = mass * mass;
+  int rc = stag_cg_block_core(l64, l32 ? l32 : ls, x, b, m2r, false, rsq, maxits, delta, info);
+  if (info && l32)
+    for (int i = 0; i < n; i++) info[i].escalations = 1;
+  qamd_links_destroy(l32);
+  return rc;
+}
+
+}  // extern "C"
+
+namespace qamd {
+
+// Block CG on (m2[r] - D_eo D_oe) x_r = b_r, one m2 per rhs, FP32 Krylov
+// vectors on sloppy links ls, FP64 reliable updates on l64; x0: start from
+// the x passed in (else from zero).  Converged rhs leave the block Dslash
+// launches (active mask) as well as the vector kernels.
+int stag_cg_block_core(const qamd_links *l64, const qamd_links *ls, qamd_field *x, const qamd_field *b,
+                       const double *m2, bool x0, double rsq, int maxits, double delta, qamd_solve_info *info) {
+  const int n = b->f.nrhs;
+  if (delta <= 0 || delta >= 1) delta = 0.1;
+  double t0 = now_s();
   qamd_lattice *lat = l64->lat;
   const int ps = ls->prec;
   const bool half = (ps == QAMD_F16 || ps == QAMD_Q16);
   const bool verbose = getenv("QAMD_VERBOSE") != nullptr;
-  const double m2 = mass * mass;
   const int nb = (int)((lat->g.vh + 255) / 256);
   const dim3 gb(nb, n);
   const FinOps fops = fin_ops(n);
@@ -422,6 +452,13 @@ int qamd_stag_cg_mixed_block(const qamd_links *l64, const qamd_links *ls_in, qam
   if (hipHostMalloc((void **)&st.Bh, sizeof(BatchDev), hipHostMallocDefault) != hipSuccess) st.Bh = nullptr;
   int rc = -1, nupd = 0, its = 0, nrestart = 0;
   double bb[MAXB], r2[MAXB];
+  int its_done[MAXB] = {0};
+  auto act_mask = [&]() -> unsigned {
+    unsigned m = 0;
+    for (int i = 0; i < n; i++)
+      if (st.Bh->active[i]) m |= 1u << i;
+    return m;
+  };
   auto push = [&]() -> int {
     QAMD_HIP(hipMemcpyAsync(st.B, st.Bh, sizeof(BatchDev), hipMemcpyHostToDevice, stream()));
     return 0;
@@ -432,16 +469,19 @@ int qamd_stag_cg_mixed_block(const qamd_links *l64, const qamd_links *ls_in, qam
     return 0;
   };
   // A v on blocks: t = D v (set 1), out = m2 w - D t (set 2), finisher
+  // (the host mirror's active flags are those on the device when this runs)
   auto normal = [&](const qamd_links *L, qamd_field *out, qamd_field *vin, qamd_field *vw, qamd_field *tt) -> int {
-    if (stag_block_launch(L, tt->f, vin->f, 0.0, nullptr, 1.0, true, red_set(1), stream())) return -1;
-    if (stag_block_launch(L, out->f, tt->f, m2, &vw->f, -1.0, true, red_set(2), stream())) return -1;
+    const unsigned act = act_mask();
+    double zero[MAXB] = {0.0};
+    if (stag_block_launch_m(L, tt->f, vin->f, zero, nullptr, 1.0, true, red_set(1), stream(), act)) return -1;
+    if (stag_block_launch_m(L, out->f, tt->f, m2, &vw->f, -1.0, true, red_set(2), stream(), act)) return -1;
     fops.normal(nb, st.B, stream());
     QAMD_HIP(hipGetLastError());
     return 0;
   };
   auto reliable = [&]() -> int {
     if (normal(l64, Ax, x, x, t64)) return -1;
-    hipLaunchKernelGGL(k_bru_resid, gb, dim3(256), 0, stream(), r64->f, b->f, Ax->f, r->f, red_set(0));
+    hipLaunchKernelGGL(k_bru_resid, gb, dim3(256), 0, stream(), r64->f, b->f, Ax->f, r->f, st.B, red_set(0));
     fops.truer(nb, st.B, stream());
     QAMD_HIP(hipGetLastError());
     return pull();
@@ -450,9 +490,9 @@ int qamd_stag_cg_mixed_block(const qamd_links *l64, const qamd_links *ls_in, qam
     if (!r64 || !Ax || !t64 || !r || !p || !Ap || !xs || !t || (half && !ph) || !st.B || !st.Bh) break;
     if (red_reserve(nb)) break;
     if (qamd_field_norm2(b, bb)) break;
-    if (qamd_field_zero(x)) break;
+    if (!x0 && qamd_field_zero(x)) break;
     memset(st.Bh, 0, sizeof(BatchDev));
-    st.Bh->m2 = m2;
+    for (int i = 0; i < n; i++) st.Bh->m2[i] = m2[i];
     st.Bh->nrhs = n;
     for (int i = 0; i < n; i++) st.Bh->active[i] = 1;
     if (push() || reliable()) break;
@@ -513,6 +553,7 @@ int qamd_stag_cg_mixed_block(const qamd_links *l64, const qamd_links *ls_in, qam
         }
         bool was = st.Bh->active[i];
         if (q < rsq * bb[i]) st.Bh->active[i] = 0;
+        if (was && !st.Bh->active[i]) its_done[i] = its;
         st.Bh->reset[i] = st.Bh->active[i] && (q > 2.0 * r2s[i] || (trig[i] && q > 0.25 * r2[i]));
         nreset += st.Bh->reset[i];
         if (was) r2[i] = q;
@@ -535,13 +576,12 @@ int qamd_stag_cg_mixed_block(const qamd_links *l64, const qamd_links *ls_in, qam
     if (info)
       for (int i = 0; i < n; i++) {
         memset(&info[i], 0, sizeof info[i]);
-        info[i].iterations = its;
+        info[i].iterations = st.Bh->active[i] ? its : its_done[i];
         info[i].reliable_updates = nupd;
         info[i].true_rsq = bb[i] > 0 ? st.Bh->rsq[i] / bb[i] : 0;
         info[i].iter_rsq = info[i].true_rsq;
         info[i].converged = bb[i] == 0 || st.Bh->rsq[i] < rsq * bb[i];
         info[i].restarts = nrestart;
-        info[i].escalations = l32 != nullptr;
         info[i].seconds = now_s() - t0;
       }
     rc = 0;
@@ -557,8 +597,7 @@ int qamd_stag_cg_mixed_block(const qamd_links *l64, const qamd_links *ls_in, qam
   qamd_field_destroy(ph);
   dev_free(st.B);
   if (st.Bh) (void)hipHostFree(st.Bh);
-  qamd_links_destroy(l32);
   return rc;
 }
 
-}  // extern "C"
+}  // namespace qamd

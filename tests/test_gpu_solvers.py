@@ -353,6 +353,28 @@ def test_config1_full_32x32x32x32_mixed_vs_fp64(gpu):
     assert (d.norm2() / x64.norm2()) ** 0.5 < 1e-6
 
 
+def test_config2_cgms_q16_refinement_groups(gpu):
+    """Config 2 at 8^4 with Q16 sloppy links: the shifts are refined in
+    blocks of up to 8 rhs, lightest first, one mass per rhs; a block holding
+    a shift too light for Q16 runs on FP32 links.  Every shift reaches FP64
+    true |r|/|b| = 1e-10 with the oracle's operator."""
+    dims = (8, 8, 8, 8)
+    links, b = _solver_case(dims, 21)
+    sig, m0, shifts = _c2_shifts()
+    lat = gpu.Lattice(dims)
+    L64 = gpu.Links(lat, gpu.STAG, 0, 16).set_lex(links)
+    bf = gpu.Field(lat, 0, 3, 1, 0).set_lex(b)
+    xs = [gpu.Field(lat, 0, 3, 1, 0) for _ in shifts]
+    infos = gpu.stag_cgms_mixed(L64, L64.converted(gpu.Q16), xs, bf, m0, shifts, 1e-20, 50000)
+    esc = [i["escalations"] for i in infos]
+    assert esc[0] == 1 and esc[-1] == 0, esc                   # lightest on FP32, heaviest on Q16
+    assert all(esc[i] >= esc[i + 1] for i in range(len(esc) - 1)), esc
+    for i, s in enumerate(shifts):
+        assert infos[i]["converged"], (i, infos[i])
+        x = xs[i].get_lex()
+        assert _true_rel_resid(dims, links, np.sqrt(m0 ** 2 + s), b, x) <= 1.01e-20, i
+
+
 def test_config2_full_32x32x32x64_cgms(gpu):
     """Config 2 at full size: 12 shifts, FP32 inner multishift + FP64
     refinement; every shift at FP64 true |r|/|b| <= 1e-10."""
