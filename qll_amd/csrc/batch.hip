@@ -314,8 +314,10 @@ static MArgs<P, PO> margs(const qamd_links *l, int par, const void *in, const fl
   A.out_norm = out_n;
   A.w = (const SO *)w;
   A.w_norm = w_n;
-  for (int r = 0; r < 16; r++) A.ar[r] = a;
-  A.act = ~0u;
+  for (int r = 0; r < 16; r++) {
+    A.ar[r] = a;
+    A.rmap[r] = (unsigned char)r;
+  }
   A.b = b;
   A.par = par;
   A.nr = nr;

@@ -17,8 +17,8 @@ template <int P, int PO> struct MArgs {
   float *out_norm;
   const SO *w;
   const float *w_norm;
-  double ar[16], b;   // out_r = ar[r]*w_r + b * D in_r
-  unsigned act;       // bit r set: rhs r is computed (others are skipped)
+  double ar[16], b;        // slot q: out = ar[q]*w + b * D in, for rhs rmap[q]
+  unsigned char rmap[16];  // rhs index of slot q < nr (the rhs a launch computes)
   int par, nr;
   float lscale[16];
   RedScratch rs;   // partial sums of rhs r in slots 2r, 2r+1 (offset by the caller)
@@ -26,9 +26,9 @@ template <int P, int PO> struct MArgs {
   const uint8_t *lsign;   // set: long terms' third rows rebuilt (determinant sign bits per site)
 };
 
-// one output site cb: out_r = a_r*w_r + b * sum_t L_t in_r(x+d_t) for r < A.nr
-// with bit r of A.act set (fields rhs-outer); with RED, red[2r] += |out_r|^2,
-// red[2r+1] += |w_r|^2
+// one output site cb: for the slots q < A.nr, rhs r = A.rmap[q]:
+// out_r = ar[q]*w_r + b * sum_t L_t in_r(x+d_t) (fields rhs-outer); with
+// RED, red[2q] += |out_r|^2, red[2q+1] += |w_r|^2
 template <int P, int PO, int NT, int NR, bool RED>
 __device__ __forceinline__ void block_site(const MArgs<P, PO> &A, int cb, double (&red)[2 * NR]) {
   typedef typename Prec<P>::R R;
@@ -71,9 +71,10 @@ __device__ __forceinline__ void block_site(const MArgs<P, PO> &A, int cb, double
     if (lng && rec) long_row3<C, R>(m, (lsb >> (sl - 8)) & 1);
 #pragma unroll
     for (int r = 0; r < NR; r++) {
-      if (r < nr && ((A.act >> r) & 1u)) {
+      if (r < nr) {
+        const int64_t rr = A.rmap[r];
         C v[3];
-        VecLd<P>::ld(A.in + (int64_t)r * 3 * vp, A.in_norm ? A.in_norm + (int64_t)r * vp : nullptr, vp, nb, v,
+        VecLd<P>::ld(A.in + rr * 3 * vp, A.in_norm ? A.in_norm + rr * vp : nullptr, vp, nb, v,
                      (P == QAMD_F16 || P == QAMD_Q16) ? A.lscale[t] : (R)1);
 #pragma unroll
         for (int rr = 0; rr < 3; rr++)
@@ -85,12 +86,13 @@ __device__ __forceinline__ void block_site(const MArgs<P, PO> &A, int cb, double
   const R b = (R)A.b;
 #pragma unroll
   for (int r = 0; r < NR; r++) {
-    if (r < nr && ((A.act >> r) & 1u)) {
+    if (r < nr) {
+      const int64_t rr = A.rmap[r];
       const R a = (R)A.ar[r];
       C o[3];
       if (A.w) {
         C w[3];
-        VecLd<PO>::ld(A.w + (int64_t)r * 3 * vp, A.w_norm ? A.w_norm + (int64_t)r * vp : nullptr, vp, cb, w, (R)1);
+        VecLd<PO>::ld(A.w + rr * 3 * vp, A.w_norm ? A.w_norm + rr * vp : nullptr, vp, cb, w, (R)1);
 #pragma unroll
         for (int j = 0; j < 3; j++) {
           o[j].x = fma(a, w[j].x, b * acc[r][j].x);
@@ -107,7 +109,7 @@ __device__ __forceinline__ void block_site(const MArgs<P, PO> &A, int cb, double
       if (RED)
 #pragma unroll
         for (int j = 0; j < 3; j++) red[2 * r] += (double)o[j].x * o[j].x + (double)o[j].y * o[j].y;
-      VecSt<PO>::st(A.out + (int64_t)r * 3 * vp, A.out_norm ? A.out_norm + (int64_t)r * vp : nullptr, vp, cb, o);
+      VecSt<PO>::st(A.out + rr * 3 * vp, A.out_norm ? A.out_norm + rr * vp : nullptr, vp, cb, o);
     }
   }
 }

@@ -18,7 +18,8 @@ namespace qamd {
 
 constexpr int MAXB = 16;
 
-// out_r = a*w_r + b * sum_t L_t in_r(x+d_t) for r < nr; fields rhs-outer
+// out_r = a*w_r + b * sum_t L_t in_r(x+d_t) for the rhs r = rmap[q], q < nr;
+// fields rhs-outer
 template <int P, int PO, int NT, int NR, bool RED>
 __global__ void __launch_bounds__(256) k_stag_block(MArgs<P, PO> A) {
   int blk = A.xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
@@ -27,8 +28,18 @@ __global__ void __launch_bounds__(256) k_stag_block(MArgs<P, PO> A) {
 #pragma unroll
   for (int i = 0; i < 2 * NR; i++) red[i] = 0.0;
   if (cb < A.g.vh) block_site<P, PO, NT, NR, RED>(A, cb, red);
-  // indexed by site block, so the sums do not depend on the block order
-  if (RED) grid_partial_at<2 * NR>(red, A.rs, blk);
+  // slots of rhs r at 2r, 2r+1, indexed by site block, so the sums do not
+  // depend on the block order or on which launch computed the rhs
+  if (RED) {
+    block_sum(red);
+    if (threadIdx.x == 0)
+#pragma unroll
+      for (int q = 0; q < NR; q++)
+        if (q < A.nr) {
+          A.rs.part[(2 * A.rmap[q]) * A.rs.max_blocks + blk] = red[2 * q];
+          A.rs.part[(2 * A.rmap[q] + 1) * A.rs.max_blocks + blk] = red[2 * q + 1];
+        }
+  }
 }
 
 template <int P, int PO, int NT, int NR>
@@ -62,20 +73,14 @@ struct BlockRun {
   hipStream_t st;
   static int run(int c, void *ctx) {
     BlockRun &R = *(BlockRun *)ctx;
-    const int64_t vp = R.A.g.vh_pad;
     for (int r0 = 0; r0 < R.n; r0 += R.k[c]) {
-      MArgs<P, PO> B = R.A;
+      MArgs<P, PO> B = R.A;   // slots r0 .. r0+nr-1 of the block's slots
       B.nr = R.n - r0 < R.k[c] ? R.n - r0 : R.k[c];
-      for (int q = 0; q < B.nr; q++) B.ar[q] = R.A.ar[r0 + q];
-      B.act = R.A.act >> r0;
+      for (int q = 0; q < B.nr; q++) {
+        B.ar[q] = R.A.ar[r0 + q];
+        B.rmap[q] = R.A.rmap[r0 + q];
+      }
       B.xcd = R.xcd[c];
-      B.in = R.A.in + (int64_t)r0 * 3 * vp;
-      B.out = R.A.out + (int64_t)r0 * 3 * vp;
-      if (R.A.w) B.w = R.A.w + (int64_t)r0 * 3 * vp;
-      if (R.A.in_norm) B.in_norm = R.A.in_norm + (int64_t)r0 * vp;
-      if (R.A.out_norm) B.out_norm = R.A.out_norm + (int64_t)r0 * vp;
-      if (R.A.w_norm) B.w_norm = R.A.w_norm + (int64_t)r0 * vp;
-      if (R.A.rs.part) B.rs.part = R.A.rs.part + (int64_t)2 * r0 * R.A.rs.max_blocks;
       int rc = (R.nt == 16) ? blaunch_nt<P, PO, 16>(B, R.red, R.st) : blaunch_nt<P, PO, 8>(B, R.red, R.st);
       if (rc) return -1;
     }
@@ -99,16 +104,22 @@ static int blaunch(const qamd_links *l, Field &out, const Field &in, const doubl
   A.out_norm = out.norm;
   A.w = w ? (const SO *)w->data : nullptr;
   A.w_norm = w ? w->norm : nullptr;
-  for (int r = 0; r < in.nrhs; r++) A.ar[r] = ar[r];
-  A.act = act;
+  // the rhs to compute, in order, as the launch's slots
+  int ns = 0;
+  for (int r = 0; r < in.nrhs; r++)
+    if ((act >> r) & 1u) {
+      A.ar[ns] = ar[r];
+      A.rmap[ns++] = (unsigned char)r;
+    }
   A.b = b;
   A.par = out.parity;
-  A.nr = in.nrhs;
+  A.nr = ns;
   A.rs = rs;
   A.xcd = 1;
   A.lsign = (l->recon && l->nterms == 16) ? l->lsign[out.parity] : nullptr;
   for (int t = 0; t < 16; t++) A.lscale[t] = (P == QAMD_Q16) ? l->scale[t] / Q16_ONE : l->scale[t];
-  R.n = in.nrhs;
+  if (ns == 0) return 0;   // no rhs to compute
+  R.n = ns;
   R.nt = l->nterms;
   R.red = red;
   R.st = st;
@@ -121,8 +132,10 @@ static int blaunch(const qamd_links *l, Field &out, const Field &in, const doubl
     R.xcd[R.ncand++] = 0;
     if (kk == 1) break;
   }
-  // out = a*out + ... would compound over the timed repeats: no tuning then
-  const bool inplace = w && w->data == out.data;
+  // out = a*out + ... would compound over the timed repeats: no tuning then;
+  // nor for launches over a subset of the rhs (solvers whose converged rhs
+  // drop out): they take the default shape rather than tune once per count
+  const bool inplace = (w && w->data == out.data) || ns != in.nrhs;
   char key[128];
   snprintf(key, sizeof key, "stag_block p%d/%d nt%d n%d red%d %dx%dx%dx%d", P, PO, R.nt, R.n, (int)red, A.g.L[0],
            A.g.L[1], A.g.L[2], A.g.L[3]);
