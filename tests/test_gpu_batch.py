@@ -126,7 +126,7 @@ def test_sloppy_error_and_light_mass_fallback(gpu):
     LQ, L32 = L64.converted(gpu.Q16), L64.converted(gpu.F32)
     eq, e32 = gpu.stag_sloppy_error(L64, LQ, 0.05), gpu.stag_sloppy_error(L64, L32, 0.05)
     assert 1e-5 < eq < 1e-3 and 1e-8 < e32 < 1e-5, (eq, e32)
-    light, heavy = 0.008, 0.1   # 16-bit error > 0.6 m^2 / << 0.6 m^2
+    light, heavy = 0.008, 0.1   # 16-bit error > 0.9 m^2 / << 0.9 m^2
     b = O.gauss_vector(dims, 52, 100, parity=0)
     bf = gpu.Field(lat, 0, 3, 1, 0).set_lex(b)
     x = gpu.Field(lat, 0, 3, 1, 0)

@@ -354,21 +354,29 @@ def test_config1_full_32x32x32x32_mixed_vs_fp64(gpu):
 
 
 def test_config2_cgms_q16_refinement_groups(gpu):
-    """Config 2 at 8^4 with Q16 sloppy links: the shifts are refined in
-    blocks of up to 8 rhs, lightest first, one mass per rhs; a block holding
-    a shift too light for Q16 runs on FP32 links.  Every shift reaches FP64
-    true |r|/|b| = 1e-10 with the oracle's operator."""
+    """12 shifts at 8^4 with Q16 sloppy links (base m^2 = 1e-4, lighter than
+    config 2): the shifts are refined in blocks of up to 8 rhs, lightest
+    first, one mass per rhs; a block holding a shift whose mass is too light
+    for the 16-bit links (operator error >= 0.9 m^2) runs on FP32 links.
+    Every shift reaches FP64 true |r|/|b| = 1e-10 with the oracle's operator."""
     dims = (8, 8, 8, 8)
     links, b = _solver_case(dims, 21)
-    sig, m0, shifts = _c2_shifts()
+    m0 = 0.01
+    shifts = list(np.logspace(-4, 0, 12) - 1e-4)
     lat = gpu.Lattice(dims)
     L64 = gpu.Links(lat, gpu.STAG, 0, 16).set_lex(links)
+    LQ = L64.converted(gpu.Q16)
+    eps = gpu.stag_sloppy_error(L64, LQ, m0)
     bf = gpu.Field(lat, 0, 3, 1, 0).set_lex(b)
     xs = [gpu.Field(lat, 0, 3, 1, 0) for _ in shifts]
-    infos = gpu.stag_cgms_mixed(L64, L64.converted(gpu.Q16), xs, bf, m0, shifts, 1e-20, 50000)
-    esc = [i["escalations"] for i in infos]
-    assert esc[0] == 1 and esc[-1] == 0, esc                   # lightest on FP32, heaviest on Q16
-    assert all(esc[i] >= esc[i + 1] for i in range(len(esc) - 1)), esc
+    infos = gpu.stag_cgms_mixed(L64, LQ, xs, bf, m0, shifts, 1e-20, 50000)
+    blocks = [list(range(0, 8)), list(range(8, 12))]   # shifts are ascending
+    want = {}
+    for blk in blocks:
+        coarse = any(eps >= 0.9 * (m0 ** 2 + shifts[i]) for i in blk)
+        want.update({i: int(coarse) for i in blk})
+    assert [i["escalations"] for i in infos] == [want[i] for i in range(12)], (eps, infos)
+    assert want[0] == 1 and want[11] == 0, eps                # both link precisions are used
     for i, s in enumerate(shifts):
         assert infos[i]["converged"], (i, infos[i])
         x = xs[i].get_lex()
