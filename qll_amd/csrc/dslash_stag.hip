@@ -41,6 +41,11 @@ template <int P, int PO = P> struct StagArgs {
   double *red_out;    // receives {|out|^2, |w|^2}
   const int *skip;    // optional device flag: whole grid returns when set
   const uint8_t *lsign;   // long-term determinant signs when their third rows are rebuilt
+  // fused CG step (CGX, FP32 output): with w = p and out = a p + b D in = Ap
+  // (not stored), x += alpha p and r -= alpha Ap, |r|^2 -> red[0]; alpha is
+  // read from the device
+  float2 *cgx, *cgr;
+  const double *cg_alpha;
 };
 
 struct FinStore2 {
@@ -53,8 +58,9 @@ struct FinStore2 {
 
 // REC (16 terms): the long terms' third rows are not loaded but rebuilt
 // from the first two (FP64: 2016 instead of 2400 bytes per site)
-template <int P, int PO, int NT, bool RED, bool REC = false>
+template <int P, int PO, int NT, bool RED, bool REC = false, bool CGX = false>
 __global__ void __launch_bounds__(256) k_stag_dslash(StagArgs<P, PO> A) {
+  static_assert(!CGX || (PO == QAMD_F32 && RED), "fused CG step: FP32 vectors, with the |r|^2 sums");
   typedef typename Prec<P>::R R;
   typedef typename Prec<P>::C C;
   typedef typename Prec<P>::S S;
@@ -108,7 +114,26 @@ __global__ void __launch_bounds__(256) k_stag_dslash(StagArgs<P, PO> A) {
     }
     C o[3];
     const R b = (R)A.b;
-    if (A.w) {
+    if constexpr (CGX) {
+      C w[3];   // p
+      VecLd<PO>::ld(A.w, A.w_norm, vp, cb, w, (R)1);
+      const R a = (R)A.a, al = (R)*A.cg_alpha;
+      float2 xv[3], rv[3];
+      VecLd<QAMD_F32>::ld(A.cgx, nullptr, vp, cb, xv, 1.0f);
+      VecLd<QAMD_F32>::ld(A.cgr, nullptr, vp, cb, rv, 1.0f);
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        o[j].x = fma(a, w[j].x, b * acc[j].x);   // Ap
+        o[j].y = fma(a, w[j].y, b * acc[j].y);
+        xv[j].x = fma(al, w[j].x, xv[j].x);
+        xv[j].y = fma(al, w[j].y, xv[j].y);
+        rv[j].x = fma(-al, o[j].x, rv[j].x);
+        rv[j].y = fma(-al, o[j].y, rv[j].y);
+        red[0] += (double)rv[j].x * rv[j].x + (double)rv[j].y * rv[j].y;
+      }
+      VecSt<QAMD_F32>::st(A.cgx, nullptr, vp, cb, xv);
+      VecSt<QAMD_F32>::st(A.cgr, nullptr, vp, cb, rv);
+    } else if (A.w) {
       C w[3];
       VecLd<PO>::ld(A.w, A.w_norm, vp, cb, w, (R)1);
       const R a = (R)A.a;
@@ -125,11 +150,13 @@ __global__ void __launch_bounds__(256) k_stag_dslash(StagArgs<P, PO> A) {
         o[j].y = b * acc[j].y;
       }
     }
-    if (RED) {
+    if constexpr (!CGX) {
+      if (RED) {
 #pragma unroll
-      for (int j = 0; j < 3; j++) red[0] += (double)o[j].x * o[j].x + (double)o[j].y * o[j].y;
+        for (int j = 0; j < 3; j++) red[0] += (double)o[j].x * o[j].x + (double)o[j].y * o[j].y;
+      }
+      VecSt<PO>::st(A.out, A.out_norm, vp, cb, o);
     }
-    VecSt<PO>::st(A.out, A.out_norm, vp, cb, o);
   }
   if (RED) grid_partial<2>(red, A.rs);
 }
@@ -226,6 +253,56 @@ int stag_dslash_launch(const qamd_links *l, Field &out, const Field &in, double 
       if (half_to_single)
         return launch_p<QAMD_Q16, QAMD_F32>(l, out, in, a, w, b, red_out, st, skip, pset, finish);
       return launch_p<QAMD_Q16>(l, out, in, a, w, b, red_out, st, skip, pset, finish);
+  }
+}
+
+// Fused CG step of the mixed-precision solvers: Ap = a p - D in (FP32,
+// not stored), x += alpha p, r -= alpha Ap, |r|^2 partial sums into set pset
+// (slot 0).  in: the sloppy Dslash input (opposite parity); p, x, r: FP32
+// vectors of the output parity; alpha: device scalar.
+template <int P>
+static int cgx_p(const qamd_links *l, const Field &in, const Field &p, Field &x, Field &r, double a,
+                 const double *alpha, hipStream_t st, const int *skip, int pset) {
+  typedef typename Prec<P>::S S;
+  StagArgs<P, QAMD_F32> A;
+  memset(&A, 0, sizeof A);
+  A.skip = skip;
+  A.g = l->lat->g;
+  A.links = (const S *)l->data[r.parity];
+  A.in = (const S *)in.data;
+  A.in_norm = in.norm;
+  A.w = (const float2 *)p.data;
+  A.a = a;
+  A.b = -1.0;
+  A.par = r.parity;
+  A.lsign = (l->recon && l->nterms == 16) ? l->lsign[r.parity] : nullptr;
+  for (int t = 0; t < 16; t++) A.lscale[t] = (P == QAMD_Q16) ? l->scale[t] / Q16_ONE : l->scale[t];
+  A.cgx = (float2 *)x.data;
+  A.cgr = (float2 *)r.data;
+  A.cg_alpha = alpha;
+  const int nb = (int)((A.g.vh + 255) / 256);
+  if (red_reserve(nb)) return -1;
+  A.rs = red_set(pset);
+  if (A.lsign)
+    hipLaunchKernelGGL((k_stag_dslash<P, QAMD_F32, 16, true, true, true>), dim3(nb), dim3(256), 0, st, A);
+  else
+    hipLaunchKernelGGL((k_stag_dslash<P, QAMD_F32, 16, true, false, true>), dim3(nb), dim3(256), 0, st, A);
+  QAMD_HIP(hipGetLastError());
+  return 0;
+}
+
+int stag_cg_step_launch(const qamd_links *l, const Field &in, const Field &p, Field &x, Field &r, double a,
+                        const double *alpha, hipStream_t st, const int *skip, int pset) {
+  if (l->kind != 0 || l->nterms != 16 || in.prec != l->prec || p.prec != QAMD_F32 || x.prec != QAMD_F32 ||
+      r.prec != QAMD_F32 || in.parity == r.parity || p.parity != r.parity || x.parity != r.parity ||
+      in.nrhs != 1 || p.nrhs != 1 || x.nrhs != 1 || r.nrhs != 1 || l->prec == QAMD_F64) {
+    set_error("stag_cg_step: needs 16-term sloppy links, FP32 p, x, r of the output parity");
+    return -1;
+  }
+  switch (l->prec) {
+    case QAMD_F32: return cgx_p<QAMD_F32>(l, in, p, x, r, a, alpha, st, skip, pset);
+    case QAMD_F16: return cgx_p<QAMD_F16>(l, in, p, x, r, a, alpha, st, skip, pset);
+    default: return cgx_p<QAMD_Q16>(l, in, p, x, r, a, alpha, st, skip, pset);
   }
 }
 

@@ -92,6 +92,49 @@ __global__ void __launch_bounds__(256) k_cg_p(Field p, Field r, Field shadow, So
   }
 }
 
+// Mixed-precision CG, fused form: p = r + beta p (reset: p = r), the 16-bit
+// Dslash copy (SH), and |p|^2 partial sums, which the next p.Ap needs
+// before the fused x/r step
+template <int SH>
+__global__ void __launch_bounds__(256) k_cg_pn(Field p, Field r, Field shadow, SolverDev *S, RedScratch rs,
+                                               int reset) {
+  if (S->done) return;   // the finisher that would read the sums is skipped too
+  double red[1] = {0.0};
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < p.g->vh) {
+    float2 pv[3], rv[3];
+    VT<QAMD_F32>::ld(r, i, rv);
+    if (reset) {
+#pragma unroll
+      for (int j = 0; j < 3; j++) pv[j] = rv[j];
+    } else {
+      VT<QAMD_F32>::ld(p, i, pv);
+      const float be = (float)S->beta;
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        pv[j].x = fmaf(be, pv[j].x, rv[j].x);
+        pv[j].y = fmaf(be, pv[j].y, rv[j].y);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 3; j++) red[0] += (double)pv[j].x * pv[j].x + (double)pv[j].y * pv[j].y;
+    VT<QAMD_F32>::st(p, i, pv);
+    if constexpr (SH != 0) VT<SH>::st(shadow, i, pv);
+  }
+  grid_partial<1>(red, rs);
+}
+
+// p.Ap = m^2 |p|^2 + |D_oe p|^2 (the dslashp2 identity, lib/stagDslash.c:546-575)
+// from |t|^2 (set a) and |p|^2 (set b); alpha = rsq / p.Ap
+struct FinAlpha {
+  SolverDev *S;
+  __device__ void operator()(const double *t) const {
+    S->nrm[0] = t[0];
+    S->nrm[3] = t[1];
+    S->alpha = S->rsq / (S->m2 * t[1] + t[0]);
+  }
+};
+
 struct FinCG {
   SolverDev *S;
   __device__ void operator()(const double *t) const {
@@ -583,7 +626,6 @@ static int cg_mixed_core(const qamd_links *l64, const qamd_links *ls, qamd_field
   qamd_field *t64 = qamd_field_create(lat, QAMD_F64, 3, 1, 1);
   qamd_field *r = qamd_field_create(lat, QAMD_F32, 3, 1, 0);
   qamd_field *p = qamd_field_create(lat, QAMD_F32, 3, 1, 0);
-  qamd_field *Ap = qamd_field_create(lat, QAMD_F32, 3, 1, 0);
   qamd_field *xs = qamd_field_create(lat, QAMD_F32, 3, 1, 0);
   qamd_field *t = qamd_field_create(lat, ps, 3, 1, 1);
   qamd_field *p16 = half ? qamd_field_create(lat, ps, 3, 1, 0) : nullptr;
@@ -592,7 +634,7 @@ static int cg_mixed_core(const qamd_links *l64, const qamd_links *ls, qamd_field
   const bool verbose = getenv("QAMD_VERBOSE") != nullptr;
   double bsq = 0, r2 = 0;
   do {
-    if (!r64 || !Ax || !t64 || !r || !p || !Ap || !xs || !t || (half && !p16) || state_init(st, lat)) break;
+    if (!r64 || !Ax || !t64 || !r || !p || !xs || !t || (half && !p16) || state_init(st, lat)) break;
     if (qamd_field_norm2(b, &bsq)) break;
     if (bsq == 0) {
       if (qamd_field_zero(x)) break;
@@ -621,13 +663,34 @@ static int cg_mixed_core(const qamd_links *l64, const qamd_links *ls, qamd_field
       rc = 0;
       break;
     }
-    if (qamd_field_copy(p, r) || qamd_field_zero(xs) || (half && qamd_field_copy(p16, p))) break;
+    // p = r with its 16-bit copy and |p|^2 (partial-sum set 3)
+    auto p_update = [&](int reset) -> int {
+      if (half && ps == QAMD_F16)
+        hipLaunchKernelGGL(k_cg_pn<QAMD_F16>, grid1(lat), dim3(256), 0, stream(), p->f, r->f, p16->f, st.S,
+                           red_set(3), reset);
+      else if (half)
+        hipLaunchKernelGGL(k_cg_pn<QAMD_Q16>, grid1(lat), dim3(256), 0, stream(), p->f, r->f, p16->f, st.S,
+                           red_set(3), reset);
+      else
+        hipLaunchKernelGGL(k_cg_pn<0>, grid1(lat), dim3(256), 0, stream(), p->f, r->f, p->f, st.S, red_set(3),
+                           reset);
+      QAMD_HIP(hipGetLastError());
+      return 0;
+    };
+    if (qamd_field_zero(xs) || p_update(1)) break;
+    const int nb1 = (int)grid1(lat).x;
     int err = 0;
     while (its < maxits) {
+      // per iteration: t = D_oe p; alpha = rsq / (m^2 |p|^2 + |t|^2); the
+      // fused step x += alpha p, r -= alpha (m^2 p - D_eo t), |r|^2; beta;
+      // p = r + beta p with |p|^2
       for (int k = 0; k < BATCH && !err; k++) {
-        err |= apply_normal(lsc, st.S, Ap, pin, p, t, m2, nullptr);
-        err |= cg_xr(lat, xs, p, r, Ap, st.S);
-        err |= cg_p(lat, p, r, half ? p16 : nullptr, st.S);
+        err |= stag_dslash_launch(lsc, t->f, pin->f, 0.0, nullptr, 1.0, nullptr, stream(), nullptr, 1, false);
+        hipLaunchKernelGGL((k_finish2<1, 1, FinAlpha>), dim3(1), dim3(FT), 0, stream(), red_set(1), red_set(3), nb1,
+                           FinAlpha{st.S}, &st.S->done);
+        err |= stag_cg_step_launch(lsc, t->f, p->f, xs->f, r->f, m2, &st.S->alpha, stream(), &st.S->done, 0);
+        launch_finish<1>(red_set(0), nb1, FinCG{st.S}, stream(), &st.S->done);
+        err |= p_update(0);
       }
       if (err || pull_state(st)) {
         err = 1;
@@ -686,7 +749,7 @@ static int cg_mixed_core(const qamd_links *l64, const qamd_links *ls, qamd_field
           if (verbose) fprintf(stderr, "cg_mixed: FP16 links stalled; continuing with FP32 links\n");
         }
         if (stalled || r2 > 2.0 * r2s) {
-          if (qamd_field_copy(p, r) || (half && qamd_field_copy(p16, p))) {
+          if (p_update(1)) {
             err = 1;
             break;
           }
@@ -714,7 +777,6 @@ static int cg_mixed_core(const qamd_links *l64, const qamd_links *ls, qamd_field
   qamd_field_destroy(t64);
   qamd_field_destroy(r);
   qamd_field_destroy(p);
-  qamd_field_destroy(Ap);
   qamd_field_destroy(xs);
   qamd_field_destroy(t);
   qamd_field_destroy(p16);
