@@ -19,6 +19,10 @@ template <int P, int PO> struct MArgs {
   const float *w_norm;
   double ar[16], b;        // slot q: out = ar[q]*w + b * D in, for rhs rmap[q]
   unsigned char rmap[16];  // rhs index of slot q < nr (the rhs a launch computes)
+  // fused block-CG step (CGX, FP32 vectors): with w = p, Ap = ar p + b D in
+  // is not stored; x_r += alpha[r] p_r, r_r -= alpha[r] Ap_r, |r_r|^2 -> red[2q]
+  float2 *cgx, *cgr;
+  const double *alpha;
   int par, nr;
   float lscale[16];
   RedScratch rs;   // partial sums of rhs r in slots 2r, 2r+1 (offset by the caller)
@@ -29,8 +33,9 @@ template <int P, int PO> struct MArgs {
 // one output site cb: for the slots q < A.nr, rhs r = A.rmap[q]:
 // out_r = ar[q]*w_r + b * sum_t L_t in_r(x+d_t) (fields rhs-outer); with
 // RED, red[2q] += |out_r|^2, red[2q+1] += |w_r|^2
-template <int P, int PO, int NT, int NR, bool RED>
+template <int P, int PO, int NT, int NR, bool RED, bool CGX = false>
 __device__ __forceinline__ void block_site(const MArgs<P, PO> &A, int cb, double (&red)[2 * NR]) {
+  static_assert(!CGX || (PO == QAMD_F32 && RED), "fused block-CG step: FP32 vectors, with the |r|^2 sums");
   typedef typename Prec<P>::R R;
   typedef typename Prec<P>::C C;
   typedef typename Prec<P>::S S;
@@ -72,14 +77,14 @@ __device__ __forceinline__ void block_site(const MArgs<P, PO> &A, int cb, double
 #pragma unroll
     for (int r = 0; r < NR; r++) {
       if (r < nr) {
-        const int64_t rr = A.rmap[r];
+        const int64_t ri = A.rmap[r];
         C v[3];
-        VecLd<P>::ld(A.in + rr * 3 * vp, A.in_norm ? A.in_norm + rr * vp : nullptr, vp, nb, v,
+        VecLd<P>::ld(A.in + ri * 3 * vp, A.in_norm ? A.in_norm + ri * vp : nullptr, vp, nb, v,
                      (P == QAMD_F16 || P == QAMD_Q16) ? A.lscale[t] : (R)1);
 #pragma unroll
-        for (int rr = 0; rr < 3; rr++)
+        for (int row = 0; row < 3; row++)
 #pragma unroll
-          for (int c = 0; c < 3; c++) cmac<C, R>(acc[r][rr], m[3 * c + rr], v[c]);
+          for (int c = 0; c < 3; c++) cmac<C, R>(acc[r][row], m[3 * c + row], v[c]);
       }
     }
   }
@@ -90,6 +95,27 @@ __device__ __forceinline__ void block_site(const MArgs<P, PO> &A, int cb, double
       const int64_t rr = A.rmap[r];
       const R a = (R)A.ar[r];
       C o[3];
+      if constexpr (CGX) {
+        C w[3];   // p_rr
+        VecLd<PO>::ld(A.w + rr * 3 * vp, nullptr, vp, cb, w, (R)1);
+        const R al = (R)A.alpha[rr];
+        float2 xv[3], rv[3];
+        VecLd<QAMD_F32>::ld(A.cgx + rr * 3 * vp, nullptr, vp, cb, xv, 1.0f);
+        VecLd<QAMD_F32>::ld(A.cgr + rr * 3 * vp, nullptr, vp, cb, rv, 1.0f);
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+          o[j].x = fma(a, w[j].x, b * acc[r][j].x);   // Ap
+          o[j].y = fma(a, w[j].y, b * acc[r][j].y);
+          xv[j].x = fma(al, w[j].x, xv[j].x);
+          xv[j].y = fma(al, w[j].y, xv[j].y);
+          rv[j].x = fma(-al, o[j].x, rv[j].x);
+          rv[j].y = fma(-al, o[j].y, rv[j].y);
+          red[2 * r] += (double)rv[j].x * rv[j].x + (double)rv[j].y * rv[j].y;
+        }
+        VecSt<QAMD_F32>::st(A.cgx + rr * 3 * vp, nullptr, vp, cb, xv);
+        VecSt<QAMD_F32>::st(A.cgr + rr * 3 * vp, nullptr, vp, cb, rv);
+        continue;
+      }
       if (A.w) {
         C w[3];
         VecLd<PO>::ld(A.w + rr * 3 * vp, A.w_norm ? A.w_norm + rr * vp : nullptr, vp, cb, w, (R)1);

@@ -20,24 +20,29 @@ constexpr int MAXB = 16;
 
 // out_r = a*w_r + b * sum_t L_t in_r(x+d_t) for the rhs r = rmap[q], q < nr;
 // fields rhs-outer
-template <int P, int PO, int NT, int NR, bool RED>
+template <int P, int PO, int NT, int NR, bool RED, bool CGX = false>
 __global__ void __launch_bounds__(256) k_stag_block(MArgs<P, PO> A) {
   int blk = A.xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
   int cb = blk * 256 + threadIdx.x;
   double red[2 * NR];
 #pragma unroll
   for (int i = 0; i < 2 * NR; i++) red[i] = 0.0;
-  if (cb < A.g.vh) block_site<P, PO, NT, NR, RED>(A, cb, red);
-  // slots of rhs r at 2r, 2r+1, indexed by site block, so the sums do not
-  // depend on the block order or on which launch computed the rhs
+  if (cb < A.g.vh) block_site<P, PO, NT, NR, RED, CGX>(A, cb, red);
+  // slots of rhs r at 2r, 2r+1 (fused CG step: |r_r|^2 at slot r), indexed by
+  // site block, so the sums do not depend on the block order or on which
+  // launch computed the rhs
   if (RED) {
     block_sum(red);
     if (threadIdx.x == 0)
 #pragma unroll
       for (int q = 0; q < NR; q++)
         if (q < A.nr) {
-          A.rs.part[(2 * A.rmap[q]) * A.rs.max_blocks + blk] = red[2 * q];
-          A.rs.part[(2 * A.rmap[q] + 1) * A.rs.max_blocks + blk] = red[2 * q + 1];
+          if (CGX) {
+            A.rs.part[A.rmap[q] * A.rs.max_blocks + blk] = red[2 * q];
+          } else {
+            A.rs.part[(2 * A.rmap[q]) * A.rs.max_blocks + blk] = red[2 * q];
+            A.rs.part[(2 * A.rmap[q] + 1) * A.rs.max_blocks + blk] = red[2 * q + 1];
+          }
         }
   }
 }
@@ -174,6 +179,66 @@ int stag_block_launch_m(const qamd_links *l, Field &out, const Field &in, const 
   }
 }
 
+// Fused step of the block CG: for the rhs set in act, Ap_r = m2[r] p_r -
+// D in_r (FP32, not stored), x_r += alpha[r] p_r, r_r -= alpha[r] Ap_r, and
+// |r_r|^2 partial sums at slot r of rs.  All rhs in one launch in the XCD
+// block order (no tuning: trial launches would repeat the in-place update).
+template <int P>
+static int bcgx(const qamd_links *l, const Field &in, const Field &p, Field &x, Field &r, const double *m2,
+                const double *alpha, RedScratch rs, hipStream_t st, unsigned act) {
+  typedef typename Prec<P>::S S;
+  MArgs<P, QAMD_F32> A;
+  memset(&A, 0, sizeof A);
+  A.g = l->lat->g;
+  A.links = (const S *)l->data[r.parity];
+  A.in = (const S *)in.data;
+  A.in_norm = in.norm;
+  A.w = (const float2 *)p.data;
+  A.b = -1.0;
+  A.par = r.parity;
+  A.rs = rs;
+  A.xcd = 1;
+  A.cgx = (float2 *)x.data;
+  A.cgr = (float2 *)r.data;
+  A.alpha = alpha;
+  A.lsign = (l->recon && l->nterms == 16) ? l->lsign[r.parity] : nullptr;
+  for (int t = 0; t < 16; t++) A.lscale[t] = (P == QAMD_Q16) ? l->scale[t] / Q16_ONE : l->scale[t];
+  int ns = 0;
+  for (int q = 0; q < p.nrhs; q++)
+    if ((act >> q) & 1u) {
+      A.ar[ns] = m2[q];
+      A.rmap[ns++] = (unsigned char)q;
+    }
+  A.nr = ns;
+  if (ns == 0) return 0;
+  const int nb = (int)((A.g.vh + 255) / 256);
+#define CGXL(NR) hipLaunchKernelGGL((k_stag_block<P, QAMD_F32, 16, NR, true, true>), dim3(nb), dim3(256), 0, st, A)
+  if (ns <= 1) CGXL(1);
+  else if (ns <= 2) CGXL(2);
+  else if (ns <= 4) CGXL(4);
+  else if (ns <= 8) CGXL(8);
+  else CGXL(16);
+#undef CGXL
+  QAMD_HIP(hipGetLastError());
+  return 0;
+}
+
+static int stag_block_cg_step(const qamd_links *l, const Field &in, const Field &p, Field &x, Field &r,
+                              const double *m2, const double *alpha, RedScratch rs, hipStream_t st, unsigned act) {
+  if (l->kind != 0 || l->nterms != 16 || l->prec == QAMD_F64 || in.prec != l->prec || p.prec != QAMD_F32 ||
+      x.prec != QAMD_F32 || r.prec != QAMD_F32 || p.nrhs > MAXB || in.nrhs != p.nrhs || x.nrhs != p.nrhs ||
+      r.nrhs != p.nrhs) {
+    set_error("block CG step: needs 16-term sloppy links and FP32 p, x, r blocks");
+    return -1;
+  }
+  if (red_reserve((int)((l->lat->g.vh + 255) / 256))) return -1;
+  switch (l->prec) {
+    case QAMD_F32: return bcgx<QAMD_F32>(l, in, p, x, r, m2, alpha, rs, st, act);
+    case QAMD_F16: return bcgx<QAMD_F16>(l, in, p, x, r, m2, alpha, rs, st, act);
+    default: return bcgx<QAMD_Q16>(l, in, p, x, r, m2, alpha, rs, st, act);
+  }
+}
+
 // every rhs, one coefficient a
 int stag_block_launch(const qamd_links *l, Field &out, const Field &in, double a, const Field *w, double b,
                       bool red, RedScratch rs, hipStream_t st) {
@@ -187,6 +252,7 @@ struct BatchDev {
   double rsq[MAXB], rsq_old[MAXB], t2[MAXB], p2[MAXB], beta[MAXB], maxrr[MAXB];
   int active[MAXB], reset[MAXB];
   double m2[MAXB];   // mass^2 (plus shift) of rhs r's operator
+  double alpha[MAXB];
   int nrhs;
 };
 
@@ -245,6 +311,21 @@ template <int NR> struct FinBTrue {   // active rhs only: the others keep their 
   }
 };
 
+// alpha_r = rsq_r / (m2_r |p_r|^2 + |t_r|^2) from |t_r|^2 (slot 2r of the
+// Dslash sums) and |p_r|^2 (slot r of the p-update sums), active rhs
+template <int NR> struct FinBAlpha {
+  BatchDev *B;
+  __device__ void operator()(const double *t) const {
+#pragma unroll
+    for (int r = 0; r < NR; r++)
+      if (r < B->nrhs && B->active[r]) {
+        B->t2[r] = t[2 * r];
+        B->p2[r] = t[2 * NR + r];
+        B->alpha[r] = B->rsq[r] / (B->m2[r] * B->p2[r] + B->t2[r]);
+      }
+  }
+};
+
 // finishers sized to the block: NR = block size rounded up to 1, 2, 4, 8, 16
 template <int NR> static void fin_normal(int nb, BatchDev *B, hipStream_t st) {
   hipLaunchKernelGGL((k_finish_b2<2 * NR, FinBNormal<NR>>), dim3(1), dim3(FT), 0, st, red_set(1), red_set(2), nb,
@@ -256,12 +337,19 @@ template <int NR> static void fin_cg(int nb, BatchDev *B, hipStream_t st) {
 template <int NR> static void fin_true(int nb, BatchDev *B, hipStream_t st) {
   launch_finish<NR>(red_set(0), nb, FinBTrue<NR>{B}, st);
 }
+template <int NR> static void fin_alpha(int nb, BatchDev *B, hipStream_t st) {
+  hipLaunchKernelGGL((k_finish2<2 * NR, NR, FinBAlpha<NR>>), dim3(1), dim3(FT), 0, st, red_set(1), red_set(3), nb,
+                     FinBAlpha<NR>{B}, nullptr);
+}
 struct FinOps {
   void (*normal)(int, BatchDev *, hipStream_t);
   void (*cg)(int, BatchDev *, hipStream_t);
   void (*truer)(int, BatchDev *, hipStream_t);
+  void (*alpha)(int, BatchDev *, hipStream_t);
 };
-template <int NR> static FinOps fin_ops_nr() { return FinOps{fin_normal<NR>, fin_cg<NR>, fin_true<NR>}; }
+template <int NR> static FinOps fin_ops_nr() {
+  return FinOps{fin_normal<NR>, fin_cg<NR>, fin_true<NR>, fin_alpha<NR>};
+}
 static FinOps fin_ops(int n) {
   if (n <= 1) return fin_ops_nr<1>();
   if (n <= 2) return fin_ops_nr<2>();
@@ -290,62 +378,38 @@ template <int P> struct VB {
   }
 };
 
-// x_r += alpha_r p_r; r_r -= alpha_r Ap_r; |r_r|^2 (FP32 vectors)
-__global__ void __launch_bounds__(256) k_bcg_xr(Field x, Field p, Field r, Field Ap, BatchDev *B, RedScratch rs) {
+// p_r = r_r + beta_r p_r (mode 0, active rhs), p_r = r_r (mode 1: rhs
+// flagged in reset; mode 2: every active rhs), the 16-bit Dslash copy, and
+// |p_r|^2 partial sums (slot r), which the next alpha_r needs before the
+// fused step
+template <int SH>
+__global__ void __launch_bounds__(256) k_bcg_pn(Field p, Field r, Field sh, BatchDev *B, RedScratch rs, int mode) {
   const int rr = blockIdx.y;
+  const bool on = mode == 1 ? B->reset[rr] != 0 : B->active[rr] != 0;
+  if (!on) return;   // uniform over the block: its partial-sum slot is not read
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   double red = 0.0;
-  if (B->active[rr] && i < p.g->vh) {
-    double pAp = B->m2[rr] * B->p2[rr] + B->t2[rr];
-    float al = (float)(B->rsq[rr] / pAp);
-    float2 pv[3], rv[3], av[3], xv[3];
-    VB<QAMD_F32>::ld(p, rr, i, pv);
+  if (i < p.g->vh) {
+    float2 pv[3], rv[3];
     VB<QAMD_F32>::ld(r, rr, i, rv);
-    VB<QAMD_F32>::ld(Ap, rr, i, av);
-    VB<QAMD_F32>::ld(x, rr, i, xv);
+    if (mode == 0) {
+      const float be = (float)B->beta[rr];
+      VB<QAMD_F32>::ld(p, rr, i, pv);
 #pragma unroll
-    for (int j = 0; j < 3; j++) {
-      xv[j].x = fmaf(al, pv[j].x, xv[j].x);
-      xv[j].y = fmaf(al, pv[j].y, xv[j].y);
-      rv[j].x = fmaf(-al, av[j].x, rv[j].x);
-      rv[j].y = fmaf(-al, av[j].y, rv[j].y);
-      red += (double)rv[j].x * rv[j].x + (double)rv[j].y * rv[j].y;
+      for (int j = 0; j < 3; j++) {
+        pv[j].x = fmaf(be, pv[j].x, rv[j].x);
+        pv[j].y = fmaf(be, pv[j].y, rv[j].y);
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < 3; j++) pv[j] = rv[j];
     }
-    VB<QAMD_F32>::st(x, rr, i, xv);
-    VB<QAMD_F32>::st(r, rr, i, rv);
+#pragma unroll
+    for (int j = 0; j < 3; j++) red += (double)pv[j].x * pv[j].x + (double)pv[j].y * pv[j].y;
+    VB<QAMD_F32>::st(p, rr, i, pv);
+    if constexpr (SH != 0) VB<SH>::st(sh, rr, i, pv);
   }
   rhs_partial(red, rs);
-}
-
-// p_r = r_r + beta_r p_r, and the 16-bit Dslash copy
-template <int SH>
-__global__ void __launch_bounds__(256) k_bcg_p(Field p, Field r, Field sh, BatchDev *B) {
-  const int rr = blockIdx.y;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (!B->active[rr] || i >= p.g->vh) return;
-  float be = (float)B->beta[rr];
-  float2 pv[3], rv[3];
-  VB<QAMD_F32>::ld(p, rr, i, pv);
-  VB<QAMD_F32>::ld(r, rr, i, rv);
-#pragma unroll
-  for (int j = 0; j < 3; j++) {
-    pv[j].x = fmaf(be, pv[j].x, rv[j].x);
-    pv[j].y = fmaf(be, pv[j].y, rv[j].y);
-  }
-  VB<QAMD_F32>::st(p, rr, i, pv);
-  if constexpr (SH != 0) VB<SH>::st(sh, rr, i, pv);
-}
-
-// restart: p_r = r_r (and its 16-bit copy) for the rhs flagged in reset
-template <int SH>
-__global__ void __launch_bounds__(256) k_bcg_reset(Field p, Field r, Field sh, BatchDev *B) {
-  const int rr = blockIdx.y;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (!B->reset[rr] || i >= p.g->vh) return;
-  float2 rv[3];
-  VB<QAMD_F32>::ld(r, rr, i, rv);
-  VB<QAMD_F32>::st(p, rr, i, rv);
-  if constexpr (SH != 0) VB<SH>::st(sh, rr, i, rv);
 }
 
 // reliable update: x64 += xs, xs = 0 (active rhs)
@@ -456,7 +520,6 @@ int stag_cg_block_core(const qamd_links *l64, const qamd_links *ls, qamd_field *
   qamd_field *t64 = qamd_field_create(lat, QAMD_F64, 3, n, 1);
   qamd_field *r = qamd_field_create(lat, QAMD_F32, 3, n, 0);
   qamd_field *p = qamd_field_create(lat, QAMD_F32, 3, n, 0);
-  qamd_field *Ap = qamd_field_create(lat, QAMD_F32, 3, n, 0);
   qamd_field *xs = qamd_field_create(lat, QAMD_F32, 3, n, 0);
   qamd_field *t = qamd_field_create(lat, ps, 3, n, 1);
   qamd_field *ph = half ? qamd_field_create(lat, ps, 3, n, 0) : nullptr;
@@ -492,6 +555,17 @@ int stag_cg_block_core(const qamd_links *l64, const qamd_links *ls, qamd_field *
     QAMD_HIP(hipGetLastError());
     return 0;
   };
+  // p update (mode: 0 p = r + beta p, 1 p = r for the reset rhs, 2 p = r for
+  // all active rhs) with |p_r|^2 in partial-sum set 3
+  auto p_update = [&](int mode) -> int {
+    if (!half) hipLaunchKernelGGL(k_bcg_pn<0>, gb, dim3(256), 0, stream(), p->f, r->f, p->f, st.B, red_set(3), mode);
+    else if (ps == QAMD_F16)
+      hipLaunchKernelGGL(k_bcg_pn<QAMD_F16>, gb, dim3(256), 0, stream(), p->f, r->f, ph->f, st.B, red_set(3), mode);
+    else
+      hipLaunchKernelGGL(k_bcg_pn<QAMD_Q16>, gb, dim3(256), 0, stream(), p->f, r->f, ph->f, st.B, red_set(3), mode);
+    QAMD_HIP(hipGetLastError());
+    return 0;
+  };
   auto reliable = [&]() -> int {
     if (normal(l64, Ax, x, x, t64)) return -1;
     hipLaunchKernelGGL(k_bru_resid, gb, dim3(256), 0, stream(), r64->f, b->f, Ax->f, r->f, st.B, red_set(0));
@@ -500,7 +574,7 @@ int stag_cg_block_core(const qamd_links *l64, const qamd_links *ls, qamd_field *
     return pull();
   };
   do {
-    if (!r64 || !Ax || !t64 || !r || !p || !Ap || !xs || !t || (half && !ph) || !st.B || !st.Bh) break;
+    if (!r64 || !Ax || !t64 || !r || !p || !xs || !t || (half && !ph) || !st.B || !st.Bh) break;
     if (red_reserve(nb)) break;
     if (qamd_field_norm2(b, bb)) break;
     if (!x0 && qamd_field_zero(x)) break;
@@ -516,16 +590,20 @@ int stag_cg_block_core(const qamd_links *l64, const qamd_links *ls, qamd_field *
       nact += st.Bh->active[i];
     }
     if (push()) break;
-    if (qamd_field_copy(p, r) || qamd_field_zero(xs) || (half && qamd_field_copy(ph, p))) break;
+    if (qamd_field_zero(xs) || p_update(2)) break;
     int err = 0;
     while (nact > 0 && its < maxits && !err) {
+      // per iteration: t_r = D_oe p_r (|t_r|^2); alpha_r; the fused step
+      // x_r += alpha_r p_r, r_r -= alpha_r (m2_r p_r - D_eo t_r), |r_r|^2;
+      // beta_r; p_r = r_r + beta_r p_r with |p_r|^2
+      const unsigned act = act_mask();
+      double zero[MAXB] = {0.0};
       for (int k = 0; k < 16 && !err; k++) {
-        err |= normal(ls, Ap, pin, p, t);
-        hipLaunchKernelGGL(k_bcg_xr, gb, dim3(256), 0, stream(), xs->f, p->f, r->f, Ap->f, st.B, red_set(0));
+        err |= stag_block_launch_m(ls, t->f, pin->f, zero, nullptr, 1.0, true, red_set(1), stream(), act);
+        fops.alpha(nb, st.B, stream());
+        err |= stag_block_cg_step(ls, t->f, p->f, xs->f, r->f, m2, st.B->alpha, red_set(0), stream(), act);
         fops.cg(nb, st.B, stream());
-        if (!half) hipLaunchKernelGGL(k_bcg_p<0>, gb, dim3(256), 0, stream(), p->f, r->f, p->f, st.B);
-        else if (ps == QAMD_F16) hipLaunchKernelGGL(k_bcg_p<QAMD_F16>, gb, dim3(256), 0, stream(), p->f, r->f, ph->f, st.B);
-        else hipLaunchKernelGGL(k_bcg_p<QAMD_Q16>, gb, dim3(256), 0, stream(), p->f, r->f, ph->f, st.B);
+        err |= p_update(0);
         if (hipGetLastError() != hipSuccess) err = 1;
       }
       if (err || pull()) {
@@ -578,10 +656,7 @@ int stag_cg_block_core(const qamd_links *l64, const qamd_links *ls, qamd_field *
         break;
       }
       if (nreset) {
-        if (!half) hipLaunchKernelGGL(k_bcg_reset<0>, gb, dim3(256), 0, stream(), p->f, r->f, p->f, st.B);
-        else if (ps == QAMD_F16) hipLaunchKernelGGL(k_bcg_reset<QAMD_F16>, gb, dim3(256), 0, stream(), p->f, r->f, ph->f, st.B);
-        else hipLaunchKernelGGL(k_bcg_reset<QAMD_Q16>, gb, dim3(256), 0, stream(), p->f, r->f, ph->f, st.B);
-        if (hipGetLastError() != hipSuccess) err = 1;
+        if (p_update(1)) err = 1;
         nrestart += nreset;
       }
     }
@@ -604,7 +679,6 @@ int stag_cg_block_core(const qamd_links *l64, const qamd_links *ls, qamd_field *
   qamd_field_destroy(t64);
   qamd_field_destroy(r);
   qamd_field_destroy(p);
-  qamd_field_destroy(Ap);
   qamd_field_destroy(xs);
   qamd_field_destroy(t);
   qamd_field_destroy(ph);
