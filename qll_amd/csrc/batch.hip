@@ -33,10 +33,11 @@ struct ProbDev {
   int nrhs, nb, gstart;
   double m2;
   double2 *x, *b, *r64, *Ax;   // even parity, FP64, [rhs][3][vh_pad]
-  float2 *r, *p, *Ap, *xs;     // even parity, FP32
+  float2 *r, *p, *xs;          // even parity, FP32
   void *ph;                    // even, 16-bit copy of p (Dslash input), or null
   float *ph_n;
   double rsq[BR], rsq_old[BR], t2[BR], p2[BR], beta[BR], maxrr[BR];
+  double alpha[BR];            // this iteration's step per rhs (0 for inactive rhs)
   int active[BR], reset[BR];
   int any_active, ru;          // host-set gates: iterate / take a reliable update
 };
@@ -49,8 +50,9 @@ struct Plan {
 
 __device__ __forceinline__ bool gate_ok(const ProbDev &D, int gate) { return gate == 0 ? D.any_active : D.ru; }
 
-// fused block Dslash over the problems of one plan
-template <int P, int PO, int NT, int NR, bool RED>
+// fused block Dslash over the problems of one plan; CGX: the fused CG step
+// (x_r += alpha_r p_r, r_r -= alpha_r Ap_r, |r_r|^2 in slot r)
+template <int P, int PO, int NT, int NR, bool RED, bool CGX = false>
 __global__ void __launch_bounds__(256) k_bat_block(const MArgs<P, PO> *args, Plan pl, const ProbDev *pd,
                                                    int gate) {
   int cblk = xcd_block(blockIdx.x, gridDim.x);
@@ -62,8 +64,15 @@ __global__ void __launch_bounds__(256) k_bat_block(const MArgs<P, PO> *args, Pla
   double red[2 * NR];
 #pragma unroll
   for (int i = 0; i < 2 * NR; i++) red[i] = 0.0;
-  if (cb < D.g.vh) block_site<P, PO, NT, NR, RED>(args[p], cb, red);
-  if (RED) grid_partial_at<2 * NR>(red, args[p].rs, D.gstart + local);
+  if (cb < D.g.vh) block_site<P, PO, NT, NR, RED, CGX>(args[p], cb, red);
+  if constexpr (CGX) {
+    double rr[NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) rr[r] = red[2 * r];
+    grid_partial_at<NR>(rr, args[p].rs, D.gstart + local);
+  } else if (RED) {
+    grid_partial_at<2 * NR>(red, args[p].rs, D.gstart + local);
+  }
 }
 
 // sum of N slots over a problem's blocks [b0, b0 + nb)
@@ -81,11 +90,13 @@ __device__ __forceinline__ void range_sum(RedScratch rs, int b0, int nb, double 
   block_sum(w);
 }
 
-// one workgroup per problem: t2 = |D p|^2 (set a, slot 2r), p2 = |p|^2 (set b, 2r+1)
-__global__ void __launch_bounds__(256) k_bat_fin_normal(RedScratch a, RedScratch b, ProbDev *pd) {
+// alpha_r = rsq_r / (m^2 |p_r|^2 + |t_r|^2) from |t_r|^2 (set a, slot 2r) and
+// |p_r|^2 (set b, slot r, from the p update); 0 for inactive rhs, so the fused
+// step leaves their x and r as they are
+__global__ void __launch_bounds__(256) k_bat_fin_alpha(RedScratch a, RedScratch b, ProbDev *pd) {
   ProbDev &D = pd[blockIdx.x];
   if (!D.any_active) return;
-  double wa[2 * BR], wb[2 * BR];
+  double wa[2 * BR], wb[BR];
   range_sum(a, D.gstart, D.nb, wa);
   range_sum(b, D.gstart, D.nb, wb);
   if (threadIdx.x == 0)
@@ -93,7 +104,8 @@ __global__ void __launch_bounds__(256) k_bat_fin_normal(RedScratch a, RedScratch
     for (int r = 0; r < BR; r++)
       if (r < D.nrhs) {
         D.t2[r] = wa[2 * r];
-        D.p2[r] = wb[2 * r + 1];
+        D.p2[r] = wb[r];
+        D.alpha[r] = D.active[r] ? D.rsq[r] / (D.m2 * wb[r] + wa[2 * r]) : 0.0;
       }
 }
 
@@ -145,40 +157,6 @@ __device__ __forceinline__ T *rhs_ptr(T *base, const ProbDev &D, int r) {
   return base + (int64_t)r * 3 * D.g.vh_pad;
 }
 
-// xs += alpha p; r -= alpha Ap; |r|^2 per rhs (FP32 Krylov vectors)
-__global__ void __launch_bounds__(256) k_bat_xr(ProbDev *pd, Plan all, RedScratch rs) {
-  SiteOf s = site_in(all);
-  const ProbDev &D = pd[s.p];
-  if (!D.any_active) return;
-  const int64_t vp = D.g.vh_pad;
-  double red[BR];
-#pragma unroll
-  for (int r = 0; r < BR; r++) red[r] = 0.0;
-  if (s.i < D.g.vh) {
-#pragma unroll
-    for (int r = 0; r < BR; r++) {
-      if (r >= D.nrhs || !D.active[r]) continue;
-      float al = (float)(D.rsq[r] / (D.m2 * D.p2[r] + D.t2[r]));
-      float2 pv[3], rv[3], av[3], xv[3];
-      VecLd<QAMD_F32>::ld(rhs_ptr(D.p, D, r), nullptr, vp, s.i, pv, 1.f);
-      VecLd<QAMD_F32>::ld(rhs_ptr(D.r, D, r), nullptr, vp, s.i, rv, 1.f);
-      VecLd<QAMD_F32>::ld(rhs_ptr(D.Ap, D, r), nullptr, vp, s.i, av, 1.f);
-      VecLd<QAMD_F32>::ld(rhs_ptr(D.xs, D, r), nullptr, vp, s.i, xv, 1.f);
-#pragma unroll
-      for (int j = 0; j < 3; j++) {
-        xv[j].x = fmaf(al, pv[j].x, xv[j].x);
-        xv[j].y = fmaf(al, pv[j].y, xv[j].y);
-        rv[j].x = fmaf(-al, av[j].x, rv[j].x);
-        rv[j].y = fmaf(-al, av[j].y, rv[j].y);
-        red[r] += (double)rv[j].x * rv[j].x + (double)rv[j].y * rv[j].y;
-      }
-      VecSt<QAMD_F32>::st(rhs_ptr(D.xs, D, r), nullptr, vp, s.i, xv);
-      VecSt<QAMD_F32>::st(rhs_ptr(D.r, D, r), nullptr, vp, s.i, rv);
-    }
-  }
-  grid_partial_at<BR>(red, rs, blockIdx.x);
-}
-
 template <int SH>
 __device__ __forceinline__ void store_shadow(const ProbDev &D, int r, int i, const float2 (&v)[3]) {
   if constexpr (SH != 0) {
@@ -188,45 +166,65 @@ __device__ __forceinline__ void store_shadow(const ProbDev &D, int r, int i, con
   }
 }
 
-// p = r + beta p (active rhs), and the 16-bit Dslash copy
+// p = r + beta p (active rhs), the 16-bit Dslash copy, and |p_r|^2 partial
+// sums (slot r; 0 for inactive rhs), which the next alpha needs
 template <int SH>
-__global__ void __launch_bounds__(256) k_bat_p(ProbDev *pd, Plan all) {
+__global__ void __launch_bounds__(256) k_bat_p(ProbDev *pd, Plan all, RedScratch rs) {
   SiteOf s = site_in(all);
   const ProbDev &D = pd[s.p];
-  if (!D.any_active || s.i >= D.g.vh) return;
+  if (!D.any_active) return;   // uniform over the block
   const int64_t vp = D.g.vh_pad;
+  double red[BR];
 #pragma unroll
-  for (int r = 0; r < BR; r++) {
-    if (r >= D.nrhs || !D.active[r]) continue;
-    float be = (float)D.beta[r];
-    float2 pv[3], rv[3];
-    VecLd<QAMD_F32>::ld(rhs_ptr(D.p, D, r), nullptr, vp, s.i, pv, 1.f);
-    VecLd<QAMD_F32>::ld(rhs_ptr(D.r, D, r), nullptr, vp, s.i, rv, 1.f);
+  for (int r = 0; r < BR; r++) red[r] = 0.0;
+  if (s.i < D.g.vh) {
 #pragma unroll
-    for (int j = 0; j < 3; j++) {
-      pv[j].x = fmaf(be, pv[j].x, rv[j].x);
-      pv[j].y = fmaf(be, pv[j].y, rv[j].y);
+    for (int r = 0; r < BR; r++) {
+      if (r >= D.nrhs || !D.active[r]) continue;
+      float be = (float)D.beta[r];
+      float2 pv[3], rv[3];
+      VecLd<QAMD_F32>::ld(rhs_ptr(D.p, D, r), nullptr, vp, s.i, pv, 1.f);
+      VecLd<QAMD_F32>::ld(rhs_ptr(D.r, D, r), nullptr, vp, s.i, rv, 1.f);
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        pv[j].x = fmaf(be, pv[j].x, rv[j].x);
+        pv[j].y = fmaf(be, pv[j].y, rv[j].y);
+        red[r] += (double)pv[j].x * pv[j].x + (double)pv[j].y * pv[j].y;
+      }
+      VecSt<QAMD_F32>::st(rhs_ptr(D.p, D, r), nullptr, vp, s.i, pv);
+      store_shadow<SH>(D, r, s.i, pv);
     }
-    VecSt<QAMD_F32>::st(rhs_ptr(D.p, D, r), nullptr, vp, s.i, pv);
-    store_shadow<SH>(D, r, s.i, pv);
   }
+  grid_partial_at<BR>(red, rs, blockIdx.x);
 }
 
-// restart: p = r (and its copy) for the rhs flagged in reset
+// restart: p = r (and its copy) for the rhs flagged in reset, with their
+// |p_r|^2 partial sums (the other rhs' slots keep the last p update's)
 template <int SH>
-__global__ void __launch_bounds__(256) k_bat_reset(ProbDev *pd, Plan all) {
+__global__ void __launch_bounds__(256) k_bat_reset(ProbDev *pd, Plan all, RedScratch rs) {
   SiteOf s = site_in(all);
   const ProbDev &D = pd[s.p];
-  if (s.i >= D.g.vh) return;
   const int64_t vp = D.g.vh_pad;
+  double red[BR];
 #pragma unroll
-  for (int r = 0; r < BR; r++) {
-    if (r >= D.nrhs || !D.reset[r]) continue;
-    float2 rv[3];
-    VecLd<QAMD_F32>::ld(rhs_ptr(D.r, D, r), nullptr, vp, s.i, rv, 1.f);
-    VecSt<QAMD_F32>::st(rhs_ptr(D.p, D, r), nullptr, vp, s.i, rv);
-    store_shadow<SH>(D, r, s.i, rv);
+  for (int r = 0; r < BR; r++) red[r] = 0.0;
+  if (s.i < D.g.vh) {
+#pragma unroll
+    for (int r = 0; r < BR; r++) {
+      if (r >= D.nrhs || !D.reset[r]) continue;
+      float2 rv[3];
+      VecLd<QAMD_F32>::ld(rhs_ptr(D.r, D, r), nullptr, vp, s.i, rv, 1.f);
+#pragma unroll
+      for (int j = 0; j < 3; j++) red[r] += (double)rv[j].x * rv[j].x + (double)rv[j].y * rv[j].y;
+      VecSt<QAMD_F32>::st(rhs_ptr(D.p, D, r), nullptr, vp, s.i, rv);
+      store_shadow<SH>(D, r, s.i, rv);
+    }
   }
+  block_sum(red);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int r = 0; r < BR; r++)
+      if (r < D.nrhs && D.reset[r]) rs.part[r * rs.max_blocks + blockIdx.x] = red[r];
 }
 
 // reliable update, first half: x += xs, xs = 0 (active rhs of gated problems)
@@ -328,7 +326,7 @@ static MArgs<P, PO> margs(const qamd_links *l, int par, const void *in, const fl
 }
 
 struct Work {   // per problem, library-owned work fields
-  qamd_field *r64, *Ax, *t64, *r, *p, *Ap, *xs, *t, *ph;
+  qamd_field *r64, *Ax, *t64, *r, *p, *xs, *t, *ph;
   qamd_links *l32;   // FP32 links when the 16-bit ones are too coarse for its mass
 };
 
@@ -345,27 +343,29 @@ struct Runner {
   ProbDev *pd = nullptr;
   int nprob = 0;
 
-  template <int P, int PO, int NR, bool RED>
+  template <int P, int PO, int NR, bool RED, bool CGX>
   static void bl(const MArgs<P, PO> *a, const Plan &pl, const ProbDev *pd, int gate, hipStream_t st) {
     if (pl.nblocks)
-      hipLaunchKernelGGL((k_bat_block<P, PO, 16, NR, RED>), dim3(pl.nblocks), dim3(256), 0, st, a, pl, pd, gate);
+      hipLaunchKernelGGL((k_bat_block<P, PO, 16, NR, RED, CGX>), dim3(pl.nblocks), dim3(256), 0, st, a, pl, pd,
+                         gate);
   }
-  template <int P, int PO, bool RED>
+  template <int P, int PO, bool RED, bool CGX = false>
   void bl_all(const MArgs<P, PO> *a, int grp, int gate, hipStream_t st) {
-    bl<P, PO, 1, RED>(a, cls[grp][0], pd, gate, st);
-    bl<P, PO, 2, RED>(a, cls[grp][1], pd, gate, st);
-    bl<P, PO, 4, RED>(a, cls[grp][2], pd, gate, st);
-    bl<P, PO, 8, RED>(a, cls[grp][3], pd, gate, st);
+    bl<P, PO, 1, RED, CGX>(a, cls[grp][0], pd, gate, st);
+    bl<P, PO, 2, RED, CGX>(a, cls[grp][1], pd, gate, st);
+    bl<P, PO, 4, RED, CGX>(a, cls[grp][2], pd, gate, st);
+    bl<P, PO, 8, RED, CGX>(a, cls[grp][3], pd, gate, st);
   }
+  // t = D_oe p (|t|^2); alpha; the fused step x += alpha p, r -= alpha
+  // (m^2 p - D_eo t), |r|^2; beta; p = r + beta p with |p|^2
   void iterate(hipStream_t st) {
     bl_all<PS, PS, true>(s1, 0, 0, st);
     if (PS != QAMD_F32) bl_all<QAMD_F32, QAMD_F32, true>(f1, 1, 0, st);
-    bl_all<PS, QAMD_F32, true>(s2, 0, 0, st);
-    if (PS != QAMD_F32) bl_all<QAMD_F32, QAMD_F32, true>(f2, 1, 0, st);
-    hipLaunchKernelGGL(k_bat_fin_normal, dim3(nprob), dim3(256), 0, st, red_set(1), red_set(2), pd);
-    hipLaunchKernelGGL(k_bat_xr, dim3(all.nblocks), dim3(256), 0, st, pd, all, red_set(0));
+    hipLaunchKernelGGL(k_bat_fin_alpha, dim3(nprob), dim3(256), 0, st, red_set(1), red_set(3), pd);
+    bl_all<PS, QAMD_F32, true, true>(s2, 0, 0, st);
+    if (PS != QAMD_F32) bl_all<QAMD_F32, QAMD_F32, true, true>(f2, 1, 0, st);
     hipLaunchKernelGGL(k_bat_fin_cg, dim3(nprob), dim3(256), 0, st, red_set(0), pd);
-    hipLaunchKernelGGL((k_bat_p<SH>), dim3(all.nblocks), dim3(256), 0, st, pd, all);
+    hipLaunchKernelGGL((k_bat_p<SH>), dim3(all.nblocks), dim3(256), 0, st, pd, all, red_set(3));
   }
   void reliable(hipStream_t st) {
     hipLaunchKernelGGL(k_bat_accum, dim3(all.nblocks), dim3(256), 0, st, pd, all);
@@ -374,7 +374,9 @@ struct Runner {
     hipLaunchKernelGGL(k_bat_resid, dim3(all.nblocks), dim3(256), 0, st, pd, all, red_set(0));
     hipLaunchKernelGGL(k_bat_fin_true, dim3(nprob), dim3(256), 0, st, red_set(0), pd);
   }
-  void reset(hipStream_t st) { hipLaunchKernelGGL((k_bat_reset<SH>), dim3(all.nblocks), dim3(256), 0, st, pd, all); }
+  void reset(hipStream_t st) {
+    hipLaunchKernelGGL((k_bat_reset<SH>), dim3(all.nblocks), dim3(256), 0, st, pd, all, red_set(3));
+  }
   void release() {
     dev_free(s1);
     dev_free(s2);
@@ -407,13 +409,19 @@ int Runner<PS>::run(const qamd_stag_problem *pr, int n, std::vector<ProbDev> &H,
       const qamd_field *pin = HALF ? w.ph : w.p;
       a1[i] = margs<PS, PS>(pr[i].ls, 1, pin->f.data, pin->f.norm, w.t->f.data, w.t->f.norm, nullptr, nullptr, 0.0,
                             1.0, nr, red_set(1));
-      a2[i] = margs<PS, QAMD_F32>(pr[i].ls, 0, w.t->f.data, w.t->f.norm, w.Ap->f.data, nullptr, w.p->f.data,
-                                  nullptr, m2, -1.0, nr, red_set(2));
+      a2[i] = margs<PS, QAMD_F32>(pr[i].ls, 0, w.t->f.data, w.t->f.norm, nullptr, nullptr, w.p->f.data,
+                                  nullptr, m2, -1.0, nr, red_set(0));
+      a2[i].cgx = (float2 *)w.xs->f.data;
+      a2[i].cgr = (float2 *)w.r->f.data;
+      a2[i].alpha = pd[i].alpha;
     } else {
       c1[i] = margs<QAMD_F32, QAMD_F32>(w.l32, 1, w.p->f.data, nullptr, w.t->f.data, nullptr, nullptr, nullptr, 0.0,
                                         1.0, nr, red_set(1));
-      c2[i] = margs<QAMD_F32, QAMD_F32>(w.l32, 0, w.t->f.data, nullptr, w.Ap->f.data, nullptr, w.p->f.data, nullptr,
-                                        m2, -1.0, nr, red_set(2));
+      c2[i] = margs<QAMD_F32, QAMD_F32>(w.l32, 0, w.t->f.data, nullptr, nullptr, nullptr, w.p->f.data, nullptr,
+                                        m2, -1.0, nr, red_set(0));
+      c2[i].cgx = (float2 *)w.xs->f.data;
+      c2[i].cgr = (float2 *)w.r->f.data;
+      c2[i].alpha = pd[i].alpha;
     }
     b1[i] = margs<QAMD_F64, QAMD_F64>(pr[i].l64, 1, pr[i].x->f.data, nullptr, w.t64->f.data, nullptr, nullptr,
                                       nullptr, 0.0, 1.0, nr, red_set(3));
@@ -644,7 +652,6 @@ int qamd_stag_cg_mixed_batch(const qamd_stag_problem *pr, int n, double rsq, int
       qamd_field_destroy(w.t64);
       qamd_field_destroy(w.r);
       qamd_field_destroy(w.p);
-      qamd_field_destroy(w.Ap);
       qamd_field_destroy(w.xs);
       qamd_field_destroy(w.t);
       qamd_field_destroy(w.ph);
@@ -705,11 +712,10 @@ int qamd_stag_cg_mixed_batch(const qamd_stag_problem *pr, int n, double rsq, int
       w.t64 = qamd_field_create(lat, QAMD_F64, 3, nr, 1);
       w.r = qamd_field_create(lat, QAMD_F32, 3, nr, 0);
       w.p = qamd_field_create(lat, QAMD_F32, 3, nr, 0);
-      w.Ap = qamd_field_create(lat, QAMD_F32, 3, nr, 0);
       w.xs = qamd_field_create(lat, QAMD_F32, 3, nr, 0);
       w.t = qamd_field_create(lat, pt, 3, nr, 1);
       w.ph = (pt == QAMD_F32) ? nullptr : qamd_field_create(lat, ps, 3, nr, 0);
-      ok = w.r64 && w.Ax && w.t64 && w.r && w.p && w.Ap && w.xs && w.t && (pt == QAMD_F32 || w.ph);
+      ok = w.r64 && w.Ax && w.t64 && w.r && w.p && w.xs && w.t && (pt == QAMD_F32 || w.ph);
       ok = ok && !qamd_field_zero(pr[i].x) && !qamd_field_zero(w.xs) && !qamd_field_zero(w.p);
       if (!ok) break;
       ProbDev &D = H[i];
@@ -719,7 +725,6 @@ int qamd_stag_cg_mixed_batch(const qamd_stag_problem *pr, int n, double rsq, int
       D.Ax = (double2 *)w.Ax->f.data;
       D.r = (float2 *)w.r->f.data;
       D.p = (float2 *)w.p->f.data;
-      D.Ap = (float2 *)w.Ap->f.data;
       D.xs = (float2 *)w.xs->f.data;
       D.ph = w.ph ? w.ph->f.data : nullptr;
       D.ph_n = w.ph ? w.ph->f.norm : nullptr;
