@@ -80,8 +80,9 @@ __global__ void __launch_bounds__(256) k_stag_dslash(StagArgs<P, PO> A) {
 #pragma unroll
     for (int j = 0; j < 3; j++) acc[j].x = acc[j].y = (R)0;
 #pragma unroll
-    for (int sg = 0; sg < NT / G; sg++) {
+    for (int so = 0; so < NT / G; so++) {
       // slots sg*G .. sg*G+G-1: all 1-hop terms (slot < 8) or all 3-hop ones
+      const int sg = group_at<G, NT>(so);
       const bool lng = REC && sg * G >= 8;
       S q[9][G];   // element e of the group's G terms, one 16-byte load each
 #pragma unroll

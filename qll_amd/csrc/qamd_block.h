@@ -55,36 +55,39 @@ __device__ __forceinline__ void block_site(const MArgs<P, PO> &A, int cb, double
   const bool rec = NT == 16 && A.lsign != nullptr;
   const unsigned lsb = rec ? A.lsign[cb] : 0u;
 #pragma unroll
-  for (int sl = 0; sl < NT; sl++) {
-    const int t = slot_term(sl, NT);
-    const int k = (NT == 16) ? t / 2 : t;
-    const int hop = (NT == 16 && (t & 1)) ? 3 : 1;
-    const int mu = k / 2;
-    const int h = (k & 1) ? -hop : hop;
-    const bool lng = NT == 16 && sl >= 8;
-    int nb = nbr_cb(A.g, s, mu, h);
-    // one 16-byte group load per element serves G consecutive slots
-    if (sl % G == 0) {
-#pragma unroll
-      for (int e = 0; e < 9; e++)
-        if (!(lng && rec && e % 3 == 2)) ntload_group(lp + link_off<S, NT>(sl, e), q[e]);
-    }
-    C m[9];
+  for (int so = 0; so < NT / G; so++) {
+    // one 16-byte group load per element serves the G slots of group sg
+    const int sg = group_at<G, NT>(so);
+    const bool lng = NT == 16 && sg * G >= 8;
 #pragma unroll
     for (int e = 0; e < 9; e++)
-      if (!(lng && rec && e % 3 == 2)) m[e] = to_c(q[e][sl % G], (R)0);
-    if (lng && rec) long_row3<C, R>(m, (lsb >> (sl - 8)) & 1);
+      if (!(lng && rec && e % 3 == 2)) ntload_group(lp + link_off<S, NT>(sg * G, e), q[e]);
 #pragma unroll
-    for (int r = 0; r < NR; r++) {
-      if (r < nr) {
-        const int64_t ri = A.rmap[r];
-        C v[3];
-        VecLd<P>::ld(A.in + ri * 3 * vp, A.in_norm ? A.in_norm + ri * vp : nullptr, vp, nb, v,
-                     (P == QAMD_F16 || P == QAMD_Q16) ? A.lscale[t] : (R)1);
+    for (int gg = 0; gg < G; gg++) {
+      const int sl = sg * G + gg;
+      const int t = slot_term(sl, NT);
+      const int k = (NT == 16) ? t / 2 : t;
+      const int hop = (NT == 16 && (t & 1)) ? 3 : 1;
+      const int mu = k / 2;
+      const int h = (k & 1) ? -hop : hop;
+      int nb = nbr_cb(A.g, s, mu, h);
+      C m[9];
 #pragma unroll
-        for (int row = 0; row < 3; row++)
+      for (int e = 0; e < 9; e++)
+        if (!(lng && rec && e % 3 == 2)) m[e] = to_c(q[e][gg], (R)0);
+      if (lng && rec) long_row3<C, R>(m, (lsb >> (sl - 8)) & 1);
 #pragma unroll
-          for (int c = 0; c < 3; c++) cmac<C, R>(acc[r][row], m[3 * c + row], v[c]);
+      for (int r = 0; r < NR; r++) {
+        if (r < nr) {
+          const int64_t ri = A.rmap[r];
+          C v[3];
+          VecLd<P>::ld(A.in + ri * 3 * vp, A.in_norm ? A.in_norm + ri * vp : nullptr, vp, nb, v,
+                       (P == QAMD_F16 || P == QAMD_Q16) ? A.lscale[t] : (R)1);
+#pragma unroll
+          for (int row = 0; row < 3; row++)
+#pragma unroll
+            for (int c = 0; c < 3; c++) cmac<C, R>(acc[r][row], m[3 * c + row], v[c]);
+        }
       }
     }
   }

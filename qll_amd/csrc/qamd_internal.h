@@ -178,6 +178,15 @@ __host__ __device__ constexpr int slot_term(int sl, int nterm) {
   return nterm == 16 ? ((sl & 7) << 1) | (sl >> 3) : sl;
 }
 
+// Processing order of a site's slot groups: a 1-hop group, then the 3-hop
+// group of the same directions, and so on (QLL's interleaved term order at
+// group granularity), so that the gathers of one direction's 1- and 3-hop
+// neighbours stay close in time.  8-term sets: storage order.
+template <int G, int NT>
+__host__ __device__ constexpr int group_at(int o) {
+  return NT == 16 ? ((o & 1) ? (8 / G) + (o >> 1) : (o >> 1)) : o;
+}
+
 // Row (a run of 64 sites x G slots) of element k of slot sl inside one
 // 64-site block.  The eight terms whose third row a Dslash may rebuild (the
 // 3-hop slots 8-15 of a 16-term staggered set; all eight terms of an 8-term
