@@ -222,22 +222,33 @@ __device__ __forceinline__ int link_off(int sl, int k) {
 // Third row of a term c*W (W unitary, det W = -1 if neg) from its first
 // two: det W * conj(row0 x row1) / c, c = |row0|.  Element (r, c) is m[3c+r].
 // Scale-free, so it also holds for 16-bit terms read as raw integers.
-// Contraction is off: which products the compiler would fuse into FMAs
-// depends on the surrounding code, and every kernel instance (rhs count,
-// reduction, launch shape) must rebuild the same bits.
+// Written as explicit FMA chains with contraction off: which products the
+// compiler would fuse on its own depends on the surrounding code, and every
+// kernel instance (rhs count, reduction, launch shape) must rebuild the same
+// bits (2 products + 6 FMAs per component instead of 8 products + 6 adds).
 template <class C, class R>
 __device__ __forceinline__ void long_row3(C (&m)[9], bool neg) {
 #pragma clang fp contract(off)
-  R c2 = 0;
-#pragma unroll
-  for (int c = 0; c < 3; c++) c2 += m[3 * c].x * m[3 * c].x + m[3 * c].y * m[3 * c].y;
+  R c2 = m[0].x * m[0].x;
+  c2 = fma(m[0].y, m[0].y, c2);
+  c2 = fma(m[3].x, m[3].x, c2);
+  c2 = fma(m[3].y, m[3].y, c2);
+  c2 = fma(m[6].x, m[6].x, c2);
+  c2 = fma(m[6].y, m[6].y, c2);
   const R ic = (neg ? (R)-1 : (R)1) / sqrt(c2);
 #pragma unroll
   for (int c = 0; c < 3; c++) {
     const C a1 = m[3 * ((c + 1) % 3)], a2 = m[3 * ((c + 2) % 3)];
     const C b1 = m[3 * ((c + 1) % 3) + 1], b2 = m[3 * ((c + 2) % 3) + 1];
-    const R re = (a1.x * b2.x - a1.y * b2.y) - (a2.x * b1.x - a2.y * b1.y);
-    const R im = (a1.x * b2.y + a1.y * b2.x) - (a2.x * b1.y + a2.y * b1.x);
+    // (a1 b2 - a2 b1) componentwise in complex arithmetic
+    R re = a1.x * b2.x;
+    re = fma(-a1.y, b2.y, re);
+    re = fma(-a2.x, b1.x, re);
+    re = fma(a2.y, b1.y, re);
+    R im = a1.x * b2.y;
+    im = fma(a1.y, b2.x, im);
+    im = fma(-a2.x, b1.y, im);
+    im = fma(-a2.y, b1.x, im);
     m[3 * c + 2].x = re * ic;
     m[3 * c + 2].y = -im * ic;
   }
