@@ -17,7 +17,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
 
-FAT7_BASELINE = (5.0 / 8, -1.0 / 16, 1.0 / 64, -1.0 / 384, 0.0, -1.0 / 24)
+FAT7_ASQTAD = (5.0 / 8, -1.0 / 16, 1.0 / 64, -1.0 / 384, 0.0, -1.0 / 24)
 
 _d = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 
@@ -119,7 +119,7 @@ def gauss_vector(dims, seed, stream, parity=0, nreal=6):
     return v
 
 
-def fat7(dims, u, ulong=None, coef=FAT7_BASELINE, want_long=True):
+def fat7(dims, u, ulong=None, coef=FAT7_ASQTAD, want_long=True):
     V = volume(dims)
     u = np.ascontiguousarray(u, dtype=np.float64)
     ulong = u if ulong is None else np.ascontiguousarray(ulong, dtype=np.float64)
@@ -154,7 +154,7 @@ def plaquette(dims, u):
     return p
 
 
-def asqtad_links(dims, seed, coef=FAT7_BASELINE):
+def asqtad_links(dims, seed, coef=FAT7_ASQTAD):
     """Seeded SU(3) links -> staggered phases -> fat7/Naik -> 16-term set."""
     u = apply_stag_phases(dims, random_su3_field(dims, seed))
     fl, ll = fat7(dims, u, u, coef)

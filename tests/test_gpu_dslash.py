@@ -1,7 +1,7 @@
 """GPU parity: asqtad / staggered and Wilson even-odd Dslash against the FP64 oracle.
 
 Tolerances (relative L2 difference over the output parity):
-  FP64 1e-13, FP32 1e-5, FP16 5e-3 (10-bit mantissa
+  FP64 1e-13, FP32 1e-5 (the parity targets), FP16 5e-3 (10-bit mantissa
   storage of links and vectors with per-site max-norm vectors).
 """
 import numpy as np

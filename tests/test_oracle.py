@@ -161,7 +161,7 @@ def test_fat7_unit_gauge_closed_form():
     u = np.zeros((4, V, 18))
     u[:, :, 0] = u[:, :, 8] = u[:, :, 16] = 1.0
     u = O.apply_stag_phases(dims, u)
-    for coef in [O.FAT7_BASELINE, (0.3 - 0.1 * 15 / 16, 0.2 / 24, 0.2 / 384, 0.2 / 3072, 0.1 / 96, 0.2)]:
+    for coef in [O.FAT7_ASQTAD, (0.3 - 0.1 * 15 / 16, 0.2 / 24, 0.2 / 384, 0.2 / 3072, 0.1 / 96, 0.2)]:
         fl, ll = O.fat7(dims, u, u, coef)
         c1, c3, c5, c7, cL, naik = coef
         ceff = c1 - 6 * c3 + 24 * c5 - 48 * c7 + 6 * cL
