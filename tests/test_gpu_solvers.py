@@ -245,6 +245,38 @@ def test_mixed_cgms_all_shifts(gpu):
         assert _true_rel_resid(dims, links, meff, b, xs[i].get_lex()) <= 1.01e-20
 
 
+@pytest.mark.parametrize("dims", [(6, 4, 4, 10), (4, 6, 10, 4)])
+def test_fused_solver_steps_on_ragged_lattices(gpu, dims):
+    """The fused x/r steps (single-rhs mixed CG, block CG with one mass per
+    rhs, multishift refinement) on lattices whose site count per parity is
+    not a multiple of the 256-site block: every solution reaches the FP64
+    true residual 1e-10 with the oracle's operator."""
+    print("case: fused steps dims=%s" % (dims,), flush=True)
+    links, b = _solver_case(dims, 40)
+    lat = gpu.Lattice(dims)
+    L64 = gpu.Links(lat, gpu.STAG, 0, 16).set_lex(links)
+    LQ = L64.converted(gpu.Q16)
+    bf = gpu.Field(lat, 0, 3, 1, 0).set_lex(b)
+    mass = 0.1
+    x = gpu.Field(lat, 0, 3, 1, 0)
+    assert gpu.stag_cg_mixed(L64, LQ, x, bf, mass, 1e-20, 20000)["converged"]
+    assert _true_rel_resid(dims, links, mass, b, x.get_lex()) <= 1.01e-20
+    bs = np.stack([O.gauss_vector(dims, 50 + r, 100, parity=0) for r in range(3)])
+    bb = gpu.Field(lat, 0, 3, 3, 0).set_lex(bs)
+    xb = gpu.Field(lat, 0, 3, 3, 0)
+    infos = gpu.stag_cg_mixed_block(L64, LQ, xb, bb, mass, 1e-20, 20000)
+    xbs = xb.get_lex().reshape(3, -1, 6)
+    for r in range(3):
+        assert infos[r]["converged"]
+        assert _true_rel_resid(dims, links, mass, bs[r], xbs[r]) <= 1.01e-20, r
+    shifts = [0.0, 0.01, 0.3]
+    xs = [gpu.Field(lat, 0, 3, 1, 0) for _ in shifts]
+    infos = gpu.stag_cgms_mixed(L64, LQ, xs, bf, mass, shifts, 1e-20, 20000)
+    for i, sh in enumerate(shifts):
+        assert infos[i]["converged"]
+        assert _true_rel_resid(dims, links, np.sqrt(mass ** 2 + sh), b, xs[i].get_lex()) <= 1.01e-20, i
+
+
 # ---------------------------------------------------------------------------
 # The reference configurations.  Oracle-sized volumes are checked against the
 # oracle; the full sizes (no oracle in reach) by FP64 true residuals computed
