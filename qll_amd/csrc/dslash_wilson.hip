@@ -39,6 +39,8 @@ template <int P> struct WArgs {
   int tiled;   // sites in the tiled order of tl (else checkerboard order)
   Tiling tl;
   const uint8_t *lsign;   // set: third rows rebuilt (determinant sign bit per site and term)
+  Field rv;               // EPI 2: the residual updated in place, r -= alpha out
+  const double *alpha;    // EPI 2: per-rhs step (device)
 };
 
 template <class C> __device__ __forceinline__ C cadd(C a, C b) { a.x += b.x; a.y += b.y; return a; }
@@ -118,10 +120,13 @@ constexpr int WMAXS = 256 / 8 + 1;
 // REC: the links' third rows are not read but rebuilt from the first two
 // (every term is a scale times a unitary matrix; 8·96 instead of 8·144 link
 // bytes per site in FP64).  Used for one or two rhs, LDS staging for eight or
-// more: never both.
-template <int P, bool RED, bool LDSL, bool REC>
+// more: never both.  EPI (with RED): 0 one sum |out|^2 over the block; 1 per-
+// rhs sums |out_r|^2 (slot r); 2 the solver step r_r -= alpha_r out_r with out
+// not stored, per-rhs sums |r_r|^2.
+template <int P, bool RED, bool LDSL, bool REC, int EPI = 0>
 __global__ void __launch_bounds__(256) k_wilson(WArgs<P> A) {
   static_assert(!(LDSL && REC), "the LDS link stage reads whole links");
+  static_assert(EPI == 0 || RED, "per-rhs epilogues reduce");
   typedef typename Prec<P>::R R;
   typedef typename Prec<P>::C C;
   typedef typename Prec<P>::S S;
@@ -295,15 +300,33 @@ __global__ void __launch_bounds__(256) k_wilson(WArgs<P> A) {
         }
       }
     }
-    float on = 1.0f;
     constexpr bool NORMED = (P == QAMD_F16 || P == QAMD_Q16);
+    if constexpr (EPI == 2) {   // o <- r - alpha o, stored to r
+      const Field &fr = A.rv;
+      const R al = (R)A.alpha[r];
+      float rn = 1.0f;
+      if constexpr (NORMED) rn = fr.norm[fr.nidx(r, cb)];
+#pragma unroll
+      for (int q = 0; q < 12 / K; q++) {
+        C rvv[K];
+        Chunk<P>::ld(fr.data, fr.chunk(r, q, cb), rn, rvv);
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+          C &v = o[(q * K + k) / 3][(q * K + k) % 3];
+          v.x = fma(-al, v.x, rvv[k].x);
+          v.y = fma(-al, v.y, rvv[k].y);
+        }
+      }
+    }
+    const Field &fst = (EPI == 2) ? A.rv : fout;
+    float on = 1.0f;
     if (NORMED) {
       float mx = 0.f;
 #pragma unroll
       for (int sp = 0; sp < 4; sp++)
 #pragma unroll
         for (int c = 0; c < 3; c++) mx = fmaxf(mx, fmaxf(fabsf((float)o[sp][c].x), fabsf((float)o[sp][c].y)));
-      fout.norm[fout.nidx(r, cb)] = mx;
+      fst.norm[fst.nidx(r, cb)] = mx;
       on = mx > 0.f ? 1.0f / mx : 0.f;
     }
 #pragma unroll
@@ -318,10 +341,24 @@ __global__ void __launch_bounds__(256) k_wilson(WArgs<P> A) {
           v[k].y *= on;
         }
       }
-      Chunk<P>::st(fout.data, fout.chunk(r, q, cb), v);
+      Chunk<P>::st(fst.data, fst.chunk(r, q, cb), v);
     }
   }
-  if (RED) grid_partial_at(red, A.rs, blk);   // indexed by site block: same sums in either order
+  if constexpr (EPI > 0) {
+    // per-rhs sums in a fixed order: thread q < nr adds the values of the
+    // block's threads that hold rhs q, in thread order
+    __shared__ double shr[256];
+    shr[threadIdx.x] = red[0];
+    __syncthreads();
+    if ((int)threadIdx.x < nr) {
+      const int q = threadIdx.x, r0 = (int)(((int64_t)blk * blockDim.x) % nr);   // rhs of thread 0
+      double t = 0.0;
+      for (int tl = (q - r0 + nr) % nr; tl < (int)blockDim.x; tl += nr) t += shr[tl];
+      A.rs.part[q * A.rs.max_blocks + blk] = t;
+    }
+  } else if (RED) {
+    grid_partial_at(red, A.rs, blk);   // indexed by site block: same sums in either order
+  }
 }
 
 // launch shapes the tuner tries: block order, links staged in LDS (8 or more
@@ -338,14 +375,14 @@ constexpr int NWSHAPE = sizeof(WSHAPES) / sizeof(WSHAPES[0]);
 
 template <int P> struct WRun {
   WArgs<P> A;
-  int nb;
+  int nb, epi;
   hipStream_t st;
   int ncand, shape[NWSHAPE];
-  template <bool RED, bool REC>
+  template <bool RED, bool REC, int EPI = 0>
   static void launch_k(int lds, const WRun &R, const WArgs<P> &B) {
-    if constexpr (REC) hipLaunchKernelGGL((k_wilson<P, RED, false, true>), dim3(R.nb), dim3(256), 0, R.st, B);
-    else if (lds) hipLaunchKernelGGL((k_wilson<P, RED, true, false>), dim3(R.nb), dim3(256), 0, R.st, B);
-    else hipLaunchKernelGGL((k_wilson<P, RED, false, false>), dim3(R.nb), dim3(256), 0, R.st, B);
+    if constexpr (REC) hipLaunchKernelGGL((k_wilson<P, RED, false, true, EPI>), dim3(R.nb), dim3(256), 0, R.st, B);
+    else if (lds) hipLaunchKernelGGL((k_wilson<P, RED, true, false, EPI>), dim3(R.nb), dim3(256), 0, R.st, B);
+    else hipLaunchKernelGGL((k_wilson<P, RED, false, false, EPI>), dim3(R.nb), dim3(256), 0, R.st, B);
   }
   static int run(int c, void *ctx) {
     WRun &R = *(WRun *)ctx;
@@ -354,7 +391,15 @@ template <int P> struct WRun {
     B.xcd = sh.xcd;
     B.tiled = sh.tile[0] ? 1 : 0;
     if (B.tiled && !B.tl.init(B.g.L, sh.tile)) return -1;
-    if (B.red_out) {
+    if (R.epi == 1 || R.epi == 2) {   // per-rhs sums into B.rs, finished by the caller
+      if (R.epi == 1) {
+        if (B.lsign) launch_k<true, true, 1>(sh.lds, R, B);
+        else launch_k<true, false, 1>(sh.lds, R, B);
+      } else {
+        if (B.lsign) launch_k<true, true, 2>(sh.lds, R, B);
+        else launch_k<true, false, 2>(sh.lds, R, B);
+      }
+    } else if (B.red_out) {
       if (B.lsign) launch_k<true, true>(sh.lds, R, B);
       else launch_k<true, false>(sh.lds, R, B);
       launch_finish<1>(B.rs, R.nb, FinStore1{B.red_out}, R.st, B.skip);
@@ -369,7 +414,8 @@ template <int P> struct WRun {
 
 template <int P>
 static int wlaunch(const qamd_links *l, Field &out, const Field &in, double a, const Field *w, double b,
-                   double *red_out, hipStream_t st, const int *skip, bool dag) {
+                   double *red_out, hipStream_t st, const int *skip, bool dag, int epi = 0,
+                   const Field *rv = nullptr, const double *alpha = nullptr, RedScratch rsx = RedScratch{}) {
   typedef typename Prec<P>::S S;
   WArgs<P> A;
   memset(&A, 0, sizeof A);
@@ -396,7 +442,14 @@ static int wlaunch(const qamd_links *l, Field &out, const Field &in, double a, c
   R.A = A;
   R.nb = (int)((nthreads + 255) / 256);
   R.st = st;
-  if (red_out) {
+  R.epi = epi;
+  if (epi) {
+    R.A.rs = rsx;
+    if (epi == 2) {
+      R.A.rv = *rv;
+      R.A.alpha = alpha;
+    }
+  } else if (red_out) {
     if (red_reserve(R.nb)) return -1;
     R.A.rs = red_set(0);
     R.A.red_out = red_out;
@@ -408,15 +461,16 @@ static int wlaunch(const qamd_links *l, Field &out, const Field &in, double a, c
   for (int k = 0; k < NWSHAPE; k++) {
     const WShape &sh = WSHAPES[k];
     Tiling t;
-    if (sh.tile[0] && (red_out || !t.init(A.g.L, sh.tile))) continue;
+    if (sh.tile[0] && (red_out || epi || !t.init(A.g.L, sh.tile))) continue;
     if (sh.lds && in.nrhs < 8) continue;
     R.shape[R.ncand++] = k;
   }
-  // the tuner's trial launches must not feed the output back into an input
-  const bool alias = out.data == in.data || (w && out.data == w->data);
+  // the tuner's trial launches must not feed the output back into an input,
+  // nor repeat an in-place residual update
+  const bool alias = out.data == in.data || (w && out.data == w->data) || epi == 2;
   char key[96];
-  snprintf(key, sizeof key, "wilson p%d n%d red%d %dx%dx%dx%d", P, in.nrhs, red_out ? 1 : 0, A.g.L[0], A.g.L[1],
-           A.g.L[2], A.g.L[3]);
+  snprintf(key, sizeof key, "wilson p%d n%d red%d %dx%dx%dx%d", P, in.nrhs, red_out || epi ? 1 : 0, A.g.L[0],
+           A.g.L[1], A.g.L[2], A.g.L[3]);
   int c = alias ? 1 : tune_pick(key, R.ncand, 1, &WRun<P>::run, &R);
   return WRun<P>::run(c, &R);
 }
@@ -444,6 +498,30 @@ int wilson_dslash_launch(const qamd_links *l, Field &out, const Field &in, doubl
     case QAMD_F32: return wlaunch<QAMD_F32>(l, out, in, a, w, b, red_out, st, skip, dag);
     case QAMD_F16: return wlaunch<QAMD_F16>(l, out, in, a, w, b, red_out, st, skip, dag);
     default: return wlaunch<QAMD_Q16>(l, out, in, a, w, b, red_out, st, skip, dag);
+  }
+}
+
+// The Wilson solver's fused launches: out = a w + b H in with per-rhs sums
+// |out_r|^2 (epi 1), or r_r -= alpha_r (a w + b H in)_r without storing out
+// and per-rhs sums |r_r|^2 (epi 2); the sums go to slot r of rs over the
+// launch's blocks, ceil(vh * nrhs / 256), for the caller to finish.
+int wilson_dslash_rhs_launch(const qamd_links *l, Field &out, const Field &in, double a, const Field *w, double b,
+                             hipStream_t st, bool dag, int epi, Field *r, const double *alpha, RedScratch rs) {
+  if (l->kind != 1 || in.prec != l->prec || (w && w->prec != l->prec) || in.ncomp != 12 ||
+      (w && (w->nrhs != in.nrhs || w->parity == in.parity)) || (epi != 1 && epi != 2) ||
+      (epi == 1 && (out.prec != l->prec || out.nrhs != in.nrhs || out.parity == in.parity)) ||
+      (epi == 2 && (!r || !alpha || r->prec != l->prec || r->nrhs != in.nrhs || r->parity == in.parity)) ||
+      in.nrhs > 16) {
+    set_error("wilson_dslash_rhs: field shapes or precisions do not fit");
+    return -1;
+  }
+  if (red_reserve((int)((l->lat->g.vh * in.nrhs + 255) / 256))) return -1;
+  Field &o = epi == 2 ? *r : out;
+  switch (l->prec) {
+    case QAMD_F64: return wlaunch<QAMD_F64>(l, o, in, a, w, b, nullptr, st, nullptr, dag, epi, r, alpha, rs);
+    case QAMD_F32: return wlaunch<QAMD_F32>(l, o, in, a, w, b, nullptr, st, nullptr, dag, epi, r, alpha, rs);
+    case QAMD_F16: return wlaunch<QAMD_F16>(l, o, in, a, w, b, nullptr, st, nullptr, dag, epi, r, alpha, rs);
+    default: return wlaunch<QAMD_Q16>(l, o, in, a, w, b, nullptr, st, nullptr, dag, epi, r, alpha, rs);
   }
 }
 

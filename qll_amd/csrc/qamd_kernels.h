@@ -49,5 +49,10 @@ int sloppy_too_coarse(const qamd_links *l64, const qamd_links *ls, double mass, 
 // dag: the hermitian conjugate g5 D g5 of the hopping term
 int wilson_dslash_launch(const qamd_links *l, Field &out, const Field &in, double a, const Field *w,
                          double b, double *red_out, hipStream_t st, const int *skip = nullptr, bool dag = false);
+// the Wilson solver's fused launches (dslash_wilson.hip): epi 1 stores out =
+// a w + b H in with per-rhs |out_r|^2 in slot r of rs; epi 2 applies r_r -=
+// alpha_r (a w + b H in)_r (out not stored) with per-rhs |r_r|^2
+int wilson_dslash_rhs_launch(const qamd_links *l, Field &out, const Field &in, double a, const Field *w, double b,
+                             hipStream_t st, bool dag, int epi, Field *r, const double *alpha, RedScratch rs);
 
 }  // namespace qamd

@@ -55,20 +55,23 @@ __global__ void __launch_bounds__(256) k_rhs_dot(Field f, Field h, int mode, Red
   grid_partial<MAXRHS>(acc, rs);
 }
 
-// y_r = a_r x_r + b_r y_r  (x, y may differ in precision)
-__global__ void __launch_bounds__(256) k_rhs_axpby(Field x, Field y, const double *a, const double *b) {
+// e_r += alpha_r p_r; p_r = r_r + beta_r p_r (one pass; alpha at a, beta at b)
+__global__ void __launch_bounds__(256) k_rhs_ep(Field e, Field p, Field r, const double *a, const double *b) {
   int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int cb = (int)(g / y.nrhs), r = (int)(g % y.nrhs);
-  if (cb >= y.g->vh) return;
-  double2 vx[12], vy[12];
-  double ar = a[r], br = b[r];
-  site_load<12>(x, r, cb, vx);
-  if (br != 0.0) site_load<12>(y, r, cb, vy);
+  int cb = (int)(g / p.nrhs), q = (int)(g % p.nrhs);
+  if (cb >= p.g->vh) return;
+  double2 ve[12], vp[12], vr[12];
+  const double al = a[q], be = b[q];
+  site_load<12>(e, q, cb, ve);
+  site_load<12>(p, q, cb, vp);
+  site_load<12>(r, q, cb, vr);
 #pragma unroll
-  for (int j = 0; j < 12; j++)
-    vy[j] = br != 0.0 ? make_double2(ar * vx[j].x + br * vy[j].x, ar * vx[j].y + br * vy[j].y)
-                      : make_double2(ar * vx[j].x, ar * vx[j].y);
-  site_store<12>(y, r, cb, vy);
+  for (int j = 0; j < 12; j++) {
+    ve[j] = make_double2(ve[j].x + al * vp[j].x, ve[j].y + al * vp[j].y);
+    vp[j] = make_double2(vr[j].x + be * vp[j].x, vr[j].y + be * vp[j].y);
+  }
+  site_store<12>(e, q, cb, ve);
+  site_store<12>(p, q, cb, vp);
 }
 
 struct WS {   // work space of one precision
@@ -126,18 +129,6 @@ static int rhs_dot(WS &w, const qamd_field *f, const qamd_field *h, double *out)
   return 0;
 }
 
-static int rhs_axpby(WS &w, const double *a, const qamd_field *x, const double *b, qamd_field *y) {
-  // coefficients go through the pinned mirror; the copy is stream-ordered
-  QAMD_HIP(hipStreamSynchronize(stream()));
-  memcpy(w.coefh, a, y->f.nrhs * sizeof(double));
-  memcpy(w.coefh + 16, b, y->f.nrhs * sizeof(double));
-  QAMD_HIP(hipMemcpyAsync(w.coef, w.coefh, 32 * sizeof(double), hipMemcpyHostToDevice, stream()));
-  hipLaunchKernelGGL(k_rhs_axpby, dim3(nblk(y)), dim3(256), 0, stream(), x->f, y->f, (const double *)w.coef,
-                     (const double *)(w.coef + 16));
-  QAMD_HIP(hipGetLastError());
-  return 0;
-}
-
 // out = Mhat in, or Mhat^+ in = g5 Mhat g5 in = in - kappa^2 (g5 H_eo g5)(g5 H_oe g5) in,
 // whose hopping terms are the Dslash with every projector sign flipped
 static int apply_mhat(const qamd_links *l, WS &w, qamd_field *out, const qamd_field *in, double kappa, int dagger) {
@@ -148,11 +139,35 @@ static int apply_mhat(const qamd_links *l, WS &w, qamd_field *out, const qamd_fi
   return 0;
 }
 
-// CGNE from x = 0 on Mhat e = rhs, to |r_normal|^2 <= tol2 |r0_normal|^2 per rhs
+// per-rhs sums of slot r over nb blocks of partial-sum set 0 -> host
+static int rhs_sums_to_host(WS &w, int nb, double *out) {
+  launch_finish<MAXRHS>(red_set(0), nb, FinRhs{w.coef + 32, w.nrhs}, stream());
+  QAMD_HIP(hipGetLastError());
+  QAMD_HIP(hipMemcpyAsync(w.coefh + 32, w.coef + 32, w.nrhs * sizeof(double), hipMemcpyDeviceToHost, stream()));
+  QAMD_HIP(hipStreamSynchronize(stream()));
+  memcpy(out, w.coefh + 32, w.nrhs * sizeof(double));
+  return 0;
+}
+
+// coefficients a[16], b[16] to the device (the stream is idle when called)
+static int coef_up(WS &w, const double *a, const double *b) {
+  memcpy(w.coefh, a, w.nrhs * sizeof(double));
+  memcpy(w.coefh + 16, b, w.nrhs * sizeof(double));
+  QAMD_HIP(hipMemcpyAsync(w.coef, w.coefh, 32 * sizeof(double), hipMemcpyHostToDevice, stream()));
+  return 0;
+}
+
+// CGNE from x = 0 on Mhat e = rhs, to |r_normal|^2 <= tol2 |r0_normal|^2 per
+// rhs.  Per iteration: q = Mhat p with |q_r|^2 in the second Dslash's
+// epilogue; alpha_r = |r_r|^2 / |q_r|^2; r_r -= alpha_r Mhat^+ q_r in the
+// epilogue of Mhat^+'s second Dslash (Mhat^+ q not stored) with |r_r|^2;
+// beta_r; one pass e_r += alpha_r p_r, p_r = r_r + beta_r p_r.
 static int cgne_inner(const qamd_links *l, WS &w, qamd_field *e, const qamd_field *rhs, double kappa, double tol2,
                       int maxits, int *its_out) {
   const int n = rhs->f.nrhs;
-  double rr[MAXRHS], rr0[MAXRHS], qq[MAXRHS], rrn[MAXRHS], al[MAXRHS], be[MAXRHS], one[MAXRHS], mal[MAXRHS];
+  const int nb = (int)nblk(rhs);
+  const double k2 = -kappa * kappa;
+  double rr[MAXRHS], rr0[MAXRHS], qq[MAXRHS], rrn[MAXRHS], al[MAXRHS], be[MAXRHS];
   int done[MAXRHS];
   if (qamd_field_zero(e)) return -1;
   if (apply_mhat(l, w, w.r, rhs, kappa, 1)) return -1;
@@ -161,30 +176,39 @@ static int cgne_inner(const qamd_links *l, WS &w, qamd_field *e, const qamd_fiel
   for (int i = 0; i < n; i++) {
     rr0[i] = rr[i];
     done[i] = rr[i] == 0;
-    one[i] = 1.0;
   }
   int it = 0;
   while (it < maxits) {
     int all = 1;
     for (int i = 0; i < n; i++) all &= done[i];
     if (all) break;
-    if (apply_mhat(l, w, w.q, w.p, kappa, 0)) return -1;
-    if (rhs_dot(w, w.q, nullptr, qq)) return -1;
+    // q = Mhat p, |q_r|^2
+    if (wilson_dslash_launch(l, w.t->f, w.p->f, 0.0, nullptr, 1.0, nullptr, stream(), nullptr, false)) return -1;
+    if (wilson_dslash_rhs_launch(l, w.q->f, w.t->f, 1.0, &w.p->f, k2, stream(), false, 1, nullptr, nullptr,
+                                 red_set(0)))
+      return -1;
+    if (rhs_sums_to_host(w, nb, qq)) return -1;
     for (int i = 0; i < n; i++) {
       al[i] = (!done[i] && qq[i] > 0) ? rr[i] / qq[i] : 0.0;
-      mal[i] = -al[i];
+      be[i] = 0.0;
     }
-    if (rhs_axpby(w, al, w.p, one, e)) return -1;
-    if (apply_mhat(l, w, w.s, w.q, kappa, 1)) return -1;
-    if (rhs_axpby(w, mal, w.s, one, w.r)) return -1;
-    if (rhs_dot(w, w.r, nullptr, rrn)) return -1;
+    if (coef_up(w, al, be)) return -1;
+    // r -= alpha Mhat^+ q, |r_r|^2
+    if (wilson_dslash_launch(l, w.t->f, w.q->f, 0.0, nullptr, 1.0, nullptr, stream(), nullptr, true)) return -1;
+    if (wilson_dslash_rhs_launch(l, w.s->f, w.t->f, 1.0, &w.q->f, k2, stream(), true, 2, &w.r->f, w.coef,
+                                 red_set(0)))
+      return -1;
+    if (rhs_sums_to_host(w, nb, rrn)) return -1;
     it++;
     for (int i = 0; i < n; i++) {
       if (!done[i] && rrn[i] <= tol2 * rr0[i]) done[i] = 1;
       be[i] = (!done[i] && rr[i] > 0) ? rrn[i] / rr[i] : 0.0;
       rr[i] = rrn[i];
     }
-    if (rhs_axpby(w, one, w.r, be, w.p)) return -1;
+    if (coef_up(w, al, be)) return -1;
+    hipLaunchKernelGGL(k_rhs_ep, dim3(nb), dim3(256), 0, stream(), e->f, w.p->f, w.r->f, (const double *)w.coef,
+                       (const double *)(w.coef + 16));
+    QAMD_HIP(hipGetLastError());
   }
   *its_out = it;
   return 0;
