@@ -57,9 +57,9 @@ template <int P> struct VT {
 };
 
 // ---------------------------------------------------------------- CG kernels
-// p update; with SH = 1 an FP16 copy of p (the Dslash input) is written too
-template <int P, int SH>
-__global__ void __launch_bounds__(256) k_cg_p(Field p, Field r, Field shadow, SolverDev *S) {
+// p update of the plain CG
+template <int P>
+__global__ void __launch_bounds__(256) k_cg_p(Field p, Field r, SolverDev *S) {
   typedef typename Prec<P>::C C;
   typedef typename Prec<P>::R R;
   if (S->done) return;
@@ -84,12 +84,6 @@ __global__ void __launch_bounds__(256) k_cg_p(Field p, Field r, Field shadow, So
     }
   }
   VT<P>::st(p, i, pv);
-  if constexpr (SH != 0) {   // 16-bit copy of p (precision SH) for the Dslash
-    float2 h[3];
-#pragma unroll
-    for (int j = 0; j < 3; j++) h[j] = make_float2((float)pv[j].x, (float)pv[j].y);
-    VT<SH>::st(shadow, i, h);
-  }
 }
 
 // Mixed-precision CG, fused form: p = r + beta p (reset: p = r), the 16-bit
@@ -466,25 +460,16 @@ static int apply_normal(const qamd_links *l, SolverDev *S, qamd_field *Ap, qamd_
   return 0;
 }
 
-template <int P, int SH>
-static void launch_cg_p(const qamd_lattice *lat, qamd_field *p, qamd_field *r, qamd_field *sh, SolverDev *S) {
-  hipLaunchKernelGGL((k_cg_p<P, SH>), grid1(lat), dim3(256), 0, stream(), p->f, r->f, sh ? sh->f : p->f, S);
-}
-static int cg_p(const qamd_lattice *lat, qamd_field *p, qamd_field *r, qamd_field *sh, SolverDev *S) {
-  if (sh && (p->f.prec != QAMD_F32 || (sh->f.prec != QAMD_F16 && sh->f.prec != QAMD_Q16))) {
-    set_error("cg_p: a 16-bit shadow needs an FP32 p");
-    return -1;
-  }
+// p update of the plain CG (QLL's recurrences in mode 0)
+static int cg_p(const qamd_lattice *lat, qamd_field *p, qamd_field *r, SolverDev *S) {
+#define CGP(P) hipLaunchKernelGGL((k_cg_p<P>), grid1(lat), dim3(256), 0, stream(), p->f, r->f, S)
   switch (p->f.prec) {
-    case QAMD_F64: launch_cg_p<QAMD_F64, 0>(lat, p, r, nullptr, S); break;
-    case QAMD_F32:
-      if (!sh) launch_cg_p<QAMD_F32, 0>(lat, p, r, nullptr, S);
-      else if (sh->f.prec == QAMD_F16) launch_cg_p<QAMD_F32, QAMD_F16>(lat, p, r, sh, S);
-      else launch_cg_p<QAMD_F32, QAMD_Q16>(lat, p, r, sh, S);
-      break;
-    case QAMD_F16: launch_cg_p<QAMD_F16, 0>(lat, p, r, nullptr, S); break;
-    default: launch_cg_p<QAMD_Q16, 0>(lat, p, r, nullptr, S); break;
+    case QAMD_F64: CGP(QAMD_F64); break;
+    case QAMD_F32: CGP(QAMD_F32); break;
+    case QAMD_F16: CGP(QAMD_F16); break;
+    default: CGP(QAMD_Q16); break;
   }
+#undef CGP
   QAMD_HIP(hipGetLastError());
   return 0;
 }
@@ -576,7 +561,7 @@ int qamd_stag_cg(const qamd_links *l, qamd_field *x, const qamd_field *b, double
     int err = 0;
     while (!Sh->done) {
       for (int k = 0; k < BATCH && !err; k++) {
-        err |= cg_p(lat, p, r, nullptr, st.S);
+        err |= cg_p(lat, p, r, st.S);
         err |= apply_normal(l, st.S, Ap, p, p, t, mass * mass, &st.S->done);
         err |= cg_xr(lat, x, p, r, Ap, st.S);
       }
