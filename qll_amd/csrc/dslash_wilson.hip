@@ -370,6 +370,7 @@ constexpr WShape WSHAPES[] = {
     {0, 0, {0, 0, 0, 0}}, {1, 0, {0, 0, 0, 0}}, {1, 0, {3, 3, 3, 3}}, {1, 0, {2, 2, 2, 2}},
     {1, 0, {4, 3, 3, 3}}, {1, 0, {3, 3, 2, 2}}, {1, 0, {2, 2, 2, 4}}, {1, 0, {5, 2, 2, 2}},
     {0, 1, {0, 0, 0, 0}}, {1, 1, {0, 0, 0, 0}}, {1, 1, {3, 3, 2, 2}}, {1, 1, {5, 2, 2, 2}},
+    {1, 1, {3, 3, 3, 3}}, {1, 1, {2, 2, 2, 2}}, {1, 1, {4, 2, 2, 2}}, {1, 1, {4, 3, 3, 2}},
 };
 constexpr int NWSHAPE = sizeof(WSHAPES) / sizeof(WSHAPES[0]);
 

@@ -220,7 +220,7 @@ def test_wilson_every_launch_shape(gpu, prec, nrhs):
         L = L.converted(prec)
     yin = gpu.Field(lat, prec, 12, nrhs, 1).set_lex(ys)
     res, norms = [], []
-    for mode in [0] + list(range(2, 14)) + [1]:   # default, each candidate, tuned
+    for mode in [0] + list(range(2, 18)) + [1]:   # default, each candidate, tuned
         gpu.set_tuning(mode)
         out = gpu.Field(lat, prec, 12, nrhs, 0)
         gpu.dslash(L, out, yin, b=-1.0)
