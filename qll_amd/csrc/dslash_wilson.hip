@@ -373,6 +373,16 @@ constexpr WShape WSHAPES[] = {
     {1, 1, {3, 3, 3, 3}}, {1, 1, {2, 2, 2, 2}}, {1, 1, {4, 2, 2, 2}}, {1, 1, {4, 3, 3, 2}},
 };
 constexpr int NWSHAPE = sizeof(WSHAPES) / sizeof(WSHAPES[0]);
+// site tiling (log2 extents 32 x 4 x 4 x 4) of every reducing launch on a lattice that admits it
+constexpr int WRTILE[4] = {5, 2, 2, 2};
+constexpr bool wrtile_listed() {
+  for (const WShape &sh : WSHAPES)
+    if (!sh.lds && sh.tile[0] == WRTILE[0] && sh.tile[1] == WRTILE[1] && sh.tile[2] == WRTILE[2] &&
+        sh.tile[3] == WRTILE[3])
+      return true;
+  return false;
+}
+static_assert(wrtile_listed(), "reducing launches need a shape with the fixed tiling and no LDS stage");
 
 template <int P> struct WRun {
   WArgs<P> A;
@@ -455,24 +465,36 @@ static int wlaunch(const qamd_links *l, Field &out, const Field &in, double a, c
     R.A.rs = red_set(0);
     R.A.red_out = red_out;
   }
-  // with a reduction the sites of a block stay fixed (a tiled order would
-  // regroup the per-block partial sums and change the rounding); LDS staging
-  // needs at least 8 rhs per site (at most WMAXS sites per block)
+  // with a reduction every candidate groups the sites into blocks the same
+  // way, so the partial sums (and their rounding) do not depend on the
+  // tuner's pick: the fixed tiling WRTILE where the lattice admits it, else
+  // checkerboard order.  LDS staging needs at least 8 rhs per site (at most
+  // WMAXS sites per block)
+  const bool rsum = red_out != nullptr || epi != 0;
+  Tiling rt;
+  const bool rtiled = rsum && rt.init(A.g.L, WRTILE);
   R.ncand = 0;
   for (int k = 0; k < NWSHAPE; k++) {
     const WShape &sh = WSHAPES[k];
-    Tiling t;
-    if (sh.tile[0] && (red_out || epi || !t.init(A.g.L, sh.tile))) continue;
     if (sh.lds && in.nrhs < 8) continue;
+    if (rsum) {
+      bool fixed = true;
+      for (int i = 0; i < 4; i++) fixed = fixed && sh.tile[i] == WRTILE[i];
+      if (rtiled ? !fixed : sh.tile[0] != 0) continue;
+    } else {
+      Tiling t;
+      if (sh.tile[0] && !t.init(A.g.L, sh.tile)) continue;
+    }
     R.shape[R.ncand++] = k;
   }
+  const int def = R.ncand > 1 ? 1 : 0;
   // the tuner's trial launches must not feed the output back into an input,
   // nor repeat an in-place residual update
   const bool alias = out.data == in.data || (w && out.data == w->data) || epi == 2;
   char key[96];
   snprintf(key, sizeof key, "wilson p%d n%d red%d %dx%dx%dx%d", P, in.nrhs, red_out || epi ? 1 : 0, A.g.L[0],
            A.g.L[1], A.g.L[2], A.g.L[3]);
-  int c = alias ? 1 : tune_pick(key, R.ncand, 1, &WRun<P>::run, &R);
+  int c = alias ? def : tune_pick(key, R.ncand, def, &WRun<P>::run, &R);
   return WRun<P>::run(c, &R);
 }
 

@@ -205,6 +205,40 @@ def test_block_dslash_every_launch_shape(gpu):
         assert np.array_equal(x, res[0][1]) and its == res[0][2]
 
 
+@pytest.mark.parametrize("prec,nrhs", [(0, 1), (0, 2), (1, 16), (3, 8)])
+def test_wilson_reducing_launches_fixed_tiling(gpu, prec, nrhs):
+    """On a lattice that admits the 32x4x4x4 site tiling every reducing Wilson
+    launch uses it: all shapes the tuner can pick give the same output and the
+    same |out|^2 bit for bit, both matching the oracle's Dslash."""
+    print("case: wilson reducing shapes prec=%d nrhs=%d" % (prec, nrhs), flush=True)
+    dims = (32, 4, 4, 8)
+    links = O.wilson_doublestore(dims, O.random_su3_field(dims, 37))
+    ys = np.stack([O.gauss_vector(dims, 90 + r, 300, parity=-1, nreal=24) for r in range(nrhs)])
+    lat = gpu.Lattice(dims)
+    L = gpu.Links(lat, gpu.WILSON, 0, 8).set_lex(links)
+    if prec:
+        L = L.converted(prec)
+    yin = gpu.Field(lat, prec, 12, nrhs, 1).set_lex(ys)
+    res, norms = [], []
+    for mode in [0] + list(range(2, 18)) + [1]:   # default, each candidate, tuned
+        gpu.set_tuning(mode)
+        out = gpu.Field(lat, prec, 12, nrhs, 0)
+        norms.append(gpu.dslash(L, out, yin, b=-1.0, want_norm=True))
+        res.append(out.get_lex())
+    gpu.set_tuning(1)
+    for o in res[1:]:
+        assert np.array_equal(o, res[0])
+    assert all(n == norms[0] for n in norms)
+    got = res[0] if nrhs > 1 else res[0][None]
+    m = parity_mask(dims, 0)
+    nref = 0.0
+    for r in range(nrhs):
+        ref = O.wilson_dslash(dims, links, 0.0, ys[r], sub="even")
+        assert rel_err(got[r], ref, m) < TOL[prec], r
+        nref += float(np.sum(ref[m] ** 2))
+    assert abs(norms[0] - nref) / nref < 4 * TOL[prec]
+
+
 @pytest.mark.parametrize("prec,nrhs", [(0, 1), (1, 16), (3, 8), (0, 8)])
 def test_wilson_every_launch_shape(gpu, prec, nrhs):
     """Every Wilson launch shape the autotuner can pick (block order, links

@@ -202,12 +202,14 @@ def test_mixed_cg_block_zero_rhs(gpu):
     assert _true_rel_resid(dims, links, 0.1, bs[0], xs[0]) <= 1.01e-20
 
 
-@pytest.mark.parametrize("sloppy,nrhs", [(None, 1), (1, 4), (2, 4)])
-def test_wilson_cgne_block(gpu, sloppy, nrhs):
+@pytest.mark.parametrize("sloppy,nrhs,dims", [(None, 1, (4, 4, 4, 8)), (1, 4, (4, 4, 4, 8)), (2, 4, (4, 4, 4, 8)),
+                                            (None, 2, (32, 4, 4, 4)), (1, 16, (32, 4, 4, 4)), (3, 8, (32, 4, 4, 4))])
+def test_wilson_cgne_block(gpu, sloppy, nrhs, dims):
     """CGNE on Mhat = 1 - k^2 H_eo H_oe for a block of right-hand sides, each
     to FP64 true residual 1e-10 (checked with the oracle's Mhat) and to the
-    oracle's own CGNE solution."""
-    dims = (4, 4, 4, 8)
+    oracle's own CGNE solution.  The 32 x 4 x 4 x 4 lattice runs the reducing
+    Dslash launches (fused residual epilogues) in the tiled site order."""
+    print("case: wilson cgne sloppy=%s nrhs=%d dims=%s" % (sloppy, nrhs, dims), flush=True)
     kappa = 0.124
     links = O.wilson_doublestore(dims, O.random_su3_field(dims, 41))
     bs = np.stack([O.gauss_vector(dims, 50 + r, 400, parity=0, nreal=24) for r in range(nrhs)])
