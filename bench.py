@@ -411,6 +411,29 @@ def main():
                       "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
         print("dslash prec=%d: %s" % (prec, prof[prec]), file=sys.stderr, flush=True)
 
+    # the FP64 Dslash called on host arrays, as a caller of the QLL API's
+    # dslash2 uses it: upload the input (lexicographic FP64 host array), one
+    # launch, download the output.  PCIe-inclusive; not the bench value.
+    if 0 in precs:
+        yh = Q.Field(lat, Q.F64, 3, 1, Q.ODD).gaussian(5, 100).get_lex()
+        yin = Q.Field(lat, Q.F64, 3, 1, Q.ODD)
+        out = Q.Field(lat, Q.F64, 3, 1, Q.EVEN)
+        nh = 10
+        yin.set_lex(yh)
+        Q.dslash(L64, out, yin)
+        out.get_lex()
+        Q.sync()
+        t0 = time.perf_counter()
+        for _ in range(nh):
+            yin.set_lex(yh)
+            Q.dslash(L64, out, yin)
+            out.get_lex()
+        Q.sync()
+        ms = (time.perf_counter() - t0) * 1e3 / nh
+        prof["f64_host_arrays"] = {"ms_per_call": round(ms, 3), "host_bytes_each_way": int(yh.nbytes),
+                                   "GFLOP/s": round(FLOPS_ASQTAD * vh / (ms * 1e-3) / 1e9, 1)}
+        print("dslash f64_host_arrays: %s" % prof["f64_host_arrays"], file=sys.stderr, flush=True)
+
     # block Dslash: links loaded once for the whole block of rhs
     nbk = args.block
     for prec, L in ((1, L32), (3, LQ)):
