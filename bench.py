@@ -28,7 +28,10 @@ sys.path.insert(0, ROOT)
 FLOPS_ASQTAD = 1146          # per EO output site
 FLOPS_ASQTAD_QLL = 1152      # QLL's own count (72 per term), reported beside it
 FLOPS_WILSON, FLOPS_WILSON_QLL = 1320, 1344
-ORACLE_CFLAGS = "gcc -O3 -fopenmp -std=gnu11 (oracle/Makefile)"
+ORACLE_CFLAGS = "gcc -O3 -march=native -fopenmp -std=gnu11 (oracle.native_lib, built on this host)"
+# threads of the CPU baseline: the CPUs this process may run on, at most the
+# 16 host cores a one-GPU box shares out per GPU
+CPU_THREADS_CAP = 16
 # algorithmic bytes per EO output site: 16 link matrices + one input and one
 # output colour vector per rhs (16-bit vectors carry a 4-byte site norm)
 CPLX_BYTES = {0: 16, 1: 8, 2: 4, 3: 4}
@@ -63,6 +66,7 @@ def parse():
     ap.add_argument("--config4", action="store_true", help=argparse.SUPPRESS)   # default on; kept for old scripts
     ap.add_argument("--no-wilson", action="store_true", help="skip the 16-rhs Wilson solves")
     ap.add_argument("--no-cgms", action="store_true", help="skip the 12-shift multishift solves")
+    ap.add_argument("--no-qll-api", action="store_true", help="skip the QLL C API (host array) timings")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--kernel-child", type=str, default=None,
                     help="KIND,PREC,NRHS[,MODE]: 20 Dslash launches of one kind (stag or wilson) for a "
@@ -228,19 +232,22 @@ def config4(Q, graph=True):
 def cpu_baseline(dims, nsec_target=10.0):
     """Oracle OpenMP restatement of QLL's dslash2T loop (256-site blocks, 16
     terms), timed on the host cores, on random SU(3) links of the same shape."""
+    import tempfile
     import oracle as O
-    import numpy as np
-    nth = min(os.cpu_count() or 1, 16)
-    os.environ.setdefault("OMP_NUM_THREADS", str(nth))
+    affinity = len(os.sched_getaffinity(0))
+    nth = min(affinity, CPU_THREADS_CAP)
+    os.environ["OMP_NUM_THREADS"] = str(nth)
+    tmp = tempfile.mkdtemp(dir="/tmp")
+    NL = O.native_lib(tmp)
     t0 = time.time()
     u = O.random_su3_field(dims, 2024)
     links = O.stag_doublestore(dims, u, u)     # 16 link terms per site
     del u
     y = O.gauss_vector(dims, 7, 100, parity=1)
     gen_s = time.time() - t0
-    t1, nthreads = O.time_dslash(dims, links, y, 1)      # warm-up + size the sample
+    t1, nthreads = O.time_dslash(dims, links, y, 10, NL)      # warm-up + size the sample
     nrep = max(1, min(5000, int(nsec_target / max(t1, 1e-6))))
-    t, nthreads = O.time_dslash(dims, links, y, nrep)
+    t, nthreads = O.time_dslash(dims, links, y, nrep, NL)
     vh = O.volume(dims) // 2
     model = "unknown"
     try:
@@ -250,6 +257,8 @@ def cpu_baseline(dims, nsec_target=10.0):
                 break
     except OSError:
         pass
+    import shutil
+    shutil.rmtree(tmp, ignore_errors=True)
     return {
         "value": round(FLOPS_ASQTAD * vh / t / 1e9, 3),
         "unit": "GFLOP/s",
@@ -259,8 +268,34 @@ def cpu_baseline(dims, nsec_target=10.0):
             nrep, "x".join(map(str, dims)), t, gen_s),
         "value_qll_flops": round(FLOPS_ASQTAD_QLL * vh / t / 1e9, 3),
         "cpu_model": model,
+        "host_cpus": os.cpu_count(),
+        "affinity_cpus": affinity,
         "build": ORACLE_CFLAGS,
     }
+
+
+def qll_api_times(dims, nrep=20):
+    """dslash2D ("all" and "even") and solve2D through the QLL C API on host
+    arrays in QLL layout, run by qll_amd/lib/qll_api_bench.bin, with the
+    host <-> device copy rates that bound them: pcie_ms is the time the
+    call's host bytes need at the pinned DMA rates (half up, half down)."""
+    import subprocess
+    exe = os.path.join(ROOT, "qll_amd", "lib", "qll_api_bench.bin")
+    if not os.path.exists(exe):
+        return {"error": "qll_api_bench.bin is not built"}
+    r = subprocess.run([exe] + [str(d) for d in dims] + [str(nrep), str(QLL_MASS)], capture_output=True,
+                       text=True, timeout=600)
+    if r.returncode != 0:
+        return {"error": "qll_api_bench exited %d: %s" % (r.returncode, r.stdout[-300:] + r.stderr[-300:])}
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    c = out["copy_gbs"]
+    for k in ("all", "even"):
+        nb = out["dslash2D_%s_host_bytes" % k]
+        pcie = (nb / 2) / (c["pinned_h2d"] * 1e9) + (nb / 2) / (c["pinned_d2h"] * 1e9)
+        out["dslash2D_%s_pcie_ms" % k] = round(pcie * 1e3, 4)
+        out["dslash2D_%s_vs_pcie" % k] = round(out["dslash2D_%s_ms" % k] / (pcie * 1e3), 3)
+    print("qll api: %s" % out, file=sys.stderr, flush=True)
+    return out
 
 
 def cgms_solves(Q, dims=(32, 32, 32, 64)):
@@ -434,6 +469,10 @@ def main():
                                    "GFLOP/s": round(FLOPS_ASQTAD * vh / (ms * 1e-3) / 1e9, 1)}
         print("dslash f64_host_arrays: %s" % prof["f64_host_arrays"], file=sys.stderr, flush=True)
 
+    # the QLL C API on host arrays in QLL layout (dslash2D, solve2D), timed by
+    # the C driver built beside the library; PCIe-inclusive, not the value
+    qll_api = None if args.no_qll_api else qll_api_times(dims)
+
     # block Dslash: links loaded once for the whole block of rhs
     nbk = args.block
     for prec, L in ((1, L32), (3, LQ)):
@@ -586,6 +625,7 @@ def main():
                      "frac_of_copy": round(achieved / copy_gbs, 4)},
         "cpu_baseline": cpu,
         "dslash": {(PREC_NAME[p] if p in PREC_NAME else p): prof[p] for p in prof},
+        "qll_api": qll_api,
         "solvers": solver,
         "wilson_cgne_16rhs": wil,
         "cgms_12_shifts": cgms,

@@ -56,4 +56,47 @@ void qllSetVlen(int vlenF, int vlenD);
 #ifdef __cplusplus
 }
 #endif
+
+/* Generic names (include/qll.h:29-58): Field, fieldNew, transporterDo, ...
+ * resolve to the F or D function through P(x), which vla.h (included by
+ * common.h) selects with -DDOUBLE; the team macros open an OpenMP team with
+ * a subset's outer-site range split over the threads (ti0, ti1). */
+#ifndef QLL_NO_GENERIC_NAMES
+#define beginParallel BEGIN_PARALLEL
+#define beginParallelSubset(s) \
+  beginParallel {              \
+    SUBSET_DEFS2;              \
+    setOffSub2(s);
+#define endParallel END_PARALLEL
+#define endParallelSubset \
+  }                       \
+  endParallel
+#define singleThread _Pragma("omp single")
+
+#define Field P(Field)
+#define fieldNew P(fieldNew)
+#define fieldFree P(fieldFree)
+#define fieldSet P(fieldSet)
+#define fieldZero P(fieldZero)
+#define fieldSumElem P(fieldSumElem)
+#define fieldNorm2 P(fieldNorm2)
+
+#define Transporter P(Transporter)
+#define transporterNew P(transporterNew)
+#define transporterFree P(transporterFree)
+#define transporterStartT P(transporterStartT)
+#define transporterLocalT P(transporterLocalT)
+#define transporterWaitT P(transporterWaitT)
+#define transporterDoT P(transporterDoT)
+#define transporterDo P(transporterDo)
+
+#define gaugeDoublestore P(gaugeDoublestore)
+#define gaugePlaq P(gaugePlaq)
+#define smearFat7 P(smearFat7)
+
+/* set every real of field ff on the thread's range to r; its squared norm */
+#define F_eq_r(ff, r) X_eq_r((ff)->f, (r), (ff)->nelem, ti0, ti1)
+#define lnorm2_F(ff) norm2_X((ff)->f, (ff)->nelem, ti0, ti1)
+#define sumDoubleArray(x, n) gsum(x, n, TC)
+#endif
 #endif

@@ -189,7 +189,9 @@ typedef struct {
 
 /* QLL CG semantics (cgSolveT lib/cg.c:43-122 with dslashp2): solves
  * (m^2 - D_eo D_oe) x = b on even sites to |r|^2 < rsq |b|^2 (iterated),
- * all vectors at the links' precision, x starts at 0.  Not rescaled by mass. */
+ * all vectors at the links' precision (FP64 or FP32; 16-bit links are
+ * rejected: they run inside qamd_stag_cg_mixed), x starts at 0.  Not
+ * rescaled by mass. */
 int qamd_stag_cg(const qamd_links *l, qamd_field *x, const qamd_field *b, double mass,
                  double rsq, int maxits, qamd_solve_info *info);
 /* Mixed precision CG with reliable updates (no reference counterpart):
@@ -299,6 +301,11 @@ int qamd_time_dslash(const qamd_links *l, qamd_field *out, const qamd_field *in,
 /* streaming copy of 'bytes' repeated nrep times: read+write GB/s and ms per
  * copy (the measured-bandwidth reference; no reference counterpart) */
 int qamd_stream_copy(size_t bytes, int nrep, double *gbs, double *ms_per_copy);
+/* host <-> device copy rates of 'bytes' in GB/s (no reference counterpart):
+ * rates[0] pinned->device, [1] device->pinned (DMA alone), [2] pageable->
+ * device, [3] device->pageable (the library's staging pipeline, the path of
+ * the QLL API's host arrays) */
+int qamd_host_copy_rate(size_t bytes, int nrep, double rates[4]);
 
 #ifdef __cplusplus
 }

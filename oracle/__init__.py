@@ -26,56 +26,70 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
+def native_lib(tmpdir):
+    """The oracle compiled for this host's CPU (gcc -O3 -march=native
+    -fopenmp) into tmpdir, for the CPU baseline timing; the shipped
+    liboracle.so is built portable."""
+    path = os.path.join(tmpdir, "liboracle_native.so")
+    subprocess.run(["gcc", "-O3", "-march=native", "-fopenmp", "-fPIC", "-std=gnu11", "-shared", "-o", path,
+                    os.path.join(_HERE, "oracle.c"), "-lm"], check=True)
+    return _configure(C.CDLL(path))
+
+
 def lib():
     global _LIB
     if _LIB is None:
         path = os.path.join(_HERE, "liboracle.so")
         if not os.path.exists(path):
             build()
-        L = C.CDLL(path)
-        i4 = C.POINTER(C.c_int)
-        u64 = C.c_uint64
-        L.orc_rng64.restype = u64
-        L.orc_rng64.argtypes = [u64, u64, u64, u64]
-        L.orc_uniform.restype = C.c_double
-        L.orc_uniform.argtypes = [u64, u64, u64, u64]
-        L.orc_random_su3_field.argtypes = [i4, u64, _d]
-        L.orc_apply_stag_phases.argtypes = [i4, _d]
-        L.orc_gauss_vector.argtypes = [i4, u64, u64, C.c_int, C.c_int, _d]
-        L.orc_su3_from_gauss.argtypes = [_d, _d]
-        L.orc_fat7.argtypes = [i4, _d, _d, _d, _d, C.c_void_p]
-        L.orc_stag_doublestore.argtypes = [i4, _d, C.c_void_p, _d]
-        L.orc_wilson_doublestore.argtypes = [i4, _d, _d]
-        L.orc_plaquette.argtypes = [i4, _d, _d]
-        L.orc_dslash.argtypes = [i4, _d, C.c_int, C.c_double, _d, _d, C.c_int]
-        L.orc_normal_op.restype = C.c_double
-        L.orc_normal_op.argtypes = [i4, _d, C.c_int, C.c_double, _d, _d, _d]
-        L.orc_cg.restype = C.c_int
-        L.orc_cg.argtypes = [i4, _d, C.c_int, C.c_double, _d, _d, C.c_double,
-                             C.c_int, C.c_int, C.POINTER(C.c_double)]
-        L.orc_cgms.restype = C.c_int
-        L.orc_cgms.argtypes = [i4, _d, C.c_int, _d, C.c_int, _d,
-                               C.POINTER(C.c_void_p), C.c_double, C.c_int, C.c_int]
-        L.orc_norm2_parity.restype = C.c_double
-        L.orc_norm2_parity.argtypes = [i4, _d, C.c_int, C.c_int]
-        L.orc_gamma.argtypes = [C.c_int, _d]
-        L.orc_wilson_dslash.argtypes = [i4, _d, C.c_double, _d, _d, C.c_int]
-        L.orc_wilson_meo.argtypes = [i4, _d, C.c_double, _d, _d, C.c_int, _d]
-        L.orc_wilson_cgne.restype = C.c_int
-        L.orc_wilson_cgne.argtypes = [i4, _d, C.c_double, _d, _d, C.c_double,
-                                      C.c_int, C.POINTER(C.c_double)]
-        L.orc_layout_setup.restype = C.c_int
-        L.orc_layout_setup.argtypes = [C.c_void_p, i4, i4]
-        L.orc_layout_index.restype = C.c_int
-        L.orc_layout_index.argtypes = [C.c_void_p, i4]
-        L.orc_layout_coord.argtypes = [C.c_void_p, i4, C.c_int]
-        L.orc_to_qll.argtypes = [C.c_void_p, _d, _d, C.c_int]
-        L.orc_from_qll.argtypes = [C.c_void_p, _d, _d, C.c_int]
-        L.orc_time_dslash.restype = C.c_double
-        L.orc_time_dslash.argtypes = [i4, _d, C.c_int, _d, _d, C.c_int,
-                                      C.POINTER(C.c_int)]
-        _LIB = L
+        _LIB = _configure(C.CDLL(path))
     return _LIB
+
+
+def _configure(L):
+    """Argument and result types of the oracle's C functions."""
+    i4 = C.POINTER(C.c_int)
+    u64 = C.c_uint64
+    L.orc_rng64.restype = u64
+    L.orc_rng64.argtypes = [u64, u64, u64, u64]
+    L.orc_uniform.restype = C.c_double
+    L.orc_uniform.argtypes = [u64, u64, u64, u64]
+    L.orc_random_su3_field.argtypes = [i4, u64, _d]
+    L.orc_apply_stag_phases.argtypes = [i4, _d]
+    L.orc_gauss_vector.argtypes = [i4, u64, u64, C.c_int, C.c_int, _d]
+    L.orc_su3_from_gauss.argtypes = [_d, _d]
+    L.orc_fat7.argtypes = [i4, _d, _d, _d, _d, C.c_void_p]
+    L.orc_stag_doublestore.argtypes = [i4, _d, C.c_void_p, _d]
+    L.orc_wilson_doublestore.argtypes = [i4, _d, _d]
+    L.orc_plaquette.argtypes = [i4, _d, _d]
+    L.orc_dslash.argtypes = [i4, _d, C.c_int, C.c_double, _d, _d, C.c_int]
+    L.orc_normal_op.restype = C.c_double
+    L.orc_normal_op.argtypes = [i4, _d, C.c_int, C.c_double, _d, _d, _d]
+    L.orc_cg.restype = C.c_int
+    L.orc_cg.argtypes = [i4, _d, C.c_int, C.c_double, _d, _d, C.c_double,
+                         C.c_int, C.c_int, C.POINTER(C.c_double)]
+    L.orc_cgms.restype = C.c_int
+    L.orc_cgms.argtypes = [i4, _d, C.c_int, _d, C.c_int, _d,
+                           C.POINTER(C.c_void_p), C.c_double, C.c_int, C.c_int]
+    L.orc_norm2_parity.restype = C.c_double
+    L.orc_norm2_parity.argtypes = [i4, _d, C.c_int, C.c_int]
+    L.orc_gamma.argtypes = [C.c_int, _d]
+    L.orc_wilson_dslash.argtypes = [i4, _d, C.c_double, _d, _d, C.c_int]
+    L.orc_wilson_meo.argtypes = [i4, _d, C.c_double, _d, _d, C.c_int, _d]
+    L.orc_wilson_cgne.restype = C.c_int
+    L.orc_wilson_cgne.argtypes = [i4, _d, C.c_double, _d, _d, C.c_double,
+                                  C.c_int, C.POINTER(C.c_double)]
+    L.orc_layout_setup.restype = C.c_int
+    L.orc_layout_setup.argtypes = [C.c_void_p, i4, i4]
+    L.orc_layout_index.restype = C.c_int
+    L.orc_layout_index.argtypes = [C.c_void_p, i4]
+    L.orc_layout_coord.argtypes = [C.c_void_p, i4, C.c_int]
+    L.orc_to_qll.argtypes = [C.c_void_p, _d, _d, C.c_int]
+    L.orc_from_qll.argtypes = [C.c_void_p, _d, _d, C.c_int]
+    L.orc_time_dslash.restype = C.c_double
+    L.orc_time_dslash.argtypes = [i4, _d, C.c_int, _d, _d, C.c_int,
+                                  C.POINTER(C.c_int)]
+    return L
 
 
 def _L(dims):
@@ -278,11 +292,12 @@ def from_qll(lo, q, nelem):
     return nat
 
 
-def time_dslash(dims, links, y, nrep):
-    """Seconds per EO Dslash call of the OpenMP CPU restatement."""
+def time_dslash(dims, links, y, nrep, L=None):
+    """Seconds per EO Dslash call of the OpenMP CPU restatement (library L,
+    default the shipped oracle)."""
     x = np.zeros_like(y)
     nth = C.c_int()
-    t = lib().orc_time_dslash(_L(dims), np.ascontiguousarray(links).reshape(-1),
+    t = (L or lib()).orc_time_dslash(_L(dims), np.ascontiguousarray(links).reshape(-1),
                               links.shape[0], np.ascontiguousarray(y).reshape(-1),
                               x.reshape(-1), nrep, C.byref(nth))
     return t, nth.value

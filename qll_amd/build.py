@@ -4,7 +4,8 @@
 
 HIP sources (csrc/*.hip) are compiled with hipcc for gfx950 only; the QLL
 C API (csrc/host/*.c) with gcc as C99/C11; everything is linked into
-qll_amd/lib/libqll_amd.so, which travels with the source tree.
+qll_amd/lib/libqll_amd.so, which travels with the source tree.  The QLL-API
+timing driver (csrc/bench/qll_api_bench.c) is built beside it.
 """
 import concurrent.futures as cf
 import glob
@@ -55,9 +56,29 @@ def build(verbose=False, jobs=8):
                 sys.stderr.write(err)
     objs = [c[c.index("-o") + 1] for c in jobs_list]
     tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs + ["-fopenmp", "-lm"])
+    # The OpenMP teams of the C host API (BEGIN_PARALLEL, gsum, and the
+    # caller's Linop callbacks that run inside them) must use the same
+    # runtime as the QLL programs calling it, which gcc -fopenmp builds
+    # against GNU libgomp; hipcc's -fopenmp would link LLVM's libomp, whose
+    # teams do not see the caller's libgomp barriers.
+    gomp = subprocess.run(["gcc", "-print-file-name=libgomp.so"], capture_output=True, text=True,
+                          check=True).stdout.strip()
+    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs + [gomp, "-lm"])
     os.replace(tmp, LIB)
+    build_api_bench()
     return LIB
+
+
+API_BENCH_SRC = os.path.join(CSRC, "bench", "qll_api_bench.c")
+API_BENCH = os.path.join(LIBDIR, "qll_api_bench.bin")
+
+
+def build_api_bench():
+    """The QLL-API timing driver bench.py runs (a C program against the QLL
+    headers and libqll_amd.so, found next to it through its rpath)."""
+    _run(["gcc"] + CFLAGS + ["-I", os.path.join(ROOT, "include"), "-o", API_BENCH, API_BENCH_SRC,
+                             "-L", LIBDIR, "-lqll_amd", "-Wl,-rpath,$ORIGIN", "-lm"])
+    return API_BENCH
 
 
 if __name__ == "__main__":

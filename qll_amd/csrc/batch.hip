@@ -456,6 +456,37 @@ int Runner<PS>::run(const qamd_stag_problem *pr, int n, std::vector<ProbDev> &H,
   };
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // Capture BITS iterations into a graph.  On any failure the stream is
+  // taken out of capture mode, partial graph objects are destroyed and false
+  // is returned, so the solve continues with eager launches and every later
+  // library call runs normally.  QAMD_TEST_FAIL_CAPTURE=1 makes instantiation
+  // report a failure (host-side; tests the recovery without a GPU fault).
+  auto capture = [&](hipGraph_t &g, hipGraphExec_t &x) -> bool {
+    g = nullptr;
+    x = nullptr;
+    hipError_t e = hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+      for (int k = 0; k < BITS; k++) iterate(stream());
+      e = hipStreamEndCapture(stream(), &g);
+      const char *inj = getenv("QAMD_TEST_FAIL_CAPTURE");
+      if (e == hipSuccess && inj && inj[0] == '1') e = hipErrorInvalidValue;
+      if (e == hipSuccess) e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    }
+    if (e == hipSuccess) return true;
+    set_error("cg_mixed_batch: graph capture: %s", hipGetErrorString(e));
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(stream(), &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) {
+      hipGraph_t g2 = nullptr;
+      (void)hipStreamEndCapture(stream(), &g2);
+      if (g2) (void)hipGraphDestroy(g2);
+    }
+    if (x) (void)hipGraphExecDestroy(x);
+    if (g) (void)hipGraphDestroy(g);
+    x = nullptr;
+    g = nullptr;
+    (void)hipGetLastError();
+    return false;
+  };
   int rc = -1;
   std::vector<int> its(n, 0), nupd(n, 0), nrst(n, 0);
   std::vector<double> bb(n * BR, 0.0), r2(n * BR, 0.0);
@@ -485,12 +516,8 @@ int Runner<PS>::run(const qamd_stag_problem *pr, int n, std::vector<ProbDev> &H,
     for (int i = 0; i < n; i++)
       for (int r = 0; r < BR; r++) hp[i].reset[r] = 0;
     if (push()) break;
-    if (use_graph && nact) {   // capture the 16-iteration step once
-      QAMD_HIP(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal));
-      for (int k = 0; k < BITS; k++) iterate(stream());
-      QAMD_HIP(hipStreamEndCapture(stream(), &graph));
-      QAMD_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-    }
+    if (use_graph && nact && !capture(graph, exec) && verbose)
+      fprintf(stderr, "cg_batch: graph capture failed (%s); eager launches\n", qamd_last_error());
     int err = 0;
     while (nact > 0 && !err) {
       if (exec) {

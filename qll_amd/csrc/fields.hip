@@ -289,14 +289,8 @@ int qamd_field_copy(qamd_field *dst, const qamd_field *src) {
 }
 
 static int import_common(qamd_field *f, const void *host, size_t bytes, int hprec, SrcMap m) {
-  void *stage = dev_alloc(bytes);
-  if (!stage) return -1;
-  hipError_t e = hipMemcpyAsync(stage, host, bytes, hipMemcpyHostToDevice, stream());
-  if (e != hipSuccess) {
-    dev_free(stage);
-    set_error("import copy: %s", hipGetErrorString(e));
-    return -1;
-  }
+  void *stage = stage_dev(bytes);
+  if (!stage || h2d_pageable(stage, host, bytes)) return -1;
   dim3 grid = site_grid(f->lat->g, f->f.nrhs);
   const bool c3 = f->f.ncomp == 3;
   if (hprec == QAMD_F64)
@@ -305,9 +299,7 @@ static int import_common(qamd_field *f, const void *host, size_t bytes, int hpre
   else
     hipLaunchKernelGGL((c3 ? k_import<float, 3> : k_import<float, 12>), grid, dim3(256), 0, stream(), f->f,
                        (const float *)stage, m);
-  e = hipGetLastError();
-  if (e == hipSuccess) e = hipStreamSynchronize(stream());
-  dev_free(stage);
+  hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     set_error("import kernel: %s", hipGetErrorString(e));
     return -1;
@@ -315,31 +307,26 @@ static int import_common(qamd_field *f, const void *host, size_t bytes, int hpre
   return 0;
 }
 
+// prefill: the staging buffer first receives the host array, so that host
+// elements the export does not write (the other parity of a lexicographic
+// array) keep their values
 static int export_common(const qamd_field *f, void *host, size_t bytes, int hprec, SrcMap m, int prefill) {
-  void *stage = dev_alloc(bytes);
-  if (!stage) return -1;
-  hipError_t e = hipSuccess;
-  if (prefill) e = hipMemcpyAsync(stage, host, bytes, hipMemcpyHostToDevice, stream());
+  void *stage = stage_dev(bytes);
+  if (!stage || (prefill && h2d_pageable(stage, host, bytes))) return -1;
   dim3 grid = site_grid(f->lat->g, f->f.nrhs);
-  if (e == hipSuccess) {
-    const bool c3 = f->f.ncomp == 3;
-    if (hprec == QAMD_F64)
-      hipLaunchKernelGGL((c3 ? k_export<double, 3> : k_export<double, 12>), grid, dim3(256), 0, stream(), f->f,
-                         (double *)stage, m);
-    else
-      hipLaunchKernelGGL((c3 ? k_export<float, 3> : k_export<float, 12>), grid, dim3(256), 0, stream(), f->f,
-                         (float *)stage, m);
-    e = hipGetLastError();
-  }
-  if (e == hipSuccess) e = hipStreamSynchronize(stream());
-  if (e == hipSuccess) e = hipMemcpyAsync(host, stage, bytes, hipMemcpyDeviceToHost, stream());
-  if (e == hipSuccess) e = hipStreamSynchronize(stream());
-  dev_free(stage);
+  const bool c3 = f->f.ncomp == 3;
+  if (hprec == QAMD_F64)
+    hipLaunchKernelGGL((c3 ? k_export<double, 3> : k_export<double, 12>), grid, dim3(256), 0, stream(), f->f,
+                       (double *)stage, m);
+  else
+    hipLaunchKernelGGL((c3 ? k_export<float, 3> : k_export<float, 12>), grid, dim3(256), 0, stream(), f->f,
+                       (float *)stage, m);
+  hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
-    set_error("export: %s", hipGetErrorString(e));
+    set_error("export kernel: %s", hipGetErrorString(e));
     return -1;
   }
-  return 0;
+  return d2h_pageable(host, stage, bytes);
 }
 
 int qamd_field_from_lex(qamd_field *f, const double *host) {
@@ -423,6 +410,7 @@ static int sites_xfer(qamd_field *f, const int *coords, int64_t nsites, void *ho
   size_t bytes = (size_t)nsites * m.nreal * (hprec == QAMD_F64 ? 8 : 4);
   // export: sites of the other parity keep the caller's values
   int rc = to_host ? export_common(f, host, bytes, hprec, m, 1) : import_common(f, host, bytes, hprec, m);
+  (void)hipStreamSynchronize(stream());   // the import kernel reads inv
   dev_free(inv);
   return rc;
 }

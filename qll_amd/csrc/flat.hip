@@ -110,40 +110,29 @@ int qamd_qll_shift(const qamd_qll_layout *l, const int disp[4], int nelem, int n
   int64_t V = (int64_t)l->physGeom[0] * l->physGeom[1] * l->physGeom[2] * l->physGeom[3];
   size_t esz = hprec == QAMD_F64 ? 8 : 4;
   size_t bytes = (size_t)V * nelem * esz, ubytes = (size_t)V * 2 * nc * nc * esz;
-  void *ds = dev_alloc(bytes), *dd = dev_alloc(bytes), *du = nullptr;
-  if (u && td > 0) du = dev_alloc(ubytes);
-  if (!ds || !dd || (u && td > 0 && !du)) {
-    dev_free(ds);
-    dev_free(dd);
-    dev_free(du);
+  const bool mat = u && td > 0;
+  // one device buffer for source, destination and matrices (grown as needed)
+  size_t off_d = (bytes + 255) & ~(size_t)255, off_u = 2 * off_d;
+  char *buf = (char *)stage_dev(off_u + (mat ? ubytes : 0));
+  if (!buf) return -1;
+  void *ds = buf, *dd = buf + off_d, *du = mat ? buf + off_u : nullptr;
+  if (h2d_pageable(ds, src, bytes) || h2d_pageable(dd, dest, bytes) || (mat && h2d_pageable(du, u, ubytes)))
     return -1;
-  }
-  int rc = 0;
-  hipError_t e = hipMemcpyAsync(ds, src, bytes, hipMemcpyHostToDevice, stream());
-  if (e == hipSuccess) e = hipMemcpyAsync(dd, dest, bytes, hipMemcpyHostToDevice, stream());
-  if (e == hipSuccess && du) e = hipMemcpyAsync(du, u, ubytes, hipMemcpyHostToDevice, stream());
-  if (e == hipSuccess) {
-    unsigned nb = (unsigned)((V + 255) / 256);
-    if (hprec == QAMD_F64)
-      hipLaunchKernelGGL(k_qll_shift<double>, dim3(nb), dim3(256), 0, stream(), lay, disp[0], disp[1], disp[2],
-                         disp[3], nelem, nc, td, (const double *)du, (const double *)ds, (double *)dd, beginOuter,
-                         endOuter, accumulate, adjoint);
-    else
-      hipLaunchKernelGGL(k_qll_shift<float>, dim3(nb), dim3(256), 0, stream(), lay, disp[0], disp[1], disp[2],
-                         disp[3], nelem, nc, td, (const float *)du, (const float *)ds, (float *)dd, beginOuter,
-                         endOuter, accumulate, adjoint);
-    e = hipGetLastError();
-  }
-  if (e == hipSuccess) e = hipMemcpyAsync(dest, dd, bytes, hipMemcpyDeviceToHost, stream());
-  if (e == hipSuccess) e = hipStreamSynchronize(stream());
+  unsigned nb = (unsigned)((V + 255) / 256);
+  if (hprec == QAMD_F64)
+    hipLaunchKernelGGL(k_qll_shift<double>, dim3(nb), dim3(256), 0, stream(), lay, disp[0], disp[1], disp[2],
+                       disp[3], nelem, nc, td, (const double *)du, (const double *)ds, (double *)dd, beginOuter,
+                       endOuter, accumulate, adjoint);
+  else
+    hipLaunchKernelGGL(k_qll_shift<float>, dim3(nb), dim3(256), 0, stream(), lay, disp[0], disp[1], disp[2],
+                       disp[3], nelem, nc, td, (const float *)du, (const float *)ds, (float *)dd, beginOuter,
+                       endOuter, accumulate, adjoint);
+  hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     set_error("qll_shift: %s", hipGetErrorString(e));
-    rc = -1;
+    return -1;
   }
-  dev_free(ds);
-  dev_free(dd);
-  dev_free(du);
-  return rc;
+  return d2h_pageable(dest, dd, bytes);
 }
 
 static int flat_reduce(int64_t n, int prec, const void *x, const void *y, int mode, double *out) {
@@ -168,15 +157,9 @@ static int flat_reduce(int64_t n, int prec, const void *x, const void *y, int mo
 int qamd_host_sum(const void *host, int64_t n, int hprec, int squares, double *out) {
   if (ensure_init()) return -1;
   size_t bytes = (size_t)n * (hprec == QAMD_F64 ? 8 : 4);
-  void *d = dev_alloc(bytes);
-  if (!d) return -1;
-  int rc = -1;
-  if (hipMemcpyAsync(d, host, bytes, hipMemcpyHostToDevice, stream()) == hipSuccess)
-    rc = flat_reduce(n, hprec, d, nullptr, squares ? 1 : 0, out);
-  else
-    set_error("host_sum: copy failed");
-  dev_free(d);
-  return rc;
+  void *d = stage_dev(bytes);
+  if (!d || h2d_pageable(d, host, bytes)) return -1;
+  return flat_reduce(n, hprec, d, nullptr, squares ? 1 : 0, out);
 }
 
 int qamd_flat_axpby(int64_t n, int prec, double a, const void *x, double b, void *y) {

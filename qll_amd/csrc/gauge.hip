@@ -374,18 +374,16 @@ void qamd_gauge_destroy(qamd_gauge *u) {
 static int gauge_xfer_lex(qamd_gauge *u, double *host, int to_host) {
   const Geom &g = u->lat->g;
   size_t bytes = (size_t)4 * g.V * 18 * sizeof(double);
-  double *st = (double *)dev_alloc(bytes);
+  double *st = (double *)stage_dev(bytes);
   if (!st) return -1;
   SrcMap m;
   memset(&m, 0, sizeof m);
-  if (!to_host) QAMD_HIP(hipMemcpyAsync(st, host, bytes, hipMemcpyHostToDevice, stream()));
+  if (!to_host && h2d_pageable(st, host, bytes)) return -1;
   for (int mu = 0; mu < 4; mu++)
     hipLaunchKernelGGL(k_gauge_xfer<double>, dim3((unsigned)((g.vh + 255) / 256), 2), dim3(256), 0, stream(),
                        u->d, st, g, mu, m, to_host, 0);
   QAMD_HIP(hipGetLastError());
-  if (to_host) QAMD_HIP(hipMemcpyAsync(host, st, bytes, hipMemcpyDeviceToHost, stream()));
-  QAMD_HIP(hipStreamSynchronize(stream()));
-  dev_free(st);
+  if (to_host) return d2h_pageable(host, st, bytes);
   return 0;
 }
 
@@ -401,24 +399,23 @@ static int gauge_xfer_qll(qamd_gauge *u, const qamd_qll_layout *l, void *const h
     }
   size_t esz = hprec == QAMD_F64 ? 8 : 4;
   size_t bytes = (size_t)g.V * 18 * esz;
-  void *st = dev_alloc(bytes);
+  void *st = stage_dev(bytes);
   if (!st) return -1;
   SrcMap m;
   memset(&m, 0, sizeof m);
   m.kind = 1;
   m.lay = to_lay(l);
   for (int mu = 0; mu < 4; mu++) {
-    if (!to_host) QAMD_HIP(hipMemcpyAsync(st, host[mu], bytes, hipMemcpyHostToDevice, stream()));
+    // stream order: the upload of direction mu + 1 waits for kernel mu
+    if (!to_host && h2d_pageable(st, host[mu], bytes)) return -1;
     dim3 grid((unsigned)((g.vh + 255) / 256), 2);
     if (hprec == QAMD_F64)
       hipLaunchKernelGGL(k_gauge_xfer<double>, grid, dim3(256), 0, stream(), u->d, (double *)st, g, mu, m, to_host, 0);
     else
       hipLaunchKernelGGL(k_gauge_xfer<float>, grid, dim3(256), 0, stream(), u->d, (float *)st, g, mu, m, to_host, 0);
     QAMD_HIP(hipGetLastError());
-    if (to_host) QAMD_HIP(hipMemcpyAsync(host[mu], st, bytes, hipMemcpyDeviceToHost, stream()));
-    QAMD_HIP(hipStreamSynchronize(stream()));
+    if (to_host && d2h_pageable(host[mu], st, bytes)) return -1;
   }
-  dev_free(st);
   return 0;
 }
 

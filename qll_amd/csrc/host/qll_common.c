@@ -9,7 +9,8 @@
 #include <stdlib.h>
 #include <string.h>
 
-#include "qll.h"
+#define QLL_NO_GENERIC_NAMES
+#include "common.h"
 #include "qll_amd.h"
 #include "qll_host.h"
 
@@ -206,6 +207,72 @@ void layoutShift(Layout *l, LayoutIndex *li, LayoutIndex *li2, int *disp)
   layoutCoord(l, x, li2);
   for (int i = 0; i < nd; i++) x[i] = (x[i] + disp[i] + l->physGeom[i]) % l->physGeom[i];
   layoutIndex(l, li, x);
+}
+
+/* makeShiftMultiSub (lib/shifts.c:516-736) for one rank: si[n] describes
+ * dst(x) = src(x - disp[n]) on the outer sites of subs[n].  libqll_amd's
+ * shifts and transporters only read disp (the device computes neighbours
+ * from coordinates); the site tables are filled for callers that inspect
+ * them, as QLL fills them when nothing crosses a rank: sidx[i] = the outer
+ * site holding the source of lane 0 of outer site i, pidx[i] the same or
+ * -(that)-2 when the lanes arrive permuted (perm = the source lane of lane
+ * 0), -1 outside the subset.  No message member is set. */
+void makeShiftMultiSub(ShiftIndices *si[], Layout *l, int *disp[], char *subs[], int ndisp)
+{
+  int nd = l->nDim, vv = l->nSitesOuter, NI = l->nSitesInner;
+  int perm = 0;
+  for (int n = 0; n < ndisp; n++) {
+    ShiftIndices *s = si[n];
+    memset(s, 0, sizeof *s);
+    s->disp = myalloc(nd * sizeof(int));
+    memcpy(s->disp, disp[n], nd * sizeof(int));
+    s->pidx = myalloc(vv * sizeof(int));
+    s->sidx = myalloc(vv * sizeof(int));
+    s->sendSites = myalloc(vv * sizeof(int));
+    s->vv = vv;
+    Subset sub;
+    layoutSubset(&sub, l, subs[n]);
+    int back[8];
+    for (int k = 0; k < nd; k++) back[k] = ((-disp[n][k]) % l->physGeom[k] + l->physGeom[k]) % l->physGeom[k];
+    for (int i = 0; i < vv; i++) {
+      s->pidx[i] = s->sidx[i] = -1;
+      if (i < sub.beginOuter || i >= sub.endOuter) continue;
+      LayoutIndex dst = {0, i * NI}, src;
+      layoutShift(l, &src, &dst, back);
+      int so = src.index / NI, lane = src.index % NI;
+      s->sidx[i] = so;
+      s->pidx[i] = so;
+      if (lane != 0) {
+        perm = lane;
+        s->pidx[i] = -so - 2;
+      }
+    }
+  }
+  for (int n = 0; n < ndisp; n++) si[n]->perm = perm;
+}
+
+void makeShiftMulti(ShiftIndices *si[], Layout *l, int *disp[], int ndisp)
+{
+  char **subs = malloc((ndisp > 0 ? ndisp : 1) * sizeof(char *));
+  for (int k = 0; k < ndisp; k++) subs[k] = "all";
+  makeShiftMultiSub(si, l, disp, subs, ndisp);
+  free(subs);
+}
+
+void makeShiftSub(ShiftIndices *si, Layout *l, int *disp, char *sub)
+{
+  makeShiftMultiSub(&si, l, &disp, &sub, 1);
+}
+
+void makeShift(ShiftIndices *si, Layout *l, int *disp) { makeShiftSub(si, l, disp, "all"); }
+
+void freeShift(ShiftIndices *si)
+{
+  free(si->disp);
+  free(si->pidx);
+  free(si->sidx);
+  free(si->sendSites);
+  si->disp = si->pidx = si->sidx = si->sendSites = NULL;
 }
 
 /* partitionGeom (lib/stagDslash.c:751-786) */

@@ -302,14 +302,17 @@ static int import_terms(qamd_links *l, const void *const terms[], int hprec, Src
     dst = qamd_links_create(l->lat, l->kind, QAMD_F64, l->nterms);
     if (!dst) return -1;
   }
-  void *stage = dev_alloc(term_bytes);
-  if (!stage) return -1;
+  void *stage = stage_dev(term_bytes);
+  if (!stage) {
+    if (dst != l) qamd_links_destroy(dst);
+    return -1;
+  }
   const Geom &g = l->lat->g;
   dim3 grid((unsigned)((g.vh + 255) / 256), 2);
   int rc = 0;
   for (int t = 0; t < l->nterms && !rc; t++) {
-    if (hipMemcpyAsync(stage, terms[t], term_bytes, hipMemcpyHostToDevice, stream()) != hipSuccess) {
-      set_error("links import: copy failed");
+    // stream order: the upload of term t + 1 waits for the import kernel of t
+    if (h2d_pageable(stage, terms[t], term_bytes)) {
       rc = -1;
       break;
     }
@@ -328,12 +331,15 @@ static int import_terms(qamd_links *l, const void *const terms[], int hprec, Src
         hipLaunchKernelGGL((k_link_import<float, float2>), grid, dim3(256), 0, stream(), (float2 *)dst->data[0],
                            (float2 *)dst->data[1], (const float *)stage, g, t, l->nterms, m);
     }
-    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(stream()) != hipSuccess) {
+    if (hipGetLastError() != hipSuccess) {
       set_error("links import kernel failed");
       rc = -1;
     }
   }
-  dev_free(stage);
+  if (!rc && hipStreamSynchronize(stream()) != hipSuccess) {
+    set_error("links import kernel failed");
+    rc = -1;
+  }
   if (dst != l) {   // 16-bit: checked in FP64, then converted (the copy inherits the result)
     if (!rc) rc = long_link_check(dst);
     if (!rc) rc = qamd_links_convert(l, dst);

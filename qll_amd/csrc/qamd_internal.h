@@ -43,6 +43,14 @@ hipStream_t stream();
 int ensure_init();
 void *dev_alloc(size_t bytes);   // returns NULL and sets the error on failure
 void dev_free(void *p);
+// transfers of pageable host arrays (hostio.hip): a persistent device staging
+// buffer of at least `bytes` (valid until the next stage_dev call), and
+// chunked copies through pinned buffers on the library stream.  h2d returns
+// once the host data has been read (the DMA may still be in flight, ordered
+// on the stream); d2h returns with the host data complete.
+void *stage_dev(size_t bytes);
+int h2d_pageable(void *dst_dev, const void *src_host, size_t bytes);
+int d2h_pageable(void *dst_host, const void *src_dev, size_t bytes);
 
 // ---------------------------------------------- fast division by invariants
 // Quotient of a non-negative 31-bit integer by a fixed divisor with one

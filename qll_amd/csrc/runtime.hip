@@ -128,17 +128,14 @@ void *qamd_malloc(size_t bytes) { return dev_alloc(bytes); }
 void qamd_free(void *p) { dev_free(p); }
 
 int qamd_memcpy_h2d(void *dst, const void *src, size_t bytes) {
-  if (ensure_init()) return -1;
-  QAMD_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_stream));
+  if (ensure_init() || h2d_pageable(dst, src, bytes)) return -1;
   QAMD_HIP(hipStreamSynchronize(g_stream));
   return 0;
 }
 
 int qamd_memcpy_d2h(void *dst, const void *src, size_t bytes) {
   if (ensure_init()) return -1;
-  QAMD_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_stream));
-  QAMD_HIP(hipStreamSynchronize(g_stream));
-  return 0;
+  return d2h_pageable(dst, src, bytes);
 }
 
 qamd_lattice *qamd_lattice_create(const int dims[4]) {

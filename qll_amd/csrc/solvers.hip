@@ -533,6 +533,10 @@ extern "C" {
 int qamd_stag_cg(const qamd_links *l, qamd_field *x, const qamd_field *b, double mass, double rsq, int maxits,
                  qamd_solve_info *info) {
   if (!l || l->kind != 0) { set_error("stag_cg: staggered links required"); return -1; }
+  if (l->prec != QAMD_F64 && l->prec != QAMD_F32) {
+    set_error("stag_cg: links must be FP64 or FP32 (16-bit links run inside qamd_stag_cg_mixed)");
+    return -1;
+  }
   if (check_vec(x, l->prec, 0, "stag_cg x") || check_vec(b, l->prec, 0, "stag_cg b")) return -1;
   double t0 = now_s();
   qamd_lattice *lat = l->lat;

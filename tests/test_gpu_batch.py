@@ -60,6 +60,34 @@ def test_batch_matches_oracle_and_graph_replay(gpu, sloppy):
     assert [i["iterations"] for i in infos_g] == [i["iterations"] for i in infos]
 
 
+def test_batch_graph_capture_failure_recovers(gpu, monkeypatch):
+    """A graph capture that fails (forced on the host by
+    QAMD_TEST_FAIL_CAPTURE=1, reported at instantiation) leaves the library
+    stream out of capture mode: the solve falls back to eager launches with
+    the eager run's bits, and a later Dslash still matches the oracle."""
+    print("case: batch graph capture failure", flush=True)
+    probs, ref = _small_batch(gpu, 1)
+    infos = gpu.stag_cg_mixed_batch(probs, 1e-20, 20000, use_graph=False)
+    eager = [p[2].get_lex().reshape(p[2].nrhs, -1, 6).copy() for p in probs]
+    monkeypatch.setenv("QAMD_TEST_FAIL_CAPTURE", "1")
+    infos_f = gpu.stag_cg_mixed_batch(probs, 1e-20, 20000, use_graph=True)
+    monkeypatch.delenv("QAMD_TEST_FAIL_CAPTURE")
+    for p, xe in zip(probs, eager):
+        assert np.array_equal(p[2].get_lex().reshape(p[2].nrhs, -1, 6), xe)
+    assert [i["iterations"] for i in infos_f] == [i["iterations"] for i in infos]
+    dims, links = ref[3][0], ref[3][1]
+    lat = probs[3][0].lat
+    y = O.gauss_vector(dims, 7, 100, parity=1)
+    out = gpu.Field(lat, 0, 3, 1, 0)
+    gpu.dslash(probs[3][0], out, gpu.Field(lat, 0, 3, 1, 1).set_lex(y))
+    want = O.dslash(dims, links, 0.0, y, sub="even")
+    assert rel_err(out.get_lex(), want, parity_mask(dims, 0)) < 1e-13
+    # and a captured run works again afterwards
+    gpu.stag_cg_mixed_batch(probs, 1e-20, 20000, use_graph=True)
+    for p, xe in zip(probs, eager):
+        assert np.array_equal(p[2].get_lex().reshape(p[2].nrhs, -1, 6), xe)
+
+
 def test_batch_zero_source_and_maxits(gpu):
     """A zero source converges at once with x = 0; a problem that hits maxits
     leaves the batch unconverged without holding up the others."""

@@ -49,6 +49,11 @@ def test_dslash_argument_errors(gpu):
     _raises(gpu, lambda: gpu.dslash(L32, gpu.Field(lat, 1, 3, 17, 0), gpu.Field(lat, 1, 3, 17, 1)),
             "field shapes do not match")
     _raises(gpu, lambda: gpu.stag_cg(W, e3, e3, 0.1, 1e-10, 10), "staggered links required")
+    LQ = L.converted(gpu.Q16)
+    eq, bq = gpu.Field(lat, gpu.Q16, 3, 1, 0), gpu.Field(lat, gpu.Q16, 3, 1, 0)
+    _raises(gpu, lambda: gpu.stag_cg(LQ, eq, bq, 0.1, 1e-10, 10), "must be FP64 or FP32")
+    _raises(gpu, lambda: gpu.stag_cg(L.converted(gpu.F16), gpu.Field(lat, gpu.F16, 3, 1, 0),
+                                     gpu.Field(lat, gpu.F16, 3, 1, 0), 0.1, 1e-10, 10), "must be FP64 or FP32")
     _raises(gpu, lambda: gpu.stag_cgms(L, [e3] * 33, e3, 0.1, [0.0] * 33, 1e-10, 10), "shifts")
     _raises(gpu, lambda: gpu.stag_cg_mixed_block(L, L.converted(1), gpu.Field(lat, 0, 3, 9, 0),
                                                  gpu.Field(lat, 0, 3, 9, 0), 0.1, 1e-10, 10), "1..8 rhs")

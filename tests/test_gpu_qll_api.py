@@ -75,3 +75,41 @@ def test_qll_testdslash_vs_qop(prec, vlen, dims):
     print(r.stderr[-2000:])
     assert r.returncode == 0, r.stdout[-4000:]
     assert "PASSED" in r.stdout
+
+
+FT_SRC = os.path.join(ROOT, "tests", "qll_fieldtest.c")
+
+
+def _fieldtest(prec):
+    """qll_fieldtest.c, a QLL program written against common.h and qll.h's
+    generic names, built per precision like QLL's own drivers."""
+    b = os.path.join(ROOT, "tests", "qll_fieldtest_%s.bin" % prec)
+    if not os.path.exists(b) or os.path.getmtime(b) < os.path.getmtime(FT_SRC):
+        subprocess.run(["gcc", "-O1", "-fopenmp", "-std=gnu11"] + (["-DDOUBLE"] if prec == "D" else []) +
+                       ["-I", os.path.join(ROOT, "include"), "-o", b, FT_SRC,
+                        "-L", os.path.join(ROOT, "qll_amd", "lib"), "-lqll_amd",
+                        "-L", os.path.join(ROOT, "oracle"), "-lqop" + prec, "-loracle",
+                        "-Wl,-rpath," + os.path.join(ROOT, "qll_amd", "lib"),
+                        "-Wl,-rpath," + os.path.join(ROOT, "oracle"), "-lm"], check=True)
+    return b
+
+
+@pytest.mark.parametrize("prec,vlen,dims", [
+    ("D", 1, (4, 4, 4, 8)),
+    ("D", 4, (4, 4, 8, 8)),
+    ("F", 8, (4, 4, 8, 8)),
+    ("F", 1, (6, 4, 4, 8)),
+])
+def test_qll_fieldtest_transporters(prec, vlen, dims):
+    """fieldtest.c:183-204 (transport forth and back, matrix transport vs
+    qopTransportM), split-phase transporters in a team, makeShift for an
+    arbitrary displacement, and a setOffSubC2/gsum Linop in cgSolve vs
+    solve2."""
+    print("case: qll_fieldtest %s vlen=%d dims=%s" % (prec, vlen, dims), flush=True)
+    env = dict(os.environ, OMP_NUM_THREADS="4")
+    r = subprocess.run([_fieldtest(prec), str(vlen)] + [str(d) for d in dims], capture_output=True, text=True,
+                       timeout=300, env=env)
+    print(r.stdout[-6000:])
+    print(r.stderr[-2000:])
+    assert r.returncode == 0, r.stdout[-4000:]
+    assert "PASSED" in r.stdout
