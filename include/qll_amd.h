@@ -170,6 +170,19 @@ int qamd_links_reconstructs(const qamd_links *l);
 int qamd_dslash(const qamd_links *l, qamd_field *out, const qamd_field *in,
                 double a, const qamd_field *w, double b, double *norm2_out);
 
+/* Allocation-free form of qamd_dslash for callers that cannot allocate
+ * device memory (code under hipGraph capture, tight serving loops, harnesses
+ * that forbid hipMalloc); no counterpart in the reference.  Same arguments
+ * and results as qamd_dslash for staggered links and single-rhs fields
+ * (every precision).  The |out|^2 reduction uses a fixed device array of
+ * 1 MiB per GPU (held until the process ends) instead of library scratch:
+ * at most 2^24 sites per parity (lattices up to 2^25 sites, e.g. 64^4);
+ * larger lattices, blocks of rhs and Wilson links are rejected with an
+ * error.  Needs qamd_init to have run; calls no hipMalloc/hipFree.  Calls
+ * that overlap on one GPU (from several host threads) race on the array. */
+int qamd_dslash_noalloc(const qamd_links *l, qamd_field *out, const qamd_field *in,
+                        double a, const qamd_field *w, double b, double *norm2_out);
+
 /* ---- normal operator (replaces dslashp2 lib/stagDslash.c:546-575):
  *      tmp = D_oe p, Ap = m^2 p - D_eo tmp, pAp = m^2|p|^2 + |tmp|^2 ---- */
 int qamd_stag_normal(const qamd_links *l, qamd_field *Ap, const qamd_field *p,

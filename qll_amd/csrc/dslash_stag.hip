@@ -182,7 +182,8 @@ static void launch_t(const StagArgs<P, PO> &A, bool red, hipStream_t st) {
 
 template <int P, int PO = P>
 static int launch_p(const qamd_links *l, Field &out, const Field &in, double a, const Field *w,
-                    double b, double *red_out, hipStream_t st, const int *skip, int pset, bool finish) {
+                    double b, double *red_out, hipStream_t st, const int *skip, int pset, bool finish,
+                    const RedScratch *rs_fixed) {
   typedef typename Prec<P>::S S;
   typedef typename Prec<PO>::S SO;
   StagArgs<P, PO> A;
@@ -205,8 +206,12 @@ static int launch_p(const qamd_links *l, Field &out, const Field &in, double a, 
   const bool red = red_out != nullptr || !finish;
   const int nb = (int)((A.g.vh + 255) / 256);
   if (red) {
-    if (red_reserve(nb)) return -1;
-    A.rs = red_set(pset);
+    if (rs_fixed) {
+      A.rs = *rs_fixed;
+    } else {
+      if (red_reserve(nb)) return -1;
+      A.rs = red_set(pset);
+    }
     A.red_out = red_out;
   }
   if (l->nterms == 16) launch_t<P, PO, 16>(A, red, st);
@@ -217,7 +222,8 @@ static int launch_p(const qamd_links *l, Field &out, const Field &in, double a, 
 }
 
 int stag_dslash_launch(const qamd_links *l, Field &out, const Field &in, double a, const Field *w,
-                       double b, double *red_out, hipStream_t st, const int *skip, int pset, bool finish) {
+                       double b, double *red_out, hipStream_t st, const int *skip, int pset, bool finish,
+                       const RedScratch *rs_fixed) {
   if (l->kind != 0) {
     set_error("stag_dslash: links are not staggered");
     return -1;
@@ -244,16 +250,16 @@ int stag_dslash_launch(const qamd_links *l, Field &out, const Field &in, double 
     return -1;
   }
   switch (l->prec) {
-    case QAMD_F64: return launch_p<QAMD_F64>(l, out, in, a, w, b, red_out, st, skip, pset, finish);
-    case QAMD_F32: return launch_p<QAMD_F32>(l, out, in, a, w, b, red_out, st, skip, pset, finish);
+    case QAMD_F64: return launch_p<QAMD_F64>(l, out, in, a, w, b, red_out, st, skip, pset, finish, rs_fixed);
+    case QAMD_F32: return launch_p<QAMD_F32>(l, out, in, a, w, b, red_out, st, skip, pset, finish, rs_fixed);
     case QAMD_F16:
       if (half_to_single)
-        return launch_p<QAMD_F16, QAMD_F32>(l, out, in, a, w, b, red_out, st, skip, pset, finish);
-      return launch_p<QAMD_F16>(l, out, in, a, w, b, red_out, st, skip, pset, finish);
+        return launch_p<QAMD_F16, QAMD_F32>(l, out, in, a, w, b, red_out, st, skip, pset, finish, rs_fixed);
+      return launch_p<QAMD_F16>(l, out, in, a, w, b, red_out, st, skip, pset, finish, rs_fixed);
     default:
       if (half_to_single)
-        return launch_p<QAMD_Q16, QAMD_F32>(l, out, in, a, w, b, red_out, st, skip, pset, finish);
-      return launch_p<QAMD_Q16>(l, out, in, a, w, b, red_out, st, skip, pset, finish);
+        return launch_p<QAMD_Q16, QAMD_F32>(l, out, in, a, w, b, red_out, st, skip, pset, finish, rs_fixed);
+      return launch_p<QAMD_Q16>(l, out, in, a, w, b, red_out, st, skip, pset, finish, rs_fixed);
   }
 }
 
@@ -307,11 +313,74 @@ int stag_cg_step_launch(const qamd_links *l, const Field &in, const Field &p, Fi
   }
 }
 
+// ------------------------------------------------- no-allocation variant
+// Partial sums and totals of qamd_dslash_noalloc live in this fixed array
+// (one copy per GPU, held until the process ends): row i of the partial
+// sums at [i * NOALLOC_BLOCKS + block], the two totals at 2 * NOALLOC_BLOCKS.
+// Every launch writes each partial and total it later reads, so nothing has
+// to start at zero.  Host code uses the per-device address from
+// hipGetSymbolAddress, never the array's name.
+constexpr int NOALLOC_BLOCKS = 65536;   // 256-site blocks: up to 2^24 sites per parity
+static __device__ __align__(256) double scratch[2 * NOALLOC_BLOCKS + 8];
+static double *g_scratch_dev[64];       // per device ordinal
+
+static int noalloc_scratch(double **ptr) {
+  int dev = 0;
+  QAMD_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) {
+    set_error("dslash_noalloc: device ordinal %d out of range", dev);
+    return -1;
+  }
+  if (!g_scratch_dev[dev]) {
+    void *q = nullptr;
+    QAMD_HIP(hipGetSymbolAddress(&q, HIP_SYMBOL(scratch)));
+    g_scratch_dev[dev] = (double *)q;
+  }
+  *ptr = g_scratch_dev[dev];
+  return 0;
+}
+
 }  // namespace qamd
 
 using namespace qamd;
 
 extern "C" {
+
+int qamd_dslash_noalloc(const qamd_links *l, qamd_field *out, const qamd_field *in, double a,
+                        const qamd_field *w, double b, double *norm2_out) {
+  if (!l || !out || !in) {
+    set_error("dslash_noalloc: NULL argument");
+    return -1;
+  }
+  if (!initialized()) {
+    set_error("dslash_noalloc: call qamd_init first (this entry point allocates nothing)");
+    return -1;
+  }
+  if (l->kind != 0 || in->f.nrhs != 1 || out->f.nrhs != 1) {
+    set_error("dslash_noalloc: needs staggered links and single-rhs fields");
+    return -1;
+  }
+  const int64_t nb = (l->lat->g.vh + 255) / 256;
+  if (nb > NOALLOC_BLOCKS) {
+    set_error("dslash_noalloc: %lld sites per parity exceed the scratch limit of %lld", (long long)l->lat->g.vh,
+              (long long)NOALLOC_BLOCKS * 256);
+    return -1;
+  }
+  double *s = nullptr;
+  if (norm2_out && noalloc_scratch(&s)) return -1;
+  const RedScratch rs{s, NOALLOC_BLOCKS};
+  double *tot = s ? s + 2 * NOALLOC_BLOCKS : nullptr;
+  if (stag_dslash_launch(l, out->f, in->f, a, w ? &w->f : nullptr, b, tot, stream(), nullptr, 0, true,
+                         norm2_out ? &rs : nullptr))
+    return -1;
+  if (norm2_out) {
+    double t[2];
+    QAMD_HIP(hipMemcpyAsync(t, tot, sizeof t, hipMemcpyDeviceToHost, stream()));
+    QAMD_HIP(hipStreamSynchronize(stream()));
+    *norm2_out = t[0];
+  }
+  return 0;
+}
 
 int qamd_dslash(const qamd_links *l, qamd_field *out, const qamd_field *in, double a,
                 const qamd_field *w, double b, double *norm2_out) {

@@ -41,6 +41,7 @@ void set_error(const char *fmt, ...);
 
 hipStream_t stream();
 int ensure_init();
+bool initialized();   // qamd_init (or a first library call) has run
 void *dev_alloc(size_t bytes);   // returns NULL and sets the error on failure
 void dev_free(void *p);
 // transfers of pageable host arrays (hostio.hip): a persistent device staging

@@ -20,10 +20,11 @@ double *scalar_slot();
 // dslash_stag.hip / dslash_wilson.hip
 // Reduction: with red_out, {|out|^2, |w|^2} is reduced into partial-sum set
 // pset and (finish = true) a finisher stores the totals at red_out; with
-// finish = false the partial sums are left in set pset for a caller's finisher.
+// finish = false the partial sums are left in set pset for a caller's finisher;
+// rs_fixed: partial sums go there instead of set pset (no scratch reserved).
 int stag_dslash_launch(const qamd_links *l, Field &out, const Field &in, double a, const Field *w,
                        double b, double *red_out, hipStream_t st, const int *skip = nullptr, int pset = 0,
-                       bool finish = true);
+                       bool finish = true, const RedScratch *rs_fixed = nullptr);
 // stag_block.hip: staggered Dslash on a block of up to 16 rhs (8 in FP64),
 // links loaded once per site; with red, {|out_r|^2, |w_r|^2} for each rhs r
 // go to partial-sum set rs (2*nrhs values)

@@ -42,6 +42,7 @@ int ensure_init() {
 }
 
 hipStream_t stream() { return g_stream; }
+bool initialized() { return g_init != 0; }
 
 void *dev_alloc(size_t bytes) {
   if (ensure_init()) return nullptr;
