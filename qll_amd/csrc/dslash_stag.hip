@@ -21,14 +21,16 @@
 
 namespace qamd {
 
-// links and input in precision P, output and w in precision PO (PO = P, or
-// FP32 output from FP16 links/input for the mixed-precision solvers)
-template <int P, int PO = P> struct StagArgs {
+// links in precision P, input in precision PI, output and w in precision PO
+// (PI = PO = P, or FP32 input and/or output with 16-bit links for the
+// mixed-precision solvers)
+template <int P, int PO = P, int PI = P> struct StagArgs {
   typedef typename Prec<P>::S S;
   typedef typename Prec<PO>::S SO;
+  typedef typename Prec<PI>::S SI;
   Geom g;
   const S *links;
-  const S *in;
+  const SI *in;
   const float *in_norm;
   SO *out;
   float *out_norm;
@@ -41,11 +43,11 @@ template <int P, int PO = P> struct StagArgs {
   double *red_out;    // receives {|out|^2, |w|^2}
   const int *skip;    // optional device flag: whole grid returns when set
   const uint8_t *lsign;   // long-term determinant signs when their third rows are rebuilt
-  // fused CG step (CGX, FP32 output): with w = p and out = a p + b D in = Ap
-  // (not stored), x += alpha p and r -= alpha Ap, |r|^2 -> red[0]; alpha is
-  // read from the device
-  float2 *cgx, *cgr;
-  const double *cg_alpha;
+  // single-reduction CG step (CGX = 2, FP32 vectors, input t): Ar =
+  // a r + b D t (not stored); p = r + beta p, s = Ar + beta s, x += alpha p,
+  // r -= alpha s, |r|^2 -> red[0]; alpha and beta read from the device
+  float2 *cgx, *cgr, *cgp, *cgs;
+  const double *cg_alpha, *cg_beta;
 };
 
 struct FinStore2 {
@@ -58,8 +60,10 @@ struct FinStore2 {
 
 // REC (16 terms): the long terms' third rows are not loaded but rebuilt
 // from the first two (FP64: 2016 instead of 2400 bytes per site)
-template <int P, int PO, int NT, bool RED, bool REC = false, bool CGX = false>
-__global__ void __launch_bounds__(256) k_stag_dslash(StagArgs<P, PO> A) {
+template <int P, int PO, int NT, bool RED, bool REC = false, int CGX = 0, int PI = P>
+__global__ void __launch_bounds__(256) k_stag_dslash(StagArgs<P, PO, PI> A) {
+  static_assert(PI == P || (PI == QAMD_F32 && (P == QAMD_F16 || P == QAMD_Q16)),
+                "input precision: the links' or FP32 with 16-bit links");
   static_assert(!CGX || (PO == QAMD_F32 && RED), "fused CG step: FP32 vectors, with the |r|^2 sums");
   typedef typename Prec<P>::R R;
   typedef typename Prec<P>::C C;
@@ -100,7 +104,17 @@ __global__ void __launch_bounds__(256) k_stag_dslash(StagArgs<P, PO> A) {
         const int h = (k & 1) ? -hop : hop;
         int nb = nbr_cb(A.g, s, mu, h);
         C v[3];
-        VecLd<P>::ld(A.in, A.in_norm, vp, nb, v, (P == QAMD_F16 || P == QAMD_Q16) ? A.lscale[t] : (R)1);
+        if constexpr (PI == P) {
+          VecLd<P>::ld(A.in, A.in_norm, vp, nb, v, (P == QAMD_F16 || P == QAMD_Q16) ? A.lscale[t] : (R)1);
+        } else {   // FP32 input, 16-bit links: the term's link scale applied here
+          VecLd<PI>::ld(A.in, A.in_norm, vp, nb, v, (R)1);
+          const R sc = A.lscale[t];
+#pragma unroll
+          for (int j = 0; j < 3; j++) {
+            v[j].x *= sc;
+            v[j].y *= sc;
+          }
+        }
         C m[9];
 #pragma unroll
         for (int e = 0; e < 9; e++)
@@ -115,23 +129,28 @@ __global__ void __launch_bounds__(256) k_stag_dslash(StagArgs<P, PO> A) {
     }
     C o[3];
     const R b = (R)A.b;
-    if constexpr (CGX) {
-      C w[3];   // p
-      VecLd<PO>::ld(A.w, A.w_norm, vp, cb, w, (R)1);
-      const R a = (R)A.a, al = (R)*A.cg_alpha;
-      float2 xv[3], rv[3];
-      VecLd<QAMD_F32>::ld(A.cgx, nullptr, vp, cb, xv, 1.0f);
+    if constexpr (CGX == 2) {
+      float2 rv[3], pv[3], sv[3], xv[3];
       VecLd<QAMD_F32>::ld(A.cgr, nullptr, vp, cb, rv, 1.0f);
+      VecLd<QAMD_F32>::ld(A.cgp, nullptr, vp, cb, pv, 1.0f);
+      VecLd<QAMD_F32>::ld(A.cgs, nullptr, vp, cb, sv, 1.0f);
+      VecLd<QAMD_F32>::ld(A.cgx, nullptr, vp, cb, xv, 1.0f);
+      const float a = (float)A.a, al = (float)*A.cg_alpha, be = (float)*A.cg_beta;
 #pragma unroll
       for (int j = 0; j < 3; j++) {
-        o[j].x = fma(a, w[j].x, b * acc[j].x);   // Ap
-        o[j].y = fma(a, w[j].y, b * acc[j].y);
-        xv[j].x = fma(al, w[j].x, xv[j].x);
-        xv[j].y = fma(al, w[j].y, xv[j].y);
-        rv[j].x = fma(-al, o[j].x, rv[j].x);
-        rv[j].y = fma(-al, o[j].y, rv[j].y);
+        const float wx = fmaf(a, rv[j].x, (float)b * acc[j].x), wy = fmaf(a, rv[j].y, (float)b * acc[j].y);
+        pv[j].x = fmaf(be, pv[j].x, rv[j].x);
+        pv[j].y = fmaf(be, pv[j].y, rv[j].y);
+        sv[j].x = fmaf(be, sv[j].x, wx);
+        sv[j].y = fmaf(be, sv[j].y, wy);
+        xv[j].x = fmaf(al, pv[j].x, xv[j].x);
+        xv[j].y = fmaf(al, pv[j].y, xv[j].y);
+        rv[j].x = fmaf(-al, sv[j].x, rv[j].x);
+        rv[j].y = fmaf(-al, sv[j].y, rv[j].y);
         red[0] += (double)rv[j].x * rv[j].x + (double)rv[j].y * rv[j].y;
       }
+      VecSt<QAMD_F32>::st(A.cgp, nullptr, vp, cb, pv);
+      VecSt<QAMD_F32>::st(A.cgs, nullptr, vp, cb, sv);
       VecSt<QAMD_F32>::st(A.cgx, nullptr, vp, cb, xv);
       VecSt<QAMD_F32>::st(A.cgr, nullptr, vp, cb, rv);
     } else if (A.w) {
@@ -162,36 +181,37 @@ __global__ void __launch_bounds__(256) k_stag_dslash(StagArgs<P, PO> A) {
   if (RED) grid_partial<2>(red, A.rs);
 }
 
-template <int P, int PO, int NT>
-static void launch_t(const StagArgs<P, PO> &A, bool red, hipStream_t st) {
+template <int P, int PO, int NT, int PI>
+static void launch_t(const StagArgs<P, PO, PI> &A, bool red, hipStream_t st) {
   int nb = (int)((A.g.vh + 255) / 256);
   if constexpr (NT == 16) {
     if (A.lsign) {
       if (red)
-        hipLaunchKernelGGL((k_stag_dslash<P, PO, NT, true, true>), dim3(nb), dim3(256), 0, st, A);
+        hipLaunchKernelGGL((k_stag_dslash<P, PO, NT, true, true, 0, PI>), dim3(nb), dim3(256), 0, st, A);
       else
-        hipLaunchKernelGGL((k_stag_dslash<P, PO, NT, false, true>), dim3(nb), dim3(256), 0, st, A);
+        hipLaunchKernelGGL((k_stag_dslash<P, PO, NT, false, true, 0, PI>), dim3(nb), dim3(256), 0, st, A);
       return;
     }
   }
   if (red)
-    hipLaunchKernelGGL((k_stag_dslash<P, PO, NT, true>), dim3(nb), dim3(256), 0, st, A);
+    hipLaunchKernelGGL((k_stag_dslash<P, PO, NT, true, false, 0, PI>), dim3(nb), dim3(256), 0, st, A);
   else
-    hipLaunchKernelGGL((k_stag_dslash<P, PO, NT, false>), dim3(nb), dim3(256), 0, st, A);
+    hipLaunchKernelGGL((k_stag_dslash<P, PO, NT, false, false, 0, PI>), dim3(nb), dim3(256), 0, st, A);
 }
 
-template <int P, int PO = P>
+template <int P, int PO = P, int PI = P>
 static int launch_p(const qamd_links *l, Field &out, const Field &in, double a, const Field *w,
                     double b, double *red_out, hipStream_t st, const int *skip, int pset, bool finish,
                     const RedScratch *rs_fixed) {
   typedef typename Prec<P>::S S;
   typedef typename Prec<PO>::S SO;
-  StagArgs<P, PO> A;
+  typedef typename Prec<PI>::S SI;
+  StagArgs<P, PO, PI> A;
   memset(&A, 0, sizeof A);
   A.skip = skip;
   A.g = l->lat->g;
   A.links = (const S *)l->data[out.parity];
-  A.in = (const S *)in.data;
+  A.in = (const SI *)in.data;
   A.in_norm = in.norm;
   A.out = (SO *)out.data;
   A.out_norm = out.norm;
@@ -214,8 +234,8 @@ static int launch_p(const qamd_links *l, Field &out, const Field &in, double a, 
     }
     A.red_out = red_out;
   }
-  if (l->nterms == 16) launch_t<P, PO, 16>(A, red, st);
-  else launch_t<P, PO, 8>(A, red, st);
+  if (l->nterms == 16) launch_t<P, PO, 16, PI>(A, red, st);
+  else launch_t<P, PO, 8, PI>(A, red, st);
   if (red && finish) launch_finish<2>(A.rs, nb, FinStore2{red_out}, st, skip);
   QAMD_HIP(hipGetLastError());
   return 0;
@@ -232,8 +252,11 @@ int stag_dslash_launch(const qamd_links *l, Field &out, const Field &in, double 
     set_error("stag_dslash: bad partial-sum set %d", pset);
     return -1;
   }
-  bool half_to_single = ((l->prec == QAMD_F16 || l->prec == QAMD_Q16) && out.prec == QAMD_F32);
-  if (in.prec != l->prec || (out.prec != l->prec && !half_to_single) || (w && w->prec != out.prec)) {
+  const bool half = (l->prec == QAMD_F16 || l->prec == QAMD_Q16);
+  bool half_to_single = half && out.prec == QAMD_F32;
+  const bool single_in = half && in.prec == QAMD_F32;   // FP32 in and out, 16-bit links
+  if ((in.prec != l->prec && !(single_in && half_to_single)) || (out.prec != l->prec && !half_to_single) ||
+      (w && w->prec != out.prec)) {
     set_error("stag_dslash: field precisions in %d / out %d do not fit links %d", in.prec, out.prec, l->prec);
     return -1;
   }
@@ -253,63 +276,68 @@ int stag_dslash_launch(const qamd_links *l, Field &out, const Field &in, double 
     case QAMD_F64: return launch_p<QAMD_F64>(l, out, in, a, w, b, red_out, st, skip, pset, finish, rs_fixed);
     case QAMD_F32: return launch_p<QAMD_F32>(l, out, in, a, w, b, red_out, st, skip, pset, finish, rs_fixed);
     case QAMD_F16:
+      if (single_in)
+        return launch_p<QAMD_F16, QAMD_F32, QAMD_F32>(l, out, in, a, w, b, red_out, st, skip, pset, finish,
+                                                      rs_fixed);
       if (half_to_single)
         return launch_p<QAMD_F16, QAMD_F32>(l, out, in, a, w, b, red_out, st, skip, pset, finish, rs_fixed);
       return launch_p<QAMD_F16>(l, out, in, a, w, b, red_out, st, skip, pset, finish, rs_fixed);
     default:
+      if (single_in)
+        return launch_p<QAMD_Q16, QAMD_F32, QAMD_F32>(l, out, in, a, w, b, red_out, st, skip, pset, finish,
+                                                      rs_fixed);
       if (half_to_single)
         return launch_p<QAMD_Q16, QAMD_F32>(l, out, in, a, w, b, red_out, st, skip, pset, finish, rs_fixed);
       return launch_p<QAMD_Q16>(l, out, in, a, w, b, red_out, st, skip, pset, finish, rs_fixed);
   }
 }
 
-// Fused CG step of the mixed-precision solvers: Ap = a p - D in (FP32,
-// not stored), x += alpha p, r -= alpha Ap, |r|^2 partial sums into set pset
-// (slot 0).  in: the sloppy Dslash input (opposite parity); p, x, r: FP32
-// vectors of the output parity; alpha: device scalar.
+// single-reduction CG step: t the D_oe output (opposite parity, FP32); r,
+// p, s, x FP32 of the output parity; links FP32, FP16 or Q16
 template <int P>
-static int cgx_p(const qamd_links *l, const Field &in, const Field &p, Field &x, Field &r, double a,
-                 const double *alpha, hipStream_t st, const int *skip, int pset) {
+static int cgcg_p(const qamd_links *l, const Field &t, Field &r, Field &p, Field &s, Field &x, double a,
+                  const double *alpha, const double *beta, hipStream_t st, const int *skip, int pset) {
   typedef typename Prec<P>::S S;
-  StagArgs<P, QAMD_F32> A;
+  StagArgs<P, QAMD_F32, QAMD_F32> A;
   memset(&A, 0, sizeof A);
   A.skip = skip;
   A.g = l->lat->g;
   A.links = (const S *)l->data[r.parity];
-  A.in = (const S *)in.data;
-  A.in_norm = in.norm;
-  A.w = (const float2 *)p.data;
+  A.in = (const float2 *)t.data;
   A.a = a;
   A.b = -1.0;
   A.par = r.parity;
   A.lsign = (l->recon && l->nterms == 16) ? l->lsign[r.parity] : nullptr;
-  for (int t = 0; t < 16; t++) A.lscale[t] = (P == QAMD_Q16) ? l->scale[t] / Q16_ONE : l->scale[t];
-  A.cgx = (float2 *)x.data;
+  for (int k = 0; k < 16; k++) A.lscale[k] = (P == QAMD_Q16) ? l->scale[k] / Q16_ONE : l->scale[k];
   A.cgr = (float2 *)r.data;
+  A.cgp = (float2 *)p.data;
+  A.cgs = (float2 *)s.data;
+  A.cgx = (float2 *)x.data;
   A.cg_alpha = alpha;
+  A.cg_beta = beta;
   const int nb = (int)((A.g.vh + 255) / 256);
   if (red_reserve(nb)) return -1;
   A.rs = red_set(pset);
   if (A.lsign)
-    hipLaunchKernelGGL((k_stag_dslash<P, QAMD_F32, 16, true, true, true>), dim3(nb), dim3(256), 0, st, A);
+    hipLaunchKernelGGL((k_stag_dslash<P, QAMD_F32, 16, true, true, 2, QAMD_F32>), dim3(nb), dim3(256), 0, st, A);
   else
-    hipLaunchKernelGGL((k_stag_dslash<P, QAMD_F32, 16, true, false, true>), dim3(nb), dim3(256), 0, st, A);
+    hipLaunchKernelGGL((k_stag_dslash<P, QAMD_F32, 16, true, false, 2, QAMD_F32>), dim3(nb), dim3(256), 0, st, A);
   QAMD_HIP(hipGetLastError());
   return 0;
 }
 
-int stag_cg_step_launch(const qamd_links *l, const Field &in, const Field &p, Field &x, Field &r, double a,
-                        const double *alpha, hipStream_t st, const int *skip, int pset) {
-  if (l->kind != 0 || l->nterms != 16 || in.prec != l->prec || p.prec != QAMD_F32 || x.prec != QAMD_F32 ||
-      r.prec != QAMD_F32 || in.parity == r.parity || p.parity != r.parity || x.parity != r.parity ||
-      in.nrhs != 1 || p.nrhs != 1 || x.nrhs != 1 || r.nrhs != 1 || l->prec == QAMD_F64) {
-    set_error("stag_cg_step: needs 16-term sloppy links, FP32 p, x, r of the output parity");
+int stag_cgcg_step_launch(const qamd_links *l, const Field &t, Field &r, Field &p, Field &s, Field &x, double a,
+                          const double *alpha, const double *beta, hipStream_t st, const int *skip, int pset) {
+  if (l->kind != 0 || l->nterms != 16 || l->prec == QAMD_F64 || t.prec != QAMD_F32 || t.parity == r.parity ||
+      r.prec != QAMD_F32 || p.prec != QAMD_F32 || s.prec != QAMD_F32 || x.prec != QAMD_F32 ||
+      p.parity != r.parity || s.parity != r.parity || x.parity != r.parity || t.nrhs != 1 || r.nrhs != 1) {
+    set_error("stag_cgcg_step: needs 16-term sloppy links and single-rhs FP32 t (opposite parity) and r, p, s, x");
     return -1;
   }
   switch (l->prec) {
-    case QAMD_F32: return cgx_p<QAMD_F32>(l, in, p, x, r, a, alpha, st, skip, pset);
-    case QAMD_F16: return cgx_p<QAMD_F16>(l, in, p, x, r, a, alpha, st, skip, pset);
-    default: return cgx_p<QAMD_Q16>(l, in, p, x, r, a, alpha, st, skip, pset);
+    case QAMD_F32: return cgcg_p<QAMD_F32>(l, t, r, p, s, x, a, alpha, beta, st, skip, pset);
+    case QAMD_F16: return cgcg_p<QAMD_F16>(l, t, r, p, s, x, a, alpha, beta, st, skip, pset);
+    default: return cgcg_p<QAMD_Q16>(l, t, r, p, s, x, a, alpha, beta, st, skip, pset);
   }
 }
 

@@ -41,6 +41,11 @@ void set_error(const char *fmt, ...);
 
 hipStream_t stream();
 int ensure_init();
+// Capture what enqueue(ctx) launches on the library stream into an
+// executable graph (kernel launches only).  On any failure the stream leaves
+// capture mode, partial objects are destroyed and NULL is returned, so the
+// caller can launch eagerly; QAMD_TEST_FAIL_CAPTURE=1 forces that failure.
+hipGraphExec_t capture_stream(int (*enqueue)(void *), void *ctx);
 bool initialized();   // qamd_init (or a first library call) has run
 void *dev_alloc(size_t bytes);   // returns NULL and sets the error on failure
 void dev_free(void *p);

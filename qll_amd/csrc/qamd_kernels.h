@@ -34,11 +34,13 @@ int stag_block_launch(const qamd_links *l, Field &out, const Field &in, double a
 // is set in act (the others' outputs and sums are left untouched / zero)
 int stag_block_launch_m(const qamd_links *l, Field &out, const Field &in, const double *ar, const Field *w,
                         double b, bool red, RedScratch rs, hipStream_t st, unsigned act);
-// dslash_stag.hip: fused step of the mixed-precision CG: Ap = a p - D in
-// (not stored), x += alpha p, r -= alpha Ap (FP32 p, x, r), |r|^2 partial
-// sums in set pset; alpha read from the device
-int stag_cg_step_launch(const qamd_links *l, const Field &in, const Field &p, Field &x, Field &r, double a,
-                        const double *alpha, hipStream_t st, const int *skip, int pset);
+// dslash_stag.hip: fused step of the single-reduction (Chronopoulos-Gear)
+// mixed-precision CG, FP32 vectors and sloppy (FP32/FP16/Q16) links:
+// Ar = a r - D t (not stored), p = r + beta p, s = Ar + beta s,
+// x += alpha p, r -= alpha s with |r|^2 partial sums in set pset; alpha and
+// beta read from the device
+int stag_cgcg_step_launch(const qamd_links *l, const Field &t, Field &r, Field &p, Field &s, Field &x, double a,
+                          const double *alpha, const double *beta, hipStream_t st, const int *skip, int pset);
 // stag_block.hip: block CG on (m2[r] - D_eo D_oe) x_r = b_r (one m2 per rhs,
 // 1..8 rhs, FP64 x and b), sloppy links ls, FP64 reliable updates on l64;
 // x0: start from the x passed in

@@ -42,6 +42,32 @@ int ensure_init() {
 }
 
 hipStream_t stream() { return g_stream; }
+
+hipGraphExec_t capture_stream(int (*enqueue)(void *), void *ctx) {
+  hipGraph_t g = nullptr;
+  hipGraphExec_t x = nullptr;
+  hipError_t e = hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal);
+  int rc = 0;
+  if (e == hipSuccess) {
+    rc = enqueue(ctx);
+    e = hipStreamEndCapture(g_stream, &g);   // ends the capture in every case
+    const char *inj = getenv("QAMD_TEST_FAIL_CAPTURE");
+    if (e == hipSuccess && inj && inj[0] == '1') e = hipErrorInvalidValue;
+    if (e == hipSuccess && !rc) e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  }
+  if (g) (void)hipGraphDestroy(g);   // the executable graph keeps what it needs
+  if (e == hipSuccess && !rc) return x;
+  if (e != hipSuccess) set_error("graph capture: %s", hipGetErrorString(e));
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(g_stream, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) {
+    hipGraph_t g2 = nullptr;
+    (void)hipStreamEndCapture(g_stream, &g2);
+    if (g2) (void)hipGraphDestroy(g2);
+  }
+  if (x) (void)hipGraphExecDestroy(x);
+  (void)hipGetLastError();
+  return nullptr;
+}
 bool initialized() { return g_init != 0; }
 
 void *dev_alloc(size_t bytes) {

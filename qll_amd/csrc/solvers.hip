@@ -42,6 +42,10 @@ struct SolverDev {
   double xb[MAXSHIFT];                  // coefficients for the x update
   int xrun;                             // 1: this iteration's x update is due
   int drop;                             // 1: shifts stop updating once converged
+  // single-reduction CG (mixed solver): the previous iteration's |r|^2 and
+  // alpha, and restart = 1 to take beta = 0 in the next iteration
+  double gam_old, al_old;
+  int restart;
 };
 
 template <int P> struct VT {
@@ -86,46 +90,32 @@ __global__ void __launch_bounds__(256) k_cg_p(Field p, Field r, SolverDev *S) {
   VT<P>::st(p, i, pv);
 }
 
-// Mixed-precision CG, fused form: p = r + beta p (reset: p = r), the 16-bit
-// Dslash copy (SH), and |p|^2 partial sums, which the next p.Ap needs
-// before the fused x/r step
-template <int SH>
-__global__ void __launch_bounds__(256) k_cg_pn(Field p, Field r, Field shadow, SolverDev *S, RedScratch rs,
-                                               int reset) {
-  if (S->done) return;   // the finisher that would read the sums is skipped too
-  double red[1] = {0.0};
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < p.g->vh) {
-    float2 pv[3], rv[3];
-    VT<QAMD_F32>::ld(r, i, rv);
-    if (reset) {
-#pragma unroll
-      for (int j = 0; j < 3; j++) pv[j] = rv[j];
-    } else {
-      VT<QAMD_F32>::ld(p, i, pv);
-      const float be = (float)S->beta;
-#pragma unroll
-      for (int j = 0; j < 3; j++) {
-        pv[j].x = fmaf(be, pv[j].x, rv[j].x);
-        pv[j].y = fmaf(be, pv[j].y, rv[j].y);
-      }
-    }
-#pragma unroll
-    for (int j = 0; j < 3; j++) red[0] += (double)pv[j].x * pv[j].x + (double)pv[j].y * pv[j].y;
-    VT<QAMD_F32>::st(p, i, pv);
-    if constexpr (SH != 0) VT<SH>::st(shadow, i, pv);
-  }
-  grid_partial<1>(red, rs);
-}
-
-// p.Ap = m^2 |p|^2 + |D_oe p|^2 (the dslashp2 identity, lib/stagDslash.c:546-575)
-// from |t|^2 (set a) and |p|^2 (set b); alpha = rsq / p.Ap
-struct FinAlpha {
+// Single-reduction CG (Chronopoulos & Gear, J. Comput. Appl. Math. 25
+// (1989) 153-168) on A = m^2 - D_eo D_oe: from gamma = |r|^2 (set a) and
+// |t|^2 = |D_oe r|^2 (set b), delta = r.Ar = m^2 gamma + |t|^2 (the dslashp2
+// identity, lib/stagDslash.c:546-575), beta = gamma / gamma_old and
+// alpha = gamma / (delta - beta gamma / alpha_old); a restart takes beta = 0,
+// alpha = gamma / delta.
+struct FinCGCG {
   SolverDev *S;
   __device__ void operator()(const double *t) const {
-    S->nrm[0] = t[0];
-    S->nrm[3] = t[1];
-    S->alpha = S->rsq / (S->m2 * t[1] + t[0]);
+    const double g = t[0], d = S->m2 * g + t[1];
+    double be = 0.0, al = g / d;
+    if (S->restart) {
+      S->restart = 0;
+    } else {
+      be = g / S->gam_old;
+      al = g / (d - be * g / S->al_old);
+    }
+    S->rsq_old = S->rsq;
+    S->rsq = g;
+    S->beta = be;
+    S->alpha = al;
+    S->gam_old = g;
+    S->al_old = al;
+    S->nrm[0] = t[1];
+    S->maxrr = fmax(S->maxrr, sqrt(g));
+    S->its += 1;
   }
 };
 
@@ -593,18 +583,25 @@ int qamd_stag_cg(const qamd_links *l, qamd_field *x, const qamd_field *b, double
   return rc;
 }
 
-// Mixed-precision CG with reliable updates.  The Krylov vectors r, p, Ap and
-// the solution increment live in FP32; the Dslash runs at the sloppy links'
-// precision (FP32, or FP16 with an FP16 copy of p as input and FP32 output);
-// x, b and the true residual are FP64.  Reliable updates happen at batch
-// boundaries when |r| has dropped below delta * max|r| since the last one.
-// x is used as the initial guess.
+// Mixed-precision CG with reliable updates.  The Krylov vectors r, p,
+// s = Ap, the solution increment and the odd temporary t live in FP32; the
+// Dslash reads the sloppy links (FP32, FP16 or Q16) and applies them in FP32
+// arithmetic, so the sloppy operator is one fixed linear map (the 16-bit
+// rounding stays in the links; none is applied to the vectors); x, b and
+// the true residual are FP64.  The inner iteration is the single-reduction
+// (Chronopoulos-Gear) form of CG: one application of the operator to r and
+// one global reduction per iteration, so an iteration is three launches:
+// t = D_oe r with |t|^2; the finisher (alpha, beta); the fused step
+// Ar = m^2 r - D_eo t (not stored), p = r + beta p, s = Ar + beta s,
+// x += alpha p, r -= alpha s with |r|^2.  The recurrence for s = Ap needs
+// the operator to be linear, which the FP32 vectors keep.  Reliable updates
+// happen at batch boundaries when |r| has dropped below delta * max|r| since
+// the last one.  x is used as the initial guess.
 static int cg_mixed_core(const qamd_links *l64, const qamd_links *ls, qamd_field *x, const qamd_field *b,
                          double m2, double rsq, int maxits, double delta, qamd_solve_info *info) {
   double t0 = now_s();
   qamd_lattice *lat = l64->lat;
   const int ps = ls->prec;
-  bool half = (ps == QAMD_F16 || ps == QAMD_Q16);   // 16-bit Dslash input copy of p
   const qamd_links *lsc = ls;        // current sloppy links
   qamd_links *l32own = nullptr;      // FP32 links made after an FP16 stall
   int nesc = 0, nrestart = 0;
@@ -615,15 +612,14 @@ static int cg_mixed_core(const qamd_links *l64, const qamd_links *ls, qamd_field
   qamd_field *t64 = qamd_field_create(lat, QAMD_F64, 3, 1, 1);
   qamd_field *r = qamd_field_create(lat, QAMD_F32, 3, 1, 0);
   qamd_field *p = qamd_field_create(lat, QAMD_F32, 3, 1, 0);
+  qamd_field *sv = qamd_field_create(lat, QAMD_F32, 3, 1, 0);
   qamd_field *xs = qamd_field_create(lat, QAMD_F32, 3, 1, 0);
-  qamd_field *t = qamd_field_create(lat, ps, 3, 1, 1);
-  qamd_field *p16 = half ? qamd_field_create(lat, ps, 3, 1, 0) : nullptr;
-  qamd_field *pin = half ? p16 : p;
+  qamd_field *t = qamd_field_create(lat, QAMD_F32, 3, 1, 1);
   int rc = -1, nupd = 0, its = 0;
   const bool verbose = getenv("QAMD_VERBOSE") != nullptr;
   double bsq = 0, r2 = 0;
   do {
-    if (!r64 || !Ax || !t64 || !r || !p || !xs || !t || (half && !p16) || state_init(st, lat)) break;
+    if (!r64 || !Ax || !t64 || !r || !p || !sv || !xs || !t || state_init(st, lat)) break;
     if (qamd_field_norm2(b, &bsq)) break;
     if (bsq == 0) {
       if (qamd_field_zero(x)) break;
@@ -637,7 +633,8 @@ static int cg_mixed_core(const qamd_links *l64, const qamd_links *ls, qamd_field
     Sh->mode = 1;
     Sh->rsq_old = 1.0;
     if (push_state(st)) break;
-    // true residual r64 = b - A x in FP64, r = r64 in FP32, |r64|^2
+    // true residual r64 = b - A x in FP64, r = r64 in FP32, |r64|^2 in
+    // partial-sum set 0 and on the host
     auto reliable = [&](void) -> int {
       if (apply_normal(l64, st.S, Ax, x, x, t64, m2, nullptr)) return -1;
       hipLaunchKernelGGL(k_ru_resid<QAMD_F32>, grid1(lat), dim3(256), 0, stream(), r64->f, b->f, Ax->f, r->f, st.S,
@@ -646,41 +643,34 @@ static int cg_mixed_core(const qamd_links *l64, const qamd_links *ls, qamd_field
       QAMD_HIP(hipGetLastError());
       return pull_state(st);
     };
+    // the next iteration takes beta = 0: p = r, s = A r
+    auto restart = [&](void) -> int {
+      Sh->restart = 1;
+      return push_state(st);
+    };
     if (reliable()) break;
     r2 = Sh->rsq;
     if (r2 < stop) {
       rc = 0;
       break;
     }
-    // p = r with its 16-bit copy and |p|^2 (partial-sum set 3)
-    auto p_update = [&](int reset) -> int {
-      if (half && ps == QAMD_F16)
-        hipLaunchKernelGGL(k_cg_pn<QAMD_F16>, grid1(lat), dim3(256), 0, stream(), p->f, r->f, p16->f, st.S,
-                           red_set(3), reset);
-      else if (half)
-        hipLaunchKernelGGL(k_cg_pn<QAMD_Q16>, grid1(lat), dim3(256), 0, stream(), p->f, r->f, p16->f, st.S,
-                           red_set(3), reset);
-      else
-        hipLaunchKernelGGL(k_cg_pn<0>, grid1(lat), dim3(256), 0, stream(), p->f, r->f, p->f, st.S, red_set(3),
-                           reset);
-      QAMD_HIP(hipGetLastError());
-      return 0;
-    };
-    if (qamd_field_zero(xs) || p_update(1)) break;
+    if (qamd_field_zero(xs) || qamd_field_zero(p) || qamd_field_zero(sv) || restart()) break;
     const int nb1 = (int)grid1(lat).x;
     int err = 0;
-    while (its < maxits) {
-      // per iteration: t = D_oe p; alpha = rsq / (m^2 |p|^2 + |t|^2); the
-      // fused step x += alpha p, r -= alpha (m^2 p - D_eo t), |r|^2; beta;
-      // p = r + beta p with |p|^2
-      for (int k = 0; k < BATCH && !err; k++) {
-        err |= stag_dslash_launch(lsc, t->f, pin->f, 0.0, nullptr, 1.0, nullptr, stream(), nullptr, 1, false);
-        hipLaunchKernelGGL((k_finish2<1, 1, FinAlpha>), dim3(1), dim3(FT), 0, stream(), red_set(1), red_set(3), nb1,
-                           FinAlpha{st.S}, &st.S->done);
-        err |= stag_cg_step_launch(lsc, t->f, p->f, xs->f, r->f, m2, &st.S->alpha, stream(), &st.S->done, 0);
-        launch_finish<1>(red_set(0), nb1, FinCG{st.S}, stream(), &st.S->done);
-        err |= p_update(0);
+    auto batch = [&]() -> int {
+      int e = 0;
+      for (int k = 0; k < BATCH && !e; k++) {
+        e |= stag_dslash_launch(lsc, t->f, r->f, 0.0, nullptr, 1.0, nullptr, stream(), nullptr, 1, false);
+        hipLaunchKernelGGL((k_finish2<1, 1, FinCGCG>), dim3(1), dim3(FT), 0, stream(), red_set(0), red_set(1), nb1,
+                           FinCGCG{st.S}, &st.S->done);
+        e |= stag_cgcg_step_launch(lsc, t->f, r->f, p->f, sv->f, xs->f, m2, &st.S->alpha, &st.S->beta, stream(),
+                                   &st.S->done, 0);
+        e |= hipGetLastError() != hipSuccess;
       }
+      return e;
+    };
+    while (its < maxits) {
+      err |= batch();
       if (err || pull_state(st)) {
         err = 1;
         break;
@@ -688,8 +678,8 @@ static int cg_mixed_core(const qamd_links *l64, const qamd_links *ls, qamd_field
       its += BATCH;
       double rn = sqrt(Sh->rsq);
       if (verbose)
-        fprintf(stderr, "cg_mixed prec %d its %d rel|r| %.3e max %.3e pAp %.3e alpha %.3e beta %.3e\n", ps, its,
-                rn / sqrt(bsq), Sh->maxrr / sqrt(bsq), Sh->m2 * Sh->nrm[3] + Sh->nrm[0], Sh->alpha, Sh->beta);
+        fprintf(stderr, "cg_mixed prec %d its %d rel|r| %.3e max %.3e rAr %.3e alpha %.3e beta %.3e\n", ps, its,
+                rn / sqrt(bsq), Sh->maxrr / sqrt(bsq), Sh->m2 * Sh->rsq + Sh->nrm[0], Sh->alpha, Sh->beta);
       // update on a delta drop, on convergence, at the end, and when the
       // iterated residual grew well above the last true one (divergence)
       if (!isfinite(Sh->rsq) || rn < delta * Sh->maxrr || Sh->rsq < stop || its >= maxits ||
@@ -712,10 +702,10 @@ static int cg_mixed_core(const qamd_links *l64, const qamd_links *ls, qamd_field
         }
         if (r2 < stop) break;
         // The sloppy operator's error shows up as drift between the iterated
-        // and the true residual.  A large drift leaves p inconsistent with the
-        // new r: restart the search direction.  When a whole cycle did not
-        // reduce the true residual, the sloppy links are too coarse for this
-        // operator: continue with FP32 links.
+        // and the true residual.  A large drift leaves p and s inconsistent
+        // with the new r: restart the search direction.  When a whole cycle
+        // did not reduce the true residual, the sloppy links are too coarse
+        // for this operator: continue with FP32 links.
         const bool stalled = r2 > 0.25 * r2prev;
         if (stalled && lsc->prec != QAMD_F32) {
           if (!l32own) {
@@ -726,24 +716,23 @@ static int cg_mixed_core(const qamd_links *l64, const qamd_links *ls, qamd_field
             }
           }
           lsc = l32own;
-          qamd_field_destroy(t);
-          t = qamd_field_create(lat, QAMD_F32, 3, 1, 1);
-          if (!t) {
-            err = 1;
-            break;
-          }
-          pin = p;
-          half = false;
           nesc++;
           if (verbose) fprintf(stderr, "cg_mixed: FP16 links stalled; continuing with FP32 links\n");
         }
-        if (stalled || r2 > 2.0 * r2s) {
-          if (p_update(1)) {
-            err = 1;
-            break;
-          }
-          nrestart++;
+        // Restart (p = r) at every reliable update.  The single-reduction
+        // recurrences infer p.Ap from the previous iteration's scalars, which
+        // assumes r orthogonal to the old p and a Hermitian sloppy operator;
+        // the true residual is not orthogonal to it (it differs from the
+        // iterated one by the sloppy operator's drift), and with 16-bit links
+        // keeping p after an update made the next cycle diverge (measured at
+        // 32^4, QLL mass 0.02: the residual grew from 9e-2 to 3e-1), also
+        // with p.s measured exactly at the update.  A restart costs ~2% more
+        // iterations; the iteration itself is 3 launches instead of 5.
+        if (restart()) {
+          err = 1;
+          break;
         }
+        nrestart++;
       }
     }
     if (err) break;
@@ -767,8 +756,8 @@ static int cg_mixed_core(const qamd_links *l64, const qamd_links *ls, qamd_field
   qamd_field_destroy(r);
   qamd_field_destroy(p);
   qamd_field_destroy(xs);
+  qamd_field_destroy(sv);
   qamd_field_destroy(t);
-  qamd_field_destroy(p16);
   state_free(st);
   return rc;
 }
