@@ -69,8 +69,9 @@ def parse():
     ap.add_argument("--no-qll-api", action="store_true", help="skip the QLL C API (host array) timings")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--kernel-child", type=str, default=None,
-                    help="KIND,PREC,NRHS[,MODE]: 20 Dslash launches of one kind (stag or wilson) for a "
-                         "profiler; MODE is the tuning mode (1 tuned, >= 2 forces launch shape MODE-2)")
+                    help="KIND,PREC,NRHS[,MODE]: 20 Dslash launches of one kind (stag, stagfb or wilson) "
+                         "for a profiler, after 3 copies of 256 MiB (the counters' scale); MODE is the "
+                         "tuning mode (1 tuned, >= 2 forces launch shape MODE-2)")
     ap.add_argument("--block", type=int, default=8, help="sources in the batched mixed-precision solve")
     ap.add_argument("--dims", type=str, default=None, help="override lattice, e.g. 16,16,16,16")
     ap.add_argument("--precs", type=str, default="0,1,2,3",
@@ -99,16 +100,19 @@ def pmc_child(dims):
 
 def kernel_child(dims, spec):
     """20 launches of one Dslash for a profiler run: asqtad ("stag", 16 terms,
-    fat7/Naik links) or Wilson, precision PREC, NRHS rhs; the launch shape is
-    fixed beforehand (MODE), so every profiled dispatch is the same kernel."""
+    fat7/Naik links; "stagfb", the 8 forward links) or Wilson, precision
+    PREC, NRHS rhs; the launch shape is fixed beforehand (MODE), so every
+    profiled dispatch is the same kernel.  A copy kernel of known size runs
+    first (the scale of the FETCH_SIZE / WRITE_SIZE counters)."""
     import qll_amd as Q
     kind, prec, nrhs = spec.split(",")[:3]
     mode = int(spec.split(",")[3]) if spec.count(",") >= 3 else 1
     prec, nrhs = int(prec), int(nrhs)
     Q.init()
+    Q.stream_copy(COPY_BYTES, 3)
     lat = Q.Lattice(dims)
-    if kind == "stag":
-        L = Q.asqtad_links_device(lat, seed=1)
+    if kind in ("stag", "stagfb"):
+        L = Q.asqtad_links_device(lat, seed=1, kind=Q.STAG_FB if kind == "stagfb" else Q.STAG)
         nc = 3
     else:
         g = Q.Gauge(lat).random_su3(3)
@@ -445,6 +449,29 @@ def main():
                       "GB/s": round(gbs, 1),
                       "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
         print("dslash prec=%d: %s" % (prec, prof[prec]), file=sys.stderr, flush=True)
+
+    # forward-only links (createStaggeredLinksFb): half the link storage; the
+    # backward terms read the other parity's forward links at the neighbour,
+    # so the algorithmic bytes per output site equal the doublestored set's
+    # (16 link matrices, long ones as two rows), reused through L2 / MALL
+    for prec, L in ((0, L64), (1, L32), (3, LQ)):
+        if prec not in precs:
+            continue
+        Lf = Q.asqtad_links_device(lat, seed=1, prec=prec, kind=Q.STAG_FB)
+        yin = Q.Field(lat, prec, 3, 1, Q.ODD).gaussian(5, 100)
+        out = Q.Field(lat, prec, 3, 1, Q.EVEN)
+        Q.time_dslash(Lf, out, yin, args.warmup)
+        ms = Q.time_dslash(Lf, out, yin, args.steps)
+        bs = link_bytes(Lf, prec) + 2 * VEC_BYTES[prec]
+        gbs = bs * vh / (ms * 1e-3) / 1e9
+        key = "%s_fb" % PREC_NAME[prec]
+        prof[key] = {"ms_per_call": round(ms, 5), "bytes_per_site": bs,
+                     "link_bytes_stored": Lf.nbytes, "link_bytes_stored_doublestored": L.nbytes,
+                     "GFLOP/s": round(FLOPS_ASQTAD * vh / (ms * 1e-3) / 1e9, 1),
+                     "GB/s": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+                     "vs_doublestored": round(rates[prec] / ms, 3) if prec in rates else None}
+        print("dslash %s: %s" % (key, prof[key]), file=sys.stderr, flush=True)
+        del Lf, yin, out
 
     # the FP64 Dslash called on host arrays, as a caller of the QLL API's
     # dslash2 uses it: upload the input (lexicographic FP64 host array), one

@@ -130,24 +130,35 @@ int qamd_fat7(qamd_gauge *fl, qamd_gauge *ll, const double coef[6],
 /* plaquettes in the pair order of gaugePlaq (lib/gauge.c:114-183) */
 int qamd_gauge_plaquette(const qamd_gauge *u, double plaq[6]);
 
-/* ---- Dslash link sets (replaces createStaggeredLinks lib/stagDslash.c:422-462
- *      and createWilsonLinks lib/wilsonDslash.c:181-202) ----
- * kind: 0 = asqtad/staggered (8 or 16 terms), 1 = Wilson (8 terms) */
+/* ---- Dslash link sets (replaces createStaggeredLinks lib/stagDslash.c:422-462,
+ *      createStaggeredLinksFb lib/stagDslash.c:464-519 and createWilsonLinks
+ *      lib/wilsonDslash.c:181-202) ----
+ * kind: 0 = asqtad/staggered, doublestored (8 or 16 terms);
+ *       1 = Wilson (8 terms);
+ *       2 = asqtad/staggered, forward links only (4 fat or 8 = fat + long
+ *           terms): half the storage of kind 0; the Dslash forms each
+ *           backward term -U_mu(x-mu)^+ y(x-mu) (long: x-3mu) from the other
+ *           parity's forward link, as dslashFb2T (lib/stagDslash.c:110-176).
+ *           Single-rhs operators and solvers only (no block kernels). */
 qamd_links *qamd_links_create(qamd_lattice *lat, int kind, int prec, int nterms);
 void qamd_links_destroy(qamd_links *l);
-/* host [nterms][V][18] lexicographic, QLL term order */
+/* host [nterms][V][18] lexicographic, QLL term order (kind 2: U_0..U_3,
+ * then L_0..L_3) */
 int qamd_links_from_lex(qamd_links *l, const double *host);
 /* QLL arrays: staggered u[8] (+ u3[8] for 16 terms, term 2k=u[k], 2k+1=u3[k]);
- * Wilson u[8] */
+ * Wilson u[8]; forward-only sets read the forward entries u[2mu] (and
+ * u3[2mu]) only */
 int qamd_links_from_qll(qamd_links *l, const qamd_qll_layout *lay, const void *const u[],
                         const void *const u3[], int host_prec);
 /* doublestore from device fat/long links: t[2mu]=F(x), t[2mu+1]=sgn*F(x-mu)^+
- * (sgn = -1 staggered, +1 Wilson), long links likewise with x-3mu */
+ * (sgn = -1 staggered, +1 Wilson), long links likewise with x-3mu; a
+ * forward-only set stores F_mu(x) and L_mu(x) as they are */
 int qamd_links_from_gauge(qamd_links *l, const qamd_gauge *fat, const qamd_gauge *lng);
 /* copy with precision conversion (same kind and term count) */
 int qamd_links_convert(qamd_links *dst, const qamd_links *src);
 /* 1 if the Dslash on these links rebuilds third rows instead of loading
- * them: set when every long term of a 16-term FP64 staggered link set (the
+ * them: set when every long term of a 16-term (or forward-only 8-term) FP64
+ * staggered link set (the
  * case for fat7/Naik links built from SU(3) gauge fields with phases), or
  * every term of an FP64 Wilson set, is a scale times a unitary matrix to
  * 1e-12, after each import, doublestore or conversion.  FP32/FP16/Q16 copies
@@ -159,6 +170,9 @@ int qamd_links_convert(qamd_links *dst, const qamd_links *src);
  * reconstruction is QUDA's two-row compression, QLL's next-step list names
  * it for the long links. */
 int qamd_links_reconstructs(const qamd_links *l);
+/* device bytes held by a link set (both parities, with the determinant-sign
+ * bytes of a rebuilding set); no counterpart in QLL */
+size_t qamd_links_bytes(const qamd_links *l);
 
 /* ---- Dslash (replaces dslash2 lib/stagDslash.c:531-544 and wilsonDslash
  *      lib/wilsonDslash.c:272-285 for one output parity) ----

@@ -63,6 +63,7 @@ def _configure(L):
     L.orc_wilson_doublestore.argtypes = [i4, _d, _d]
     L.orc_plaquette.argtypes = [i4, _d, _d]
     L.orc_dslash.argtypes = [i4, _d, C.c_int, C.c_double, _d, _d, C.c_int]
+    L.orc_dslash_fb.argtypes = [i4, _d, C.c_void_p, C.c_double, _d, _d, C.c_int]
     L.orc_normal_op.restype = C.c_double
     L.orc_normal_op.argtypes = [i4, _d, C.c_int, C.c_double, _d, _d, _d]
     L.orc_cg.restype = C.c_int
@@ -183,6 +184,19 @@ def dslash(dims, links, mass, y, sub="all", x=None):
     x = np.zeros_like(y) if x is None else x.copy()
     lib().orc_dslash(_L(dims), np.ascontiguousarray(links).reshape(-1), links.shape[0],
                      mass, y.reshape(-1), x.reshape(-1), SUB[sub])
+    return x
+
+
+def dslash_fb(dims, fl, ll, mass, y, sub="all", x=None):
+    """dslash2 on forward-only links (fl[mu], ll[mu] or None), see orc_dslash_fb."""
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    x = np.zeros_like(y) if x is None else x.copy()
+    llp = None
+    if ll is not None:
+        ll = np.ascontiguousarray(ll, dtype=np.float64)
+        llp = ll.ctypes.data
+    lib().orc_dslash_fb(_L(dims), np.ascontiguousarray(fl, dtype=np.float64).reshape(-1), llp,
+                        mass, y.reshape(-1), x.reshape(-1), SUB[sub])
     return x
 
 

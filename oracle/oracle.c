@@ -419,6 +419,39 @@ void orc_dslash(const int L[4], const double *links, int nterms, double mass,
   }
 }
 
+/* dslashFb2T (lib/stagDslash.c:110-176) on the links of createStaggeredLinksFb
+ * (lib/stagDslash.c:464-519): forward links only, fl[mu] (and ll[mu]);
+ *   x(s) = massFac*mass*y(s) + sum_mu fl_mu(s) y(s+mu) - fl_mu(s-mu)^+ y(s-mu)
+ *                            + sum_mu ll_mu(s) y(s+3mu) - ll_mu(s-3mu)^+ y(s-3mu)
+ * (the backward terms by VmeqMAXtimesVXp, :164-170) */
+void orc_dslash_fb(const int L[4], const double *fl, const double *ll, double mass,
+                   const double *y, double *x, int sub)
+{
+  int64_t V = orc_volume(L), n = V * 18;
+  double mf = (sub == 0) ? mass : 0.0;
+#pragma omp parallel for
+  for (int64_t s = 0; s < V; s++) {
+    int p = orc_parity(L, s);
+    if (sub == 1 && p != 0) continue;
+    if (sub == 2 && p != 1) continue;
+    double acc[6];
+    for (int i = 0; i < 6; i++) acc[i] = mf * y[s * 6 + i];
+    for (int naik = 0; naik < (ll ? 2 : 1); naik++) {
+      const double *u = naik ? ll : fl;
+      int hop = naik ? 3 : 1;
+      for (int mu = 0; mu < 4; mu++) {
+        mv_acc(acc, &u[mu * n + s * 18], &y[orc_neighbor(L, s, mu, hop) * 6]);
+        int64_t b = orc_neighbor(L, s, mu, -hop);
+        double t[18], a[6] = {0, 0, 0, 0, 0, 0};
+        madj(t, &u[mu * n + b * 18]);
+        mv_acc(a, t, &y[b * 6]);
+        for (int i = 0; i < 6; i++) acc[i] -= a[i];
+      }
+    }
+    for (int i = 0; i < 6; i++) x[s * 6 + i] = acc[i];
+  }
+}
+
 double orc_norm2_parity(const int L[4], const double *v, int nreal, int parity)
 {
   int64_t V = orc_volume(L);

@@ -53,6 +53,8 @@ void orc_plaquette(const int L[4], const double *u, double plaq[6]);
 /* ---- staggered operator (lib/stagDslash.c) ---- */
 void orc_dslash(const int L[4], const double *links, int nterms, double mass,
                 const double *y, double *x, int sub /*0 all,1 even,2 odd*/);
+void orc_dslash_fb(const int L[4], const double *fl /*[4][V][18]*/, const double *ll /*or NULL*/,
+                   double mass, const double *y, double *x, int sub);
 double orc_normal_op(const int L[4], const double *links, int nterms, double mass,
                      const double *p, double *Ap, double *tmp);
 int orc_cg(const int L[4], const double *links, int nterms, double mass,

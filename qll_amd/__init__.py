@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libqll_amd.so")
 
 F64, F32, F16, Q16 = 0, 1, 2, 3
 EVEN, ODD = 0, 1
-STAG, WILSON = 0, 1
+STAG, WILSON, STAG_FB = 0, 1, 2   # STAG_FB: forward links only (createStaggeredLinksFb)
 
 # fat7 + Naik coefficients {one_link, three_staple, five_staple, seven_staple,
 # lepage, naik} of the benchmark configurations
@@ -91,6 +91,7 @@ def lib():
             "qamd_links_from_gauge": (C.c_int, [_vp, _vp, _vp]),
             "qamd_links_convert": (C.c_int, [_vp, _vp]),
             "qamd_links_reconstructs": (C.c_int, [_vp]),
+            "qamd_links_bytes": (C.c_size_t, [_vp]),
             "qamd_dslash": (C.c_int, [_vp, _vp, _vp, C.c_double, _vp, C.c_double, C.POINTER(C.c_double)]),
             "qamd_stag_normal": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.POINTER(C.c_double)]),
             "qamd_stag_cg": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_double, C.c_int, C.POINTER(SolveInfo)]),
@@ -313,6 +314,11 @@ class Links:
         return self
 
     @property
+    def nbytes(self):
+        """Device bytes of the link set."""
+        return int(lib().qamd_links_bytes(self.h))
+
+    @property
     def reconstructs(self):
         """FP64 Dslash rebuilds the long terms' third rows (fewer bytes)."""
         return bool(lib().qamd_links_reconstructs(self.h))
@@ -420,10 +426,11 @@ def time_dslash(links, out, inp, nrep):
     return ms.value
 
 
-def asqtad_links_device(lat, seed, coef=FAT7_ASQTAD, prec=F64):
-    """Seeded SU(3) -> staggered phases -> fat7/Naik -> 16-term link set, all on the GPU."""
+def asqtad_links_device(lat, seed, coef=FAT7_ASQTAD, prec=F64, kind=STAG):
+    """Seeded SU(3) -> staggered phases -> fat7/Naik -> 16-term link set (or
+    the 8-term forward-only set, kind=STAG_FB), all on the GPU."""
     u = Gauge(lat).random_su3(seed).stag_phases()
     fl, ll = fat7(u, coef)
-    l64 = Links(lat, STAG, F64, 16).from_gauge(fl, ll)
+    l64 = Links(lat, kind, F64, 16 if kind == STAG else 8).from_gauge(fl, ll)
     del fl, ll, u
     return l64 if prec == F64 else l64.converted(prec)

@@ -10,6 +10,13 @@
 // with Naik, term 2k is the 1-hop link u[k] and 2k+1 the 3-hop link u3[k];
 // k = 2mu gathers from x+mu, k = 2mu+1 from x-mu.  In storage the 1-hop terms
 // come first (slots 0-7) and the 3-hop terms after them (slots 8-15).
+//
+// Forward-only link sets (QLL's createStaggeredLinksFb, lib/stagDslash.c:
+// 464-519, applied by dslashFb2T, lib/stagDslash.c:110-176) store U_mu(x) in
+// slot mu and L_mu(x) in slot 4 + mu for both parities, half the doublestored
+// storage; the backward terms -U_mu(x-mu)^+ in(x-mu) and -L_mu(x-3mu)^+
+// in(x-3mu) read the other parity's forward link at the neighbour, the same
+// site index as the input vector they multiply.
 #include "qamd_internal.h"
 #include "qamd_objects.h"
 #include "qamd_kernels.h"
@@ -43,6 +50,8 @@ template <int P, int PO = P, int PI = P> struct StagArgs {
   double *red_out;    // receives {|out|^2, |w|^2}
   const int *skip;    // optional device flag: whole grid returns when set
   const uint8_t *lsign;   // long-term determinant signs when their third rows are rebuilt
+  const S *links_o;       // forward-only sets: the input parity's links (backward terms)
+  const uint8_t *lsign_o; // and their long-term determinant signs
   // single-reduction CG step (CGX = 2, FP32 vectors, input t): Ar =
   // a r + b D t (not stored); p = r + beta p, s = Ar + beta s, x += alpha p,
   // r -= alpha s, |r|^2 -> red[0]; alpha and beta read from the device
@@ -58,13 +67,160 @@ struct FinStore2 {
   }
 };
 
+// input vector at site nb for the term in link slot t (16-bit links: the
+// term's scale folded into the vector)
+template <int P, int PO, int PI>
+__device__ __forceinline__ void stag_vin(const StagArgs<P, PO, PI> &A, int64_t vp, int nb, int t,
+                                         typename Prec<P>::C (&v)[3]) {
+  typedef typename Prec<P>::R R;
+  if constexpr (PI == P) {
+    VecLd<P>::ld(A.in, A.in_norm, vp, nb, v, (P == QAMD_F16 || P == QAMD_Q16) ? A.lscale[t] : (R)1);
+  } else {   // FP32 input, 16-bit links: the term's link scale applied here
+    VecLd<PI>::ld(A.in, A.in_norm, vp, nb, v, (R)1);
+    const R sc = A.lscale[t];
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      v[j].x *= sc;
+      v[j].y *= sc;
+    }
+  }
+}
+
+// acc -= m^+ v  (element (r,c) of m at m[3c+r])
+template <class C>
+__device__ __forceinline__ void cmac_adj_sub(C (&acc)[3], const C (&m)[9], const C (&v)[3]) {
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      const C a = m[3 * r + c];
+      acc[r].x = fma(-a.x, v[c].x, acc[r].x);
+      acc[r].x = fma(-a.y, v[c].y, acc[r].x);
+      acc[r].y = fma(-a.x, v[c].y, acc[r].y);
+      acc[r].y = fma(a.y, v[c].x, acc[r].y);
+    }
+}
+
+// One slot of a forward-only set at the site whose stream starts at base
+// (layout: link_off_fb): whole 16-byte words, the third-row words skipped
+// when they are rebuilt (lng).  NT: non-temporal loads (a site's own forward
+// links, read once); otherwise cached (a neighbour's, whose other slots
+// other sites read).
+template <class S, bool NT>
+__device__ __forceinline__ uvec4_t fb_word(const S *p) {
+  if constexpr (NT) return __builtin_nontemporal_load(reinterpret_cast<const uvec4_t *>(p));
+  else return *reinterpret_cast<const uvec4_t *>(p);
+}
+template <class S, bool NT>
+__device__ __forceinline__ S fb_elem(const S *p) {
+  if constexpr (NT) return ntload(p);
+  else return *p;
+}
+template <int P, bool NT>
+__device__ __forceinline__ void fb_load_slot(const typename Prec<P>::S *base, int sl, bool lng,
+                                             typename Prec<P>::C (&m)[9]) {
+  typedef typename Prec<P>::S S;
+  typedef typename Prec<P>::R R;
+  constexpr int G = LinkG<S>::G;
+  if constexpr (G == 1) {
+#pragma unroll
+    for (int e = 0; e < 9; e++)
+      if (!(lng && e % 3 == 2)) m[e] = to_c(fb_elem<S, NT>(base + link_off_fb<S>(sl, e)), (R)0);
+  } else if constexpr (G == 2) {   // words {0,1} {3,4} {6,7} {2,5}, then element 8
+    S w[2];
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      unpack_group(fb_word<S, NT>(base + link_off_fb<S>(sl, 3 * j)), w);
+      m[3 * j] = to_c(w[0], (R)0);
+      m[3 * j + 1] = to_c(w[1], (R)0);
+    }
+    if (!lng) {
+      unpack_group(fb_word<S, NT>(base + link_off_fb<S>(sl, 2)), w);
+      m[2] = to_c(w[0], (R)0);
+      m[5] = to_c(w[1], (R)0);
+      m[8] = to_c(fb_elem<S, NT>(base + link_off_fb<S>(sl, 8)), (R)0);
+    }
+  } else {   // G == 4: words {0,1,3,4} {6,7,2,5}, then element 8
+    S w[4];
+    unpack_group(fb_word<S, NT>(base + link_off_fb<S>(sl, 0)), w);
+    m[0] = to_c(w[0], (R)0);
+    m[1] = to_c(w[1], (R)0);
+    m[3] = to_c(w[2], (R)0);
+    m[4] = to_c(w[3], (R)0);
+    const S *p1 = base + link_off_fb<S>(sl, 6);
+    if (lng) {   // elements 6, 7: the leading 8 bytes of the second word
+      uint64_t u;
+      if constexpr (NT) u = __builtin_nontemporal_load(reinterpret_cast<const uint64_t *>(p1));
+      else u = *reinterpret_cast<const uint64_t *>(p1);
+      S h[2];
+      __builtin_memcpy(h, &u, 8);
+      m[6] = to_c(h[0], (R)0);
+      m[7] = to_c(h[1], (R)0);
+    } else {
+      unpack_group(fb_word<S, NT>(p1), w);
+      m[6] = to_c(w[0], (R)0);
+      m[7] = to_c(w[1], (R)0);
+      m[2] = to_c(w[2], (R)0);
+      m[5] = to_c(w[3], (R)0);
+      m[8] = to_c(fb_elem<S, NT>(base + link_off_fb<S>(sl, 8)), (R)0);
+    }
+  }
+}
+
+// Forward-only link set (8 storage slots: fat mu in slot mu, long mu in slot
+// 4 + mu; FBT = 4 without long links, 8 with them):
+//   acc = sum_mu U_mu(x) in(x+mu) - U_mu(x-mu)^+ in(x-mu)
+//       + sum_mu L_mu(x) in(x+3mu) - L_mu(x-3mu)^+ in(x-3mu)
+// The forward links are streamed from this site's block; a backward link is
+// read at the neighbour, through the caches (the neighbour's other slots
+// serve the sites on its other sides).  REC: long third rows rebuilt from
+// the first two.
+template <int P, int PO, int PI, int FBT, bool REC>
+__device__ __forceinline__ void stag_acc_fb(const StagArgs<P, PO, PI> &A, const Site &s, int cb, int64_t vp,
+                                            typename Prec<P>::C (&acc)[3]) {
+  typedef typename Prec<P>::R R;
+  typedef typename Prec<P>::C C;
+  typedef typename Prec<P>::S S;
+  const S *lp = link_base<S, 8>(A.links, cb);
+  [[maybe_unused]] unsigned lsb = 0;
+  if constexpr (REC) lsb = A.lsign[cb];
+#pragma unroll
+  for (int sl = 0; sl < FBT; sl++) {
+    const int mu = sl & 3, hop = sl < 4 ? 1 : 3;
+    const bool lng = REC && sl >= 4;
+    C m[9];
+    fb_load_slot<P, true>(lp, sl, lng, m);
+    C v[3];
+    stag_vin(A, vp, nbr_cb(A.g, s, mu, hop), sl, v);
+    if (lng) long_row3<C, R>(m, (lsb >> (sl - 4)) & 1);
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+      for (int c = 0; c < 3; c++) cmac<C, R>(acc[r], m[3 * c + r], v[c]);
+  }
+#pragma unroll
+  for (int sl = 0; sl < FBT; sl++) {
+    const int mu = sl & 3, hop = sl < 4 ? 1 : 3;
+    const bool lng = REC && sl >= 4;
+    const int nb = nbr_cb(A.g, s, mu, -hop);
+    C m[9];
+    fb_load_slot<P, false>(link_base<S, 8>(A.links_o, nb), sl, lng, m);
+    C v[3];
+    stag_vin(A, vp, nb, sl, v);
+    if (lng) long_row3<C, R>(m, (A.lsign_o[nb] >> (sl - 4)) & 1);
+    cmac_adj_sub(acc, m, v);
+  }
+}
+
 // REC (16 terms): the long terms' third rows are not loaded but rebuilt
-// from the first two (FP64: 2016 instead of 2400 bytes per site)
-template <int P, int PO, int NT, bool RED, bool REC = false, int CGX = 0, int PI = P>
+// from the first two (FP64: 2016 instead of 2400 bytes per site).
+// FBT != 0: forward-only link set (NT = 8 storage slots), see stag_acc_fb.
+template <int P, int PO, int NT, bool RED, bool REC = false, int CGX = 0, int PI = P, int FBT = 0>
 __global__ void __launch_bounds__(256) k_stag_dslash(StagArgs<P, PO, PI> A) {
   static_assert(PI == P || (PI == QAMD_F32 && (P == QAMD_F16 || P == QAMD_Q16)),
                 "input precision: the links' or FP32 with 16-bit links");
   static_assert(!CGX || (PO == QAMD_F32 && RED), "fused CG step: FP32 vectors, with the |r|^2 sums");
+  static_assert(!FBT || (NT == 8 && (FBT == 4 || FBT == 8) && (!REC || FBT == 8)), "forward-only link set");
   typedef typename Prec<P>::R R;
   typedef typename Prec<P>::C C;
   typedef typename Prec<P>::S S;
@@ -76,15 +232,16 @@ __global__ void __launch_bounds__(256) k_stag_dslash(StagArgs<P, PO, PI> A) {
   if (cb < A.g.vh) {
     Site s = site_of(A.g, cb, A.par);
     constexpr int G = LinkG<S>::G;
-    static_assert(!REC || NT == 16, "long-row rebuild: 16-term links only");
-    const S *lp = link_base<S, NT>(A.links, cb);
-    [[maybe_unused]] unsigned lsb = 0;
-    if constexpr (REC) lsb = A.lsign[cb];
+    static_assert(!REC || NT == 16 || FBT, "long-row rebuild: 16-term or forward-only links");
     C acc[3];
 #pragma unroll
     for (int j = 0; j < 3; j++) acc[j].x = acc[j].y = (R)0;
+    if constexpr (FBT) stag_acc_fb<P, PO, PI, FBT, REC>(A, s, cb, vp, acc);
+    [[maybe_unused]] const S *lp = link_base<S, NT>(A.links, cb);
+    [[maybe_unused]] unsigned lsb = 0;
+    if constexpr (REC && !FBT) lsb = A.lsign[cb];
 #pragma unroll
-    for (int so = 0; so < NT / G; so++) {
+    for (int so = 0; so < (FBT ? 0 : NT / G); so++) {
       // slots sg*G .. sg*G+G-1: all 1-hop terms (slot < 8) or all 3-hop ones
       const int sg = group_at<G, NT>(so);
       const bool lng = REC && sg * G >= 8;
@@ -104,17 +261,7 @@ __global__ void __launch_bounds__(256) k_stag_dslash(StagArgs<P, PO, PI> A) {
         const int h = (k & 1) ? -hop : hop;
         int nb = nbr_cb(A.g, s, mu, h);
         C v[3];
-        if constexpr (PI == P) {
-          VecLd<P>::ld(A.in, A.in_norm, vp, nb, v, (P == QAMD_F16 || P == QAMD_Q16) ? A.lscale[t] : (R)1);
-        } else {   // FP32 input, 16-bit links: the term's link scale applied here
-          VecLd<PI>::ld(A.in, A.in_norm, vp, nb, v, (R)1);
-          const R sc = A.lscale[t];
-#pragma unroll
-          for (int j = 0; j < 3; j++) {
-            v[j].x *= sc;
-            v[j].y *= sc;
-          }
-        }
+        stag_vin(A, vp, nb, t, v);
         C m[9];
 #pragma unroll
         for (int e = 0; e < 9; e++)
@@ -181,9 +328,23 @@ __global__ void __launch_bounds__(256) k_stag_dslash(StagArgs<P, PO, PI> A) {
   if (RED) grid_partial<2>(red, A.rs);
 }
 
-template <int P, int PO, int NT, int PI>
+template <int P, int PO, int NT, int PI, int FBT = 0>
 static void launch_t(const StagArgs<P, PO, PI> &A, bool red, hipStream_t st) {
   int nb = (int)((A.g.vh + 255) / 256);
+  if constexpr (FBT) {
+    if (FBT == 8 && A.lsign) {
+      if (red)
+        hipLaunchKernelGGL((k_stag_dslash<P, PO, 8, true, FBT == 8, 0, PI, FBT>), dim3(nb), dim3(256), 0, st, A);
+      else
+        hipLaunchKernelGGL((k_stag_dslash<P, PO, 8, false, FBT == 8, 0, PI, FBT>), dim3(nb), dim3(256), 0, st, A);
+      return;
+    }
+    if (red)
+      hipLaunchKernelGGL((k_stag_dslash<P, PO, 8, true, false, 0, PI, FBT>), dim3(nb), dim3(256), 0, st, A);
+    else
+      hipLaunchKernelGGL((k_stag_dslash<P, PO, 8, false, false, 0, PI, FBT>), dim3(nb), dim3(256), 0, st, A);
+    return;
+  }
   if constexpr (NT == 16) {
     if (A.lsign) {
       if (red)
@@ -220,7 +381,11 @@ static int launch_p(const qamd_links *l, Field &out, const Field &in, double a, 
   A.a = a;
   A.b = b;
   A.par = out.parity;
-  A.lsign = (l->recon && l->nterms == 16) ? l->lsign[out.parity] : nullptr;
+  A.lsign = (l->recon && (l->nterms == 16 || l->kind == 2)) ? l->lsign[out.parity] : nullptr;
+  if (l->kind == 2) {
+    A.links_o = (const S *)l->data[1 - out.parity];
+    A.lsign_o = A.lsign ? l->lsign[1 - out.parity] : nullptr;
+  }
   // Q16 links are read as raw integers: fold 1/32767 into the term scale
   for (int t = 0; t < 16; t++) A.lscale[t] = (P == QAMD_Q16) ? l->scale[t] / Q16_ONE : l->scale[t];
   const bool red = red_out != nullptr || !finish;
@@ -234,8 +399,14 @@ static int launch_p(const qamd_links *l, Field &out, const Field &in, double a, 
     }
     A.red_out = red_out;
   }
-  if (l->nterms == 16) launch_t<P, PO, 16, PI>(A, red, st);
-  else launch_t<P, PO, 8, PI>(A, red, st);
+  if (l->kind == 2) {
+    if (l->nterms == 8) launch_t<P, PO, 8, PI, 8>(A, red, st);
+    else launch_t<P, PO, 8, PI, 4>(A, red, st);
+  } else if (l->nterms == 16) {
+    launch_t<P, PO, 16, PI>(A, red, st);
+  } else {
+    launch_t<P, PO, 8, PI>(A, red, st);
+  }
   if (red && finish) launch_finish<2>(A.rs, nb, FinStore2{red_out}, st, skip);
   QAMD_HIP(hipGetLastError());
   return 0;
@@ -244,7 +415,7 @@ static int launch_p(const qamd_links *l, Field &out, const Field &in, double a, 
 int stag_dslash_launch(const qamd_links *l, Field &out, const Field &in, double a, const Field *w,
                        double b, double *red_out, hipStream_t st, const int *skip, int pset, bool finish,
                        const RedScratch *rs_fixed) {
-  if (l->kind != 0) {
+  if (l->kind != 0 && l->kind != 2) {
     set_error("stag_dslash: links are not staggered");
     return -1;
   }
@@ -307,7 +478,11 @@ static int cgcg_p(const qamd_links *l, const Field &t, Field &r, Field &p, Field
   A.a = a;
   A.b = -1.0;
   A.par = r.parity;
-  A.lsign = (l->recon && l->nterms == 16) ? l->lsign[r.parity] : nullptr;
+  A.lsign = l->recon ? l->lsign[r.parity] : nullptr;
+  if (l->kind == 2) {
+    A.links_o = (const S *)l->data[1 - r.parity];
+    A.lsign_o = A.lsign ? l->lsign[1 - r.parity] : nullptr;
+  }
   for (int k = 0; k < 16; k++) A.lscale[k] = (P == QAMD_Q16) ? l->scale[k] / Q16_ONE : l->scale[k];
   A.cgr = (float2 *)r.data;
   A.cgp = (float2 *)p.data;
@@ -318,20 +493,28 @@ static int cgcg_p(const qamd_links *l, const Field &t, Field &r, Field &p, Field
   const int nb = (int)((A.g.vh + 255) / 256);
   if (red_reserve(nb)) return -1;
   A.rs = red_set(pset);
-  if (A.lsign)
+  if (l->kind == 2) {
+    if (A.lsign)
+      hipLaunchKernelGGL((k_stag_dslash<P, QAMD_F32, 8, true, true, 2, QAMD_F32, 8>), dim3(nb), dim3(256), 0, st, A);
+    else
+      hipLaunchKernelGGL((k_stag_dslash<P, QAMD_F32, 8, true, false, 2, QAMD_F32, 8>), dim3(nb), dim3(256), 0, st, A);
+  } else if (A.lsign) {
     hipLaunchKernelGGL((k_stag_dslash<P, QAMD_F32, 16, true, true, 2, QAMD_F32>), dim3(nb), dim3(256), 0, st, A);
-  else
+  } else {
     hipLaunchKernelGGL((k_stag_dslash<P, QAMD_F32, 16, true, false, 2, QAMD_F32>), dim3(nb), dim3(256), 0, st, A);
+  }
   QAMD_HIP(hipGetLastError());
   return 0;
 }
 
 int stag_cgcg_step_launch(const qamd_links *l, const Field &t, Field &r, Field &p, Field &s, Field &x, double a,
                           const double *alpha, const double *beta, hipStream_t st, const int *skip, int pset) {
-  if (l->kind != 0 || l->nterms != 16 || l->prec == QAMD_F64 || t.prec != QAMD_F32 || t.parity == r.parity ||
+  if (!((l->kind == 0 && l->nterms == 16) || (l->kind == 2 && l->nterms == 8)) || l->prec == QAMD_F64 ||
+      t.prec != QAMD_F32 || t.parity == r.parity ||
       r.prec != QAMD_F32 || p.prec != QAMD_F32 || s.prec != QAMD_F32 || x.prec != QAMD_F32 ||
       p.parity != r.parity || s.parity != r.parity || x.parity != r.parity || t.nrhs != 1 || r.nrhs != 1) {
-    set_error("stag_cgcg_step: needs 16-term sloppy links and single-rhs FP32 t (opposite parity) and r, p, s, x");
+    set_error("stag_cgcg_step: needs sloppy asqtad links (16 terms or forward-only fat + long) and single-rhs "
+              "FP32 t (opposite parity) and r, p, s, x");
     return -1;
   }
   switch (l->prec) {
@@ -384,7 +567,7 @@ int qamd_dslash_noalloc(const qamd_links *l, qamd_field *out, const qamd_field *
     set_error("dslash_noalloc: call qamd_init first (this entry point allocates nothing)");
     return -1;
   }
-  if (l->kind != 0 || in->f.nrhs != 1 || out->f.nrhs != 1) {
+  if ((l->kind != 0 && l->kind != 2) || in->f.nrhs != 1 || out->f.nrhs != 1) {
     set_error("dslash_noalloc: needs staggered links and single-rhs fields");
     return -1;
   }

@@ -11,7 +11,11 @@
 
 namespace qamd {
 
-static int64_t link_elems(const qamd_links *l) { return l->lat->g.vh_pad * l->nterms * 9; }
+// storage slots per site: the term count, except forward-only staggered sets
+// (kind 2), which always hold 8 slots (fat mu in slot mu, long mu in slot
+// 4 + mu; without Naik terms the long slots stay zero)
+static int nslots(const qamd_links *l) { return l->kind == 2 ? 8 : l->nterms; }
+static int64_t link_elems(const qamd_links *l) { return l->lat->g.vh_pad * nslots(l) * 9; }
 
 template <class S> __device__ inline S mk(double2 v);
 template <> __device__ inline double2 mk<double2>(double2 v) { return v; }
@@ -36,9 +40,27 @@ static_assert(link_rows_ok<1>(16) && link_rows_ok<2>(16) && link_rows_ok<4>(16) 
                   link_rows_ok<2>(8) && link_rows_ok<4>(8),
               "link block rows");
 
+// The forward-only layout is a permutation of the block's 8 x 9 elements; a
+// slot's first two rows fill the leading positions of its words.
+template <int G>
+constexpr bool fb_rows_ok() {
+  bool seen[8 * 9] = {};
+  for (int sl = 0; sl < 8; sl++)
+    for (int k = 0; k < 9; k++) {
+      const int r = fb_row<G>(sl, k), q = fb_pos<G>(sl, k), i = r * G + q;
+      if (q < 0 || q >= G || i < 0 || i >= 8 * 9 || seen[i]) return false;
+      seen[i] = true;
+      if (G > 1 && k != 8 && r / (G == 2 ? 4 : 2) != sl) return false;
+      if (G == 2 && k % 3 != 2 && fb_row<G>(sl, k) % 4 == 3) return false;
+      if (G == 4 && k % 3 != 2 && k >= 6 && q >= 2) return false;
+    }
+  return true;
+}
+static_assert(fb_rows_ok<1>() && fb_rows_ok<2>() && fb_rows_ok<4>(), "forward-only link rows");
+
 // one term from a host-ordered staging array into the blocked layout
 template <class H, class S>
-__global__ void k_link_import(S *d0, S *d1, const H *src, Geom g, int t, int nterm, SrcMap m) {
+__global__ void k_link_import(S *d0, S *d1, const H *src, Geom g, int t, int nterm, bool fb, SrcMap m) {
   int cb = blockIdx.x * blockDim.x + threadIdx.x;
   int par = blockIdx.y;
   if (cb >= g.vh) return;
@@ -55,8 +77,27 @@ __global__ void k_link_import(S *d0, S *d1, const H *src, Geom g, int t, int nte
   }
   S *d = par ? d1 : d0;
   for (int e = 0; e < 9; e++)
-    d[link_idx<S>(cb, t, e, nterm)] =
+    d[link_idx<S>(cb, t, e, nterm, fb)] =
         mk<S>(make_double2((double)src[base + 2 * e * stride], (double)src[base + (2 * e + 1) * stride]));
+}
+
+// forward-only staggered set from FP64 gauge fields: slot mu = fat U_mu(x),
+// slot 4 + mu = long L_mu(x) (zero without long links), at the sites of
+// each parity
+template <class S>
+__global__ void k_fb_from_gauge(S *d0, S *d1, const double2 *fat, const double2 *lng, Geom g) {
+  int cb = blockIdx.x * blockDim.x + threadIdx.x;
+  int par = blockIdx.y;
+  if (cb >= g.vh) return;
+  S *d = par ? d1 : d0;
+  const int64_t vp = g.vh_pad;
+  for (int mu = 0; mu < 4; mu++) {
+    const int64_t o = ((int64_t)(mu * 2 + par) * 9) * vp + cb;
+    for (int e = 0; e < 9; e++) {
+      d[link_idx<S>(cb, mu, e, 8, true)] = mk<S>(fat[o + e * vp]);
+      d[link_idx<S>(cb, 4 + mu, e, 8, true)] = mk<S>(lng ? lng[o + e * vp] : make_double2(0.0, 0.0));
+    }
+  }
 }
 
 // doublestore from FP64 gauge fields: term for (k, naik) at output parity par
@@ -108,7 +149,7 @@ template <> __device__ inline double2 ld2<short2>(const short2 *p, int64_t i) { 
 // uints).  Each wavefront reduces one 64-site block (all terms) in registers
 // and issues one atomic per term.
 template <class S>
-__global__ void k_link_absmax(const S *src, int64_t nblk, int nterm, unsigned *mx) {
+__global__ void k_link_absmax(const S *src, int64_t nblk, int nterm, bool fb, unsigned *mx) {
   int lane = threadIdx.x & 63;
   int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
@@ -117,7 +158,7 @@ __global__ void k_link_absmax(const S *src, int64_t nblk, int nterm, unsigned *m
   for (int64_t b = wave; b < nblk; b += nwaves)
     for (int t = 0; t < nterm; t++)
       for (int e = 0; e < 9; e++) {
-        double2 v = ld2<S>(src, link_idx<S>(b * 64 + lane, t, e, nterm));
+        double2 v = ld2<S>(src, link_idx<S>(b * 64 + lane, t, e, nterm, fb));
         m[t] = fmaxf(m[t], (float)fmax(fabs(v.x), fabs(v.y)));
       }
   for (int t = 0; t < nterm; t++) {
@@ -128,7 +169,7 @@ __global__ void k_link_absmax(const S *src, int64_t nblk, int nterm, unsigned *m
 }
 
 template <class SI, class SO>
-__global__ void k_link_convert(SO *dst, const SI *src, int64_t n, int nterm, const float *iscale,
+__global__ void k_link_convert(SO *dst, const SI *src, int64_t n, int nterm, bool fb, const float *iscale,
                                const float *oscale) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     // (site block, term, element, lane) of work item i; consecutive threads
@@ -138,8 +179,8 @@ __global__ void k_link_convert(SO *dst, const SI *src, int64_t n, int nterm, con
     q /= 9;
     const int t = (int)(q % nterm);
     const int64_t cb = (q / nterm) * 64 + lane;
-    const int64_t o = link_idx<SO>(cb, t, k, nterm);
-    double2 v = ld2<SI>(src, link_idx<SI>(cb, t, k, nterm));
+    const int64_t o = link_idx<SO>(cb, t, k, nterm, fb);
+    double2 v = ld2<SI>(src, link_idx<SI>(cb, t, k, nterm, fb));
     double f = (double)iscale[t] / (double)oscale[t];
     v.x *= f;
     v.y *= f;
@@ -150,17 +191,19 @@ __global__ void k_link_convert(SO *dst, const SI *src, int64_t n, int nterm, con
 }
 
 // Terms of the form c * W with W unitary (c > 0, det W = +-1): then the
-// third row is det W * conj(row0 x row1) / c.  Checks the eight terms
-// t0 + dt*k of FP64 links (16-term staggered: the long terms 2k+1; Wilson:
-// all eight) to 1e-12 relative, records det W per (site, k) and flags any
-// term that is not of that form.
-__global__ void k_long_check(const double2 *d, uint8_t *sign, Geom g, int nterm, int t0, int dt, unsigned *bad) {
+// third row is det W * conj(row0 x row1) / c.  Checks the nk terms
+// t0 + dt*k of FP64 links (16-term staggered: the eight long terms 2k+1;
+// Wilson: all eight; forward-only staggered: the four long slots 4 + mu) to
+// 1e-12 relative, records det W per (site, k) and flags any term that is not
+// of that form.
+__global__ void k_long_check(const double2 *d, uint8_t *sign, Geom g, int nterm, bool fb, int t0, int dt, int nk,
+                             unsigned *bad) {
   const int cb = blockIdx.x * blockDim.x + threadIdx.x;
   if (cb >= g.vh) return;
   unsigned bits = 0, fail = 0;
-  for (int k = 0; k < 8; k++) {
+  for (int k = 0; k < nk; k++) {
     double2 m[9];
-    for (int e = 0; e < 9; e++) m[e] = d[link_idx<double2>(cb, t0 + dt * k, e, nterm)];
+    for (int e = 0; e < 9; e++) m[e] = d[link_idx<double2>(cb, t0 + dt * k, e, nterm, fb)];
     // row r holds elements 3c + r
     double c2 = 0, n2 = 0;
     double2 dot = make_double2(0, 0);
@@ -200,15 +243,17 @@ __global__ void k_long_check(const double2 *d, uint8_t *sign, Geom g, int nterm,
 }
 
 // link sets whose Dslash may rebuild third rows: 16-term staggered (the long
-// terms) and Wilson (every term)
-static bool rebuild_kind(const qamd_links *l) { return (l->kind == 0 && l->nterms == 16) || l->kind == 1; }
+// terms), Wilson (every term) and forward-only staggered with long links
+static bool rebuild_kind(const qamd_links *l) {
+  return (l->kind == 0 && l->nterms == 16) || l->kind == 1 || (l->kind == 2 && l->nterms == 8);
+}
 
 // after FP64 links change: decide whether the Dslash may rebuild the third
 // rows (every checked term passes)
 static int long_link_check(qamd_links *l) {
   l->recon = 0;
   if (!rebuild_kind(l) || l->prec != QAMD_F64) return 0;
-  const int t0 = l->kind == 0 ? 1 : 0, dt = l->kind == 0 ? 2 : 1;
+  const int t0 = l->kind == 0 ? 1 : l->kind == 2 ? 4 : 0, dt = l->kind == 0 ? 2 : 1, nk = l->kind == 2 ? 4 : 8;
   const Geom &g = l->lat->g;
   for (int p = 0; p < 2; p++)
     if (!l->lsign[p] && !(l->lsign[p] = (uint8_t *)dev_alloc(g.vh_pad))) return -1;
@@ -218,7 +263,7 @@ static int long_link_check(qamd_links *l) {
   hipError_t e = hipMemsetAsync(bad, 0, sizeof(unsigned), stream());
   for (int p = 0; p < 2 && e == hipSuccess; p++) {
     hipLaunchKernelGGL(k_long_check, dim3((unsigned)((g.vh + 255) / 256)), dim3(256), 0, stream(),
-                       (const double2 *)l->data[p], l->lsign[p], g, l->nterms, t0, dt, bad);
+                       (const double2 *)l->data[p], l->lsign[p], g, nslots(l), l->kind == 2, t0, dt, nk, bad);
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipMemcpyAsync(&hb, bad, sizeof hb, hipMemcpyDeviceToHost, stream());
@@ -261,7 +306,11 @@ qamd_links *qamd_links_create(qamd_lattice *lat, int kind, int prec, int nterms)
   if (!lat) { set_error("links_create: NULL lattice"); return nullptr; }
   if (kind == 0 && nterms != 8 && nterms != 16) { set_error("staggered links need 8 or 16 terms"); return nullptr; }
   if (kind == 1 && nterms != 8) { set_error("Wilson links need 8 terms"); return nullptr; }
-  if (kind != 0 && kind != 1) { set_error("links_create: bad kind"); return nullptr; }
+  if (kind == 2 && nterms != 4 && nterms != 8) {
+    set_error("forward-only staggered links need 4 (fat) or 8 (fat + long) terms");
+    return nullptr;
+  }
+  if (kind < 0 || kind > 2) { set_error("links_create: bad kind"); return nullptr; }
   if (prec < QAMD_F64 || prec > QAMD_Q16) { set_error("links_create: bad precision"); return nullptr; }
   qamd_links *l = (qamd_links *)calloc(1, sizeof(qamd_links));
   l->lat = lat;
@@ -316,20 +365,22 @@ static int import_terms(qamd_links *l, const void *const terms[], int hprec, Src
       rc = -1;
       break;
     }
+    const int ns = nslots(l);
+    const bool fb = l->kind == 2;
     if (dst->prec == QAMD_F64) {
       if (hprec == QAMD_F64)
         hipLaunchKernelGGL((k_link_import<double, double2>), grid, dim3(256), 0, stream(), (double2 *)dst->data[0],
-                           (double2 *)dst->data[1], (const double *)stage, g, t, l->nterms, m);
+                           (double2 *)dst->data[1], (const double *)stage, g, t, ns, fb, m);
       else
         hipLaunchKernelGGL((k_link_import<float, double2>), grid, dim3(256), 0, stream(), (double2 *)dst->data[0],
-                           (double2 *)dst->data[1], (const float *)stage, g, t, l->nterms, m);
+                           (double2 *)dst->data[1], (const float *)stage, g, t, ns, fb, m);
     } else {
       if (hprec == QAMD_F64)
         hipLaunchKernelGGL((k_link_import<double, float2>), grid, dim3(256), 0, stream(), (float2 *)dst->data[0],
-                           (float2 *)dst->data[1], (const double *)stage, g, t, l->nterms, m);
+                           (float2 *)dst->data[1], (const double *)stage, g, t, ns, fb, m);
       else
         hipLaunchKernelGGL((k_link_import<float, float2>), grid, dim3(256), 0, stream(), (float2 *)dst->data[0],
-                           (float2 *)dst->data[1], (const float *)stage, g, t, l->nterms, m);
+                           (float2 *)dst->data[1], (const float *)stage, g, t, ns, fb, m);
     }
     if (hipGetLastError() != hipSuccess) {
       set_error("links import kernel failed");
@@ -372,7 +423,16 @@ int qamd_links_from_qll(qamd_links *l, const qamd_qll_layout *lay, const void *c
   m.kind = 1;
   m.lay = to_lay(lay);
   const void *terms[16];
-  if (l->nterms == 16) {
+  if (l->kind == 2) {   // forward links only: QLL's u[2mu] (and u3[2mu])
+    if (l->nterms == 8 && !u3) {
+      set_error("links_from_qll: forward-only links with Naik terms need u3");
+      return -1;
+    }
+    for (int mu = 0; mu < 4; mu++) {
+      terms[mu] = u[2 * mu];
+      if (l->nterms == 8) terms[4 + mu] = u3[2 * mu];
+    }
+  } else if (l->nterms == 16) {
     if (!u3) {
       set_error("links_from_qll: 16 terms need u3");
       return -1;
@@ -389,7 +449,7 @@ int qamd_links_from_qll(qamd_links *l, const qamd_qll_layout *lay, const void *c
 }
 
 int qamd_links_from_gauge(qamd_links *l, const qamd_gauge *fat, const qamd_gauge *lng) {
-  if (l->nterms == 16 && !lng) {
+  if ((l->nterms == 16 || (l->kind == 2 && l->nterms == 8)) && !lng) {
     set_error("links_from_gauge: 16 terms need long links");
     return -1;
   }
@@ -402,7 +462,15 @@ int qamd_links_from_gauge(qamd_links *l, const qamd_gauge *fat, const qamd_gauge
   dim3 grid((unsigned)((g.vh + 255) / 256), 2);
   double sgn = l->kind == 0 ? -1.0 : 1.0;
   const double2 *lp = lng ? lng->d : fat->d;
-  if (dst->prec == QAMD_F64)
+  if (l->kind == 2) {
+    const double2 *lf = l->nterms == 8 ? lng->d : nullptr;
+    if (dst->prec == QAMD_F64)
+      hipLaunchKernelGGL(k_fb_from_gauge<double2>, grid, dim3(256), 0, stream(), (double2 *)dst->data[0],
+                         (double2 *)dst->data[1], fat->d, lf, g);
+    else
+      hipLaunchKernelGGL(k_fb_from_gauge<float2>, grid, dim3(256), 0, stream(), (float2 *)dst->data[0],
+                         (float2 *)dst->data[1], fat->d, lf, g);
+  } else if (dst->prec == QAMD_F64)
     hipLaunchKernelGGL(k_link_from_gauge<double2>, grid, dim3(256), 0, stream(), (double2 *)dst->data[0],
                        (double2 *)dst->data[1], fat->d, lp, g, l->nterms, sgn);
   else
@@ -425,6 +493,14 @@ int qamd_links_from_gauge(qamd_links *l, const qamd_gauge *fat, const qamd_gauge
 
 int qamd_links_reconstructs(const qamd_links *l) { return l && l->recon ? 1 : 0; }
 
+size_t qamd_links_bytes(const qamd_links *l) {
+  if (!l) return 0;
+  size_t n = 2 * (size_t)link_elems(l) * prec_bytes(l->prec);
+  for (int p = 0; p < 2; p++)
+    if (l->lsign[p]) n += (size_t)l->lat->g.vh_pad;
+  return n;
+}
+
 int qamd_links_convert(qamd_links *dst, const qamd_links *src) {
   if (dst->lat->g.V != src->lat->g.V || dst->nterms != src->nterms || dst->kind != src->kind) {
     set_error("links_convert: mismatch");
@@ -445,18 +521,18 @@ int qamd_links_convert(qamd_links *dst, const qamd_links *src) {
     QAMD_HIP(hipMemsetAsync(mx, 0, 16 * sizeof(unsigned), stream()));
     for (int p = 0; p < 2; p++) {
       if (src->prec == QAMD_F64)
-        hipLaunchKernelGGL(k_link_absmax<double2>, dim3(2048), dim3(256), 0, stream(), (const double2 *)src->data[p], nblk, src->nterms, mx);
+        hipLaunchKernelGGL(k_link_absmax<double2>, dim3(2048), dim3(256), 0, stream(), (const double2 *)src->data[p], nblk, nslots(src), src->kind == 2, mx);
       else if (src->prec == QAMD_F32)
-        hipLaunchKernelGGL(k_link_absmax<float2>, dim3(2048), dim3(256), 0, stream(), (const float2 *)src->data[p], nblk, src->nterms, mx);
+        hipLaunchKernelGGL(k_link_absmax<float2>, dim3(2048), dim3(256), 0, stream(), (const float2 *)src->data[p], nblk, nslots(src), src->kind == 2, mx);
       else if (src->prec == QAMD_F16)
-        hipLaunchKernelGGL(k_link_absmax<__half2>, dim3(2048), dim3(256), 0, stream(), (const __half2 *)src->data[p], nblk, src->nterms, mx);
+        hipLaunchKernelGGL(k_link_absmax<__half2>, dim3(2048), dim3(256), 0, stream(), (const __half2 *)src->data[p], nblk, nslots(src), src->kind == 2, mx);
       else
-        hipLaunchKernelGGL(k_link_absmax<short2>, dim3(2048), dim3(256), 0, stream(), (const short2 *)src->data[p], nblk, src->nterms, mx);
+        hipLaunchKernelGGL(k_link_absmax<short2>, dim3(2048), dim3(256), 0, stream(), (const short2 *)src->data[p], nblk, nslots(src), src->kind == 2, mx);
     }
     unsigned hm[16];
     QAMD_HIP(hipMemcpyAsync(hm, mx, sizeof hm, hipMemcpyDeviceToHost, stream()));
     QAMD_HIP(hipStreamSynchronize(stream()));
-    for (int t = 0; t < src->nterms; t++) {
+    for (int t = 0; t < nslots(src); t++) {
       float m;
       memcpy(&m, &hm[t], 4);
       m *= src->scale[t];
@@ -468,7 +544,7 @@ int qamd_links_convert(qamd_links *dst, const qamd_links *src) {
   for (int p = 0; p < 2; p++) {
 #define CONV(SI, SO)                                                                                   \
   hipLaunchKernelGGL((k_link_convert<SI, SO>), dim3(2048), dim3(256), 0, stream(), (SO *)dst->data[p], \
-                     (const SI *)src->data[p], n, src->nterms, sc, sc + 16)
+                     (const SI *)src->data[p], n, nslots(src), src->kind == 2, sc, sc + 16)
     int a = src->prec, b = dst->prec;
     if (a == QAMD_F64 && b == QAMD_F64) CONV(double2, double2);
     else if (a == QAMD_F64 && b == QAMD_F32) CONV(double2, float2);

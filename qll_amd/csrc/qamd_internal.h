@@ -215,12 +215,46 @@ __host__ __device__ constexpr int link_row(int sl, int k, int nterm) {
   return (k % 3 != 2) ? base + g * 6 + (k / 3) * 2 + k % 3 : base + (8 / G) * 6 + g * 3 + k / 3;
 }
 
-// blocked link index: element k (0..8) of term t for site cb
+// Forward-only (8-slot) sets keep each slot's elements together instead, so
+// that a Dslash reading a single slot of a neighbour's set (a backward term)
+// uses every byte of its 16-byte words.  Per 64-site block (rows of 64 sites
+// x 16 bytes):
+//   G = 1: the 8-term layout above;
+//   G = 2: slot sl in rows 4sl..4sl+3 as the words {0,1} {3,4} {6,7} {2,5},
+//          element 8 of slots 2j, 2j+1 in row 32 + j;
+//   G = 4: slot sl in rows 2sl, 2sl+1 as {0,1,3,4} {6,7,2,5}, element 8 of
+//          slots 4j..4j+3 in row 16 + j.
+// The first two rows of a term (elements 0,1,3,4,6,7) are the leading
+// elements of its words.
+template <int G>
+__host__ __device__ constexpr int fb_row(int sl, int k) {
+  if (G == 1) return link_row<1>(sl, k, 8);
+  if (k == 8) return 8 * (8 / G) + sl / G;
+  if (G == 2) return sl * 4 + (k % 3 == 2 ? 3 : k / 3);
+  return sl * 2 + ((k % 3 == 2 || k >= 6) ? 1 : 0);
+}
+template <int G>
+__host__ __device__ constexpr int fb_pos(int sl, int k) {
+  if (G == 1) return 0;
+  if (k == 8) return sl % G;
+  if (G == 2) return (k % 3 == 2) ? k / 3 : k % 3;
+  return (k % 3 == 2) ? 2 + k / 3 : (k >= 6 ? k - 6 : k % 3 + 2 * (k / 3));
+}
+
+// blocked link index: element k (0..8) of term t for site cb (fb: the
+// forward-only layout, nterm = 8)
 template <class S>
-__host__ __device__ __forceinline__ int64_t link_idx(int64_t cb, int t, int k, int nterm) {
+__host__ __device__ __forceinline__ int64_t link_idx(int64_t cb, int t, int k, int nterm, bool fb = false) {
   constexpr int G = LinkG<S>::G;
+  if (fb) return ((((cb >> 6) * (8 / G) * 9 + fb_row<G>(t, k)) << 6) + (cb & 63)) * G + fb_pos<G>(t, k);
   const int sl = term_slot(t, nterm);
   return ((((cb >> 6) * (nterm / G) * 9 + link_row<G>(sl, k, nterm)) << 6) + (cb & 63)) * G + sl % G;
+}
+// offset of (slot, k) from link_base<S, 8> in the forward-only layout
+template <class S>
+__device__ __forceinline__ int link_off_fb(int sl, int k) {
+  constexpr int G = LinkG<S>::G;
+  return fb_row<G>(sl, k) * 64 * G + fb_pos<G>(sl, k);
 }
 // first element of site cb's link stream, and the offset of (slot, k) from it
 template <class S, int NT>

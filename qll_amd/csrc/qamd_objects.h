@@ -21,18 +21,21 @@ struct qamd_gauge {
 // output sites of parity p in the blocked layout of qamd_internal.h.
 struct qamd_links {
   qamd_lattice *lat;
-  int kind;     // 0 staggered, 1 Wilson
+  int kind;     // 0 staggered (doublestored), 1 Wilson, 2 staggered forward-only
   int prec;
   int nterms;
   void *data[2];
-  float scale[16];   // FP16 only: per-term scale (stored = link / scale)
+  float scale[16];   // 16-bit only: per-slot scale (stored = link / scale)
   // links whose rebuildable terms (the long terms of a 16-term staggered set,
-  // every Wilson term) are all a scale times a unitary matrix, checked in
-  // FP64: the Dslash loads their first two rows only and rebuilds the third;
-  // lsign[p][cb] bit k = determinant sign of rebuildable term k
+  // every Wilson term, the long slots 4-7 of a forward-only set) are all a
+  // scale times a unitary matrix, checked in FP64: the Dslash loads their
+  // first two rows only and rebuilds the third; lsign[p][cb] bit k =
+  // determinant sign of rebuildable term k
   int recon;
   uint8_t *lsign[2];
 };
+
+inline bool is_stag(const qamd_links *l) { return l->kind == 0 || l->kind == 2; }
 
 namespace qamd {
 inline double2 *gauge_ptr(const qamd_gauge *u, int mu, int par) {

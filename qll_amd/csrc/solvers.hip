@@ -505,7 +505,7 @@ int sloppy_too_coarse(const qamd_links *l64, const qamd_links *ls, double mass, 
   double eps;
   if (qamd_stag_sloppy_error(l64, ls, mass, &eps)) return -1;
   if (eps < sloppy_max() * mass * mass) return 0;
-  *l32 = qamd_links_create(l64->lat, 0, QAMD_F32, l64->nterms);
+  *l32 = qamd_links_create(l64->lat, l64->kind, QAMD_F32, l64->nterms);
   if (!*l32 || qamd_links_convert(*l32, l64)) {
     qamd_links_destroy(*l32);
     *l32 = nullptr;
@@ -522,7 +522,7 @@ extern "C" {
 // QLL CG: all vectors at the links' precision, recurrences of cgSolveT
 int qamd_stag_cg(const qamd_links *l, qamd_field *x, const qamd_field *b, double mass, double rsq, int maxits,
                  qamd_solve_info *info) {
-  if (!l || l->kind != 0) { set_error("stag_cg: staggered links required"); return -1; }
+  if (!l || !is_stag(l)) { set_error("stag_cg: staggered links required"); return -1; }
   if (l->prec != QAMD_F64 && l->prec != QAMD_F32) {
     set_error("stag_cg: links must be FP64 or FP32 (16-bit links run inside qamd_stag_cg_mixed)");
     return -1;
@@ -709,7 +709,7 @@ static int cg_mixed_core(const qamd_links *l64, const qamd_links *ls, qamd_field
         const bool stalled = r2 > 0.25 * r2prev;
         if (stalled && lsc->prec != QAMD_F32) {
           if (!l32own) {
-            l32own = qamd_links_create(lat, 0, QAMD_F32, l64->nterms);
+            l32own = qamd_links_create(lat, l64->kind, QAMD_F32, l64->nterms);
             if (!l32own || qamd_links_convert(l32own, l64)) {
               err = 1;
               break;
@@ -763,7 +763,7 @@ static int cg_mixed_core(const qamd_links *l64, const qamd_links *ls, qamd_field
 }
 
 int qamd_stag_sloppy_error(const qamd_links *l64, const qamd_links *ls, double mass, double *eps) {
-  if (!l64 || !ls || !eps || l64->kind != 0 || ls->kind != 0 || l64->prec != QAMD_F64 || ls->prec == QAMD_F64 ||
+  if (!l64 || !ls || !eps || !is_stag(l64) || ls->kind != l64->kind || l64->prec != QAMD_F64 || ls->prec == QAMD_F64 ||
       l64->lat->g.V != ls->lat->g.V || l64->nterms != ls->nterms) {
     set_error("stag_sloppy_error: need FP64 and sloppy staggered links on one lattice");
     return -1;
@@ -803,8 +803,8 @@ int qamd_stag_sloppy_error(const qamd_links *l64, const qamd_links *ls, double m
 
 int qamd_stag_cg_mixed(const qamd_links *l64, const qamd_links *ls, qamd_field *x, const qamd_field *b,
                        double mass, double rsq, int maxits, double delta, qamd_solve_info *info) {
-  if (!l64 || !ls || l64->kind != 0 || ls->kind != 0 || l64->prec != QAMD_F64 || ls->prec == QAMD_F64 ||
-      l64->lat->g.V != ls->lat->g.V) {
+  if (!l64 || !ls || !is_stag(l64) || ls->kind != l64->kind || l64->nterms != ls->nterms ||
+      l64->prec != QAMD_F64 || ls->prec == QAMD_F64 || l64->lat->g.V != ls->lat->g.V) {
     set_error("stag_cg_mixed: need FP64 links and FP32/FP16 sloppy links on one lattice");
     return -1;
   }
@@ -832,7 +832,7 @@ int qamd_stag_cgms(const qamd_links *l, qamd_field *const x[], const qamd_field 
 // QLL's multishift CG; with drop, converged shifts i > 0 stop updating
 static int cgms_core(const qamd_links *l, qamd_field *const x[], const qamd_field *b, double mass,
                      const double shifts[], int n, double rsq, int maxits, qamd_solve_info *info, bool drop) {
-  if (!l || l->kind != 0) { set_error("stag_cgms: staggered links required"); return -1; }
+  if (!l || !is_stag(l)) { set_error("stag_cgms: staggered links required"); return -1; }
   if (l->prec == QAMD_Q16) { set_error("stag_cgms: Q16 links are for the mixed-precision solvers only"); return -1; }
   if (n < 1 || n > MAXSHIFT) { set_error("stag_cgms: 1..%d shifts", MAXSHIFT); return -1; }
   if (check_vec(b, l->prec, 0, "stag_cgms b")) return -1;
@@ -935,7 +935,8 @@ static int cgms_core(const qamd_links *l, qamd_field *const x[], const qamd_fiel
 
 int qamd_stag_cgms_mixed(const qamd_links *l64, const qamd_links *ls, qamd_field *const x[], const qamd_field *b,
                          double mass, const double shifts[], int n, double rsq, int maxits, qamd_solve_info *info) {
-  if (!l64 || !ls || l64->prec != QAMD_F64 || ls->prec == QAMD_F64 || l64->kind != 0 || ls->kind != 0) {
+  if (!l64 || !ls || l64->prec != QAMD_F64 || ls->prec == QAMD_F64 || !is_stag(l64) || ls->kind != l64->kind ||
+      l64->nterms != ls->nterms) {
     set_error("stag_cgms_mixed: need FP64 and sloppy staggered links");
     return -1;
   }
@@ -948,7 +949,7 @@ int qamd_stag_cgms_mixed(const qamd_links *l64, const qamd_links *ls, qamd_field
   const qamd_links *l32 = ls;
   qamd_links *own = nullptr;
   if (ls->prec != QAMD_F32) {
-    own = qamd_links_create(lat, 0, QAMD_F32, l64->nterms);
+    own = qamd_links_create(lat, l64->kind, QAMD_F32, l64->nterms);
     if (!own || qamd_links_convert(own, l64)) {
       qamd_links_destroy(own);
       return -1;

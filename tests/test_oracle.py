@@ -124,6 +124,37 @@ def test_dslash_antihermitian_and_eo_blocks():
     assert rel_err(full, parts + mass * b) < 1e-14
 
 
+@pytest.mark.parametrize("naik", [False, True])
+def test_dslash_fb_matches_doublestored(naik):
+    """dslashFb2T on forward links equals dslash2 on the doublestored set built
+    from them (t[2mu+1] = -F(x-mu)^+, lib/gauge.c:34-63), as testdslash.c:338-351
+    checks both against one qopDslash; antihermitian; a backward term with
+    the wrong sign or without the adjoint is caught."""
+    dims = (4, 6, 4, 8)
+    u = O.apply_stag_phases(dims, O.random_su3_field(dims, 4))
+    fl, ll = O.fat7(dims, u, u, want_long=naik)
+    y = O.gauss_vector(dims, 3, 7, parity=-1)
+    links = O.stag_doublestore(dims, fl, ll)
+    for sub, mass in (("all", 0.3), ("even", 0.0), ("odd", 0.0)):
+        ref = O.dslash(dims, links, mass, y, sub=sub)
+        got = O.dslash_fb(dims, fl, ll, mass, y, sub=sub)
+        assert rel_err(got, ref) < 1e-14, sub
+    ip = lambda x, z: np.vdot(cplx(x).ravel(), cplx(z).ravel())
+    b = O.gauss_vector(dims, 3, 8, parity=-1)
+    assert abs(ip(y, O.dslash_fb(dims, fl, ll, 0, b)) + ip(O.dslash_fb(dims, fl, ll, 0, y), b)) < 1e-11
+    # planted mistakes in the comparison set: +F(x-mu)^+ and -F(x-mu)
+    nt = links.shape[0]
+    bwd = [t for t in range(nt) if ((t // 2 if nt == 16 else t) & 1)]
+    wrong_sign = links.copy()
+    wrong_sign[bwd] *= -1
+    no_adj = links.copy()
+    for t in bwd:   # transpose instead of conjugate transpose
+        no_adj[t] = unmats(mats(links[t]).swapaxes(-1, -2))
+    ref = O.dslash_fb(dims, fl, ll, 0.3, y)
+    for bad in (wrong_sign, no_adj):
+        assert rel_err(O.dslash(dims, bad, 0.3, y), ref) > 1e-2
+
+
 def _gauge_transform(dims, links16, g):
     """L_t(x) -> g(x) L_t(x) g(x + d_t)^+ for the 16-term set."""
     V = O.volume(dims)
