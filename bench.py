@@ -72,6 +72,8 @@ def parse():
                     help="KIND,PREC,NRHS[,MODE]: 20 Dslash launches of one kind (stag, stagfb or wilson) "
                          "for a profiler, after 3 copies of 256 MiB (the counters' scale); MODE is the "
                          "tuning mode (1 tuned, >= 2 forces launch shape MODE-2)")
+    ap.add_argument("--solver-child", type=str, default=None,
+                    help="fp64 | mixed_q16 | mixed_q16_fb: one 32^4 solve of the bench's source, for a profiler")
     ap.add_argument("--block", type=int, default=8, help="sources in the batched mixed-precision solve")
     ap.add_argument("--dims", type=str, default=None, help="override lattice, e.g. 16,16,16,16")
     ap.add_argument("--precs", type=str, default="0,1,2,3",
@@ -128,6 +130,29 @@ def kernel_child(dims, spec):
     ms = Q.time_dslash(L, out, yin, 20)
     Q.sync()
     print("%s prec %d nrhs %d mode %d: %.4f ms" % (kind, prec, nrhs, mode, ms), flush=True)
+
+
+def solver_child(dims, kind):
+    """One solve of the bench's source and mass (as in the solver lines), for
+    a profiler run: FP64 CG, or mixed CG on Q16 doublestored / forward-only
+    sloppy links."""
+    import qll_amd as Q
+    Q.init()
+    lat = Q.Lattice(dims)
+    L64 = Q.asqtad_links_device(lat, seed=1)
+    b = Q.Field(lat, Q.F64, 3, 1, Q.EVEN).gaussian(11, 100)
+    x = Q.Field(lat, Q.F64, 3, 1, Q.EVEN)
+    Ls = None
+    if kind == "mixed_q16":
+        Ls = L64.converted(Q.Q16)
+    elif kind == "mixed_q16_fb":
+        Ls = Q.asqtad_links_device(lat, seed=1, prec=Q.Q16, kind=Q.STAG_FB)
+    Q.sync()
+    t0 = time.time()
+    info = Q.stag_cg(L64, x, b, QLL_MASS, RSQ, 50000) if Ls is None else \
+        Q.stag_cg_mixed(L64, Ls, x, b, QLL_MASS, RSQ, 50000, 0.1)
+    Q.sync()
+    print("%s: %.4f s, %d iterations" % (kind, time.time() - t0, info["iterations"]), flush=True)
 
 
 def _pmc_pass(counter, dims, tmp):
@@ -400,6 +425,9 @@ def main():
     if args.kernel_child:
         kernel_child(dims, args.kernel_child)
         return
+    if args.solver_child:
+        solver_child(dims, args.solver_child)
+        return
     traffic = None
     if not args.no_pmc:
         try:
@@ -572,7 +600,10 @@ def main():
         solver["fp64_cg"] = {"seconds": round(dt, 4), "iterations": info["iterations"],
                              "true_rel_resid": true_rsq() ** 0.5, "solves_per_s": round(1 / dt, 4)}
         print("fp64 cg: %s" % solver["fp64_cg"], file=sys.stderr, flush=True)
-        for name, Ls in (("mixed_fp32", L32), ("mixed_q16", LQ), ("mixed_fp16", L16)):
+        # the sloppy operator on forward-only Q16 links (half the storage;
+        # FP64 reliable updates still on the doublestored FP64 links)
+        LQfb = Q.asqtad_links_device(lat, seed=1, prec=Q.Q16, kind=Q.STAG_FB)
+        for name, Ls in (("mixed_fp32", L32), ("mixed_q16", LQ), ("mixed_fp16", L16), ("mixed_q16_fb", LQfb)):
             x.zero()
             Q.sync()
             t0 = time.time()
@@ -585,6 +616,7 @@ def main():
                             "true_rel_resid": true_rsq() ** 0.5, "solves_per_s": round(1 / dt, 4),
                             "speedup_vs_fp64": round(solver["fp64_cg"]["seconds"] / dt, 3)}
             print("%s: %s" % (name, solver[name]), file=sys.stderr, flush=True)
+        del LQfb
 
         # batched: a block of sources solved together (one block-Dslash pair per
         # iteration for the whole block), solves/s = block size / wall time
@@ -601,6 +633,20 @@ def main():
             r2 = ApB.norm2() if nb > 1 else [ApB.norm2()]
             return max(r2[i] / bBsq[i] for i in range(nb))
 
+        # the plain FP64 block CG on the same sources: separates what batching
+        # gains from what the lower precision gains
+        xB.zero()
+        Q.sync()
+        t0 = time.time()
+        infos = Q.stag_cg_block(L64, xB, bB, QLL_MASS, RSQ, 50000)
+        Q.sync()
+        dt = time.time() - t0
+        key64 = "fp64_block%d" % nb
+        solver[key64] = {"seconds": round(dt, 4), "rhs": nb, "iterations": max(i["iterations"] for i in infos),
+                         "converged": all(i["converged"] for i in infos),
+                         "max_true_rel_resid": true_rsq_block() ** 0.5, "solves_per_s": round(nb / dt, 4),
+                         "speedup_vs_fp64": round(nb / dt / solver["fp64_cg"]["solves_per_s"], 3)}
+        print("%s: %s" % (key64, solver[key64]), file=sys.stderr, flush=True)
         for name, Ls in (("block%d_mixed_q16" % nb, LQ), ("block%d_mixed_fp32" % nb, L32)):
             Q.stag_cg_mixed_block(L64, Ls, xB, bB, QLL_MASS, RSQ, 16, 0.1)   # untimed: launch-shape tuning
             Q.sync()
@@ -613,7 +659,8 @@ def main():
                             "reliable_updates": infos[0]["reliable_updates"], "restarts": infos[0]["restarts"],
                             "converged": all(i["converged"] for i in infos),
                             "max_true_rel_resid": true_rsq_block() ** 0.5, "solves_per_s": round(sps, 4),
-                            "speedup_vs_fp64": round(sps / solver["fp64_cg"]["solves_per_s"], 3)}
+                            "speedup_vs_fp64": round(sps / solver["fp64_cg"]["solves_per_s"], 3),
+                            "speedup_vs_fp64_block": round(sps / solver[key64]["solves_per_s"], 3)}
             print("%s: %s" % (name, solver[name]), file=sys.stderr, flush=True)
 
     wil = None if args.no_wilson else wilson_solves(Q)

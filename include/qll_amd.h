@@ -246,6 +246,15 @@ int qamd_stag_sloppy_error(const qamd_links *l64, const qamd_links *ls, double m
 int qamd_stag_cg_mixed_block(const qamd_links *l64, const qamd_links *ls, qamd_field *x,
                              const qamd_field *b, double mass, double rsq, int maxits,
                              double delta, qamd_solve_info *info);
+/* Plain FP64 block CG (no reference counterpart; the baseline the batched
+ * mixed solver is measured against): the nrhs (1..8) sources of the even
+ * FP64 block b are solved together with textbook CG per rhs (cgSolveT's
+ * iteration, lib/cg.c:43-122, unscaled), every vector FP64, the normal
+ * operator applied by the block Dslash (FP64 doublestored links loaded once
+ * for all rhs); each rhs stops on its iterated |r|^2 < rsq |b_r|^2, checked
+ * every 16 iterations.  x starts at 0.  info: nrhs entries. */
+int qamd_stag_cg_block(const qamd_links *l, qamd_field *x, const qamd_field *b, double mass, double rsq,
+                       int maxits, qamd_solve_info *info);
 /* One problem of a heterogeneous batch: its own lattice, gauge field
  * (links), mass and block of 1..8 sources. */
 typedef struct {

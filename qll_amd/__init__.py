@@ -101,6 +101,7 @@ def lib():
             "qamd_stag_sloppy_error": (C.c_int, [_vp, _vp, C.c_double, C.POINTER(C.c_double)]),
             "qamd_stag_cg_mixed_batch": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_double, C.c_int,
                                                    C.POINTER(SolveInfo)]),
+            "qamd_stag_cg_block": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_double, C.c_int, C.POINTER(SolveInfo)]),
             "qamd_stag_cg_mixed_block": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_double, C.c_int, C.c_double,
                                              C.POINTER(SolveInfo)]),
             "qamd_stag_cgms": (C.c_int, [_vp, C.POINTER(_vp), _vp, C.c_double, _d, C.c_int, C.c_double, C.c_int,
@@ -352,6 +353,13 @@ def stag_cg_mixed(l64, ls, x, b, mass, rsq, maxits, delta=0.1):
     info = SolveInfo()
     _check(lib().qamd_stag_cg_mixed(l64.h, ls.h, x.h, b.h, mass, rsq, maxits, delta, C.byref(info)))
     return info.as_dict()
+
+
+def stag_cg_block(links, x, b, mass, rsq, maxits):
+    """Plain FP64 block CG on every rhs of the even block b; one info dict per rhs."""
+    infos = (SolveInfo * b.nrhs)()
+    _check(lib().qamd_stag_cg_block(links.h, x.h, b.h, mass, rsq, maxits, infos))
+    return [i.as_dict() for i in infos]
 
 
 def stag_cg_mixed_block(l64, ls, x, b, mass, rsq, maxits, delta=0.1):

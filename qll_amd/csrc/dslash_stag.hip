@@ -54,10 +54,66 @@ template <int P, int PO = P, int PI = P> struct StagArgs {
   const uint8_t *lsign_o; // and their long-term determinant signs
   // single-reduction CG step (CGX = 2, FP32 vectors, input t): Ar =
   // a r + b D t (not stored); p = r + beta p, s = Ar + beta s, x += alpha p,
-  // r -= alpha s, |r|^2 -> red[0]; alpha and beta read from the device
+  // r -= alpha s, |r|^2 -> red[0]; alpha and beta formed by every block from
+  // the partial sums of |r|^2 (cg_rin) and |t|^2 (cg_tin), state in cg_state
   float2 *cgx, *cgr, *cgp, *cgs;
-  const double *cg_alpha, *cg_beta;
+  CGScal *cg_state;
+  int cg_ph, cg_nb;
+  RedScratch cg_rin, cg_tin;
 };
+
+// The CG step's alpha and beta, formed by every block in the same fixed
+// order (so all blocks agree bit for bit): thread k adds the partial sums of
+// blocks k, k + blockDim, ... in order, then the block adds its threads in
+// order; the recurrences are those of Chronopoulos & Gear (1989).
+template <class Args>
+__device__ __forceinline__ void cg_block_scalars(const Args &A, float &al_out, float &be_out, float &alp_out) {
+  __shared__ double sab[3];
+  double w[2] = {0.0, 0.0};
+  for (int b0 = threadIdx.x; b0 < A.cg_nb; b0 += 4 * blockDim.x) {
+    double v[4][2];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int b = b0 + u * blockDim.x;
+      v[u][0] = b < A.cg_nb ? A.cg_rin.part[b] : 0.0;
+      v[u][1] = b < A.cg_nb ? A.cg_tin.part[b] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      w[0] += v[u][0];
+      w[1] += v[u][1];
+    }
+  }
+  block_sum(w);
+  if (threadIdx.x == 0) {
+    CGScal *S = A.cg_state;
+    const int ph = A.cg_ph;
+    const double g = w[0], d = A.a * g + w[1];
+    double be = 0.0, al = g / d;
+    if (!S->restart[ph]) {
+      be = g / S->gam[ph];
+      al = g / (d - be * g / S->al[ph]);
+    }
+    sab[0] = al;
+    sab[1] = be;
+    sab[2] = S->al[ph];   // the previous iteration's alpha
+    if (blockIdx.x == 0) {
+      S->gam[1 - ph] = g;
+      S->al[1 - ph] = al;
+      S->restart[1 - ph] = 0;
+      S->rsq = g;
+      S->tt = w[1];
+      S->alpha = al;
+      S->beta = be;
+      S->maxrr = fmax(S->maxrr, sqrt(g));
+      S->its += 1;
+    }
+  }
+  __syncthreads();
+  al_out = (float)sab[0];
+  be_out = (float)sab[1];
+  alp_out = (float)sab[2];
+}
 
 struct FinStore2 {
   double *dst;
@@ -216,7 +272,7 @@ __device__ __forceinline__ void stag_acc_fb(const StagArgs<P, PO, PI> &A, const 
 // from the first two (FP64: 2016 instead of 2400 bytes per site).
 // FBT != 0: forward-only link set (NT = 8 storage slots), see stag_acc_fb.
 template <int P, int PO, int NT, bool RED, bool REC = false, int CGX = 0, int PI = P, int FBT = 0>
-__global__ void __launch_bounds__(256) k_stag_dslash(StagArgs<P, PO, PI> A) {
+__device__ __forceinline__ void stag_dslash_body(const StagArgs<P, PO, PI> &A) {
   static_assert(PI == P || (PI == QAMD_F32 && (P == QAMD_F16 || P == QAMD_Q16)),
                 "input precision: the links' or FP32 with 16-bit links");
   static_assert(!CGX || (PO == QAMD_F32 && RED), "fused CG step: FP32 vectors, with the |r|^2 sums");
@@ -229,13 +285,13 @@ __global__ void __launch_bounds__(256) k_stag_dslash(StagArgs<P, PO, PI> A) {
   int blk = xcd_block(blockIdx.x, gridDim.x);
   int cb = blk * 256 + threadIdx.x;
   double red[2] = {0.0, 0.0};
+  C acc[3];
+#pragma unroll
+  for (int j = 0; j < 3; j++) acc[j].x = acc[j].y = (R)0;
   if (cb < A.g.vh) {
     Site s = site_of(A.g, cb, A.par);
     constexpr int G = LinkG<S>::G;
     static_assert(!REC || NT == 16 || FBT, "long-row rebuild: 16-term or forward-only links");
-    C acc[3];
-#pragma unroll
-    for (int j = 0; j < 3; j++) acc[j].x = acc[j].y = (R)0;
     if constexpr (FBT) stag_acc_fb<P, PO, PI, FBT, REC>(A, s, cb, vp, acc);
     [[maybe_unused]] const S *lp = link_base<S, NT>(A.links, cb);
     [[maybe_unused]] unsigned lsb = 0;
@@ -274,32 +330,47 @@ __global__ void __launch_bounds__(256) k_stag_dslash(StagArgs<P, PO, PI> A) {
           for (int c = 0; c < 3; c++) cmac<C, R>(acc[r], m[3 * c + r], v[c]);
       }
     }
+  }
+  // the CG step's scalars: a block-wide sum, after the link stream
+  [[maybe_unused]] float cg_al = 0.f, cg_be = 0.f, cg_alp = 0.f;
+  if constexpr (CGX == 2) cg_block_scalars(A, cg_al, cg_be, cg_alp);
+  if (cb < A.g.vh) {
     C o[3];
     const R b = (R)A.b;
     if constexpr (CGX == 2) {
-      float2 rv[3], pv[3], sv[3], xv[3];
+      // x is updated every second iteration, for two: in iteration k + 1
+      // (odd ph) x += alpha_k p_k + alpha_{k+1} p_{k+1}, with p_k the p read
+      // here, so even iterations leave x alone (one vector stream fewer)
+      float2 rv[3], pv[3], sv[3], xv[3], po[3];
       VecLd<QAMD_F32>::ld(A.cgr, nullptr, vp, cb, rv, 1.0f);
       VecLd<QAMD_F32>::ld(A.cgp, nullptr, vp, cb, pv, 1.0f);
       VecLd<QAMD_F32>::ld(A.cgs, nullptr, vp, cb, sv, 1.0f);
-      VecLd<QAMD_F32>::ld(A.cgx, nullptr, vp, cb, xv, 1.0f);
-      const float a = (float)A.a, al = (float)*A.cg_alpha, be = (float)*A.cg_beta;
+      const bool xup = A.cg_ph == 1;
+      if (xup) VecLd<QAMD_F32>::ld(A.cgx, nullptr, vp, cb, xv, 1.0f);
+      const float a = (float)A.a, al = cg_al, be = cg_be, alp = cg_alp;
 #pragma unroll
       for (int j = 0; j < 3; j++) {
         const float wx = fmaf(a, rv[j].x, (float)b * acc[j].x), wy = fmaf(a, rv[j].y, (float)b * acc[j].y);
+        po[j] = pv[j];
         pv[j].x = fmaf(be, pv[j].x, rv[j].x);
         pv[j].y = fmaf(be, pv[j].y, rv[j].y);
         sv[j].x = fmaf(be, sv[j].x, wx);
         sv[j].y = fmaf(be, sv[j].y, wy);
-        xv[j].x = fmaf(al, pv[j].x, xv[j].x);
-        xv[j].y = fmaf(al, pv[j].y, xv[j].y);
         rv[j].x = fmaf(-al, sv[j].x, rv[j].x);
         rv[j].y = fmaf(-al, sv[j].y, rv[j].y);
         red[0] += (double)rv[j].x * rv[j].x + (double)rv[j].y * rv[j].y;
       }
       VecSt<QAMD_F32>::st(A.cgp, nullptr, vp, cb, pv);
       VecSt<QAMD_F32>::st(A.cgs, nullptr, vp, cb, sv);
-      VecSt<QAMD_F32>::st(A.cgx, nullptr, vp, cb, xv);
       VecSt<QAMD_F32>::st(A.cgr, nullptr, vp, cb, rv);
+      if (xup) {
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+          xv[j].x = fmaf(al, pv[j].x, fmaf(alp, po[j].x, xv[j].x));
+          xv[j].y = fmaf(al, pv[j].y, fmaf(alp, po[j].y, xv[j].y));
+        }
+        VecSt<QAMD_F32>::st(A.cgx, nullptr, vp, cb, xv);
+      }
     } else if (A.w) {
       C w[3];
       VecLd<PO>::ld(A.w, A.w_norm, vp, cb, w, (R)1);
@@ -326,6 +397,11 @@ __global__ void __launch_bounds__(256) k_stag_dslash(StagArgs<P, PO, PI> A) {
     }
   }
   if (RED) grid_partial<2>(red, A.rs);
+}
+
+template <int P, int PO, int NT, bool RED, bool REC = false, int CGX = 0, int PI = P, int FBT = 0>
+__global__ void __launch_bounds__(256) k_stag_dslash(StagArgs<P, PO, PI> A) {
+  stag_dslash_body<P, PO, NT, RED, REC, CGX, PI, FBT>(A);
 }
 
 template <int P, int PO, int NT, int PI, int FBT = 0>
@@ -425,8 +501,9 @@ int stag_dslash_launch(const qamd_links *l, Field &out, const Field &in, double 
   }
   const bool half = (l->prec == QAMD_F16 || l->prec == QAMD_Q16);
   bool half_to_single = half && out.prec == QAMD_F32;
-  const bool single_in = half && in.prec == QAMD_F32;   // FP32 in and out, 16-bit links
-  if ((in.prec != l->prec && !(single_in && half_to_single)) || (out.prec != l->prec && !half_to_single) ||
+  // FP32 input with 16-bit links (output FP32 or in the links' format)
+  const bool single_in = half && in.prec == QAMD_F32;
+  if ((in.prec != l->prec && !single_in) || (out.prec != l->prec && !half_to_single) ||
       (w && w->prec != out.prec)) {
     set_error("stag_dslash: field precisions in %d / out %d do not fit links %d", in.prec, out.prec, l->prec);
     return -1;
@@ -447,15 +524,21 @@ int stag_dslash_launch(const qamd_links *l, Field &out, const Field &in, double 
     case QAMD_F64: return launch_p<QAMD_F64>(l, out, in, a, w, b, red_out, st, skip, pset, finish, rs_fixed);
     case QAMD_F32: return launch_p<QAMD_F32>(l, out, in, a, w, b, red_out, st, skip, pset, finish, rs_fixed);
     case QAMD_F16:
-      if (single_in)
+      if (single_in && half_to_single)
         return launch_p<QAMD_F16, QAMD_F32, QAMD_F32>(l, out, in, a, w, b, red_out, st, skip, pset, finish,
+                                                      rs_fixed);
+      if (single_in)
+        return launch_p<QAMD_F16, QAMD_F16, QAMD_F32>(l, out, in, a, w, b, red_out, st, skip, pset, finish,
                                                       rs_fixed);
       if (half_to_single)
         return launch_p<QAMD_F16, QAMD_F32>(l, out, in, a, w, b, red_out, st, skip, pset, finish, rs_fixed);
       return launch_p<QAMD_F16>(l, out, in, a, w, b, red_out, st, skip, pset, finish, rs_fixed);
     default:
-      if (single_in)
+      if (single_in && half_to_single)
         return launch_p<QAMD_Q16, QAMD_F32, QAMD_F32>(l, out, in, a, w, b, red_out, st, skip, pset, finish,
+                                                      rs_fixed);
+      if (single_in)
+        return launch_p<QAMD_Q16, QAMD_Q16, QAMD_F32>(l, out, in, a, w, b, red_out, st, skip, pset, finish,
                                                       rs_fixed);
       if (half_to_single)
         return launch_p<QAMD_Q16, QAMD_F32>(l, out, in, a, w, b, red_out, st, skip, pset, finish, rs_fixed);
@@ -463,18 +546,22 @@ int stag_dslash_launch(const qamd_links *l, Field &out, const Field &in, double 
   }
 }
 
-// single-reduction CG step: t the D_oe output (opposite parity, FP32); r,
-// p, s, x FP32 of the output parity; links FP32, FP16 or Q16
-template <int P>
+// single-reduction CG step: t the D_oe output (opposite parity; FP32, or
+// the links' 16-bit format); r, p, s, x FP32 of the output parity; links
+// FP32, FP16 or Q16
+template <int P, int PI>
 static int cgcg_p(const qamd_links *l, const Field &t, Field &r, Field &p, Field &s, Field &x, double a,
-                  const double *alpha, const double *beta, hipStream_t st, const int *skip, int pset) {
+                  CGScal *cs, int ph, RedScratch rin, RedScratch tin, RedScratch rout, hipStream_t st,
+                  const int *skip) {
   typedef typename Prec<P>::S S;
-  StagArgs<P, QAMD_F32, QAMD_F32> A;
+  typedef typename Prec<PI>::S SI;
+  StagArgs<P, QAMD_F32, PI> A;
   memset(&A, 0, sizeof A);
   A.skip = skip;
   A.g = l->lat->g;
   A.links = (const S *)l->data[r.parity];
-  A.in = (const float2 *)t.data;
+  A.in = (const SI *)t.data;
+  A.in_norm = t.norm;
   A.a = a;
   A.b = -1.0;
   A.par = r.parity;
@@ -488,39 +575,47 @@ static int cgcg_p(const qamd_links *l, const Field &t, Field &r, Field &p, Field
   A.cgp = (float2 *)p.data;
   A.cgs = (float2 *)s.data;
   A.cgx = (float2 *)x.data;
-  A.cg_alpha = alpha;
-  A.cg_beta = beta;
   const int nb = (int)((A.g.vh + 255) / 256);
-  if (red_reserve(nb)) return -1;
-  A.rs = red_set(pset);
+  A.cg_state = cs;
+  A.cg_ph = ph;
+  A.cg_nb = nb;
+  A.cg_rin = rin;
+  A.cg_tin = tin;
+  A.rs = rout;
   if (l->kind == 2) {
     if (A.lsign)
-      hipLaunchKernelGGL((k_stag_dslash<P, QAMD_F32, 8, true, true, 2, QAMD_F32, 8>), dim3(nb), dim3(256), 0, st, A);
+      hipLaunchKernelGGL((k_stag_dslash<P, QAMD_F32, 8, true, true, 2, PI, 8>), dim3(nb), dim3(256), 0, st, A);
     else
-      hipLaunchKernelGGL((k_stag_dslash<P, QAMD_F32, 8, true, false, 2, QAMD_F32, 8>), dim3(nb), dim3(256), 0, st, A);
+      hipLaunchKernelGGL((k_stag_dslash<P, QAMD_F32, 8, true, false, 2, PI, 8>), dim3(nb), dim3(256), 0, st, A);
   } else if (A.lsign) {
-    hipLaunchKernelGGL((k_stag_dslash<P, QAMD_F32, 16, true, true, 2, QAMD_F32>), dim3(nb), dim3(256), 0, st, A);
+    hipLaunchKernelGGL((k_stag_dslash<P, QAMD_F32, 16, true, true, 2, PI>), dim3(nb), dim3(256), 0, st, A);
   } else {
-    hipLaunchKernelGGL((k_stag_dslash<P, QAMD_F32, 16, true, false, 2, QAMD_F32>), dim3(nb), dim3(256), 0, st, A);
+    hipLaunchKernelGGL((k_stag_dslash<P, QAMD_F32, 16, true, false, 2, PI>), dim3(nb), dim3(256), 0, st, A);
   }
   QAMD_HIP(hipGetLastError());
   return 0;
 }
 
 int stag_cgcg_step_launch(const qamd_links *l, const Field &t, Field &r, Field &p, Field &s, Field &x, double a,
-                          const double *alpha, const double *beta, hipStream_t st, const int *skip, int pset) {
+                          CGScal *cs, int ph, RedScratch rin, RedScratch tin, RedScratch rout, hipStream_t st,
+                          const int *skip) {
   if (!((l->kind == 0 && l->nterms == 16) || (l->kind == 2 && l->nterms == 8)) || l->prec == QAMD_F64 ||
-      t.prec != QAMD_F32 || t.parity == r.parity ||
+      (t.prec != QAMD_F32 && t.prec != l->prec) || t.parity == r.parity ||
       r.prec != QAMD_F32 || p.prec != QAMD_F32 || s.prec != QAMD_F32 || x.prec != QAMD_F32 ||
-      p.parity != r.parity || s.parity != r.parity || x.parity != r.parity || t.nrhs != 1 || r.nrhs != 1) {
+      p.parity != r.parity || s.parity != r.parity || x.parity != r.parity || t.nrhs != 1 || r.nrhs != 1 ||
+      (ph != 0 && ph != 1)) {
     set_error("stag_cgcg_step: needs sloppy asqtad links (16 terms or forward-only fat + long) and single-rhs "
               "FP32 t (opposite parity) and r, p, s, x");
     return -1;
   }
   switch (l->prec) {
-    case QAMD_F32: return cgcg_p<QAMD_F32>(l, t, r, p, s, x, a, alpha, beta, st, skip, pset);
-    case QAMD_F16: return cgcg_p<QAMD_F16>(l, t, r, p, s, x, a, alpha, beta, st, skip, pset);
-    default: return cgcg_p<QAMD_Q16>(l, t, r, p, s, x, a, alpha, beta, st, skip, pset);
+    case QAMD_F32: return cgcg_p<QAMD_F32, QAMD_F32>(l, t, r, p, s, x, a, cs, ph, rin, tin, rout, st, skip);
+    case QAMD_F16:
+      if (t.prec == QAMD_F16) return cgcg_p<QAMD_F16, QAMD_F16>(l, t, r, p, s, x, a, cs, ph, rin, tin, rout, st, skip);
+      return cgcg_p<QAMD_F16, QAMD_F32>(l, t, r, p, s, x, a, cs, ph, rin, tin, rout, st, skip);
+    default:
+      if (t.prec == QAMD_Q16) return cgcg_p<QAMD_Q16, QAMD_Q16>(l, t, r, p, s, x, a, cs, ph, rin, tin, rout, st, skip);
+      return cgcg_p<QAMD_Q16, QAMD_F32>(l, t, r, p, s, x, a, cs, ph, rin, tin, rout, st, skip);
   }
 }
 

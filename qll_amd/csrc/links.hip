@@ -250,7 +250,11 @@ static bool rebuild_kind(const qamd_links *l) {
 
 // after FP64 links change: decide whether the Dslash may rebuild the third
 // rows (every checked term passes)
+static uint64_t g_links_ver = 0;
+static void new_version(qamd_links *l) { l->ver = __atomic_add_fetch(&g_links_ver, 1, __ATOMIC_RELAXED); }
+
 static int long_link_check(qamd_links *l) {
+  new_version(l);
   l->recon = 0;
   if (!rebuild_kind(l) || l->prec != QAMD_F64) return 0;
   const int t0 = l->kind == 0 ? 1 : l->kind == 2 ? 4 : 0, dt = l->kind == 0 ? 2 : 1, nk = l->kind == 2 ? 4 : 8;
@@ -282,6 +286,7 @@ static int long_link_check(qamd_links *l) {
 // are the roundings of a scaled unitary matrix's, so the rebuilt row carries
 // the storage precision's error, as a stored one would.
 static int inherit_recon(qamd_links *dst, const qamd_links *src) {
+  new_version(dst);
   dst->recon = 0;
   if (!src->recon || !rebuild_kind(dst)) return 0;
   const Geom &g = dst->lat->g;

@@ -34,13 +34,29 @@ int stag_block_launch(const qamd_links *l, Field &out, const Field &in, double a
 // is set in act (the others' outputs and sums are left untouched / zero)
 int stag_block_launch_m(const qamd_links *l, Field &out, const Field &in, const double *ar, const Field *w,
                         double b, bool red, RedScratch rs, hipStream_t st, unsigned act);
+// Scalars of the single-reduction CG step.  Iteration parity ph reads slot
+// ph of gam/al/restart and writes slot 1 - ph, so every block of a launch can
+// read the previous iteration's values while block 0 stores the new ones;
+// block 0 also leaves this iteration's values for the host.
+struct CGScal {
+  double gam[2], al[2];   // previous |r|^2 and alpha
+  int restart[2];         // 1: this iteration takes beta = 0 (p = r, s = A r)
+  double rsq, tt, alpha, beta, maxrr;
+  int its;
+};
 // dslash_stag.hip: fused step of the single-reduction (Chronopoulos-Gear)
-// mixed-precision CG, FP32 vectors and sloppy (FP32/FP16/Q16) links:
+// mixed-precision CG, FP32 vectors and sloppy (FP32/FP16/Q16) links.  Every
+// block first sums, in fixed order, the partial sums of gamma = |r|^2 (set
+// rin, one value per block) and |t|^2 (set tin) and forms alpha and beta
+// (identically in every block; no finisher launch), then
 // Ar = a r - D t (not stored), p = r + beta p, s = Ar + beta s,
-// x += alpha p, r -= alpha s with |r|^2 partial sums in set pset; alpha and
-// beta read from the device
+// r -= alpha s, with the |r|^2 partial sums into set rout; x += alpha p is
+// applied two iterations at a time, in the odd ones (ph = 1): x += alpha_prev
+// p_prev + alpha p, so an even number of steps leaves x complete.
+// The sets come from red_set after a red_reserve covering the lattice.
 int stag_cgcg_step_launch(const qamd_links *l, const Field &t, Field &r, Field &p, Field &s, Field &x, double a,
-                          const double *alpha, const double *beta, hipStream_t st, const int *skip, int pset);
+                          CGScal *cs, int ph, RedScratch rin, RedScratch tin, RedScratch rout, hipStream_t st,
+                          const int *skip);
 // stag_block.hip: block CG on (m2[r] - D_eo D_oe) x_r = b_r (one m2 per rhs,
 // 1..8 rhs, FP64 x and b), sloppy links ls, FP64 reliable updates on l64;
 // x0: start from the x passed in

@@ -33,9 +33,26 @@ struct qamd_links {
   // determinant sign of rebuildable term k
   int recon;
   uint8_t *lsign[2];
+  // content version: a process-wide counter value, new after every import,
+  // doublestore or conversion into these links
+  uint64_t ver;
+  // the last sloppy-operator error measured with these links as the sloppy
+  // set (qamd_stag_sloppy_error): against links of version err_ref_ver, at
+  // mass err_mass, with these links at version err_ver
+  mutable uint64_t err_ref_ver, err_ver;
+  mutable double err_mass, err_eps;
 };
 
 inline bool is_stag(const qamd_links *l) { return l->kind == 0 || l->kind == 2; }
+// staggered with long (Naik) terms: 16 doublestored or 8 forward-only terms
+inline bool stag_naik(const qamd_links *l) {
+  return (l->kind == 0 && l->nterms == 16) || (l->kind == 2 && l->nterms == 8);
+}
+// two staggered link sets of the same operator form (both with or both
+// without Naik terms), stored doublestored or forward-only
+inline bool stag_pair(const qamd_links *a, const qamd_links *b) {
+  return is_stag(a) && is_stag(b) && stag_naik(a) == stag_naik(b);
+}
 
 namespace qamd {
 inline double2 *gauge_ptr(const qamd_gauge *u, int mu, int par) {

@@ -42,10 +42,7 @@ struct SolverDev {
   double xb[MAXSHIFT];                  // coefficients for the x update
   int xrun;                             // 1: this iteration's x update is due
   int drop;                             // 1: shifts stop updating once converged
-  // single-reduction CG (mixed solver): the previous iteration's |r|^2 and
-  // alpha, and restart = 1 to take beta = 0 in the next iteration
-  double gam_old, al_old;
-  int restart;
+  CGScal cg;   // single-reduction CG (mixed solver)
 };
 
 template <int P> struct VT {
@@ -96,29 +93,6 @@ __global__ void __launch_bounds__(256) k_cg_p(Field p, Field r, SolverDev *S) {
 // identity, lib/stagDslash.c:546-575), beta = gamma / gamma_old and
 // alpha = gamma / (delta - beta gamma / alpha_old); a restart takes beta = 0,
 // alpha = gamma / delta.
-struct FinCGCG {
-  SolverDev *S;
-  __device__ void operator()(const double *t) const {
-    const double g = t[0], d = S->m2 * g + t[1];
-    double be = 0.0, al = g / d;
-    if (S->restart) {
-      S->restart = 0;
-    } else {
-      be = g / S->gam_old;
-      al = g / (d - be * g / S->al_old);
-    }
-    S->rsq_old = S->rsq;
-    S->rsq = g;
-    S->beta = be;
-    S->alpha = al;
-    S->gam_old = g;
-    S->al_old = al;
-    S->nrm[0] = t[1];
-    S->maxrr = fmax(S->maxrr, sqrt(g));
-    S->its += 1;
-  }
-};
-
 struct FinCG {
   SolverDev *S;
   __device__ void operator()(const double *t) const {
@@ -502,8 +476,18 @@ static double sloppy_max() {
 int sloppy_too_coarse(const qamd_links *l64, const qamd_links *ls, double mass, qamd_links **l32) {
   *l32 = nullptr;
   if (ls->prec != QAMD_F16 && ls->prec != QAMD_Q16) return 0;
+  // the error of these links against these FP64 links at this mass is
+  // measured once (a few Dslash launches) and kept with the sloppy links
   double eps;
-  if (qamd_stag_sloppy_error(l64, ls, mass, &eps)) return -1;
+  if (l64->ver && ls->ver && ls->err_ref_ver == l64->ver && ls->err_ver == ls->ver && ls->err_mass == mass) {
+    eps = ls->err_eps;
+  } else {
+    if (qamd_stag_sloppy_error(l64, ls, mass, &eps)) return -1;
+    ls->err_ref_ver = l64->ver;
+    ls->err_ver = ls->ver;
+    ls->err_mass = mass;
+    ls->err_eps = eps;
+  }
   if (eps < sloppy_max() * mass * mass) return 0;
   *l32 = qamd_links_create(l64->lat, l64->kind, QAMD_F32, l64->nterms);
   if (!*l32 || qamd_links_convert(*l32, l64)) {
@@ -584,19 +568,21 @@ int qamd_stag_cg(const qamd_links *l, qamd_field *x, const qamd_field *b, double
 }
 
 // Mixed-precision CG with reliable updates.  The Krylov vectors r, p,
-// s = Ap, the solution increment and the odd temporary t live in FP32; the
-// Dslash reads the sloppy links (FP32, FP16 or Q16) and applies them in FP32
-// arithmetic, so the sloppy operator is one fixed linear map (the 16-bit
-// rounding stays in the links; none is applied to the vectors); x, b and
-// the true residual are FP64.  The inner iteration is the single-reduction
-// (Chronopoulos-Gear) form of CG: one application of the operator to r and
-// one global reduction per iteration, so an iteration is three launches:
-// t = D_oe r with |t|^2; the finisher (alpha, beta); the fused step
+// s = Ap and the solution increment live in FP32; the Dslash reads the
+// sloppy links (FP32, FP16 or Q16) and accumulates in FP32; the odd
+// temporary t = D_oe r is kept in the links' format (16-bit with a per-site
+// norm for 16-bit links); x, b and the true residual are FP64.  The inner
+// iteration is the single-reduction (Chronopoulos-Gear) form of CG: one
+// application of the operator to r and one global reduction per iteration,
+// in two launches: t = D_oe r with |t|^2 partial sums, then the fused step,
+// whose blocks form alpha and beta from the partial sums of |r|^2 and |t|^2,
 // Ar = m^2 r - D_eo t (not stored), p = r + beta p, s = Ar + beta s,
-// x += alpha p, r -= alpha s with |r|^2.  The recurrence for s = Ap needs
-// the operator to be linear, which the FP32 vectors keep.  Reliable updates
-// happen at batch boundaries when |r| has dropped below delta * max|r| since
-// the last one.  x is used as the initial guess.
+// r -= alpha s with |r|^2 partial sums, and x += alpha p (two iterations at
+// a time).  The recurrence for s = Ap assumes a linear operator: the links'
+// rounding is fixed, and the one rounding applied to a vector (t) is at the
+// 16-bit links' own level; restarts at every reliable update reset s.
+// Reliable updates happen at batch boundaries when |r| has dropped below
+// delta * max|r| since the last one.  The caller passes x = 0.
 static int cg_mixed_core(const qamd_links *l64, const qamd_links *ls, qamd_field *x, const qamd_field *b,
                          double m2, double rsq, int maxits, double delta, qamd_solve_info *info) {
   double t0 = now_s();
@@ -614,7 +600,9 @@ static int cg_mixed_core(const qamd_links *l64, const qamd_links *ls, qamd_field
   qamd_field *p = qamd_field_create(lat, QAMD_F32, 3, 1, 0);
   qamd_field *sv = qamd_field_create(lat, QAMD_F32, 3, 1, 0);
   qamd_field *xs = qamd_field_create(lat, QAMD_F32, 3, 1, 0);
-  qamd_field *t = qamd_field_create(lat, QAMD_F32, 3, 1, 1);
+  // D_oe r is stored in the links' format (16-bit t with a per-site norm:
+  // 8 bytes per site fewer written and gathered than FP32)
+  qamd_field *t = qamd_field_create(lat, ps, 3, 1, 1);
   int rc = -1, nupd = 0, its = 0;
   const bool verbose = getenv("QAMD_VERBOSE") != nullptr;
   double bsq = 0, r2 = 0;
@@ -635,17 +623,32 @@ static int cg_mixed_core(const qamd_links *l64, const qamd_links *ls, qamd_field
     if (push_state(st)) break;
     // true residual r64 = b - A x in FP64, r = r64 in FP32, |r64|^2 in
     // partial-sum set 0 and on the host
+    // Iteration k (counted from the start, kit) reads the partial sums of
+    // |r|^2 from set R[k & 1] and writes the next ones to R[1 - (k & 1)];
+    // the D_oe launch leaves those of |t|^2 in set 1.
+    const int nb1 = (int)grid1(lat).x;
+    if (red_reserve(nb1)) break;
+    const RedScratch rset[2] = {red_set(0), red_set(2)};
+    long kit = 0;
+    bool x_zero = true;   // the caller passes x = 0: the first residual is b
     auto reliable = [&](void) -> int {
-      if (apply_normal(l64, st.S, Ax, x, x, t64, m2, nullptr)) return -1;
+      if (x_zero) {
+        if (qamd_field_zero(Ax)) return -1;
+        x_zero = false;
+      } else if (apply_normal(l64, st.S, Ax, x, x, t64, m2, nullptr)) {
+        return -1;
+      }
+      const RedScratch rr = rset[kit & 1];
       hipLaunchKernelGGL(k_ru_resid<QAMD_F32>, grid1(lat), dim3(256), 0, stream(), r64->f, b->f, Ax->f, r->f, st.S,
-                         red_set(0));
-      launch_finish<1>(red_set(0), (int)grid1(lat).x, FinTrue{st.S}, stream());
+                         rr);
+      launch_finish<1>(rr, nb1, FinTrue{st.S}, stream());
       QAMD_HIP(hipGetLastError());
       return pull_state(st);
     };
     // the next iteration takes beta = 0: p = r, s = A r
     auto restart = [&](void) -> int {
-      Sh->restart = 1;
+      Sh->cg.restart[kit & 1] = 1;
+      Sh->cg.maxrr = sqrt(Sh->rsq);
       return push_state(st);
     };
     if (reliable()) break;
@@ -655,16 +658,17 @@ static int cg_mixed_core(const qamd_links *l64, const qamd_links *ls, qamd_field
       break;
     }
     if (qamd_field_zero(xs) || qamd_field_zero(p) || qamd_field_zero(sv) || restart()) break;
-    const int nb1 = (int)grid1(lat).x;
     int err = 0;
+    // two launches per iteration: t = D_oe r (|t|^2 partial sums), then the
+    // fused step, whose blocks form alpha and beta themselves; the step
+    // updates x in pairs of iterations, so a batch must hold an even number
+    static_assert(BATCH % 2 == 0, "the fused CG step updates x every second iteration");
     auto batch = [&]() -> int {
       int e = 0;
-      for (int k = 0; k < BATCH && !e; k++) {
+      for (int k = 0; k < BATCH && !e; k++, kit++) {
         e |= stag_dslash_launch(lsc, t->f, r->f, 0.0, nullptr, 1.0, nullptr, stream(), nullptr, 1, false);
-        hipLaunchKernelGGL((k_finish2<1, 1, FinCGCG>), dim3(1), dim3(FT), 0, stream(), red_set(0), red_set(1), nb1,
-                           FinCGCG{st.S}, &st.S->done);
-        e |= stag_cgcg_step_launch(lsc, t->f, r->f, p->f, sv->f, xs->f, m2, &st.S->alpha, &st.S->beta, stream(),
-                                   &st.S->done, 0);
+        e |= stag_cgcg_step_launch(lsc, t->f, r->f, p->f, sv->f, xs->f, m2, &st.S->cg, (int)(kit & 1),
+                                   rset[kit & 1], red_set(1), rset[1 - (kit & 1)], stream(), nullptr);
         e |= hipGetLastError() != hipSuccess;
       }
       return e;
@@ -676,16 +680,15 @@ static int cg_mixed_core(const qamd_links *l64, const qamd_links *ls, qamd_field
         break;
       }
       its += BATCH;
-      double rn = sqrt(Sh->rsq);
+      const CGScal &cg = Sh->cg;
+      double rn = sqrt(cg.rsq);
       if (verbose)
         fprintf(stderr, "cg_mixed prec %d its %d rel|r| %.3e max %.3e rAr %.3e alpha %.3e beta %.3e\n", ps, its,
-                rn / sqrt(bsq), Sh->maxrr / sqrt(bsq), Sh->m2 * Sh->rsq + Sh->nrm[0], Sh->alpha, Sh->beta);
+                rn / sqrt(bsq), cg.maxrr / sqrt(bsq), Sh->m2 * cg.rsq + cg.tt, cg.alpha, cg.beta);
       // update on a delta drop, on convergence, at the end, and when the
       // iterated residual grew well above the last true one (divergence)
-      if (!isfinite(Sh->rsq) || rn < delta * Sh->maxrr || Sh->rsq < stop || its >= maxits ||
-          Sh->rsq > 4.0 * r2) {
+      if (!isfinite(cg.rsq) || rn < delta * cg.maxrr || cg.rsq < stop || its >= maxits || cg.rsq > 4.0 * r2) {
         // reliable update: x += xs (xs = 0), r = b - A x in FP64
-        const double r2s = Sh->rsq;
         hipLaunchKernelGGL(k_ru_accum<QAMD_F32>, grid1(lat), dim3(256), 0, stream(), x->f, xs->f);
         if (hipGetLastError() != hipSuccess || reliable()) {
           err = 1;
@@ -716,6 +719,13 @@ static int cg_mixed_core(const qamd_links *l64, const qamd_links *ls, qamd_field
             }
           }
           lsc = l32own;
+          if (t->f.prec != QAMD_F32) {
+            qamd_field_destroy(t);
+            if (!(t = qamd_field_create(lat, QAMD_F32, 3, 1, 1))) {
+              err = 1;
+              break;
+            }
+          }
           nesc++;
           if (verbose) fprintf(stderr, "cg_mixed: FP16 links stalled; continuing with FP32 links\n");
         }
@@ -763,8 +773,8 @@ static int cg_mixed_core(const qamd_links *l64, const qamd_links *ls, qamd_field
 }
 
 int qamd_stag_sloppy_error(const qamd_links *l64, const qamd_links *ls, double mass, double *eps) {
-  if (!l64 || !ls || !eps || !is_stag(l64) || ls->kind != l64->kind || l64->prec != QAMD_F64 || ls->prec == QAMD_F64 ||
-      l64->lat->g.V != ls->lat->g.V || l64->nterms != ls->nterms) {
+  if (!l64 || !ls || !eps || !stag_pair(l64, ls) || l64->prec != QAMD_F64 || ls->prec == QAMD_F64 ||
+      l64->lat->g.V != ls->lat->g.V) {
     set_error("stag_sloppy_error: need FP64 and sloppy staggered links on one lattice");
     return -1;
   }
@@ -803,8 +813,8 @@ int qamd_stag_sloppy_error(const qamd_links *l64, const qamd_links *ls, double m
 
 int qamd_stag_cg_mixed(const qamd_links *l64, const qamd_links *ls, qamd_field *x, const qamd_field *b,
                        double mass, double rsq, int maxits, double delta, qamd_solve_info *info) {
-  if (!l64 || !ls || !is_stag(l64) || ls->kind != l64->kind || l64->nterms != ls->nterms ||
-      l64->prec != QAMD_F64 || ls->prec == QAMD_F64 || l64->lat->g.V != ls->lat->g.V) {
+  if (!l64 || !ls || !stag_pair(l64, ls) || l64->prec != QAMD_F64 || ls->prec == QAMD_F64 ||
+      l64->lat->g.V != ls->lat->g.V) {
     set_error("stag_cg_mixed: need FP64 links and FP32/FP16 sloppy links on one lattice");
     return -1;
   }
@@ -935,9 +945,10 @@ static int cgms_core(const qamd_links *l, qamd_field *const x[], const qamd_fiel
 
 int qamd_stag_cgms_mixed(const qamd_links *l64, const qamd_links *ls, qamd_field *const x[], const qamd_field *b,
                          double mass, const double shifts[], int n, double rsq, int maxits, qamd_solve_info *info) {
-  if (!l64 || !ls || l64->prec != QAMD_F64 || ls->prec == QAMD_F64 || !is_stag(l64) || ls->kind != l64->kind ||
-      l64->nterms != ls->nterms) {
-    set_error("stag_cgms_mixed: need FP64 and sloppy staggered links");
+  if (!l64 || !ls || l64->prec != QAMD_F64 || ls->prec == QAMD_F64 || l64->kind != 0 || ls->kind != 0 ||
+      l64->nterms != ls->nterms || l64->lat->g.V != ls->lat->g.V) {
+    set_error("stag_cgms_mixed: need FP64 and sloppy doublestored staggered links (the refinement runs the "
+              "block kernels)");
     return -1;
   }
   if (n < 1 || n > MAXSHIFT) { set_error("stag_cgms_mixed: 1..%d shifts", MAXSHIFT); return -1; }

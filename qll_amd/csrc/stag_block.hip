@@ -454,6 +454,84 @@ __global__ void k_bru_resid(Field r64, Field b, Field Ax, Field r32, BatchDev *B
   rhs_partial(red, rs);
 }
 
+// ----------------------------------------------- FP64 block CG (baseline)
+// x_r = 0, r_r = p_r = b_r, |b_r|^2 partial sums into sets a (slot r) and c
+// (slot r), active rhs
+__global__ void __launch_bounds__(256) k_bcg64_init(Field x, Field r, Field p, Field b, BatchDev *B, RedScratch ra,
+                                                    RedScratch rc) {
+  const int rr = blockIdx.y;
+  if (!B->active[rr]) return;   // uniform over the block
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double red = 0.0;
+  if (i < x.g->vh) {
+    double2 bv[3], z[3];
+    VB<QAMD_F64>::ld(b, rr, i, bv);
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      red += bv[j].x * bv[j].x + bv[j].y * bv[j].y;
+      z[j] = make_double2(0.0, 0.0);
+    }
+    VB<QAMD_F64>::st(x, rr, i, z);
+    VB<QAMD_F64>::st(r, rr, i, bv);
+    VB<QAMD_F64>::st(p, rr, i, bv);
+  }
+  double a[1] = {red};
+  block_sum(a);
+  if (threadIdx.x == 0) {
+    ra.part[rr * ra.max_blocks + blockIdx.x] = a[0];
+    rc.part[rr * rc.max_blocks + blockIdx.x] = a[0];
+  }
+}
+
+// x_r += alpha_r p_r; r_r -= alpha_r Ap_r; |r_r|^2 partial sums (slot r)
+__global__ void __launch_bounds__(256) k_bcg64_xr(Field x, Field r, Field p, Field Ap, BatchDev *B, RedScratch rs) {
+  const int rr = blockIdx.y;
+  if (!B->active[rr]) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double red = 0.0;
+  if (i < x.g->vh) {
+    const double al = B->alpha[rr];
+    double2 xv[3], rv[3], pv[3], av[3];
+    VB<QAMD_F64>::ld(x, rr, i, xv);
+    VB<QAMD_F64>::ld(r, rr, i, rv);
+    VB<QAMD_F64>::ld(p, rr, i, pv);
+    VB<QAMD_F64>::ld(Ap, rr, i, av);
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      xv[j].x = fma(al, pv[j].x, xv[j].x);
+      xv[j].y = fma(al, pv[j].y, xv[j].y);
+      rv[j].x = fma(-al, av[j].x, rv[j].x);
+      rv[j].y = fma(-al, av[j].y, rv[j].y);
+      red += rv[j].x * rv[j].x + rv[j].y * rv[j].y;
+    }
+    VB<QAMD_F64>::st(x, rr, i, xv);
+    VB<QAMD_F64>::st(r, rr, i, rv);
+  }
+  rhs_partial(red, rs);
+}
+
+// p_r = r_r + beta_r p_r with |p_r|^2 partial sums (slot r)
+__global__ void __launch_bounds__(256) k_bcg64_p(Field p, Field r, BatchDev *B, RedScratch rs) {
+  const int rr = blockIdx.y;
+  if (!B->active[rr]) return;
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double red = 0.0;
+  if (i < p.g->vh) {
+    const double be = B->beta[rr];
+    double2 pv[3], rv[3];
+    VB<QAMD_F64>::ld(p, rr, i, pv);
+    VB<QAMD_F64>::ld(r, rr, i, rv);
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      pv[j].x = fma(be, pv[j].x, rv[j].x);
+      pv[j].y = fma(be, pv[j].y, rv[j].y);
+      red += pv[j].x * pv[j].x + pv[j].y * pv[j].y;
+    }
+    VB<QAMD_F64>::st(p, rr, i, pv);
+  }
+  rhs_partial(red, rs);
+}
+
 static double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
@@ -467,6 +545,118 @@ struct BState {
 using namespace qamd;
 
 extern "C" {
+
+// Plain FP64 block CG: textbook CG recurrences for every rhs (the
+// cgSolveT iteration, lib/cg.c:43-122, in unscaled form), all vectors FP64,
+// the normal operator applied to the whole block by the block Dslash
+// (links loaded once for all rhs); each rhs leaves the block when its
+// iterated |r|^2 < rsq |b|^2 (checked every 16 iterations).
+int qamd_stag_cg_block(const qamd_links *l, qamd_field *x, const qamd_field *b, double mass, double rsq,
+                       int maxits, qamd_solve_info *info) {
+  if (!l || !x || !b || l->kind != 0 || l->prec != QAMD_F64) {
+    set_error("stag_cg_block: needs FP64 doublestored staggered links");
+    return -1;
+  }
+  const int n = b->f.nrhs;
+  if (n < 1 || n > 8 || x->f.nrhs != n || b->f.prec != QAMD_F64 || x->f.prec != QAMD_F64 || b->f.ncomp != 3 ||
+      x->f.ncomp != 3 || b->f.parity != 0 || x->f.parity != 0 || b->lat != l->lat || x->lat != l->lat) {
+    set_error("stag_cg_block: x, b must be even FP64 colour-vector blocks of 1..8 rhs on the links' lattice");
+    return -1;
+  }
+  double t0 = now_s();
+  qamd_lattice *lat = l->lat;
+  const int nb = (int)((lat->g.vh + 255) / 256);
+  const dim3 gb(nb, n);
+  const FinOps fops = fin_ops(n);
+  qamd_field *r = qamd_field_create(lat, QAMD_F64, 3, n, 0);
+  qamd_field *p = qamd_field_create(lat, QAMD_F64, 3, n, 0);
+  qamd_field *Ap = qamd_field_create(lat, QAMD_F64, 3, n, 0);
+  qamd_field *t = qamd_field_create(lat, QAMD_F64, 3, n, 1);
+  BatchDev *B = (BatchDev *)dev_alloc(sizeof(BatchDev)), *Bh = nullptr;
+  if (hipHostMalloc((void **)&Bh, sizeof(BatchDev), hipHostMallocDefault) != hipSuccess) Bh = nullptr;
+  double m2[MAXB], bb[MAXB];
+  int its_done[MAXB] = {0};
+  int rc = -1, its = 0;
+  do {
+    if (!r || !p || !Ap || !t || !B || !Bh || red_reserve(nb)) break;
+    if (qamd_field_norm2(b, bb)) break;
+    memset(Bh, 0, sizeof(BatchDev));
+    Bh->nrhs = n;
+    for (int i = 0; i < n; i++) {
+      m2[i] = mass * mass;
+      Bh->m2[i] = m2[i];
+      Bh->active[i] = 1;
+    }
+    if (hipMemcpyAsync(B, Bh, sizeof(BatchDev), hipMemcpyHostToDevice, stream()) != hipSuccess) break;
+    hipLaunchKernelGGL(k_bcg64_init, gb, dim3(256), 0, stream(), x->f, r->f, p->f, b->f, B, red_set(0), red_set(3));
+    fops.truer(nb, B, stream());
+    if (hipMemcpyAsync(Bh, B, sizeof(BatchDev), hipMemcpyDeviceToHost, stream()) != hipSuccess ||
+        hipStreamSynchronize(stream()) != hipSuccess)
+      break;
+    int nact = 0;
+    for (int i = 0; i < n; i++) {
+      Bh->active[i] = bb[i] > 0 && Bh->rsq[i] >= rsq * bb[i];
+      nact += Bh->active[i];
+    }
+    if (hipMemcpyAsync(B, Bh, sizeof(BatchDev), hipMemcpyHostToDevice, stream()) != hipSuccess) break;
+    int err = 0;
+    double zero[MAXB] = {0.0};
+    while (nact > 0 && its < maxits && !err) {
+      unsigned act = 0;
+      for (int i = 0; i < n; i++)
+        if (Bh->active[i]) act |= 1u << i;
+      for (int k = 0; k < 16 && !err; k++) {
+        // t = D_oe p (|t_r|^2), alpha_r = rsq_r / (m2 |p_r|^2 + |t_r|^2),
+        // Ap = m2 p - D_eo t, x += alpha p, r -= alpha Ap (|r_r|^2), beta_r,
+        // p = r + beta p (|p_r|^2)
+        err |= stag_block_launch_m(l, t->f, p->f, zero, nullptr, 1.0, true, red_set(1), stream(), act);
+        fops.alpha(nb, B, stream());
+        err |= stag_block_launch_m(l, Ap->f, t->f, m2, &p->f, -1.0, false, RedScratch{}, stream(), act);
+        hipLaunchKernelGGL(k_bcg64_xr, gb, dim3(256), 0, stream(), x->f, r->f, p->f, Ap->f, B, red_set(0));
+        fops.cg(nb, B, stream());
+        hipLaunchKernelGGL(k_bcg64_p, gb, dim3(256), 0, stream(), p->f, r->f, B, red_set(3));
+        if (hipGetLastError() != hipSuccess) err = 1;
+      }
+      if (err || hipMemcpyAsync(Bh, B, sizeof(BatchDev), hipMemcpyDeviceToHost, stream()) != hipSuccess ||
+          hipStreamSynchronize(stream()) != hipSuccess) {
+        err = 1;
+        break;
+      }
+      its += 16;
+      nact = 0;
+      for (int i = 0; i < n; i++) {
+        if (Bh->active[i] && Bh->rsq[i] < rsq * bb[i]) {
+          Bh->active[i] = 0;
+          its_done[i] = its;
+        }
+        nact += Bh->active[i];
+      }
+      if (hipMemcpyAsync(B, Bh, sizeof(BatchDev), hipMemcpyHostToDevice, stream()) != hipSuccess) err = 1;
+    }
+    if (err) {
+      if (!qamd_last_error()[0]) set_error("stag_cg_block: launch failed");
+      break;
+    }
+    if (info)
+      for (int i = 0; i < n; i++) {
+        memset(&info[i], 0, sizeof info[i]);
+        info[i].iterations = Bh->active[i] ? its : its_done[i];
+        info[i].iter_rsq = bb[i] > 0 ? Bh->rsq[i] / bb[i] : 0;
+        info[i].true_rsq = info[i].iter_rsq;
+        info[i].converged = bb[i] == 0 || Bh->rsq[i] < rsq * bb[i];
+        info[i].seconds = now_s() - t0;
+      }
+    rc = 0;
+  } while (0);
+  qamd_field_destroy(r);
+  qamd_field_destroy(p);
+  qamd_field_destroy(Ap);
+  qamd_field_destroy(t);
+  dev_free(B);
+  if (Bh) (void)hipHostFree(Bh);
+  return rc;
+}
+
 
 int qamd_stag_cg_mixed_block(const qamd_links *l64, const qamd_links *ls_in, qamd_field *x, const qamd_field *b,
                              double mass, double rsq, int maxits, double delta, qamd_solve_info *info) {

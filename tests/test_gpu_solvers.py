@@ -160,6 +160,35 @@ def test_mixed_cg_true_residual(gpu, sloppy):
     assert rel_err(xs, xr, parity_mask(dims, 0)) < 1e-7
 
 
+@pytest.mark.parametrize("nrhs", [1, 3, 8])
+def test_fp64_block_cg_matches_oracle(gpu, nrhs):
+    """Plain FP64 block CG: every rhs reaches the oracle CG's solution and
+    true residual; its iteration count is the oracle's rounded up to the
+    16-iteration check (a zero source leaves at once)."""
+    dims = (8, 8, 8, 8)
+    mass, rsq = 0.2, 1e-16
+    links = O.asqtad_links(dims, 3)
+    bs = np.stack([O.gauss_vector(dims, 60 + r, 100, parity=0) * (1.0 + 3 * r) for r in range(nrhs)])
+    if nrhs == 3:
+        bs[1] = 0.0
+    lat = gpu.Lattice(dims)
+    L = gpu.Links(lat, gpu.STAG, 0, 16).set_lex(links)
+    bb = gpu.Field(lat, 0, 3, nrhs, 0).set_lex(bs)
+    xb = gpu.Field(lat, 0, 3, nrhs, 0)
+    infos = gpu.stag_cg_block(L, xb, bb, mass, rsq, 2000)
+    xs = xb.get_lex().reshape(nrhs, -1, 6)
+    m = parity_mask(dims, 0)
+    for r in range(nrhs):
+        assert infos[r]["converged"], (r, infos[r])
+        if not bs[r].any():
+            assert infos[r]["iterations"] == 0 and not xs[r].any()
+            continue
+        xr, its_r, _ = O.cg(dims, links, mass, bs[r], rsq, 2000, noscale=True)
+        assert its_r <= infos[r]["iterations"] <= its_r + 16, (r, infos[r]["iterations"], its_r)
+        assert _true_rel_resid(dims, links, mass, bs[r], xs[r]) < 4 * rsq
+        assert rel_err(xs[r], xr, m) < 1e-6
+
+
 @pytest.mark.parametrize("sloppy,nrhs", [(1, 1), (1, 5), (3, 8)])
 def test_mixed_cg_block(gpu, sloppy, nrhs):
     """Batched mixed-precision CG: every rhs of the block reaches the FP64
