@@ -72,6 +72,12 @@ def parse():
                     help="KIND,PREC,NRHS[,MODE]: 20 Dslash launches of one kind (stag, stagfb or wilson) "
                          "for a profiler, after 3 copies of 256 MiB (the counters' scale); MODE is the "
                          "tuning mode (1 tuned, >= 2 forces launch shape MODE-2)")
+    ap.add_argument("--pmc-scale-child", action="store_true",
+                    help="kernels of known read bytes and load widths (a 16-byte-per-thread copy, then one "
+                         "even-field norm each in FP64, FP32 and Q16), for checking the FETCH_SIZE scale")
+    ap.add_argument("--fat7-child", action="store_true",
+                    help="3 fat7 + Naik smearings of a random SU(3) field, after 3 copies of 256 MiB (the "
+                         "counters' scale), for a profiler")
     ap.add_argument("--solver-child", type=str, default=None,
                     help="fp64 | mixed_q16 | mixed_q16_fb: one 32^4 solve of the bench's source, for a profiler")
     ap.add_argument("--block", type=int, default=8, help="sources in the batched mixed-precision solve")
@@ -130,6 +136,39 @@ def kernel_child(dims, spec):
     ms = Q.time_dslash(L, out, yin, 20)
     Q.sync()
     print("%s prec %d nrhs %d mode %d: %.4f ms" % (kind, prec, nrhs, mode, ms), flush=True)
+
+
+def pmc_scale_child(dims):
+    """Kernels whose read bytes are known: the copy kernel (16-byte loads,
+    COPY_BYTES read), then k_norm_dot over one even colour-vector field in
+    FP64 (16-byte loads, 48 B/site), FP32 (8-byte, 24 B/site) and Q16 (4-byte
+    components + a 4-byte norm, 16 B/site), in that dispatch order."""
+    import qll_amd as Q
+    Q.init()
+    Q.stream_copy(COPY_BYTES, 1)
+    lat = Q.Lattice(dims)
+    vh = lat.V // 2
+    for prec, per_site in ((Q.F64, 48), (Q.F32, 24), (Q.Q16, 16)):
+        f = Q.Field(lat, prec, 3, 1, Q.EVEN).gaussian(1, 1)
+        f.norm2()
+        print("norm2 prec %d: %d bytes read" % (prec, per_site * vh), flush=True)
+
+
+def fat7_child(dims):
+    """Three fat7 + Naik smearings (the bench's coefficients) for a profiler
+    run; a copy kernel of known size runs first."""
+    import qll_amd as Q
+    Q.init()
+    Q.stream_copy(COPY_BYTES, 3)
+    lat = Q.Lattice(dims)
+    u = Q.Gauge(lat).random_su3(1).stag_phases()
+    for _ in range(3):
+        Q.sync()
+        t0 = time.time()
+        fl, ll = Q.fat7(u)
+        Q.sync()
+        print("fat7 + long links: %.3f ms" % ((time.time() - t0) * 1e3), flush=True)
+        del fl, ll
 
 
 def solver_child(dims, kind):
@@ -424,6 +463,12 @@ def main():
         return
     if args.kernel_child:
         kernel_child(dims, args.kernel_child)
+        return
+    if args.pmc_scale_child:
+        pmc_scale_child(dims)
+        return
+    if args.fat7_child:
+        fat7_child(dims)
         return
     if args.solver_child:
         solver_child(dims, args.solver_child)

@@ -1,15 +1,14 @@
 // Gauge fields on the device: seeded SU(3) generation, staggered phases,
-// fat7 + Naik smearing as a staple-product kernel, plaquette, host transfer.
+// fat7 + Naik smearing in three staple passes per direction, plaquette,
+// host transfer.
 //
-// fat7 (lib/gauge.c:188-416) is evaluated per (site, direction) as the sum of
-// nested staples, all in FP64:
+// fat7 (lib/gauge.c:188-416) in FP64, with
 //   St_nu[X](x) = U_nu(x) X(x+nu) U_nu(x+mu)^+ + U_nu(x-nu)^+ X(x-nu) U_nu(x-nu+mu)
 //   fat_mu = (c1 - 6 cL) U_mu + c3 sum_nu St_nu[U_mu] + cL sum_nu St_nu[St_nu[U_mu]]
 //          + c5 sum_{nu,rho} St_rho[St_nu[U_mu]]
 //          + c7 sum_{nu,rho,sig} St_sig[St_rho[St_nu[U_mu]]]
-// which is the same sum of path products QLL's genStaple recursion builds with
-// transporters and temporary fields; here every site recomputes its own
-// paths from U (no temporaries, one pass, gathers served by L2/MALL).
+// the sum QLL's genStaple recursion builds with transporters and temporary
+// fields.
 #include "qamd_internal.h"
 #include "qamd_objects.h"
 #include "qamd_site.h"
@@ -88,6 +87,39 @@ __device__ inline M3 mul_na(const M3 &A, const M3 &B) {
     }
   return C;
 }
+// A += a B C (nn) and A += a B C^+ (na), without a temporary for the product
+__device__ inline void madd_nn(M3 &A, double a, const M3 &B, const M3 &C) {
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      double2 s = make_double2(0, 0);
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        double2 t = cmul(B.e[3 * k + r], C.e[3 * c + k]);
+        s.x += t.x;
+        s.y += t.y;
+      }
+      A.e[3 * c + r].x += a * s.x;
+      A.e[3 * c + r].y += a * s.y;
+    }
+}
+__device__ inline void madd_na(M3 &A, double a, const M3 &B, const M3 &C) {
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      double2 s = make_double2(0, 0);
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        double2 t = cmulc(B.e[3 * k + r], C.e[3 * k + c]);
+        s.x += t.x;
+        s.y += t.y;
+      }
+      A.e[3 * c + r].x += a * s.x;
+      A.e[3 * c + r].y += a * s.y;
+    }
+}
 __device__ inline void madd(M3 &A, double a, const M3 &B) {
 #pragma unroll
   for (int i = 0; i < 9; i++) {
@@ -96,111 +128,190 @@ __device__ inline void madd(M3 &A, double a, const M3 &B) {
   }
 }
 
+// site coordinates as four named values: a direction known only at run time
+// selects among them (no indexed array, which would live in scratch memory)
+struct SC {
+  int c0, c1, c2, c3;
+};
+__device__ __forceinline__ SC sc_of(const Site &s) { return SC{s.c[0], s.c[1], s.c[2], s.c[3]}; }
+
 struct GView {
   const double2 *d;
   Geom g;
-  __device__ M3 load(int mu, const int c[4]) const {
-    int lex = c[0] + g.stride[1] * c[1] + g.stride[2] * c[2] + g.stride[3] * c[3];
-    int par = (c[0] + c[1] + c[2] + c[3]) & 1;
-    const double2 *p = d + ((int64_t)(mu * 2 + par) * 9) * g.vh_pad + (lex >> 1);
+  __device__ M3 load(int f, const SC &c) const {   // field f (a direction) at site c
+    const int lex = c.c0 + g.stride[1] * c.c1 + g.stride[2] * c.c2 + g.stride[3] * c.c3;
+    const int par = (c.c0 + c.c1 + c.c2 + c.c3) & 1;
+    const double2 *p = d + ((int64_t)(f * 2 + par) * 9) * g.vh_pad + (lex >> 1);
     M3 m;
 #pragma unroll
     for (int e = 0; e < 9; e++) m.e[e] = p[e * g.vh_pad];
     return m;
   }
-  __device__ void shift(int c[4], int mu, int h) const {
-    int v = c[mu] + h;
-    v += (v < 0) ? g.L[mu] : 0;
-    v -= (v >= g.L[mu]) ? g.L[mu] : 0;
-    c[mu] = v;
+  __device__ SC shifted(SC c, int mu, int h) const {   // c + h mu (periodic)
+    const int L = sel4(mu, g.L[0], g.L[1], g.L[2], g.L[3]);
+    int v = sel4(mu, c.c0, c.c1, c.c2, c.c3) + h;
+    v += (v < 0) ? L : 0;
+    v -= (v >= L) ? L : 0;
+    c.c0 = mu == 0 ? v : c.c0;
+    c.c1 = mu == 1 ? v : c.c1;
+    c.c2 = mu == 2 ? v : c.c2;
+    c.c3 = mu == 3 ? v : c.c3;
+    return c;
   }
 };
-
-// explicit depth unrolling (device recursion is avoided)
-template <int LEVEL>
-__device__ M3 nestedT(const GView &U, int mu, const int *dirs, const int c[4]);
-template <>
-__device__ M3 nestedT<0>(const GView &U, int mu, const int *, const int c[4]) { return U.load(mu, c); }
-template <int LEVEL>
-__device__ M3 nestedT(const GView &U, int mu, const int *dirs, const int c[4]) {
-  int nu = dirs[LEVEL - 1];
-  int b[4] = {c[0], c[1], c[2], c[3]};
-  M3 Un = U.load(nu, b);
-  U.shift(b, nu, 1);
-  M3 t = mul_nn(Un, nestedT<LEVEL - 1>(U, mu, dirs, b));
-  int d[4] = {c[0], c[1], c[2], c[3]};
-  U.shift(d, mu, 1);
-  M3 res = mul_na(t, U.load(nu, d));
-  int e[4] = {c[0], c[1], c[2], c[3]};
-  U.shift(e, nu, -1);
-  M3 t2 = mul_an(U.load(nu, e), nestedT<LEVEL - 1>(U, mu, dirs, e));
-  U.shift(e, mu, 1);
-  madd(res, 1.0, mul_nn(t2, U.load(nu, e)));
-  return res;
-}
 
 struct Fat7Coef {
   double c1, c3, c5, c7, cL, naik;
 };
 
-// one thread per (site, mu): grid.x over checkerboard sites, grid.y = 2*mu+par
-__global__ void __launch_bounds__(128) k_fat7(double2 *fl, const double2 *u, Geom g, Fat7Coef C) {
-  int cb = blockIdx.x * blockDim.x + threadIdx.x;
-  int mu = blockIdx.y >> 1, par = blockIdx.y & 1;
-  if (cb >= g.vh) return;
-  Site s = site_of(g, cb, par);
-  GView U{u, g};
-  M3 F = U.load(mu, s.c);
-#pragma unroll
-  for (int i = 0; i < 9; i++) {
-    F.e[i].x *= C.c1 - 6 * C.cL;
-    F.e[i].y *= C.c1 - 6 * C.cL;
+// ------------------------------------------- fat7 as three field passes
+// The staples are built once per (site, direction) and shared, as QLL's
+// genStaple sequence does (lib/gauge.c:264-366), instead of recomputed along
+// every path: for each direction mu,
+//   pass 1: S3_nu = St_nu[U_mu] for the three nu != mu (stored),
+//           fat_mu = (c1 - 6 cL) U_mu + c3 sum_nu S3_nu
+//   pass 2: T5_sig = sum over the two orderings (nu, rho) of the directions
+//           other than mu and sig of St_rho[S3_nu] (stored),
+//   pass 3: fat_mu += cL sum_nu St_nu[S3_nu] + c5 sum_sig T5_sig
+//                   + c7 sum_sig St_sig[T5_sig]
+// (each 7-staple St_sig[St_rho[St_nu[U]]] has sig = the one direction left,
+// so the staple is linear in its argument and the two 5-staples that feed
+// one sig share it).  Temporaries: 12 + 12 matrix fields in the gauge
+// layout, field 3 mu + j for the j-th direction other than mu.
+//
+// Thread mapping: a 256-thread block is 64 sites x the 4 directions (wave w
+// computes direction w), blocks alternate parity over the same site range,
+// and consecutive blocks run on one die (XCD order), so the four directions'
+// loads of a site and the neighbours of the other parity meet in one L2.
+// Each pass holds one accumulator (two would not fit the 256 registers of
+// two waves per SIMD: the staple's six operand loads are issued together).
+
+// the j-th direction other than mu, and the inverse
+__device__ __forceinline__ int fat_dir(int mu, int j) { return j + (j >= mu); }
+__device__ __forceinline__ int fat_idx(int mu, int nu) { return nu - (nu > mu); }
+
+// St_nu[X](x) = U_nu(x) X(x+nu) U_nu(x+mu)^+ + U_nu(x-nu)^+ X(x-nu) U_nu(x-nu+mu)
+// for the mu-direction field fx of view X
+__device__ M3 fat_staple(const GView &U, const GView &X, int fx, int mu, int nu, const SC &c) {
+  M3 t = mul_nn(U.load(nu, c), X.load(fx, U.shifted(c, nu, 1)));
+  M3 res = mul_na(t, U.load(nu, U.shifted(c, mu, 1)));
+  const SC e = U.shifted(c, nu, -1);
+  M3 t2 = mul_an(U.load(nu, e), X.load(fx, e));
+  madd(res, 1.0, mul_nn(t2, U.load(nu, U.shifted(e, mu, 1))));
+  return res;
+}
+// A += a St_nu[X](x), the staple not held as a matrix
+__device__ void fat_staple_acc(M3 &A, double a, const GView &U, const GView &X, int fx, int mu, int nu,
+                               const SC &c) {
+  {
+    const M3 t = mul_nn(U.load(nu, c), X.load(fx, U.shifted(c, nu, 1)));
+    madd_na(A, a, t, U.load(nu, U.shifted(c, mu, 1)));
   }
-  bool have5 = C.c5 != 0 || C.c7 != 0 || C.cL != 0;
-  for (int nu = 0; nu < 4; nu++) {
-    if (nu == mu) continue;
-    int dirs[3] = {nu, 0, 0};
-    if (C.c3 != 0 || have5) madd(F, C.c3, nestedT<1>(U, mu, dirs, s.c));
-    if (C.cL != 0) {
-      dirs[1] = nu;
-      madd(F, C.cL, nestedT<2>(U, mu, dirs, s.c));
-    }
-    if (C.c5 != 0 || C.c7 != 0) {
-      for (int rho = 0; rho < 4; rho++) {
-        if (rho == mu || rho == nu) continue;
-        dirs[1] = rho;
-        madd(F, C.c5, nestedT<2>(U, mu, dirs, s.c));
-        if (C.c7 != 0) {
-          for (int sig = 0; sig < 4; sig++) {
-            if (sig == mu || sig == nu || sig == rho) continue;
-            dirs[2] = sig;
-            madd(F, C.c7, nestedT<3>(U, mu, dirs, s.c));
-          }
-        }
-      }
-    }
-  }
-  double2 *p = fl + ((int64_t)(mu * 2 + par) * 9) * g.vh_pad + cb;
-#pragma unroll
-  for (int e = 0; e < 9; e++) p[e * g.vh_pad] = F.e[e];
+  const SC e = U.shifted(c, nu, -1);
+  const M3 t2 = mul_an(U.load(nu, e), X.load(fx, e));
+  madd_nn(A, a, t2, U.load(nu, U.shifted(e, mu, 1)));
 }
 
-// long links: naik * U(x) U(x+mu) U(x+2mu)   (lib/gauge.c:369-392)
-__global__ void k_longlink(double2 *ll, const double2 *u, Geom g, double naik) {
-  int cb = blockIdx.x * blockDim.x + threadIdx.x;
-  int mu = blockIdx.y >> 1, par = blockIdx.y & 1;
-  if (cb >= g.vh) return;
-  Site s = site_of(g, cb, par);
-  GView U{u, g};
-  int c[4] = {s.c[0], s.c[1], s.c[2], s.c[3]};
-  M3 A = U.load(mu, c);
-  U.shift(c, mu, 1);
-  A = mul_nn(A, U.load(mu, c));
-  U.shift(c, mu, 1);
-  A = mul_nn(A, U.load(mu, c));
-  double2 *p = ll + ((int64_t)(mu * 2 + par) * 9) * g.vh_pad + cb;
+__device__ __forceinline__ void fat_store(double2 *base, const Geom &g, int f, int par, int cb, const M3 &m) {
+  double2 *p = base + ((int64_t)(f * 2 + par) * 9) * g.vh_pad + cb;
 #pragma unroll
-  for (int e = 0; e < 9; e++) p[e * g.vh_pad] = make_double2(naik * A.e[e].x, naik * A.e[e].y);
+  for (int e = 0; e < 9; e++) p[e * g.vh_pad] = m.e[e];
+}
+__device__ __forceinline__ void fat_accum(double2 *base, const Geom &g, int f, int par, int cb, const M3 &m) {
+  double2 *p = base + ((int64_t)(f * 2 + par) * 9) * g.vh_pad + cb;
+#pragma unroll
+  for (int e = 0; e < 9; e++) {
+    double2 v = p[e * g.vh_pad];
+    p[e * g.vh_pad] = make_double2(v.x + m.e[e].x, v.y + m.e[e].y);
+  }
+}
+
+struct FatSite {
+  int mu, par, cb, k;
+};
+// nk: blocks per 64-site run (k = 0..nk-1, consecutive, so they share a die)
+__device__ __forceinline__ FatSite fat_site(int nk = 1) {
+  const int b = xcd_block(blockIdx.x, gridDim.x), sb = b / nk;
+  return FatSite{(int)(threadIdx.x >> 6), sb & 1, (sb >> 1) * 64 + (int)(threadIdx.x & 63), b - sb * nk};
+}
+static dim3 fat_grid(const Geom &g, int nk = 1) { return dim3((unsigned)(2 * nk * ((g.vh + 63) / 64))); }
+
+// also the long links (ll != NULL): naik ul_mu(x) ul_mu(x+mu) ul_mu(x+2mu)
+// (lib/gauge.c:369-392), whose loads share the block's U traffic when ul = u
+__global__ void __launch_bounds__(256) k_fat7_p1(double2 *fl, double2 *s3, double2 *ll, const double2 *u,
+                                                 const double2 *ul, Geom g, Fat7Coef C, int keep) {
+  const FatSite f = fat_site();
+  if (f.cb >= g.vh) return;
+  const SC c = sc_of(site_of(g, f.cb, f.par));
+  GView U{u, g};
+  if (ll) {
+    GView L{ul, g};
+    const SC c1 = L.shifted(c, f.mu, 1);
+    M3 A = mul_nn(L.load(f.mu, c), L.load(f.mu, c1));
+    A = mul_nn(A, L.load(f.mu, L.shifted(c1, f.mu, 1)));
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+      A.e[i].x *= C.naik;
+      A.e[i].y *= C.naik;
+    }
+    fat_store(ll, g, f.mu, f.par, f.cb, A);
+  }
+  M3 F = U.load(f.mu, c);
+  const double c1 = C.c1 - 6 * C.cL;
+#pragma unroll
+  for (int i = 0; i < 9; i++) {
+    F.e[i].x *= c1;
+    F.e[i].y *= c1;
+  }
+#pragma unroll 1
+  for (int j = 0; j < 3; j++) {
+    M3 S = fat_staple(U, U, f.mu, f.mu, fat_dir(f.mu, j), c);
+    if (keep) fat_store(s3, g, 3 * f.mu + j, f.par, f.cb, S);
+    madd(F, C.c3, S);
+  }
+  fat_store(fl, g, f.mu, f.par, f.cb, F);
+}
+
+// three blocks per site run: T5 for sig = the k-th direction other than mu
+__global__ void __launch_bounds__(256) k_fat7_p2(double2 *t5, const double2 *s3, const double2 *u, Geom g) {
+  const FatSite f = fat_site(3);
+  if (f.cb >= g.vh) return;
+  const int mu = f.mu, k = f.k, sig = fat_dir(mu, k);
+  const SC c = sc_of(site_of(g, f.cb, f.par));
+  GView U{u, g}, S3{s3, g};
+  M3 T;
+#pragma unroll
+  for (int i = 0; i < 9; i++) T.e[i] = make_double2(0.0, 0.0);
+#pragma unroll 1
+  for (int nu = 0; nu < 4; nu++) {
+    if (nu == mu || nu == sig) continue;
+    const int rho = 6 - mu - sig - nu;   // the fourth direction
+    fat_staple_acc(T, 1.0, U, S3, 3 * mu + fat_idx(mu, nu), mu, rho, c);
+  }
+  fat_store(t5, g, 3 * mu + k, f.par, f.cb, T);
+}
+
+// fat_mu += cL sum_nu St_nu[S3_nu] + c5 sum_sig T5_sig + c7 sum_sig St_sig[T5_sig]
+__global__ void __launch_bounds__(256) k_fat7_p3(double2 *fl, const double2 *s3, const double2 *t5, const double2 *u,
+                                                 Geom g, Fat7Coef C) {
+  const FatSite f = fat_site();
+  if (f.cb >= g.vh) return;
+  const int mu = f.mu;
+  const SC c = sc_of(site_of(g, f.cb, f.par));
+  GView U{u, g}, S3{s3, g}, T5{t5, g};
+  M3 F;
+#pragma unroll
+  for (int i = 0; i < 9; i++) F.e[i] = make_double2(0.0, 0.0);
+  if (C.cL != 0)
+#pragma unroll 1
+    for (int j = 0; j < 3; j++) fat_staple_acc(F, C.cL, U, S3, 3 * mu + j, mu, fat_dir(mu, j), c);
+  if (C.c5 != 0 || C.c7 != 0)
+#pragma unroll 1
+    for (int k = 0; k < 3; k++) {
+      if (C.c5 != 0) madd(F, C.c5, T5.load(3 * mu + k, c));
+      if (C.c7 != 0) fat_staple_acc(F, C.c7, U, T5, 3 * mu + k, mu, fat_dir(mu, k), c);
+    }
+  fat_accum(fl, g, mu, f.par, f.cb, F);
 }
 
 // Gram-Schmidt in the column order of makeUsite (lib/vla.c:1635-1654), then
@@ -290,16 +401,14 @@ __global__ void k_plaq(const double2 *u, Geom g, RedScratch rs) {
   GView U{u, g};
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < 2 * g.vh; i += (int64_t)gridDim.x * blockDim.x) {
     int par = (int)(i / g.vh), cb = (int)(i % g.vh);
-    Site s = site_of(g, cb, par);
+    const SC c = sc_of(site_of(g, cb, par));
     int l = 0;
+#pragma unroll
     for (int mu = 0; mu < 4; mu++)
+#pragma unroll
       for (int nu = 0; nu < mu; nu++) {
-        int a[4] = {s.c[0], s.c[1], s.c[2], s.c[3]};
-        int b[4] = {s.c[0], s.c[1], s.c[2], s.c[3]};
-        U.shift(a, mu, 1);
-        U.shift(b, nu, 1);
-        M3 A = mul_nn(U.load(mu, s.c), U.load(nu, a));
-        M3 B = mul_nn(U.load(nu, s.c), U.load(mu, b));
+        M3 A = mul_nn(U.load(mu, c), U.load(nu, U.shifted(c, mu, 1)));
+        M3 B = mul_nn(U.load(nu, c), U.load(mu, U.shifted(c, nu, 1)));
         M3 P = mul_na(A, B);
         acc[l++] += P.e[0].x + P.e[4].x + P.e[8].x;
       }
@@ -496,12 +605,25 @@ int qamd_fat7(qamd_gauge *fl, qamd_gauge *ll, const double coef[6], const qamd_g
   if (fl == u || (ll && (ll == u || ll == ulong))) { set_error("fat7: outputs must not alias inputs"); return -1; }
   const Geom &g = u->lat->g;
   Fat7Coef C = {coef[0], coef[1], coef[2], coef[3], coef[4], coef[5]};
-  hipLaunchKernelGGL(k_fat7, gauge_grid(g, 128), dim3(128), 0, stream(), fl->d, u->d, g, C);
-  QAMD_HIP(hipGetLastError());
-  if (ll) {
-    const qamd_gauge *ul = ulong ? ulong : u;
-    hipLaunchKernelGGL(k_longlink, gauge_grid(g, 256), dim3(256), 0, stream(), ll->d, ul->d, g, coef[5]);
-    QAMD_HIP(hipGetLastError());
+  {
+    const bool have5 = C.c5 != 0 || C.c7 != 0 || C.cL != 0;   // S3 kept for the nested staples
+    const bool have57 = C.c5 != 0 || C.c7 != 0;                 // the T5 sums needed
+    const size_t nf = (size_t)2 * 9 * g.vh_pad;   // double2 per matrix field
+    double2 *tmp = (double2 *)dev_alloc(((have5 ? 12 : 0) + (have57 ? 12 : 0)) * nf * sizeof(double2) + 256);
+    if (!tmp) return -1;
+    double2 *s3 = tmp, *t5 = tmp + (have5 ? 12 : 0) * nf;
+    const dim3 grid = fat_grid(g);
+    hipLaunchKernelGGL(k_fat7_p1, grid, dim3(256), 0, stream(), fl->d, s3, ll ? ll->d : nullptr, u->d,
+                       ulong ? ulong->d : u->d, g, C, have5 ? 1 : 0);
+    if (have57) hipLaunchKernelGGL(k_fat7_p2, fat_grid(g, 3), dim3(256), 0, stream(), t5, s3, u->d, g);
+    if (have5) hipLaunchKernelGGL(k_fat7_p3, grid, dim3(256), 0, stream(), fl->d, s3, t5, u->d, g, C);
+    hipError_t e = hipGetLastError();
+    (void)hipStreamSynchronize(stream());
+    dev_free(tmp);
+    if (e != hipSuccess) {
+      set_error("fat7: %s", hipGetErrorString(e));
+      return -1;
+    }
   }
   return 0;
 }

@@ -43,6 +43,27 @@ def test_fat7_with_lepage(gpu):
     assert rel_err(ll.get_lex(), ref_ll) < 1e-13
 
 
+@pytest.mark.parametrize("coef", [
+    (0.7, 0.0, 0.0, 0.0, 0.0, 0.0),          # one-link term only (no staple pass)
+    (0.6, -0.05, 0.0, 0.0, 0.0, 0.0),        # 3-staples only
+    (0.6, -0.05, 0.02, 0.0, 0.0, 0.0),       # no 7-staples: the 5-staple sums are not stored
+    (0.6, 0.0, 0.0, 0.0, 0.03, 0.0),         # Lepage only
+    (0.6, -0.05, 0.02, -0.003, 0.03, -0.04),
+])
+def test_fat7_coefficient_subsets(gpu, coef):
+    """Each staple pass switched on or off by its coefficients (the passes and
+    their temporaries are skipped when unused) against the oracle."""
+    dims = (4, 6, 4, 8)
+    ref_u = O.random_su3_field(dims, 21)
+    ref_fl, ref_ll = O.fat7(dims, ref_u, ref_u, coef)
+    lat = gpu.Lattice(dims)
+    u = gpu.Gauge(lat).set_lex(ref_u)
+    fl, ll = gpu.fat7(u, coef)
+    assert rel_err(fl.get_lex(), ref_fl) < 1e-13
+    if coef[5] != 0:
+        assert rel_err(ll.get_lex(), ref_ll) < 1e-13
+
+
 def test_plaquette(gpu):
     dims = (4, 4, 4, 6)
     ref_u = O.random_su3_field(dims, 8)
