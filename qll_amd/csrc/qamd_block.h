@@ -26,7 +26,8 @@ template <int P, int PO> struct MArgs {
   int par, nr;
   float lscale[16];
   RedScratch rs;   // partial sums of rhs r in slots 2r, 2r+1 (offset by the caller)
-  int xcd;         // 1: XCD-aware block order
+  int xcd;         // block order: 0 plain, 1 XCD ranges (xcd_block), 2 slab sweep (slab_block)
+  SlabOrder slab;
   const uint8_t *lsign;   // set: long terms' third rows rebuilt (determinant sign bits per site)
 };
 

@@ -310,6 +310,31 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
   return xcd * q + (xcd < r ? xcd : r) + idx;
 }
 
+// Slab-sweep block order for 256-site blocks of one parity.  Logical block
+// blk covers checkerboard sites blk*256 ..; a (z, t) plane holds P of them
+// (P = Lx Ly / 512).  Die x owns the piece (y part x % py, z range x / py)
+// of every plane and walks it plane by plane in t: all dies advance through
+// t together, and a die's gathers from t +- 3 land in the 7 t-slices of its
+// own piece it has just touched, which fit its L2 (the contiguous ranges of
+// xcd_block put whole t-slices on a die: 7 of them do not fit).  A
+// permutation of the logical blocks, so partial sums indexed by logical
+// block do not change.
+struct SlabOrder {
+  int P, py, bpy, zs, Lz;   // blocks per plane, y pieces, blocks per piece per plane, z per piece
+};
+__device__ __forceinline__ int slab_block(int b, const SlabOrder &s) {
+  const int x = b % NXCD, j = b / NXCD;
+  const int hy = x % s.py, zq = x / s.py;
+  const int per_t = s.zs * s.bpy;
+  const int t = j / per_t, r = j - t * per_t;
+  const int zl = r / s.bpy, pb = r - zl * s.bpy;
+  return (t * s.Lz + zq * s.zs + zl) * s.P + hy * s.bpy + pb;
+}
+// the slab order for g, or false when the lattice does not split that way
+// (planes not a whole number of blocks, or no split of the 8 dies into y and
+// z pieces that divides both)
+bool slab_order(const Geom &g, SlabOrder &s);
+
 // --------------------------------------------------------------- precisions
 // Complex loads/stores for the three storage precisions.  Accumulation is in
 // double for FP64 storage and float for FP32/FP16 storage.

@@ -70,6 +70,28 @@ hipGraphExec_t capture_stream(int (*enqueue)(void *), void *ctx) {
 }
 bool initialized() { return g_init != 0; }
 
+bool slab_order(const Geom &g, SlabOrder &s) {
+  const int64_t plane = (int64_t)g.L[0] * g.L[1] / 2;   // sites of one parity per (z, t)
+  if (plane % 256 != 0 || g.vh % 256 != 0) return false;
+  const int P = (int)(plane / 256);
+  // pieces as thick as possible in both y and z (the gathers reach 3 sites)
+  int best = -1;
+  for (int py = 1; py <= NXCD; py *= 2) {
+    const int pz = NXCD / py;
+    if (P % py != 0 || g.L[2] % pz != 0) continue;
+    const int ty = g.L[1] / py, tz = g.L[2] / pz, th = ty < tz ? ty : tz;
+    if (th > best) {
+      best = th;
+      s.P = P;
+      s.py = py;
+      s.bpy = P / py;
+      s.zs = tz;
+      s.Lz = g.L[2];
+    }
+  }
+  return best > 0;
+}
+
 void *dev_alloc(size_t bytes) {
   if (ensure_init()) return nullptr;
   void *p = nullptr;

@@ -22,7 +22,7 @@ constexpr int MAXB = 16;
 // fields rhs-outer
 template <int P, int PO, int NT, int NR, bool RED, bool CGX = false>
 __global__ void __launch_bounds__(256) k_stag_block(MArgs<P, PO> A) {
-  int blk = A.xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+  const int blk = A.xcd == 2 ? slab_block(blockIdx.x, A.slab) : A.xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
   int cb = blk * 256 + threadIdx.x;
   double red[2 * NR];
 #pragma unroll
@@ -74,7 +74,7 @@ static int blaunch_nt(const MArgs<P, PO> &A, bool red, hipStream_t st) {
 template <int P, int PO>
 struct BlockRun {
   MArgs<P, PO> A;
-  int n, nt, red, ncand, k[8], xcd[8];
+  int n, nt, red, ncand, k[10], xcd[10];
   hipStream_t st;
   static int run(int c, void *ctx) {
     BlockRun &R = *(BlockRun *)ctx;
@@ -129,13 +129,24 @@ static int blaunch(const qamd_links *l, Field &out, const Field &in, const doubl
   R.red = red;
   R.st = st;
   // candidates: all rhs in one launch (default), then halves, quarters, ...,
-  // each with and without the XCD block order
+  // each with the XCD block order and without; all rhs in one launch in the
+  // slab-sweep order when the lattice allows it
+  const bool slab = slab_order(A.g, A.slab);
   for (int kk = R.n; R.ncand < 8; kk = (kk + 1) / 2) {
     R.k[R.ncand] = kk;
     R.xcd[R.ncand++] = 1;
     R.k[R.ncand] = kk;
     R.xcd[R.ncand++] = 0;
     if (kk == 1) break;
+  }
+  if (slab) {
+    R.k[R.ncand] = R.n;
+    R.xcd[R.ncand++] = 2;
+  }
+  if (const char *o = getenv("QAMD_BLOCK_ORDER")) {   // force an order (measurements)
+    R.ncand = 1;
+    R.k[0] = R.n;
+    R.xcd[0] = (atoi(o) == 2 && !slab) ? 1 : atoi(o);
   }
   // out = a*out + ... would compound over the timed repeats: no tuning then;
   // nor for launches over a subset of the rhs (solvers whose converged rhs

@@ -178,11 +178,13 @@ def test_qll_layout_transfer(gpu, inner):
     assert np.array_equal(back, q)
 
 
-def test_block_dslash_every_launch_shape(gpu):
-    """Every launch shape the autotuner can pick (rhs per launch, block order)
-    gives the block Dslash and the block CG of the default shape bit for bit
-    (partial sums are indexed by site block, not by launch order)."""
-    dims = (8, 8, 8, 8)
+@pytest.mark.parametrize("dims", [(8, 8, 8, 8), (32, 16, 8, 8)])
+def test_block_dslash_every_launch_shape(gpu, dims):
+    """Every launch shape the autotuner can pick (rhs per launch, block order:
+    plain, XCD ranges, and on (32, 16, 8, 8) the slab sweep) gives the block
+    Dslash and the block CG of the default shape bit for bit (partial sums
+    are indexed by site block, not by launch order)."""
+    print("case: block shapes dims=%s" % (dims,), flush=True)
     links, _ = _stag_case(dims, 23)
     ys = np.stack([O.gauss_vector(dims, 90 + r, 100, parity=1) for r in range(6)])
     bs = np.stack([O.gauss_vector(dims, 95 + r, 100, parity=0) for r in range(6)])
@@ -192,7 +194,7 @@ def test_block_dslash_every_launch_shape(gpu):
     yin = gpu.Field(lat, 1, 3, 6, 1).set_lex(ys)
     b = gpu.Field(lat, 0, 3, 6, 0).set_lex(bs)
     res = []
-    for mode in [0] + list(range(2, 10)) + [1]:   # default, each candidate, tuned
+    for mode in [0] + list(range(2, 11)) + [1]:   # default, each candidate, tuned
         gpu.set_tuning(mode)
         out = gpu.Field(lat, 1, 3, 6, 0)
         gpu.dslash(L, out, yin)
