@@ -223,17 +223,25 @@ int qamd_stag_cg(const qamd_links *l, qamd_field *x, const qamd_field *b, double
                  double rsq, int maxits, qamd_solve_info *info);
 /* Mixed precision CG with reliable updates (no reference counterpart):
  * outer FP64 (links l64, x, b), inner sloppy links ls (FP32, FP16 or Q16,
- * escalating to FP32 when 16-bit links stall); stops on
- * the FP64 true residual |b - A x|^2 <= rsq |b|^2.  delta: reliable-update
- * threshold on |r| relative to its maximum since the last update. */
+ * escalating to FP32 when 16-bit links stall; doublestored or forward-only,
+ * independently of l64); stops on the FP64 true residual
+ * |b - A x|^2 <= rsq |b|^2.  The inner iteration is the single-reduction
+ * (Chronopoulos-Gear) CG with FP32 vectors, two launches per iteration,
+ * restarted at every reliable update.  delta: reliable-update threshold on
+ * |r| relative to its maximum since the last update.  x is overwritten (the
+ * solve starts from 0). */
 int qamd_stag_cg_mixed(const qamd_links *l64, const qamd_links *ls, qamd_field *x,
                        const qamd_field *b, double mass, double rsq, int maxits,
                        double delta, qamd_solve_info *info);
-/* Absolute error of the sloppy normal operator the mixed solvers apply
- * (sloppy input copy, sloppy odd temporary, FP32 output), |(A_s - A) v|/|v|
- * for a Gaussian v against A = m^2 - D_eo D_oe in FP64 (no reference
- * counterpart).  The mixed solvers measure it at set-up and use FP32 links
- * in place of 16-bit ones when it is not small against m^2. */
+/* Absolute error of the sloppy normal operator of the block and batched
+ * mixed solvers (input and odd temporary in the links' format, FP32
+ * output), |(A_s - A) v|/|v| for a Gaussian v against A = m^2 - D_eo D_oe in
+ * FP64 (no reference counterpart); the single-rhs mixed CG rounds only the
+ * odd temporary, so its operator's error is at most this.  The mixed solvers
+ * measure it at set-up (the single-rhs one keeps it with the sloppy links
+ * for later solves at the same mass) and use FP32 links in place of 16-bit
+ * ones when it is not small against m^2.  ls may be doublestored or
+ * forward-only. */
 int qamd_stag_sloppy_error(const qamd_links *l64, const qamd_links *ls, double mass, double *eps);
 /* Batched mixed-precision CG (no reference counterpart; the batched
  * counterpart of calling solve2, lib/stagDslash.c:594-612, once per source):
