@@ -162,8 +162,9 @@ def test_fb_mixed_cg(gpu, sloppy):
 
 
 def test_fb_rejected_where_unsupported(gpu):
-    """Forward-only sets run single-rhs operators only; a block of rhs and a
-    mix with doublestored links are refused with an error."""
+    """Forward-only sets run single-rhs operators only: a block of rhs, the
+    block and the mixed multishift solvers refuse them with an error; a set
+    without long links does not pair with one that has them."""
     dims = (4, 4, 4, 8)
     fl, ll, _ = _fb_case(dims, 3)
     lat = gpu.Lattice(dims)
@@ -172,8 +173,34 @@ def test_fb_rejected_where_unsupported(gpu):
     ob = gpu.Field(lat, 0, 3, 4, 0)
     with pytest.raises(gpu.QllAmdError):
         gpu.dslash(L, ob, yb)
-    Lds = gpu.Links(lat, gpu.STAG, 0, 16).set_lex(O.stag_doublestore(dims, fl, ll))
+    xb = gpu.Field(lat, 0, 3, 4, 0)
+    bb = gpu.Field(lat, 0, 3, 4, 0)
+    with pytest.raises(gpu.QllAmdError):
+        gpu.stag_cg_mixed_block(L, L.converted(1), xb, bb, 0.1, 1e-10, 10)
     x = gpu.Field(lat, 0, 3, 1, 0)
     bf = gpu.Field(lat, 0, 3, 1, 0)
     with pytest.raises(gpu.QllAmdError):
-        gpu.stag_cg_mixed(L, Lds.converted(1), x, bf, 0.1, 1e-10, 10)
+        gpu.stag_cgms_mixed(L, L.converted(1), [x], bf, 0.1, [0.0], 1e-10, 10)
+    Lfat = _fb_links(gpu, lat, fl, None)
+    with pytest.raises(gpu.QllAmdError):
+        gpu.stag_cg_mixed(L, Lfat.converted(1), x, bf, 0.1, 1e-10, 10)
+
+
+@pytest.mark.parametrize("sloppy", [1, 3])
+def test_mixed_cg_doublestored_fp64_forward_only_sloppy(gpu, sloppy):
+    """Doublestored FP64 links with forward-only sloppy links of the same
+    operator: the mixed solve reaches the FP64 true residual 1e-10 with the
+    oracle's operator."""
+    dims = (8, 8, 8, 8)
+    mass = 0.05
+    fl, ll, _ = _fb_case(dims, 17)
+    links = O.stag_doublestore(dims, fl, ll)
+    b = O.gauss_vector(dims, 18, 100, parity=0)
+    lat = gpu.Lattice(dims)
+    L64 = gpu.Links(lat, gpu.STAG, 0, 16).set_lex(links)
+    Ls = _fb_links(gpu, lat, fl, ll, sloppy)
+    bf = gpu.Field(lat, 0, 3, 1, 0).set_lex(b)
+    x = gpu.Field(lat, 0, 3, 1, 0)
+    info = gpu.stag_cg_mixed(L64, Ls, x, bf, mass, 1e-20, 20000)
+    assert info["converged"], info
+    assert _true_rel_resid(dims, links, mass, b, x.get_lex()) <= 1e-20 * 1.01
