@@ -659,6 +659,10 @@ def main():
                             "restarts": info["restarts"], "escalated_to_fp32": info["escalations"],
                             "true_rel_resid": true_rsq() ** 0.5, "solves_per_s": round(1 / dt, 4),
                             "speedup_vs_fp64": round(solver["fp64_cg"]["seconds"] / dt, 3)}
+            if info["escalations"]:
+                # the 16-bit links' operator error was not small against m^2:
+                # the inner solve ran on FP32 links, so this is not a 16-bit result
+                solver[name]["inner_links"] = "fp32 (16-bit links too coarse for this mass)"
             print("%s: %s" % (name, solver[name]), file=sys.stderr, flush=True)
         del LQfb
 
