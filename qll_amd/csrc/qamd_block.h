@@ -31,6 +31,21 @@ template <int P, int PO> struct MArgs {
   const uint8_t *lsign;   // set: long terms' third rows rebuilt (determinant sign bits per site)
 };
 
+// Slots per step of the term loop: each step loads the links of SPS
+// consecutive slots (in group_at order), then runs the rhs loop over them, so
+// the gathers of one rhs from those slots' neighbours -- the 1- and 3-hop
+// neighbours along one direction, which share cache lines -- are issued back
+// to back instead of one slot (all rhs) apart.  With eight rhs a wavefront's
+// slots are several microseconds apart and the shared lines are gone from L2
+// by the next slot (32^4, 8 rhs, one MI355X: FP64 0.58 -> 0.43 ms with 2
+// slots per step, FP32 0.27 -> 0.20 ms with 4, Q16 0.197 -> 0.180 ms with one
+// 4-slot group).  Every rhs still adds its terms in slot order, so the sums
+// are the same bits for any step size.
+template <int P, int NR> struct BlockStep {
+  static constexpr int G = LinkG<typename Prec<P>::S>::G;
+  static constexpr int SPS = P == QAMD_F64 ? (NR >= 2 ? 2 : 1) : P == QAMD_F32 ? (NR >= 2 && NR <= 8 ? 4 : 2) : G;
+};
+
 // one output site cb: for the slots q < A.nr, rhs r = A.rmap[q]:
 // out_r = ar[q]*w_r + b * sum_t L_t in_r(x+d_t) (fields rhs-outer); with
 // RED, red[2q] += |out_r|^2, red[2q+1] += |w_r|^2
@@ -44,9 +59,11 @@ __device__ __forceinline__ void block_site(const MArgs<P, PO> &A, int cb, double
   const int nr = A.nr;
   Site s = site_of(A.g, cb, A.par);
   constexpr int G = LinkG<S>::G;
+  constexpr int SPS = BlockStep<P, NR>::SPS;
+  constexpr int GPS = SPS / G;   // slot groups per step
+  static_assert(SPS % G == 0 && (NT / G) % GPS == 0, "a step is whole slot groups");
   const S *lp = link_base<S, NT>(A.links, cb);
   C acc[NR][3];
-  S q[9][G];
 #pragma unroll
   for (int r = 0; r < NR; r++)
 #pragma unroll
@@ -56,38 +73,48 @@ __device__ __forceinline__ void block_site(const MArgs<P, PO> &A, int cb, double
   const bool rec = NT == 16 && A.lsign != nullptr;
   const unsigned lsb = rec ? A.lsign[cb] : 0u;
 #pragma unroll
-  for (int so = 0; so < NT / G; so++) {
-    // one 16-byte group load per element serves the G slots of group sg
-    const int sg = group_at<G, NT>(so);
-    const bool lng = NT == 16 && sg * G >= 8;
+  for (int st = 0; st < NT / G / GPS; st++) {
+    C m[SPS][9];
+    int nb[SPS];
+    R sc[SPS];
 #pragma unroll
-    for (int e = 0; e < 9; e++)
-      if (!(lng && rec && e % 3 == 2)) ntload_group(lp + link_off<S, NT>(sg * G, e), q[e]);
-#pragma unroll
-    for (int gg = 0; gg < G; gg++) {
-      const int sl = sg * G + gg;
-      const int t = slot_term(sl, NT);
-      const int k = (NT == 16) ? t / 2 : t;
-      const int hop = (NT == 16 && (t & 1)) ? 3 : 1;
-      const int mu = k / 2;
-      const int h = (k & 1) ? -hop : hop;
-      int nb = nbr_cb(A.g, s, mu, h);
-      C m[9];
+    for (int gi = 0; gi < GPS; gi++) {
+      // one 16-byte group load per element serves the G slots of group sg
+      const int sg = group_at<G, NT>(st * GPS + gi);
+      const bool lng = NT == 16 && sg * G >= 8;
+      S q[9][G];
 #pragma unroll
       for (int e = 0; e < 9; e++)
-        if (!(lng && rec && e % 3 == 2)) m[e] = to_c(q[e][gg], (R)0);
-      if (lng && rec) long_row3<C, R>(m, (lsb >> (sl - 8)) & 1);
+        if (!(lng && rec && e % 3 == 2)) ntload_group(lp + link_off<S, NT>(sg * G, e), q[e]);
 #pragma unroll
-      for (int r = 0; r < NR; r++) {
-        if (r < nr) {
-          const int64_t ri = A.rmap[r];
+      for (int gg = 0; gg < G; gg++) {
+        const int j = gi * G + gg;
+        const int sl = sg * G + gg;
+        const int t = slot_term(sl, NT);
+        const int k = (NT == 16) ? t / 2 : t;
+        const int hop = (NT == 16 && (t & 1)) ? 3 : 1;
+        const int mu = k / 2;
+        const int h = (k & 1) ? -hop : hop;
+        nb[j] = nbr_cb(A.g, s, mu, h);
+        sc[j] = (P == QAMD_F16 || P == QAMD_Q16) ? A.lscale[t] : (R)1;
+#pragma unroll
+        for (int e = 0; e < 9; e++)
+          if (!(lng && rec && e % 3 == 2)) m[j][e] = to_c(q[e][gg], (R)0);
+        if (lng && rec) long_row3<C, R>(m[j], (lsb >> (sl - 8)) & 1);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+      if (r < nr) {
+        const int64_t ri = A.rmap[r];
+#pragma unroll
+        for (int j = 0; j < SPS; j++) {
           C v[3];
-          VecLd<P>::ld(A.in + ri * 3 * vp, A.in_norm ? A.in_norm + ri * vp : nullptr, vp, nb, v,
-                       (P == QAMD_F16 || P == QAMD_Q16) ? A.lscale[t] : (R)1);
+          VecLd<P>::ld(A.in + ri * 3 * vp, A.in_norm ? A.in_norm + ri * vp : nullptr, vp, nb[j], v, sc[j]);
 #pragma unroll
           for (int row = 0; row < 3; row++)
 #pragma unroll
-            for (int c = 0; c < 3; c++) cmac<C, R>(acc[r][row], m[3 * c + row], v[c]);
+            for (int c = 0; c < 3; c++) cmac<C, R>(acc[r][row], m[j][3 * c + row], v[c]);
         }
       }
     }
