@@ -29,8 +29,8 @@ FLOPS_ASQTAD = 1146          # per EO output site
 FLOPS_ASQTAD_QLL = 1152      # QLL's own count (72 per term), reported beside it
 FLOPS_WILSON, FLOPS_WILSON_QLL = 1320, 1344
 ORACLE_CFLAGS = "gcc -O3 -march=native -fopenmp -std=gnu11 (oracle.native_lib, built on this host)"
-# threads of the CPU baseline: the CPUs this process may run on, at most the
-# 16 host cores a one-GPU box shares out per GPU
+# threads of the CPU baseline: the CPUs this process may run on, at most 16
+# (the host cores one GPU of a shared node is given)
 CPU_THREADS_CAP = 16
 # algorithmic bytes per EO output site: 16 link matrices + one input and one
 # output colour vector per rhs (16-bit vectors carry a 4-byte site norm)
