@@ -90,7 +90,36 @@ __device__ __forceinline__ void cg_block_scalars(const Args &A, float &al_out, f
     const int ph = A.cg_ph;
     const double g = w[0], d = A.a * g + w[1];
     double be = 0.0, al = g / d;
-    if (!S->restart[ph]) {
+    int keep_next = 0;
+    double rho_next = 0.0, pap_next = 0.0;
+    if (S->keep[ph] == 1) {
+      // After a reliable update that keeps p: the true residual r is not
+      // orthogonal to p_old, so the single-reduction identities do not hold
+      // for this step and the next.  Here beta makes p = r + beta p_old
+      // A-conjugate to p_old, p.Ap is formed from the measured products and
+      // alpha is the exact line search (r_next orthogonal to p).
+      const double kr = S->kr[ph], kp = S->kp[ph], kq = S->kq[ph];
+      if (kp > 0.0) {
+        const double b1 = -kr / kp, pap = d + b1 * kr;   // d + 2 b kr + b^2 kp with b = -kr/kp
+        if (pap > 0.0) {
+          be = b1;
+          al = (g + b1 * kq) / pap;
+          keep_next = 2;
+          rho_next = g - al * (d + b1 * kr);   // r_next . r
+          pap_next = pap;
+        }
+      }
+      if (!keep_next) al = g / d;   // non-positive measured p.Ap: restart
+    } else if (S->keep[ph] == 2) {
+      // the step after: beta conjugate to the previous p (r.r_prev = rho is
+      // not zero), alpha = gamma / p.Ap; from the next step on r is
+      // orthogonal to the previous r and p, and the usual recurrences resume
+      const double pp = S->pap[ph], b1 = (g - S->rho[ph]) / (S->al[ph] * pp), pap = d - b1 * b1 * pp;
+      if (pap > 0.0) {
+        be = b1;
+        al = g / pap;
+      }
+    } else if (!S->restart[ph]) {
       be = g / S->gam[ph];
       al = g / (d - be * g / S->al[ph]);
     }
@@ -101,6 +130,9 @@ __device__ __forceinline__ void cg_block_scalars(const Args &A, float &al_out, f
       S->gam[1 - ph] = g;
       S->al[1 - ph] = al;
       S->restart[1 - ph] = 0;
+      S->keep[1 - ph] = keep_next;
+      S->rho[1 - ph] = rho_next;
+      S->pap[1 - ph] = pap_next;
       S->rsq = g;
       S->tt = w[1];
       S->alpha = al;

@@ -41,6 +41,12 @@ int stag_block_launch_m(const qamd_links *l, Field &out, const Field &in, const 
 struct CGScal {
   double gam[2], al[2];   // previous |r|^2 and alpha
   int restart[2];         // 1: this iteration takes beta = 0 (p = r, s = A r)
+  // keep[ph] 1: the first iteration after a reliable update that keeps p,
+  // with kr = r.s_old, kp = p_old.s_old, kq = r.p_old measured (s_old =
+  // A p_old); 2: the iteration after it, with rho = r.r_prev and pap =
+  // p_prev.A p_prev carried over (see cg_block_scalars)
+  int keep[2];
+  double kr[2], kp[2], kq[2], rho[2], pap[2];
   double rsq, tt, alpha, beta, maxrr;
   int its;
 };

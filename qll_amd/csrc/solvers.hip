@@ -175,6 +175,37 @@ struct FinTrue {
   }
 };
 
+// r.s, p.s and r.p (real parts) of FP32 fields, for a reliable update that
+// keeps p
+__global__ void __launch_bounds__(256) k_cg_keep_dots(Field r, Field p, Field s, RedScratch rs) {
+  double red[3] = {0.0, 0.0, 0.0};
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < r.g->vh) {
+    float2 rv[3], pv[3], sv[3];
+    VT<QAMD_F32>::ld(r, i, rv);
+    VT<QAMD_F32>::ld(p, i, pv);
+    VT<QAMD_F32>::ld(s, i, sv);
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      red[0] += (double)rv[j].x * sv[j].x + (double)rv[j].y * sv[j].y;
+      red[1] += (double)pv[j].x * sv[j].x + (double)pv[j].y * sv[j].y;
+      red[2] += (double)rv[j].x * pv[j].x + (double)rv[j].y * pv[j].y;
+    }
+  }
+  grid_partial<3>(red, rs);
+}
+struct FinKeep {
+  SolverDev *S;
+  int ph;
+  __device__ void operator()(const double *t) const {
+    S->cg.kr[ph] = t[0];
+    S->cg.kp[ph] = t[1];
+    S->cg.kq[ph] = t[2];
+    S->cg.keep[ph] = 1;
+    S->cg.maxrr = sqrt(S->rsq);
+  }
+};
+
 // reliable update, part 2: rhi = b - Ax; rs = rhi; |rhi|^2
 template <int PS>
 __global__ void k_ru_resid(Field rhi, Field b, Field Ax, Field rs, SolverDev *S, RedScratch red_s) {
@@ -651,6 +682,15 @@ static int cg_mixed_core(const qamd_links *l64, const qamd_links *ls, qamd_field
       Sh->cg.maxrr = sqrt(Sh->rsq);
       return push_state(st);
     };
+    // the next iteration keeps p: r.s and p.s measured on the device
+    auto keep = [&](void) -> int {
+      hipLaunchKernelGGL(k_cg_keep_dots, grid1(lat), dim3(256), 0, stream(), r->f, p->f, sv->f, red_set(1));
+      launch_finish<3>(red_set(1), nb1, FinKeep{st.S, (int)(kit & 1)}, stream());
+      QAMD_HIP(hipGetLastError());
+      return 0;
+    };
+    // QAMD_CG_KEEP=0: restart at every reliable update instead (measurements)
+    const bool keep_p = !(getenv("QAMD_CG_KEEP") && atoi(getenv("QAMD_CG_KEEP")) == 0);
     if (reliable()) break;
     r2 = Sh->rsq;
     if (r2 < stop) {
@@ -729,20 +769,28 @@ static int cg_mixed_core(const qamd_links *l64, const qamd_links *ls, qamd_field
           nesc++;
           if (verbose) fprintf(stderr, "cg_mixed: FP16 links stalled; continuing with FP32 links\n");
         }
-        // Restart (p = r) at every reliable update.  The single-reduction
-        // recurrences infer p.Ap from the previous iteration's scalars, which
-        // assumes r orthogonal to the old p and a Hermitian sloppy operator;
-        // the true residual is not orthogonal to it (it differs from the
-        // iterated one by the sloppy operator's drift), and with 16-bit links
-        // keeping p after an update made the next cycle diverge (measured at
-        // 32^4, QLL mass 0.02: the residual grew from 9e-2 to 3e-1), also
-        // with p.s measured exactly at the update.  A restart costs ~2% more
-        // iterations; the iteration itself is 3 launches instead of 5.
-        if (restart()) {
-          err = 1;
-          break;
+        // Keep p across the reliable update.  The single-reduction
+        // recurrences infer p.Ap from the previous step's scalars, which
+        // assumes r orthogonal to the old p; the true residual is not (it
+        // differs from the iterated one by the sloppy operator's drift), so
+        // the next step measures r.s, p.s and r.p (k_cg_keep_dots) and takes
+        // a conjugate beta and an exact line search, and the step after it
+        // one more corrected step (cg_block_scalars).  Against restarting
+        // (p = r) at every update, 32^4, QLL mass 0.02: Q16 links 1936 ->
+        // 1872 iterations, FP32 links 1904 -> 1824.  Before new sloppy links
+        // (escalation) the search restarts.
+        if (keep_p && !stalled) {
+          if (keep()) {
+            err = 1;
+            break;
+          }
+        } else {
+          if (restart()) {
+            err = 1;
+            break;
+          }
+          nrestart++;
         }
-        nrestart++;
       }
     }
     if (err) break;
