@@ -181,6 +181,35 @@ def test_mixed_cg_true_residual(gpu, sloppy):
     assert rel_err(xs, xr, parity_mask(dims, 0)) < 1e-7
 
 
+@pytest.mark.parametrize("sloppy", [1, 3])
+def test_mixed_cg_keep_and_restart(gpu, sloppy, monkeypatch):
+    """Reliable updates that keep the search direction (measured r.s, p.s,
+    r.p; the default) and ones that restart it (QAMD_CG_KEEP=0) both reach
+    the FP64 true residual and the oracle's solution; keeping takes no more
+    iterations and restarts only on a stall."""
+    dims = (8, 8, 8, 8)
+    mass = 0.05
+    links, b = _solver_case(dims, 12)
+    lat = gpu.Lattice(dims)
+    L64 = gpu.Links(lat, gpu.STAG, 0, 16).set_lex(links)
+    Ls = L64.converted(sloppy)
+    bf = gpu.Field(lat, 0, 3, 1, 0).set_lex(b)
+    xr, _, _ = O.cg(dims, links, mass, b, 1e-24, 20000, noscale=True)
+    res = {}
+    for keep in ("1", "0"):
+        monkeypatch.setenv("QAMD_CG_KEEP", keep)
+        x = gpu.Field(lat, 0, 3, 1, 0)
+        info = gpu.stag_cg_mixed(L64, Ls, x, bf, mass, 1e-20, 20000)
+        assert info["converged"] and info["reliable_updates"] >= 2, (keep, info)
+        xs = x.get_lex()
+        assert _true_rel_resid(dims, links, mass, b, xs) <= 1e-20 * 1.01, keep
+        assert rel_err(xs, xr, parity_mask(dims, 0)) < 1e-7, keep
+        res[keep] = info
+    monkeypatch.delenv("QAMD_CG_KEEP")
+    assert res["1"]["restarts"] == 0 and res["0"]["restarts"] == res["0"]["reliable_updates"] - 1
+    assert res["1"]["iterations"] <= res["0"]["iterations"]
+
+
 @pytest.mark.parametrize("nrhs", [1, 3, 8])
 def test_fp64_block_cg_matches_oracle(gpu, nrhs):
     """Plain FP64 block CG: every rhs reaches the oracle CG's solution and
