@@ -229,8 +229,15 @@ __device__ __forceinline__ void fat_accum(double2 *base, const Geom &g, int f, i
 struct FatSite {
   int mu, par, cb, k;
 };
-// nk: blocks per 64-site run (k = 0..nk-1, consecutive, so they share a die)
-__device__ __forceinline__ FatSite fat_site(int nk = 1) {
+// nk: blocks per 64-site run (k = 0..nk-1, consecutive, so they share a die).
+// so.P > 0: the runs in slab-sweep order (die x walks its piece of every
+// t-slice, slab_block over 64-site runs), else XCD-contiguous ranges
+__device__ __forceinline__ FatSite fat_site(const SlabOrder &so, int nk = 1) {
+  if (so.P > 0) {
+    const int x = blockIdx.x % NXCD, j = blockIdx.x / NXCD, sb = j / nk;
+    const int run = slab_block((sb >> 1) * NXCD + x, so);
+    return FatSite{(int)(threadIdx.x >> 6), sb & 1, run * 64 + (int)(threadIdx.x & 63), j - sb * nk};
+  }
   const int b = xcd_block(blockIdx.x, gridDim.x), sb = b / nk;
   return FatSite{(int)(threadIdx.x >> 6), sb & 1, (sb >> 1) * 64 + (int)(threadIdx.x & 63), b - sb * nk};
 }
@@ -239,8 +246,8 @@ static dim3 fat_grid(const Geom &g, int nk = 1) { return dim3((unsigned)(2 * nk 
 // also the long links (ll != NULL): naik ul_mu(x) ul_mu(x+mu) ul_mu(x+2mu)
 // (lib/gauge.c:369-392), whose loads share the block's U traffic when ul = u
 __global__ void __launch_bounds__(256) k_fat7_p1(double2 *fl, double2 *s3, double2 *ll, const double2 *u,
-                                                 const double2 *ul, Geom g, Fat7Coef C, int keep) {
-  const FatSite f = fat_site();
+                                                 const double2 *ul, Geom g, Fat7Coef C, int keep, SlabOrder so) {
+  const FatSite f = fat_site(so);
   if (f.cb >= g.vh) return;
   const SC c = sc_of(site_of(g, f.cb, f.par));
   GView U{u, g};
@@ -273,8 +280,9 @@ __global__ void __launch_bounds__(256) k_fat7_p1(double2 *fl, double2 *s3, doubl
 }
 
 // three blocks per site run: T5 for sig = the k-th direction other than mu
-__global__ void __launch_bounds__(256) k_fat7_p2(double2 *t5, const double2 *s3, const double2 *u, Geom g) {
-  const FatSite f = fat_site(3);
+__global__ void __launch_bounds__(256) k_fat7_p2(double2 *t5, const double2 *s3, const double2 *u, Geom g,
+                                                 SlabOrder so) {
+  const FatSite f = fat_site(so, 3);
   if (f.cb >= g.vh) return;
   const int mu = f.mu, k = f.k, sig = fat_dir(mu, k);
   const SC c = sc_of(site_of(g, f.cb, f.par));
@@ -293,8 +301,8 @@ __global__ void __launch_bounds__(256) k_fat7_p2(double2 *t5, const double2 *s3,
 
 // fat_mu += cL sum_nu St_nu[S3_nu] + c5 sum_sig T5_sig + c7 sum_sig St_sig[T5_sig]
 __global__ void __launch_bounds__(256) k_fat7_p3(double2 *fl, const double2 *s3, const double2 *t5, const double2 *u,
-                                                 Geom g, Fat7Coef C) {
-  const FatSite f = fat_site();
+                                                 Geom g, Fat7Coef C, SlabOrder so) {
+  const FatSite f = fat_site(so);
   if (f.cb >= g.vh) return;
   const int mu = f.mu;
   const SC c = sc_of(site_of(g, f.cb, f.par));
@@ -613,10 +621,17 @@ int qamd_fat7(qamd_gauge *fl, qamd_gauge *ll, const double coef[6], const qamd_g
     if (!tmp) return -1;
     double2 *s3 = tmp, *t5 = tmp + (have5 ? 12 : 0) * nf;
     const dim3 grid = fat_grid(g);
+    // runs of 64 sites in the slab-sweep order where the lattice splits that
+    // way (32^4: 9 % less fetched, 5 % faster than XCD-contiguous ranges,
+    // which QAMD_FAT7_ORDER=0 selects); same sums either way
+    SlabOrder so;
+    memset(&so, 0, sizeof so);
+    const char *fo = getenv("QAMD_FAT7_ORDER");
+    if ((fo && atoi(fo) == 0) || !slab_order(g, so, 64)) memset(&so, 0, sizeof so);
     hipLaunchKernelGGL(k_fat7_p1, grid, dim3(256), 0, stream(), fl->d, s3, ll ? ll->d : nullptr, u->d,
-                       ulong ? ulong->d : u->d, g, C, have5 ? 1 : 0);
-    if (have57) hipLaunchKernelGGL(k_fat7_p2, fat_grid(g, 3), dim3(256), 0, stream(), t5, s3, u->d, g);
-    if (have5) hipLaunchKernelGGL(k_fat7_p3, grid, dim3(256), 0, stream(), fl->d, s3, t5, u->d, g, C);
+                       ulong ? ulong->d : u->d, g, C, have5 ? 1 : 0, so);
+    if (have57) hipLaunchKernelGGL(k_fat7_p2, fat_grid(g, 3), dim3(256), 0, stream(), t5, s3, u->d, g, so);
+    if (have5) hipLaunchKernelGGL(k_fat7_p3, grid, dim3(256), 0, stream(), fl->d, s3, t5, u->d, g, C, so);
     hipError_t e = hipGetLastError();
     (void)hipStreamSynchronize(stream());
     dev_free(tmp);

@@ -332,8 +332,8 @@ __device__ __forceinline__ int slab_block(int b, const SlabOrder &s) {
 }
 // the slab order for g, or false when the lattice does not split that way
 // (planes not a whole number of blocks, or no split of the 8 dies into y and
-// z pieces that divides both)
-bool slab_order(const Geom &g, SlabOrder &s);
+// z pieces that divides both); blocks of run sites of one parity
+bool slab_order(const Geom &g, SlabOrder &s, int run = 256);
 
 // --------------------------------------------------------------- precisions
 // Complex loads/stores for the three storage precisions.  Accumulation is in

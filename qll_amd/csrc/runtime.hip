@@ -70,10 +70,10 @@ hipGraphExec_t capture_stream(int (*enqueue)(void *), void *ctx) {
 }
 bool initialized() { return g_init != 0; }
 
-bool slab_order(const Geom &g, SlabOrder &s) {
+bool slab_order(const Geom &g, SlabOrder &s, int run) {
   const int64_t plane = (int64_t)g.L[0] * g.L[1] / 2;   // sites of one parity per (z, t)
-  if (plane % 256 != 0 || g.vh % 256 != 0) return false;
-  const int P = (int)(plane / 256);
+  if (plane % run != 0 || g.vh % run != 0) return false;
+  const int P = (int)(plane / run);
   // pieces as thick as possible in both y and z (the gathers reach 3 sites)
   int best = -1;
   for (int py = 1; py <= NXCD; py *= 2) {
