@@ -326,18 +326,20 @@ __device__ __forceinline__ void stag_dslash_body(const StagArgs<P, PO, PI> &A) {
     static_assert(!REC || NT == 16 || FBT, "long-row rebuild: 16-term or forward-only links");
     if constexpr (FBT) stag_acc_fb<P, PO, PI, FBT, REC>(A, s, cb, vp, acc);
     [[maybe_unused]] const S *lp = link_base<S, NT>(A.links, cb);
+    [[maybe_unused]] const S *lp3 = link_base3<S, NT>(lp, vp);   // the 3-hop terms' region
     [[maybe_unused]] unsigned lsb = 0;
     if constexpr (REC && !FBT) lsb = A.lsign[cb];
 #pragma unroll
     for (int so = 0; so < (FBT ? 0 : NT / G); so++) {
       // slots sg*G .. sg*G+G-1: all 1-hop terms (slot < 8) or all 3-hop ones
       const int sg = group_at<G, NT>(so);
-      const bool lng = REC && sg * G >= 8;
+      const bool hop3 = NT == 16 && sg * G >= 8;
+      const bool lng = REC && hop3;
       S q[9][G];   // element e of the group's G terms, one 16-byte load each
 #pragma unroll
       for (int e = 0; e < 9; e++) {
         if (lng && e % 3 == 2) continue;   // third row of a long term: rebuilt below
-        ntload_group(lp + link_off<S, NT>(sg * G, e), q[e]);
+        ntload_group((hop3 ? lp3 : lp) + link_off<S, NT>(sg * G, e), q[e]);
       }
 #pragma unroll
       for (int gg = 0; gg < G; gg++) {

@@ -77,7 +77,7 @@ __global__ void k_link_import(S *d0, S *d1, const H *src, Geom g, int t, int nte
   }
   S *d = par ? d1 : d0;
   for (int e = 0; e < 9; e++)
-    d[link_idx<S>(cb, t, e, nterm, fb)] =
+    d[link_idx<S>(cb, t, e, nterm, fb, g.vh_pad)] =
         mk<S>(make_double2((double)src[base + 2 * e * stride], (double)src[base + (2 * e + 1) * stride]));
 }
 
@@ -94,8 +94,8 @@ __global__ void k_fb_from_gauge(S *d0, S *d1, const double2 *fat, const double2 
   for (int mu = 0; mu < 4; mu++) {
     const int64_t o = ((int64_t)(mu * 2 + par) * 9) * vp + cb;
     for (int e = 0; e < 9; e++) {
-      d[link_idx<S>(cb, mu, e, 8, true)] = mk<S>(fat[o + e * vp]);
-      d[link_idx<S>(cb, 4 + mu, e, 8, true)] = mk<S>(lng ? lng[o + e * vp] : make_double2(0.0, 0.0));
+      d[link_idx<S>(cb, mu, e, 8, true, vp)] = mk<S>(fat[o + e * vp]);
+      d[link_idx<S>(cb, 4 + mu, e, 8, true, vp)] = mk<S>(lng ? lng[o + e * vp] : make_double2(0.0, 0.0));
     }
   }
 }
@@ -117,7 +117,7 @@ __global__ void k_link_from_gauge(S *d0, S *d1, const double2 *fat, const double
       int t = (nterm == 16) ? 2 * k + n : k;
       if (!bwd) {
         const double2 *p = src + ((int64_t)(mu * 2 + par) * 9) * vp + cb;
-        for (int e = 0; e < 9; e++) d[link_idx<S>(cb, t, e, nterm)] = mk<S>(p[e * vp]);
+        for (int e = 0; e < 9; e++) d[link_idx<S>(cb, t, e, nterm, false, vp)] = mk<S>(p[e * vp]);
       } else {
         int hop = n ? 3 : 1;
         int nb = nbr_cb(g, s, mu, -hop);
@@ -125,7 +125,7 @@ __global__ void k_link_from_gauge(S *d0, S *d1, const double2 *fat, const double
         for (int r = 0; r < 3; r++)
           for (int c = 0; c < 3; c++) {
             double2 v = p[(3 * r + c) * vp];   // element (c, r) of the source
-            d[link_idx<S>(cb, t, 3 * c + r, nterm)] = mk<S>(make_double2(sgn * v.x, -sgn * v.y));
+            d[link_idx<S>(cb, t, 3 * c + r, nterm, false, vp)] = mk<S>(make_double2(sgn * v.x, -sgn * v.y));
           }
       }
     }
@@ -158,7 +158,7 @@ __global__ void k_link_absmax(const S *src, int64_t nblk, int nterm, bool fb, un
   for (int64_t b = wave; b < nblk; b += nwaves)
     for (int t = 0; t < nterm; t++)
       for (int e = 0; e < 9; e++) {
-        double2 v = ld2<S>(src, link_idx<S>(b * 64 + lane, t, e, nterm, fb));
+        double2 v = ld2<S>(src, link_idx<S>(b * 64 + lane, t, e, nterm, fb, nblk * 64));
         m[t] = fmaxf(m[t], (float)fmax(fabs(v.x), fabs(v.y)));
       }
   for (int t = 0; t < nterm; t++) {
@@ -179,8 +179,9 @@ __global__ void k_link_convert(SO *dst, const SI *src, int64_t n, int nterm, boo
     q /= 9;
     const int t = (int)(q % nterm);
     const int64_t cb = (q / nterm) * 64 + lane;
-    const int64_t o = link_idx<SO>(cb, t, k, nterm, fb);
-    double2 v = ld2<SI>(src, link_idx<SI>(cb, t, k, nterm, fb));
+    const int64_t vp = n / ((int64_t)nterm * 9);   // n covers every element of one parity's set
+    const int64_t o = link_idx<SO>(cb, t, k, nterm, fb, vp);
+    double2 v = ld2<SI>(src, link_idx<SI>(cb, t, k, nterm, fb, vp));
     double f = (double)iscale[t] / (double)oscale[t];
     v.x *= f;
     v.y *= f;
@@ -203,7 +204,7 @@ __global__ void k_long_check(const double2 *d, uint8_t *sign, Geom g, int nterm,
   unsigned bits = 0, fail = 0;
   for (int k = 0; k < nk; k++) {
     double2 m[9];
-    for (int e = 0; e < 9; e++) m[e] = d[link_idx<double2>(cb, t0 + dt * k, e, nterm, fb)];
+    for (int e = 0; e < 9; e++) m[e] = d[link_idx<double2>(cb, t0 + dt * k, e, nterm, fb, g.vh_pad)];
     // row r holds elements 3c + r
     double c2 = 0, n2 = 0;
     double2 dot = make_double2(0, 0);

@@ -63,6 +63,7 @@ __device__ __forceinline__ void block_site(const MArgs<P, PO> &A, int cb, double
   constexpr int GPS = SPS / G;   // slot groups per step
   static_assert(SPS % G == 0 && (NT / G) % GPS == 0, "a step is whole slot groups");
   const S *lp = link_base<S, NT>(A.links, cb);
+  const S *lp3 = link_base3<S, NT>(lp, vp);   // the 3-hop terms' region
   C acc[NR][3];
 #pragma unroll
   for (int r = 0; r < NR; r++)
@@ -85,7 +86,7 @@ __device__ __forceinline__ void block_site(const MArgs<P, PO> &A, int cb, double
       S q[9][G];
 #pragma unroll
       for (int e = 0; e < 9; e++)
-        if (!(lng && rec && e % 3 == 2)) ntload_group(lp + link_off<S, NT>(sg * G, e), q[e]);
+        if (!(lng && rec && e % 3 == 2)) ntload_group((lng ? lp3 : lp) + link_off<S, NT>(sg * G, e), q[e]);
 #pragma unroll
       for (int gg = 0; gg < G; gg++) {
         const int j = gi * G + gg;

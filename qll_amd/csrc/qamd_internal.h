@@ -241,13 +241,30 @@ __host__ __device__ constexpr int fb_pos(int sl, int k) {
   return (k % 3 == 2) ? 2 + k / 3 : (k >= 6 ? k - 6 : k % 3 + 2 * (k / 3));
 }
 
-// blocked link index: element k (0..8) of term t for site cb (fb: the
-// forward-only layout, nterm = 8)
+// 16-term staggered sets of FP32 and 16-bit links keep their 1-hop terms
+// (slots 0-7) and their 3-hop terms (slots 8-15) in two regions of the
+// array, each in the block layout above with (8/G)*9 rows per 64-site block:
+// region 1 starts 72*vh_pad complex elements in (8 slots x 9 elements).  A
+// Dslash wavefront then reads two dense streams instead of one interleaved
+// one; on MI355X at 32^4 that measured 1-3 % faster (FP32 single-rhs Dslash
+// 0.094 -> 0.093 ms, FP32 8-rhs block 0.180 -> 0.176 ms, Q16 8-rhs block
+// 0.156 -> 0.152 ms).  FP64 sets measured no gain and keep one region.
 template <class S>
-__host__ __device__ __forceinline__ int64_t link_idx(int64_t cb, int t, int k, int nterm, bool fb = false) {
+__host__ __device__ constexpr bool link_split(int nterm) { return nterm == 16 && LinkG<S>::G >= 2; }
+
+// blocked link index: element k (0..8) of term t for site cb (fb: the
+// forward-only layout, nterm = 8); vp = sites per parity padded to 64
+template <class S>
+__host__ __device__ __forceinline__ int64_t link_idx(int64_t cb, int t, int k, int nterm, bool fb, int64_t vp) {
   constexpr int G = LinkG<S>::G;
   if (fb) return ((((cb >> 6) * (8 / G) * 9 + fb_row<G>(t, k)) << 6) + (cb & 63)) * G + fb_pos<G>(t, k);
   const int sl = term_slot(t, nterm);
+  if (link_split<S>(nterm)) {
+    constexpr int RB = (8 / G) * 9;   // rows per 64-site block in each region
+    const int r = link_row<G>(sl, k, 16);
+    const bool hop3 = sl >= 8;
+    return (hop3 ? (int64_t)72 * vp : 0) + ((((cb >> 6) * RB + (hop3 ? r - RB : r)) << 6) + (cb & 63)) * G + sl % G;
+  }
   return ((((cb >> 6) * (nterm / G) * 9 + link_row<G>(sl, k, nterm)) << 6) + (cb & 63)) * G + sl % G;
 }
 // offset of (slot, k) from link_base<S, 8> in the forward-only layout
@@ -256,15 +273,23 @@ __device__ __forceinline__ int link_off_fb(int sl, int k) {
   constexpr int G = LinkG<S>::G;
   return fb_row<G>(sl, k) * 64 * G + fb_pos<G>(sl, k);
 }
-// first element of site cb's link stream, and the offset of (slot, k) from it
+// first element of site cb's link stream (of region 0 for split sets), the
+// first of its 3-hop region (the same pointer for sets in one region), and
+// the offset of (slot, k) from the base of the slot's region
 template <class S, int NT>
 __device__ __forceinline__ const S *link_base(const S *links, int cb) {
-  return links + (int64_t)(cb >> 6) * NT * 9 * 64 + (int64_t)(cb & 63) * LinkG<S>::G;
+  constexpr int G = LinkG<S>::G, RB = link_split<S>(NT) ? (8 / G) * 9 : NT * 9 / G;
+  return links + (int64_t)(cb >> 6) * RB * 64 * G + (int64_t)(cb & 63) * G;
+}
+template <class S, int NT>
+__device__ __forceinline__ const S *link_base3(const S *lp, int64_t vp) {
+  return link_split<S>(NT) ? lp + 72 * vp : lp;
 }
 template <class S, int NT>
 __device__ __forceinline__ int link_off(int sl, int k) {
-  constexpr int G = LinkG<S>::G;
-  return link_row<G>(sl, k, NT) * 64 * G + sl % G;
+  constexpr int G = LinkG<S>::G, RB = (8 / G) * 9;
+  const int r = link_row<G>(sl, k, NT);
+  return ((link_split<S>(NT) && sl >= 8) ? r - RB : r) * 64 * G + sl % G;
 }
 
 // Third row of a term c*W (W unitary, det W = -1 if neg) from its first
