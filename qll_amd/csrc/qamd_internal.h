@@ -346,19 +346,28 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
 // block do not change.
 struct SlabOrder {
   int P, py, bpy, zs, Lz;   // blocks per plane, y pieces, blocks per piece per plane, z per piece
+  int ny, nz, Lt;           // a die's piece swept as ny x nz sub-pieces, one t sweep each; t extent
 };
 __device__ __forceinline__ int slab_block(int b, const SlabOrder &s) {
   const int x = b % NXCD, j = b / NXCD;
   const int hy = x % s.py, zq = x / s.py;
-  const int per_t = s.zs * s.bpy;
-  const int t = j / per_t, r = j - t * per_t;
-  const int zl = r / s.bpy, pb = r - zl * s.bpy;
+  const int bs = s.bpy / s.ny, zsub = s.zs / s.nz;   // a sub-piece: bs blocks per plane, zsub planes
+  const int per_t = zsub * bs, per_sweep = s.Lt * per_t;
+  const int sub = j / per_sweep, jj = j - sub * per_sweep;
+  const int sy = sub % s.ny, sz = sub / s.ny;
+  const int t = jj / per_t, r = jj - t * per_t;
+  const int zl = sz * zsub + r / bs, pb = sy * bs + (r - (r / bs) * bs);
   return (t * s.Lz + zq * s.zs + zl) * s.P + hy * s.bpy + pb;
 }
 // the slab order for g, or false when the lattice does not split that way
 // (planes not a whole number of blocks, or no split of the 8 dies into y and
-// z pieces that divides both); blocks of run sites of one parity
+// z pieces that divides both); blocks of run sites of one parity; one sweep
+// per die (ny = nz = 1)
 bool slab_order(const Geom &g, SlabOrder &s, int run = 256);
+// sweep each die's piece as ny x nz sub-pieces (y blocks, z planes), each
+// through all t before the next: a smaller t-window per die; false (s
+// unchanged) unless both divide the piece
+bool slab_split(SlabOrder &s, int ny, int nz);
 
 // --------------------------------------------------------------- precisions
 // Complex loads/stores for the three storage precisions.  Accumulation is in

@@ -87,9 +87,18 @@ bool slab_order(const Geom &g, SlabOrder &s, int run) {
       s.bpy = P / py;
       s.zs = tz;
       s.Lz = g.L[2];
+      s.ny = s.nz = 1;
+      s.Lt = g.L[3];
     }
   }
   return best > 0;
+}
+
+bool slab_split(SlabOrder &s, int ny, int nz) {
+  if (ny < 1 || nz < 1 || s.bpy % ny != 0 || s.zs % nz != 0) return false;
+  s.ny = ny;
+  s.nz = nz;
+  return true;
 }
 
 void *dev_alloc(size_t bytes) {
