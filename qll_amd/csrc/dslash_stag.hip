@@ -303,7 +303,7 @@ __device__ __forceinline__ void stag_acc_fb(const StagArgs<P, PO, PI> &A, const 
 // REC (16 terms): the long terms' third rows are not loaded but rebuilt
 // from the first two (FP64: 2016 instead of 2400 bytes per site).
 // FBT != 0: forward-only link set (NT = 8 storage slots), see stag_acc_fb.
-template <int P, int PO, int NT, bool RED, bool REC = false, int CGX = 0, int PI = P, int FBT = 0>
+template <int P, int PO, int NT, bool RED, bool REC = false, int CGX = 0, int PI = P, int FBT = 0, int RG = 0>
 __device__ __forceinline__ void stag_dslash_body(const StagArgs<P, PO, PI> &A) {
   static_assert(PI == P || (PI == QAMD_F32 && (P == QAMD_F16 || P == QAMD_Q16)),
                 "input precision: the links' or FP32 with 16-bit links");
@@ -329,17 +329,20 @@ __device__ __forceinline__ void stag_dslash_body(const StagArgs<P, PO, PI> &A) {
     [[maybe_unused]] const S *lp3 = link_base3<S, NT>(lp, vp);   // the 3-hop terms' region
     [[maybe_unused]] unsigned lsb = 0;
     if constexpr (REC && !FBT) lsb = A.lsign[cb];
-#pragma unroll
-    for (int so = 0; so < (FBT ? 0 : NT / G); so++) {
-      // slots sg*G .. sg*G+G-1: all 1-hop terms (slot < 8) or all 3-hop ones
-      const int sg = group_at<G, NT>(so);
-      const bool hop3 = NT == 16 && sg * G >= 8;
-      const bool lng = REC && hop3;
+    // slot group so (slots sg*G .. sg*G+G-1: all 1-hop terms or all 3-hop
+    // ones; in a 16-term set the 3-hop groups are the odd positions of the
+    // group order); the first RG 3-hop groups load through the caches
+    auto group = [&](auto so_c) __attribute__((always_inline)) {
+      constexpr int so = decltype(so_c)::value;
+      constexpr int sg = group_at<G, NT>(so);
+      constexpr bool H3 = NT == 16 && sg * G >= 8;
+      constexpr bool ALLOC = H3 && so / 2 < RG;
+      const bool lng = REC && H3;
       S q[9][G];   // element e of the group's G terms, one 16-byte load each
 #pragma unroll
       for (int e = 0; e < 9; e++) {
         if (lng && e % 3 == 2) continue;   // third row of a long term: rebuilt below
-        ntload_group((hop3 ? lp3 : lp) + link_off<S, NT>(sg * G, e), q[e]);
+        link_group_load<ALLOC>((H3 ? lp3 : lp) + link_off<S, NT>(sg * G, e), q[e]);
       }
 #pragma unroll
       for (int gg = 0; gg < G; gg++) {
@@ -363,7 +366,8 @@ __device__ __forceinline__ void stag_dslash_body(const StagArgs<P, PO, PI> &A) {
 #pragma unroll
           for (int c = 0; c < 3; c++) cmac<C, R>(acc[r], m[3 * c + r], v[c]);
       }
-    }
+    };
+    if constexpr (!FBT) static_for<NT / G>(group);
   }
   // the CG step's scalars: a block-wide sum, after the link stream
   [[maybe_unused]] float cg_al = 0.f, cg_be = 0.f, cg_alp = 0.f;
@@ -433,9 +437,41 @@ __device__ __forceinline__ void stag_dslash_body(const StagArgs<P, PO, PI> &A) {
   if (RED) grid_partial<2>(red, A.rs);
 }
 
-template <int P, int PO, int NT, bool RED, bool REC = false, int CGX = 0, int PI = P, int FBT = 0>
+template <int P, int PO, int NT, bool RED, bool REC = false, int CGX = 0, int PI = P, int FBT = 0, int RG = 0>
 __global__ void __launch_bounds__(256) k_stag_dslash(StagArgs<P, PO, PI> A) {
-  stag_dslash_body<P, PO, NT, RED, REC, CGX, PI, FBT>(A);
+  stag_dslash_body<P, PO, NT, RED, REC, CGX, PI, FBT, RG>(A);
+}
+
+// MI355X's 256 MB last-level cache (MALL) keeps lines loaded through the
+// caches from one kernel launch to the next, while non-temporal loads stream
+// past it.  FP32 and 16-bit 16-term sets with rebuilt long rows load their
+// first RG 3-hop slot groups through the caches, at most MALL_LINK_MB of
+// them per set, so that those bytes come from the MALL on the next call with
+// the same links; a solver alternates the two parities' sets, and both sets'
+// shares fit together with its vectors.  FP64 sets stream every load: their
+// vectors are twice as large, and in FP64 CG a 100 MB share per set was 2 %
+// slower (one set called repeatedly: 2.5 % faster).  Measured at 32^4 (one
+// 3-hop group = 50 MB), share 0 -> 104 MB per set: FP32 Dslash 0.089 ->
+// 0.083-0.086 ms (one set repeated), 0.091 -> 0.086-0.088 ms (parities
+// alternating), mixed CG with FP32 links 0.388 -> 0.367-0.375 s; Q16 Dslash
+// 0.052 -> 0.047 ms either way, mixed CG with Q16 links 0.243 -> 0.222 s.
+// A 208 MB share per set thrashes when the parities alternate (FP32 Dslash
+// 0.094 ms).  QAMD_MALL_MB sets the share (0: every link load non-temporal).
+constexpr int MALL_LINK_MB = 104;
+template <class S>
+static int mall_groups(const Geom &g) {
+  if (LinkG<S>::G == 1) return 0;
+  static int mb = -1;
+  if (mb < 0) {
+    const char *e = getenv("QAMD_MALL_MB");
+    mb = e ? atoi(e) : MALL_LINK_MB;
+    if (mb < 0) mb = 0;
+  }
+  const int64_t per = (int64_t)6 * 16 * g.vh;   // bytes of one 3-hop group (two rows of 16-byte words)
+  int64_t n = (int64_t)mb * 1000000 / per;
+  const int ng = 8 / LinkG<S>::G;               // 3-hop groups of a 16-term set
+  if (n > ng) n = ng;
+  return n >= 4 ? 4 : n >= 2 ? 2 : (int)n;
 }
 
 template <int P, int PO, int NT, int PI, int FBT = 0>
@@ -457,10 +493,30 @@ static void launch_t(const StagArgs<P, PO, PI> &A, bool red, hipStream_t st) {
   }
   if constexpr (NT == 16) {
     if (A.lsign) {
-      if (red)
-        hipLaunchKernelGGL((k_stag_dslash<P, PO, NT, true, true, 0, PI>), dim3(nb), dim3(256), 0, st, A);
-      else
-        hipLaunchKernelGGL((k_stag_dslash<P, PO, NT, false, true, 0, PI>), dim3(nb), dim3(256), 0, st, A);
+#define QAMD_LRG(RG)                                                                                       \
+  do {                                                                                                     \
+    if (red)                                                                                               \
+      hipLaunchKernelGGL((k_stag_dslash<P, PO, NT, true, true, 0, PI, 0, RG>), dim3(nb), dim3(256), 0, st, A);  \
+    else                                                                                                   \
+      hipLaunchKernelGGL((k_stag_dslash<P, PO, NT, false, true, 0, PI, 0, RG>), dim3(nb), dim3(256), 0, st, A); \
+  } while (0)
+      constexpr int NG = 8 / LinkG<typename Prec<P>::S>::G;   // 3-hop groups
+      if constexpr (NG == 8) {   // FP64: every link load non-temporal
+        QAMD_LRG(0);
+      } else {
+        switch (mall_groups<typename Prec<P>::S>(A.g)) {
+          case 4:
+            if constexpr (NG >= 4) {
+              QAMD_LRG(4);
+              break;
+            }
+            [[fallthrough]];
+          case 2: QAMD_LRG(2); break;
+          case 1: QAMD_LRG(1); break;
+          default: QAMD_LRG(0);
+        }
+      }
+#undef QAMD_LRG
       return;
     }
   }
@@ -622,7 +678,22 @@ static int cgcg_p(const qamd_links *l, const Field &t, Field &r, Field &p, Field
     else
       hipLaunchKernelGGL((k_stag_dslash<P, QAMD_F32, 8, true, false, 2, PI, 8>), dim3(nb), dim3(256), 0, st, A);
   } else if (A.lsign) {
-    hipLaunchKernelGGL((k_stag_dslash<P, QAMD_F32, 16, true, true, 2, PI>), dim3(nb), dim3(256), 0, st, A);
+    constexpr int NG = 8 / LinkG<S>::G;   // 3-hop groups
+    switch (mall_groups<S>(A.g)) {
+      case 4:
+        if constexpr (NG >= 4) {
+          hipLaunchKernelGGL((k_stag_dslash<P, QAMD_F32, 16, true, true, 2, PI, 0, 4>), dim3(nb), dim3(256), 0, st, A);
+          break;
+        }
+        [[fallthrough]];
+      case 2:
+        hipLaunchKernelGGL((k_stag_dslash<P, QAMD_F32, 16, true, true, 2, PI, 0, 2>), dim3(nb), dim3(256), 0, st, A);
+        break;
+      case 1:
+        hipLaunchKernelGGL((k_stag_dslash<P, QAMD_F32, 16, true, true, 2, PI, 0, 1>), dim3(nb), dim3(256), 0, st, A);
+        break;
+      default: hipLaunchKernelGGL((k_stag_dslash<P, QAMD_F32, 16, true, true, 2, PI>), dim3(nb), dim3(256), 0, st, A);
+    }
   } else {
     hipLaunchKernelGGL((k_stag_dslash<P, QAMD_F32, 16, true, false, 2, PI>), dim3(nb), dim3(256), 0, st, A);
   }

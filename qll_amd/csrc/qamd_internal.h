@@ -19,6 +19,7 @@
 #include <hip/hip_fp16.h>
 #include <stdint.h>
 #include <stddef.h>
+#include <utility>
 
 #include "qll_amd.h"
 
@@ -454,6 +455,31 @@ __device__ __forceinline__ void ntload_group(const __half2 *p, __half2 (&o)[4]) 
     unsigned u = v[i];
     o[i] = *reinterpret_cast<__half2 *>(&u);
   }
+}
+// the same 16 bytes through the ordinary (cached) load path
+template <class S>
+__device__ __forceinline__ void cload_group(const S *p, S (&o)[LinkG<S>::G]) {
+  const uvec4_t v = *reinterpret_cast<const uvec4_t *>(p);
+  unpack_group(v, o);
+}
+// A link group through an allocating load (ALLOC) or a non-temporal one.
+// ALLOC must be a compile-time constant: two loads of one address that
+// differ only in the non-temporal hint are merged into one without it.
+template <bool ALLOC, class S>
+__device__ __forceinline__ void link_group_load(const S *p, S (&o)[LinkG<S>::G]) {
+  if constexpr (ALLOC) cload_group(p, o);
+  else ntload_group(p, o);
+}
+
+// f(integral_constant<int, 0>), ..., f(integral_constant<int, N - 1>): a loop
+// whose index is a compile-time constant inside the body
+template <class F, int... I>
+__device__ __forceinline__ void static_for_seq(F &&f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, class F>
+__device__ __forceinline__ void static_for(F &&f) {
+  static_for_seq(f, std::make_integer_sequence<int, N>{});
 }
 
 // ---------------------------------------------------------------- fields
