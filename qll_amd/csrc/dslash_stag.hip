@@ -22,6 +22,7 @@
 #include "qamd_kernels.h"
 #include "qamd_vec3.h"
 
+#include <atomic>
 #include <stdlib.h>
 #include <string.h>
 #include <math.h>
@@ -444,27 +445,35 @@ __global__ void __launch_bounds__(256) k_stag_dslash(StagArgs<P, PO, PI> A) {
 
 // MI355X's 256 MB last-level cache (MALL) keeps lines loaded through the
 // caches from one kernel launch to the next, while non-temporal loads stream
-// past it.  FP32 and 16-bit 16-term sets with rebuilt long rows load their
-// first RG 3-hop slot groups through the caches, at most MALL_LINK_MB of
-// them per set, so that those bytes come from the MALL on the next call with
-// the same links; a solver alternates the two parities' sets, and both sets'
-// shares fit together with its vectors.  FP64 sets stream every load: their
-// vectors are twice as large, and in FP64 CG a 100 MB share per set was 2 %
-// slower (one set called repeatedly: 2.5 % faster).  Measured at 32^4 (one
-// 3-hop group = 50 MB), share 0 -> 104 MB per set: FP32 Dslash 0.089 ->
-// 0.083-0.086 ms (one set repeated), 0.091 -> 0.086-0.088 ms (parities
-// alternating), mixed CG with FP32 links 0.388 -> 0.367-0.375 s; Q16 Dslash
-// 0.052 -> 0.047 ms either way, mixed CG with Q16 links 0.243 -> 0.222 s.
-// A 208 MB share per set thrashes when the parities alternate (FP32 Dslash
-// 0.094 ms).  QAMD_MALL_MB sets the share (0: every link load non-temporal).
-constexpr int MALL_LINK_MB = 104;
+// past it.  16-term sets with rebuilt long rows load their first RG 3-hop
+// slot groups through the caches -- at most 104 MB per FP32 or 16-bit set,
+// 52 MB per FP64 set (whose solvers' vectors are twice as large) -- so that
+// those bytes come from the MALL on the next call with the same links; a
+// solver alternates the two parities' sets, and both sets' shares fit
+// together with its vectors.  Measured at 32^4 (one 3-hop group = 50 MB;
+// one set called repeatedly / the parities alternating / the solve):
+//   FP64, share 0 -> 52 MB: Dslash 0.179 -> 0.176 ms / 0.179 -> 0.176 ms,
+//     FP64 CG 0.759 -> 0.743 s (104 MB: 0.174 / 0.181 ms, CG 0.772 s);
+//   FP32, 0 -> 104 MB: 0.089 -> 0.083-0.086 / 0.091 -> 0.086-0.088 ms,
+//     mixed CG with FP32 links 0.388 -> 0.367-0.375 s (208 MB thrashes when
+//     the parities alternate: 0.094 ms);
+//   Q16, 0 -> 104 MB: 0.052 -> 0.047 ms either way, mixed CG with Q16 links
+//     0.243 -> 0.222 s.
+// Multishift CG keeps no share (MallShareOff): its 2 vectors per shift fill
+// the MALL (12 shifts at 32^3x64, FP32 share 100 MB per set: 1.75 -> 1.79 s).
+// QAMD_MALL_MB (FP32 and 16-bit sets) and QAMD_MALL_MB64 (FP64 sets) set
+// the share per set in MB (0: every link load non-temporal).
+static std::atomic<int> g_mall_off{0};
+MallShareOff::MallShareOff() { g_mall_off++; }
+MallShareOff::~MallShareOff() { g_mall_off--; }
+
 template <class S>
 static int mall_groups(const Geom &g) {
-  if (LinkG<S>::G == 1) return 0;
+  if (g_mall_off.load() > 0) return 0;
   static int mb = -1;
   if (mb < 0) {
-    const char *e = getenv("QAMD_MALL_MB");
-    mb = e ? atoi(e) : MALL_LINK_MB;
+    const char *e = getenv(LinkG<S>::G == 1 ? "QAMD_MALL_MB64" : "QAMD_MALL_MB");
+    mb = e ? atoi(e) : (LinkG<S>::G == 1 ? 52 : 104);
     if (mb < 0) mb = 0;
   }
   const int64_t per = (int64_t)6 * 16 * g.vh;   // bytes of one 3-hop group (two rows of 16-byte words)
@@ -501,8 +510,13 @@ static void launch_t(const StagArgs<P, PO, PI> &A, bool red, hipStream_t st) {
       hipLaunchKernelGGL((k_stag_dslash<P, PO, NT, false, true, 0, PI, 0, RG>), dim3(nb), dim3(256), 0, st, A); \
   } while (0)
       constexpr int NG = 8 / LinkG<typename Prec<P>::S>::G;   // 3-hop groups
-      if constexpr (NG == 8) {   // FP64: every link load non-temporal
-        QAMD_LRG(0);
+      if constexpr (NG == 8) {   // FP64
+        switch (mall_groups<typename Prec<P>::S>(A.g)) {
+          case 4: QAMD_LRG(4); break;
+          case 2: QAMD_LRG(2); break;
+          case 1: QAMD_LRG(1); break;
+          default: QAMD_LRG(0);
+        }
       } else {
         switch (mall_groups<typename Prec<P>::S>(A.g)) {
           case 4:

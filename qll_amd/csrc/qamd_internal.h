@@ -471,6 +471,16 @@ __device__ __forceinline__ void link_group_load(const S *p, S (&o)[LinkG<S>::G])
   else ntload_group(p, o);
 }
 
+// While one of these lives (a solver whose vectors fill the 256 MB MALL, such
+// as multishift CG with two vectors per shift), the staggered Dslash loads
+// every link non-temporally instead of keeping a share of them in the MALL.
+struct MallShareOff {
+  MallShareOff();
+  ~MallShareOff();
+  MallShareOff(const MallShareOff &) = delete;
+  MallShareOff &operator=(const MallShareOff &) = delete;
+};
+
 // f(integral_constant<int, 0>), ..., f(integral_constant<int, N - 1>): a loop
 // whose index is a compile-time constant inside the body
 template <class F, int... I>

@@ -896,6 +896,7 @@ static int cgms_core(const qamd_links *l, qamd_field *const x[], const qamd_fiel
   if (check_vec(b, l->prec, 0, "stag_cgms b")) return -1;
   for (int i = 0; i < n; i++)
     if (check_vec(x[i], l->prec, 0, "stag_cgms x")) return -1;
+  MallShareOff vectors_first;   // the 2n shift vectors fill the MALL
   double t0 = now_s();
   qamd_lattice *lat = l->lat;
   State st;
