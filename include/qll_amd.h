@@ -339,6 +339,18 @@ int qamd_flat_dot(int64_t n, int prec, const void *x, const void *y, double *out
  * one if fewer) everywhere, for tests.  QAMD_VERBOSE=1 prints each choice. */
 int qamd_set_tuning(int mode);
 
+/* ---- link share kept in the MALL (no reference counterpart) ----
+ * The staggered Dslash on a 16-term link set with rebuilt long rows loads
+ * part of the set (whole 3-hop slot groups, at most the given megabytes per
+ * set) through the caches so that it stays in MI355X's 256 MB last-level
+ * cache from one call to the next, and streams the rest non-temporally.
+ * mb64 applies to FP64 sets (default 52), mb to FP32 and 16-bit sets
+ * (default 104); 0 streams every link load, a negative value restores the
+ * default (or QAMD_MALL_MB64 / QAMD_MALL_MB from the environment).  A caller
+ * whose own vectors need the MALL can set 0.  Results are the same bits for
+ * every share. */
+int qamd_set_mall_share(int mb64, int mb);
+
 /* ---- timing helpers (hipEvents on the library stream) ---- */
 int qamd_time_dslash(const qamd_links *l, qamd_field *out, const qamd_field *in,
                      int nrep, double *ms_per_call);

@@ -98,6 +98,7 @@ def lib():
             "qamd_stag_cg_mixed": (C.c_int, [_vp, _vp, _vp, _vp, C.c_double, C.c_double, C.c_int, C.c_double,
                                              C.POINTER(SolveInfo)]),
             "qamd_set_tuning": (C.c_int, [C.c_int]),
+            "qamd_set_mall_share": (C.c_int, [C.c_int, C.c_int]),
             "qamd_stag_sloppy_error": (C.c_int, [_vp, _vp, C.c_double, C.POINTER(C.c_double)]),
             "qamd_stag_cg_mixed_batch": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_double, C.c_int,
                                                    C.POINTER(SolveInfo)]),
@@ -373,6 +374,12 @@ def set_tuning(mode):
     """Launch-shape autotuning: True/1 on (default), False/0 off, k >= 2 forces
     candidate k - 2 (tests)."""
     _check(lib().qamd_set_tuning(int(mode)))
+
+
+def set_mall_share(mb64=-1, mb=-1):
+    """Megabytes of each staggered link set (FP64; FP32 and 16-bit) kept in
+    the MALL from call to call; 0 streams every link load, -1 the default."""
+    _check(lib().qamd_set_mall_share(int(mb64), int(mb)))
 
 
 def stag_sloppy_error(l64, ls, mass):

@@ -461,23 +461,28 @@ __global__ void __launch_bounds__(256) k_stag_dslash(StagArgs<P, PO, PI> A) {
 //     0.243 -> 0.222 s.
 // Multishift CG keeps no share (MallShareOff): its 2 vectors per shift fill
 // the MALL (12 shifts at 32^3x64, FP32 share 100 MB per set: 1.75 -> 1.79 s).
-// QAMD_MALL_MB (FP32 and 16-bit sets) and QAMD_MALL_MB64 (FP64 sets) set
-// the share per set in MB (0: every link load non-temporal).
+// qamd_set_mall_share, or QAMD_MALL_MB64 (FP64 sets) and QAMD_MALL_MB (FP32
+// and 16-bit sets) in the environment, set the share per set in MB (0: every
+// link load non-temporal).
 static std::atomic<int> g_mall_off{0};
 MallShareOff::MallShareOff() { g_mall_off++; }
 MallShareOff::~MallShareOff() { g_mall_off--; }
 
+static int g_mall_mb[2] = {-1, -1};   // FP64 sets, FP32/16-bit sets; -1: default
+static int mall_mb(bool fp64) {
+  int &mb = g_mall_mb[fp64 ? 0 : 1];
+  if (mb < 0) {
+    const char *e = getenv(fp64 ? "QAMD_MALL_MB64" : "QAMD_MALL_MB");
+    mb = e ? atoi(e) : (fp64 ? 52 : 104);
+    if (mb < 0) mb = 0;
+  }
+  return mb;
+}
 template <class S>
 static int mall_groups(const Geom &g) {
   if (g_mall_off.load() > 0) return 0;
-  static int mb = -1;
-  if (mb < 0) {
-    const char *e = getenv(LinkG<S>::G == 1 ? "QAMD_MALL_MB64" : "QAMD_MALL_MB");
-    mb = e ? atoi(e) : (LinkG<S>::G == 1 ? 52 : 104);
-    if (mb < 0) mb = 0;
-  }
   const int64_t per = (int64_t)6 * 16 * g.vh;   // bytes of one 3-hop group (two rows of 16-byte words)
-  int64_t n = (int64_t)mb * 1000000 / per;
+  int64_t n = (int64_t)mall_mb(LinkG<S>::G == 1) * 1000000 / per;
   const int ng = 8 / LinkG<S>::G;               // 3-hop groups of a 16-term set
   if (n > ng) n = ng;
   return n >= 4 ? 4 : n >= 2 ? 2 : (int)n;
@@ -866,6 +871,12 @@ int qamd_stag_normal(const qamd_links *l, qamd_field *Ap, const qamd_field *p, q
     QAMD_HIP(hipStreamSynchronize(stream()));
     *pAp = mass * mass * t[3] + t[0];
   }
+  return 0;
+}
+
+int qamd_set_mall_share(int mb64, int mb) {
+  g_mall_mb[0] = mb64 < 0 ? -1 : mb64;
+  g_mall_mb[1] = mb < 0 ? -1 : mb;
   return 0;
 }
 

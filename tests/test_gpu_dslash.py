@@ -271,3 +271,33 @@ def test_wilson_every_launch_shape(gpu, prec, nrhs):
     for r in range(nrhs):
         ref = O.wilson_dslash(dims, links, 0.0, ys[r], sub="even")
         assert rel_err(got[r], ref, m) < TOL[prec], r
+
+
+@pytest.mark.parametrize("prec", [0, 1, 3])
+def test_mall_share_same_bits(gpu, prec):
+    """The MALL share (how many 3-hop link groups load through the caches
+    instead of non-temporally) changes cache hints only: every share gives
+    the same bits, and they match the oracle."""
+    dims = (16, 16, 16, 16)   # one 3-hop group: 6 x 16 B x 32768 sites = 3.1 MB
+    print("case: MALL share dims=%s prec=%d" % (dims, prec), flush=True)
+    links, y = _stag_case(dims, 23)
+    lat = gpu.Lattice(dims)
+    L = gpu.Links(lat, gpu.STAG, 0, 16).set_lex(links)
+    assert L.reconstructs
+    if prec != 0:
+        L = L.converted(prec)
+    yin = gpu.Field(lat, prec, 3, 1, 1).set_lex(y)
+    out = gpu.Field(lat, prec, 3, 1, 0)
+    outs = []
+    try:
+        for mb in (0, 4, 7, 1000):   # 0, 1, 2 and every 3-hop group
+            gpu.set_mall_share(mb, mb)
+            out.zero()
+            gpu.dslash(L, out, yin)
+            outs.append(out.get_lex())
+    finally:
+        gpu.set_mall_share(-1, -1)
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
+    ref = O.dslash(dims, links, 0.0, y, sub="even")
+    assert rel_err(outs[0], ref, parity_mask(dims, 0)) < TOL[prec]
