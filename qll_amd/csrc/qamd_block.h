@@ -6,12 +6,15 @@
 
 namespace qamd {
 
-template <int P, int PO> struct MArgs {
+// links in precision P, input in PI (P, or FP32 with 16-bit links),
+// output and w in PO
+template <int P, int PO, int PI = P> struct MArgs {
   typedef typename Prec<P>::S S;
   typedef typename Prec<PO>::S SO;
+  typedef typename Prec<PI>::S SI;
   Geom g;
   const S *links;
-  const S *in;
+  const SI *in;
   const float *in_norm;
   SO *out;
   float *out_norm;
@@ -49,9 +52,11 @@ template <int P, int NR> struct BlockStep {
 // one output site cb: for the slots q < A.nr, rhs r = A.rmap[q]:
 // out_r = ar[q]*w_r + b * sum_t L_t in_r(x+d_t) (fields rhs-outer); with
 // RED, red[2q] += |out_r|^2, red[2q+1] += |w_r|^2
-template <int P, int PO, int NT, int NR, bool RED, bool CGX = false>
-__device__ __forceinline__ void block_site(const MArgs<P, PO> &A, int cb, double (&red)[2 * NR]) {
+template <int P, int PO, int NT, int NR, bool RED, bool CGX = false, int PI = P>
+__device__ __forceinline__ void block_site(const MArgs<P, PO, PI> &A, int cb, double (&red)[2 * NR]) {
   static_assert(!CGX || (PO == QAMD_F32 && RED), "fused block-CG step: FP32 vectors, with the |r|^2 sums");
+  static_assert(PI == P || (PI == QAMD_F32 && (P == QAMD_F16 || P == QAMD_Q16)),
+                "input precision: the links' or FP32 with 16-bit links");
   typedef typename Prec<P>::R R;
   typedef typename Prec<P>::C C;
   typedef typename Prec<P>::S S;
@@ -102,6 +107,13 @@ __device__ __forceinline__ void block_site(const MArgs<P, PO> &A, int cb, double
         for (int e = 0; e < 9; e++)
           if (!(lng && rec && e % 3 == 2)) m[j][e] = to_c(q[e][gg], (R)0);
         if (lng && rec) long_row3<C, R>(m[j], (lsb >> (sl - 8)) & 1);
+        if constexpr (PI != P) {   // FP32 input: the term's link scale applied to the link, once for all rhs
+#pragma unroll
+          for (int e = 0; e < 9; e++) {
+            m[j][e].x *= sc[j];
+            m[j][e].y *= sc[j];
+          }
+        }
       }
     }
 #pragma unroll
@@ -111,7 +123,10 @@ __device__ __forceinline__ void block_site(const MArgs<P, PO> &A, int cb, double
 #pragma unroll
         for (int j = 0; j < SPS; j++) {
           C v[3];
-          VecLd<P>::ld(A.in + ri * 3 * vp, A.in_norm ? A.in_norm + ri * vp : nullptr, vp, nb[j], v, sc[j]);
+          if constexpr (PI == P)
+            VecLd<P>::ld(A.in + ri * 3 * vp, A.in_norm ? A.in_norm + ri * vp : nullptr, vp, nb[j], v, sc[j]);
+          else
+            VecLd<PI>::ld(A.in + ri * 3 * vp, nullptr, vp, nb[j], v, (R)1);
 #pragma unroll
           for (int row = 0; row < 3; row++)
 #pragma unroll

@@ -20,14 +20,14 @@ constexpr int MAXB = 16;
 
 // out_r = a*w_r + b * sum_t L_t in_r(x+d_t) for the rhs r = rmap[q], q < nr;
 // fields rhs-outer
-template <int P, int PO, int NT, int NR, bool RED, bool CGX = false>
-__global__ void __launch_bounds__(256) k_stag_block(MArgs<P, PO> A) {
+template <int P, int PO, int NT, int NR, bool RED, bool CGX = false, int PI = P>
+__global__ void __launch_bounds__(256) k_stag_block(MArgs<P, PO, PI> A) {
   const int blk = A.xcd == 2 ? slab_block(blockIdx.x, A.slab) : A.xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
   int cb = blk * 256 + threadIdx.x;
   double red[2 * NR];
 #pragma unroll
   for (int i = 0; i < 2 * NR; i++) red[i] = 0.0;
-  if (cb < A.g.vh) block_site<P, PO, NT, NR, RED, CGX>(A, cb, red);
+  if (cb < A.g.vh) block_site<P, PO, NT, NR, RED, CGX, PI>(A, cb, red);
   // slots of rhs r at 2r, 2r+1 (fused CG step: |r_r|^2 at slot r), indexed by
   // site block, so the sums do not depend on the block order or on which
   // launch computed the rhs
@@ -47,15 +47,15 @@ __global__ void __launch_bounds__(256) k_stag_block(MArgs<P, PO> A) {
   }
 }
 
-template <int P, int PO, int NT, int NR>
-static void blaunch_nr(const MArgs<P, PO> &A, bool red, hipStream_t st) {
+template <int P, int PO, int NT, int NR, int PI>
+static void blaunch_nr(const MArgs<P, PO, PI> &A, bool red, hipStream_t st) {
   int nb = (int)((A.g.vh + 255) / 256);
-  if (red) hipLaunchKernelGGL((k_stag_block<P, PO, NT, NR, true>), dim3(nb), dim3(256), 0, st, A);
-  else hipLaunchKernelGGL((k_stag_block<P, PO, NT, NR, false>), dim3(nb), dim3(256), 0, st, A);
+  if (red) hipLaunchKernelGGL((k_stag_block<P, PO, NT, NR, true, false, PI>), dim3(nb), dim3(256), 0, st, A);
+  else hipLaunchKernelGGL((k_stag_block<P, PO, NT, NR, false, false, PI>), dim3(nb), dim3(256), 0, st, A);
 }
 
-template <int P, int PO, int NT>
-static int blaunch_nt(const MArgs<P, PO> &A, bool red, hipStream_t st) {
+template <int P, int PO, int NT, int PI>
+static int blaunch_nt(const MArgs<P, PO, PI> &A, bool red, hipStream_t st) {
   if (A.nr <= 1) blaunch_nr<P, PO, NT, 1>(A, red, st);
   else if (A.nr <= 2) blaunch_nr<P, PO, NT, 2>(A, red, st);
   else if (A.nr <= 4) blaunch_nr<P, PO, NT, 4>(A, red, st);
@@ -71,39 +71,40 @@ static int blaunch_nt(const MArgs<P, PO> &A, bool red, hipStream_t st) {
 // One block-Dslash application, in launches of k rhs with or without the
 // XCD block order; (k, xcd) is chosen by the autotuner per lattice,
 // precision and block size.
-template <int P, int PO>
+template <int P, int PO, int PI>
 struct BlockRun {
-  MArgs<P, PO> A;
+  MArgs<P, PO, PI> A;
   int n, nt, red, ncand, k[10], xcd[10];
   hipStream_t st;
   static int run(int c, void *ctx) {
     BlockRun &R = *(BlockRun *)ctx;
     for (int r0 = 0; r0 < R.n; r0 += R.k[c]) {
-      MArgs<P, PO> B = R.A;   // slots r0 .. r0+nr-1 of the block's slots
+      MArgs<P, PO, PI> B = R.A;   // slots r0 .. r0+nr-1 of the block's slots
       B.nr = R.n - r0 < R.k[c] ? R.n - r0 : R.k[c];
       for (int q = 0; q < B.nr; q++) {
         B.ar[q] = R.A.ar[r0 + q];
         B.rmap[q] = R.A.rmap[r0 + q];
       }
       B.xcd = R.xcd[c];
-      int rc = (R.nt == 16) ? blaunch_nt<P, PO, 16>(B, R.red, R.st) : blaunch_nt<P, PO, 8>(B, R.red, R.st);
+      int rc = (R.nt == 16) ? blaunch_nt<P, PO, 16, PI>(B, R.red, R.st) : blaunch_nt<P, PO, 8, PI>(B, R.red, R.st);
       if (rc) return -1;
     }
     return hipGetLastError() == hipSuccess ? 0 : -1;
   }
 };
 
-template <int P, int PO = P>
+template <int P, int PO = P, int PI = P>
 static int blaunch(const qamd_links *l, Field &out, const Field &in, const double *ar, const Field *w, double b,
                    bool red, RedScratch rs, hipStream_t st, unsigned act) {
   typedef typename Prec<P>::S S;
   typedef typename Prec<PO>::S SO;
-  BlockRun<P, PO> R;
+  typedef typename Prec<PI>::S SI;
+  BlockRun<P, PO, PI> R;
   memset(&R, 0, sizeof R);
-  MArgs<P, PO> &A = R.A;
+  MArgs<P, PO, PI> &A = R.A;
   A.g = l->lat->g;
   A.links = (const S *)l->data[out.parity];
-  A.in = (const S *)in.data;
+  A.in = (const SI *)in.data;
   A.in_norm = in.norm;
   A.out = (SO *)out.data;
   A.out_norm = out.norm;
@@ -153,10 +154,10 @@ static int blaunch(const qamd_links *l, Field &out, const Field &in, const doubl
   // drop out): they take the default shape rather than tune once per count
   const bool inplace = (w && w->data == out.data) || ns != in.nrhs;
   char key[128];
-  snprintf(key, sizeof key, "stag_block p%d/%d nt%d n%d red%d %dx%dx%dx%d", P, PO, R.nt, R.n, (int)red, A.g.L[0],
-           A.g.L[1], A.g.L[2], A.g.L[3]);
-  int c = inplace ? 0 : tune_pick(key, R.ncand, 0, &BlockRun<P, PO>::run, &R);
-  if (BlockRun<P, PO>::run(c, &R)) {
+  snprintf(key, sizeof key, "stag_block p%d/%d/%d nt%d n%d red%d %dx%dx%dx%d", P, PO, PI, R.nt, R.n, (int)red,
+           A.g.L[0], A.g.L[1], A.g.L[2], A.g.L[3]);
+  int c = inplace ? 0 : tune_pick(key, R.ncand, 0, &BlockRun<P, PO, PI>::run, &R);
+  if (BlockRun<P, PO, PI>::run(c, &R)) {
     set_error("block Dslash: launch failed");
     return -1;
   }
@@ -167,8 +168,12 @@ static int blaunch(const qamd_links *l, Field &out, const Field &in, const doubl
 // in act; with red, 2*nrhs partial sums {|out_r|^2, |w_r|^2} go to rs
 int stag_block_launch_m(const qamd_links *l, Field &out, const Field &in, const double *ar, const Field *w,
                         double b, bool red, RedScratch rs, hipStream_t st, unsigned act) {
-  bool h2s = ((l->prec == QAMD_F16 || l->prec == QAMD_Q16) && out.prec == QAMD_F32);
-  if (l->kind != 0 || in.prec != l->prec || (out.prec != l->prec && !h2s) || (w && w->prec != out.prec)) {
+  const bool half = (l->prec == QAMD_F16 || l->prec == QAMD_Q16);
+  const bool h2s = half && out.prec == QAMD_F32;
+  // FP32 input with 16-bit links (output FP32 or in the links' format)
+  const bool single_in = half && in.prec == QAMD_F32;
+  if (l->kind != 0 || (in.prec != l->prec && !single_in) || (out.prec != l->prec && !h2s) ||
+      (w && w->prec != out.prec)) {
     set_error("block Dslash: precision/kind mismatch");
     return -1;
   }
@@ -182,9 +187,15 @@ int stag_block_launch_m(const qamd_links *l, Field &out, const Field &in, const 
     case QAMD_F64: return blaunch<QAMD_F64>(l, out, in, ar, w, b, red, rs, st, act);
     case QAMD_F32: return blaunch<QAMD_F32>(l, out, in, ar, w, b, red, rs, st, act);
     case QAMD_F16:
+      if (single_in)
+        return h2s ? blaunch<QAMD_F16, QAMD_F32, QAMD_F32>(l, out, in, ar, w, b, red, rs, st, act)
+                   : blaunch<QAMD_F16, QAMD_F16, QAMD_F32>(l, out, in, ar, w, b, red, rs, st, act);
       return h2s ? blaunch<QAMD_F16, QAMD_F32>(l, out, in, ar, w, b, red, rs, st, act)
                  : blaunch<QAMD_F16>(l, out, in, ar, w, b, red, rs, st, act);
     default:
+      if (single_in)
+        return h2s ? blaunch<QAMD_Q16, QAMD_F32, QAMD_F32>(l, out, in, ar, w, b, red, rs, st, act)
+                   : blaunch<QAMD_Q16, QAMD_Q16, QAMD_F32>(l, out, in, ar, w, b, red, rs, st, act);
       return h2s ? blaunch<QAMD_Q16, QAMD_F32>(l, out, in, ar, w, b, red, rs, st, act)
                  : blaunch<QAMD_Q16>(l, out, in, ar, w, b, red, rs, st, act);
   }
@@ -194,15 +205,16 @@ int stag_block_launch_m(const qamd_links *l, Field &out, const Field &in, const 
 // D in_r (FP32, not stored), x_r += alpha[r] p_r, r_r -= alpha[r] Ap_r, and
 // |r_r|^2 partial sums at slot r of rs.  All rhs in one launch in the XCD
 // block order (no tuning: trial launches would repeat the in-place update).
-template <int P>
+template <int P, int PI = P>
 static int bcgx(const qamd_links *l, const Field &in, const Field &p, Field &x, Field &r, const double *m2,
                 const double *alpha, RedScratch rs, hipStream_t st, unsigned act) {
   typedef typename Prec<P>::S S;
-  MArgs<P, QAMD_F32> A;
+  typedef typename Prec<PI>::S SI;
+  MArgs<P, QAMD_F32, PI> A;
   memset(&A, 0, sizeof A);
   A.g = l->lat->g;
   A.links = (const S *)l->data[r.parity];
-  A.in = (const S *)in.data;
+  A.in = (const SI *)in.data;
   A.in_norm = in.norm;
   A.w = (const float2 *)p.data;
   A.b = -1.0;
@@ -223,7 +235,7 @@ static int bcgx(const qamd_links *l, const Field &in, const Field &p, Field &x, 
   A.nr = ns;
   if (ns == 0) return 0;
   const int nb = (int)((A.g.vh + 255) / 256);
-#define CGXL(NR) hipLaunchKernelGGL((k_stag_block<P, QAMD_F32, 16, NR, true, true>), dim3(nb), dim3(256), 0, st, A)
+#define CGXL(NR) hipLaunchKernelGGL((k_stag_block<P, QAMD_F32, 16, NR, true, true, PI>), dim3(nb), dim3(256), 0, st, A)
   if (ns <= 1) CGXL(1);
   else if (ns <= 2) CGXL(2);
   else if (ns <= 4) CGXL(4);
@@ -236,7 +248,9 @@ static int bcgx(const qamd_links *l, const Field &in, const Field &p, Field &x, 
 
 static int stag_block_cg_step(const qamd_links *l, const Field &in, const Field &p, Field &x, Field &r,
                               const double *m2, const double *alpha, RedScratch rs, hipStream_t st, unsigned act) {
-  if (l->kind != 0 || l->nterms != 16 || l->prec == QAMD_F64 || in.prec != l->prec || p.prec != QAMD_F32 ||
+  const bool single_in = l->prec != QAMD_F32 && in.prec == QAMD_F32;   // FP32 D_oe p with 16-bit links
+  if (l->kind != 0 || l->nterms != 16 || l->prec == QAMD_F64 || (in.prec != l->prec && !single_in) ||
+      p.prec != QAMD_F32 ||
       x.prec != QAMD_F32 || r.prec != QAMD_F32 || p.nrhs > MAXB || in.nrhs != p.nrhs || x.nrhs != p.nrhs ||
       r.nrhs != p.nrhs) {
     set_error("block CG step: needs 16-term sloppy links and FP32 p, x, r blocks");
@@ -245,8 +259,12 @@ static int stag_block_cg_step(const qamd_links *l, const Field &in, const Field 
   if (red_reserve((int)((l->lat->g.vh + 255) / 256))) return -1;
   switch (l->prec) {
     case QAMD_F32: return bcgx<QAMD_F32>(l, in, p, x, r, m2, alpha, rs, st, act);
-    case QAMD_F16: return bcgx<QAMD_F16>(l, in, p, x, r, m2, alpha, rs, st, act);
-    default: return bcgx<QAMD_Q16>(l, in, p, x, r, m2, alpha, rs, st, act);
+    case QAMD_F16:
+      return single_in ? bcgx<QAMD_F16, QAMD_F32>(l, in, p, x, r, m2, alpha, rs, st, act)
+                       : bcgx<QAMD_F16>(l, in, p, x, r, m2, alpha, rs, st, act);
+    default:
+      return single_in ? bcgx<QAMD_Q16, QAMD_F32>(l, in, p, x, r, m2, alpha, rs, st, act)
+                       : bcgx<QAMD_Q16>(l, in, p, x, r, m2, alpha, rs, st, act);
   }
 }
 
@@ -390,11 +408,9 @@ template <int P> struct VB {
 };
 
 // p_r = r_r + beta_r p_r (mode 0, active rhs), p_r = r_r (mode 1: rhs
-// flagged in reset; mode 2: every active rhs), the 16-bit Dslash copy, and
-// |p_r|^2 partial sums (slot r), which the next alpha_r needs before the
-// fused step
-template <int SH>
-__global__ void __launch_bounds__(256) k_bcg_pn(Field p, Field r, Field sh, BatchDev *B, RedScratch rs, int mode) {
+// flagged in reset; mode 2: every active rhs), and |p_r|^2 partial sums
+// (slot r), which the next alpha_r needs before the fused step
+__global__ void __launch_bounds__(256) k_bcg_pn(Field p, Field r, BatchDev *B, RedScratch rs, int mode) {
   const int rr = blockIdx.y;
   const bool on = mode == 1 ? B->reset[rr] != 0 : B->active[rr] != 0;
   if (!on) return;   // uniform over the block: its partial-sum slot is not read
@@ -418,7 +434,6 @@ __global__ void __launch_bounds__(256) k_bcg_pn(Field p, Field r, Field sh, Batc
 #pragma unroll
     for (int j = 0; j < 3; j++) red += (double)pv[j].x * pv[j].x + (double)pv[j].y * pv[j].y;
     VB<QAMD_F32>::st(p, rr, i, pv);
-    if constexpr (SH != 0) VB<SH>::st(sh, rr, i, pv);
   }
   rhs_partial(red, rs);
 }
@@ -710,7 +725,6 @@ int stag_cg_block_core(const qamd_links *l64, const qamd_links *ls, qamd_field *
   double t0 = now_s();
   qamd_lattice *lat = l64->lat;
   const int ps = ls->prec;
-  const bool half = (ps == QAMD_F16 || ps == QAMD_Q16);
   const bool verbose = getenv("QAMD_VERBOSE") != nullptr;
   const int nb = (int)((lat->g.vh + 255) / 256);
   const dim3 gb(nb, n);
@@ -722,9 +736,11 @@ int stag_cg_block_core(const qamd_links *l64, const qamd_links *ls, qamd_field *
   qamd_field *r = qamd_field_create(lat, QAMD_F32, 3, n, 0);
   qamd_field *p = qamd_field_create(lat, QAMD_F32, 3, n, 0);
   qamd_field *xs = qamd_field_create(lat, QAMD_F32, 3, n, 0);
+  // D_oe reads the FP32 p directly (16-bit links: the link scale goes onto
+  // the link, no 16-bit copy of p); t = D_oe p is stored in the links'
+  // format.  32^4, 8 rhs, Q16 links: 0.963 s with a Q16 copy of p, 0.926 s
+  // reading FP32 p; an FP32 t as well: 0.98 s.
   qamd_field *t = qamd_field_create(lat, ps, 3, n, 1);
-  qamd_field *ph = half ? qamd_field_create(lat, ps, 3, n, 0) : nullptr;
-  qamd_field *pin = half ? ph : p;
   st.B = (BatchDev *)dev_alloc(sizeof(BatchDev));
   if (hipHostMalloc((void **)&st.Bh, sizeof(BatchDev), hipHostMallocDefault) != hipSuccess) st.Bh = nullptr;
   int rc = -1, nupd = 0, its = 0, nrestart = 0;
@@ -759,11 +775,7 @@ int stag_cg_block_core(const qamd_links *l64, const qamd_links *ls, qamd_field *
   // p update (mode: 0 p = r + beta p, 1 p = r for the reset rhs, 2 p = r for
   // all active rhs) with |p_r|^2 in partial-sum set 3
   auto p_update = [&](int mode) -> int {
-    if (!half) hipLaunchKernelGGL(k_bcg_pn<0>, gb, dim3(256), 0, stream(), p->f, r->f, p->f, st.B, red_set(3), mode);
-    else if (ps == QAMD_F16)
-      hipLaunchKernelGGL(k_bcg_pn<QAMD_F16>, gb, dim3(256), 0, stream(), p->f, r->f, ph->f, st.B, red_set(3), mode);
-    else
-      hipLaunchKernelGGL(k_bcg_pn<QAMD_Q16>, gb, dim3(256), 0, stream(), p->f, r->f, ph->f, st.B, red_set(3), mode);
+    hipLaunchKernelGGL(k_bcg_pn, gb, dim3(256), 0, stream(), p->f, r->f, st.B, red_set(3), mode);
     QAMD_HIP(hipGetLastError());
     return 0;
   };
@@ -775,7 +787,7 @@ int stag_cg_block_core(const qamd_links *l64, const qamd_links *ls, qamd_field *
     return pull();
   };
   do {
-    if (!r64 || !Ax || !t64 || !r || !p || !xs || !t || (half && !ph) || !st.B || !st.Bh) break;
+    if (!r64 || !Ax || !t64 || !r || !p || !xs || !t || !st.B || !st.Bh) break;
     if (red_reserve(nb)) break;
     if (qamd_field_norm2(b, bb)) break;
     if (!x0 && qamd_field_zero(x)) break;
@@ -800,7 +812,7 @@ int stag_cg_block_core(const qamd_links *l64, const qamd_links *ls, qamd_field *
       const unsigned act = act_mask();
       double zero[MAXB] = {0.0};
       for (int k = 0; k < 16 && !err; k++) {
-        err |= stag_block_launch_m(ls, t->f, pin->f, zero, nullptr, 1.0, true, red_set(1), stream(), act);
+        err |= stag_block_launch_m(ls, t->f, p->f, zero, nullptr, 1.0, true, red_set(1), stream(), act);
         fops.alpha(nb, st.B, stream());
         err |= stag_block_cg_step(ls, t->f, p->f, xs->f, r->f, m2, st.B->alpha, red_set(0), stream(), act);
         fops.cg(nb, st.B, stream());
@@ -882,7 +894,6 @@ int stag_cg_block_core(const qamd_links *l64, const qamd_links *ls, qamd_field *
   qamd_field_destroy(p);
   qamd_field_destroy(xs);
   qamd_field_destroy(t);
-  qamd_field_destroy(ph);
   dev_free(st.B);
   if (st.Bh) (void)hipHostFree(st.Bh);
   return rc;

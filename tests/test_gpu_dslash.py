@@ -152,6 +152,30 @@ def test_asqtad_block_dslash(gpu, prec, nrhs):
             assert err < TOL[prec], (p, r, err)
 
 
+@pytest.mark.parametrize("prec,oprec,nrhs", [(3, 3, 8), (3, 1, 5), (2, 2, 4), (2, 1, 8)])
+def test_block_dslash_single_input(gpu, prec, oprec, nrhs):
+    """The block Dslash with 16-bit links and FP32 input (the block CG's
+    D_oe p; the links' term scale applied to the link instead of the vector)
+    gives the oracle's Dslash on every rhs, output in the links' format or
+    FP32."""
+    print("case: block single-in prec=%d out=%d nrhs=%d" % (prec, oprec, nrhs), flush=True)
+    dims = (4, 6, 4, 8)
+    links, _ = _stag_case(dims, 29)
+    ys = np.stack([O.gauss_vector(dims, 80 + r, 100, parity=-1) for r in range(nrhs)])
+    lat = gpu.Lattice(dims)
+    L = gpu.Links(lat, gpu.STAG, 0, 16).set_lex(links).converted(prec)
+    for p in (0, 1):
+        yo = gpu.Field(lat, 1, 3, nrhs, 1 - p).set_lex(ys)
+        out = gpu.Field(lat, oprec, 3, nrhs, p)
+        gpu.dslash(L, out, yo)
+        got = out.get_lex()
+        m = parity_mask(dims, p)
+        for r in range(nrhs):
+            ref = O.dslash(dims, links, 0.0, ys[r], sub="even" if p == 0 else "odd")
+            err = rel_err(got[r], ref, m)
+            assert err < TOL[prec], (p, r, err)
+
+
 @pytest.mark.parametrize("inner", [(1, 1, 1, 1), (2, 1, 1, 1), (1, 2, 2, 1), (2, 2, 2, 1)])
 def test_qll_layout_transfer(gpu, inner):
     """Device import/export of QLL's SIMD layout (lib/layout.c:120-142) for several VLEN."""
