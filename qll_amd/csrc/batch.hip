@@ -34,8 +34,6 @@ struct ProbDev {
   double m2;
   double2 *x, *b, *r64, *Ax;   // even parity, FP64, [rhs][3][vh_pad]
   float2 *r, *p, *xs;          // even parity, FP32
-  void *ph;                    // even, 16-bit copy of p (Dslash input), or null
-  float *ph_n;
   double rsq[BR], rsq_old[BR], t2[BR], p2[BR], beta[BR], maxrr[BR];
   double alpha[BR];            // this iteration's step per rhs (0 for inactive rhs)
   int active[BR], reset[BR];
@@ -52,8 +50,8 @@ __device__ __forceinline__ bool gate_ok(const ProbDev &D, int gate) { return gat
 
 // fused block Dslash over the problems of one plan; CGX: the fused CG step
 // (x_r += alpha_r p_r, r_r -= alpha_r Ap_r, |r_r|^2 in slot r)
-template <int P, int PO, int NT, int NR, bool RED, bool CGX = false>
-__global__ void __launch_bounds__(256) k_bat_block(const MArgs<P, PO> *args, Plan pl, const ProbDev *pd,
+template <int P, int PO, int NT, int NR, bool RED, bool CGX = false, int PI = P>
+__global__ void __launch_bounds__(256) k_bat_block(const MArgs<P, PO, PI> *args, Plan pl, const ProbDev *pd,
                                                    int gate) {
   int cblk = xcd_block(blockIdx.x, gridDim.x);
   int p = pl.blk2prob[cblk];
@@ -64,7 +62,7 @@ __global__ void __launch_bounds__(256) k_bat_block(const MArgs<P, PO> *args, Pla
   double red[2 * NR];
 #pragma unroll
   for (int i = 0; i < 2 * NR; i++) red[i] = 0.0;
-  if (cb < D.g.vh) block_site<P, PO, NT, NR, RED, CGX>(args[p], cb, red);
+  if (cb < D.g.vh) block_site<P, PO, NT, NR, RED, CGX, PI>(args[p], cb, red);
   if constexpr (CGX) {
     double rr[NR];
 #pragma unroll
@@ -157,18 +155,8 @@ __device__ __forceinline__ T *rhs_ptr(T *base, const ProbDev &D, int r) {
   return base + (int64_t)r * 3 * D.g.vh_pad;
 }
 
-template <int SH>
-__device__ __forceinline__ void store_shadow(const ProbDev &D, int r, int i, const float2 (&v)[3]) {
-  if constexpr (SH != 0) {
-    if (!D.ph) return;   // a problem running on FP32 links reads p itself
-    typedef typename Prec<SH>::S S;
-    VecSt<SH>::st((S *)D.ph + (int64_t)r * 3 * D.g.vh_pad, D.ph_n + (int64_t)r * D.g.vh_pad, D.g.vh_pad, i, v);
-  }
-}
-
-// p = r + beta p (active rhs), the 16-bit Dslash copy, and |p_r|^2 partial
-// sums (slot r; 0 for inactive rhs), which the next alpha needs
-template <int SH>
+// p = r + beta p (active rhs) and |p_r|^2 partial sums (slot r; 0 for
+// inactive rhs), which the next alpha needs
 __global__ void __launch_bounds__(256) k_bat_p(ProbDev *pd, Plan all, RedScratch rs) {
   SiteOf s = site_in(all);
   const ProbDev &D = pd[s.p];
@@ -192,15 +180,13 @@ __global__ void __launch_bounds__(256) k_bat_p(ProbDev *pd, Plan all, RedScratch
         red[r] += (double)pv[j].x * pv[j].x + (double)pv[j].y * pv[j].y;
       }
       VecSt<QAMD_F32>::st(rhs_ptr(D.p, D, r), nullptr, vp, s.i, pv);
-      store_shadow<SH>(D, r, s.i, pv);
     }
   }
   grid_partial_at<BR>(red, rs, blockIdx.x);
 }
 
-// restart: p = r (and its copy) for the rhs flagged in reset, with their
-// |p_r|^2 partial sums (the other rhs' slots keep the last p update's)
-template <int SH>
+// restart: p = r for the rhs flagged in reset, with their |p_r|^2 partial
+// sums (the other rhs' slots keep the last p update's)
 __global__ void __launch_bounds__(256) k_bat_reset(ProbDev *pd, Plan all, RedScratch rs) {
   SiteOf s = site_in(all);
   const ProbDev &D = pd[s.p];
@@ -217,7 +203,6 @@ __global__ void __launch_bounds__(256) k_bat_reset(ProbDev *pd, Plan all, RedScr
 #pragma unroll
       for (int j = 0; j < 3; j++) red[r] += (double)rv[j].x * rv[j].x + (double)rv[j].y * rv[j].y;
       VecSt<QAMD_F32>::st(rhs_ptr(D.p, D, r), nullptr, vp, s.i, rv);
-      store_shadow<SH>(D, r, s.i, rv);
     }
   }
   block_sum(red);
@@ -297,16 +282,18 @@ static T *upload(const std::vector<T> &v) {
 }
 
 // host-built kernel arguments for one link set / field set
-template <int P, int PO>
-static MArgs<P, PO> margs(const qamd_links *l, int par, const void *in, const float *in_n, void *out, float *out_n,
-                          const void *w, const float *w_n, double a, double b, int nr, RedScratch rs) {
+template <int P, int PO, int PI = P>
+static MArgs<P, PO, PI> margs(const qamd_links *l, int par, const void *in, const float *in_n, void *out,
+                              float *out_n, const void *w, const float *w_n, double a, double b, int nr,
+                              RedScratch rs) {
   typedef typename Prec<P>::S S;
   typedef typename Prec<PO>::S SO;
-  MArgs<P, PO> A;
+  typedef typename Prec<PI>::S SI;
+  MArgs<P, PO, PI> A;
   memset(&A, 0, sizeof A);
   A.g = l->lat->g;
   A.links = (const S *)l->data[par];
-  A.in = (const S *)in;
+  A.in = (const SI *)in;
   A.in_norm = in_n;
   A.out = (SO *)out;
   A.out_norm = out_n;
@@ -326,7 +313,7 @@ static MArgs<P, PO> margs(const qamd_links *l, int par, const void *in, const fl
 }
 
 struct Work {   // per problem, library-owned work fields
-  qamd_field *r64, *Ax, *t64, *r, *p, *xs, *t, *ph;
+  qamd_field *r64, *Ax, *t64, *r, *p, *xs, *t;
   qamd_links *l32;   // FP32 links when the 16-bit ones are too coarse for its mass
 };
 
@@ -334,8 +321,10 @@ struct Work {   // per problem, library-owned work fields
 template <int PS>
 struct Runner {
   static constexpr bool HALF = (PS == QAMD_F16 || PS == QAMD_Q16);
-  static constexpr int SH = HALF ? PS : 0;
-  MArgs<PS, PS> *s1 = nullptr;          // group 0: the common sloppy links
+  // t = D_oe p reads the FP32 p itself (16-bit links: the term scale goes
+  // onto the link)
+  static constexpr int PI1 = HALF ? QAMD_F32 : PS;
+  MArgs<PS, PS, PI1> *s1 = nullptr;     // group 0: the common sloppy links
   MArgs<PS, QAMD_F32> *s2 = nullptr;
   MArgs<QAMD_F32, QAMD_F32> *f1 = nullptr, *f2 = nullptr;   // group 1: FP32 fallback
   MArgs<QAMD_F64, QAMD_F64> *r1 = nullptr, *r2 = nullptr;
@@ -343,29 +332,29 @@ struct Runner {
   ProbDev *pd = nullptr;
   int nprob = 0;
 
-  template <int P, int PO, int NR, bool RED, bool CGX>
-  static void bl(const MArgs<P, PO> *a, const Plan &pl, const ProbDev *pd, int gate, hipStream_t st) {
+  template <int P, int PO, int NR, bool RED, bool CGX, int PI>
+  static void bl(const MArgs<P, PO, PI> *a, const Plan &pl, const ProbDev *pd, int gate, hipStream_t st) {
     if (pl.nblocks)
-      hipLaunchKernelGGL((k_bat_block<P, PO, 16, NR, RED, CGX>), dim3(pl.nblocks), dim3(256), 0, st, a, pl, pd,
-                         gate);
+      hipLaunchKernelGGL((k_bat_block<P, PO, 16, NR, RED, CGX, PI>), dim3(pl.nblocks), dim3(256), 0, st, a, pl,
+                         pd, gate);
   }
-  template <int P, int PO, bool RED, bool CGX = false>
-  void bl_all(const MArgs<P, PO> *a, int grp, int gate, hipStream_t st) {
-    bl<P, PO, 1, RED, CGX>(a, cls[grp][0], pd, gate, st);
-    bl<P, PO, 2, RED, CGX>(a, cls[grp][1], pd, gate, st);
-    bl<P, PO, 4, RED, CGX>(a, cls[grp][2], pd, gate, st);
-    bl<P, PO, 8, RED, CGX>(a, cls[grp][3], pd, gate, st);
+  template <int P, int PO, bool RED, bool CGX = false, int PI = P>
+  void bl_all(const MArgs<P, PO, PI> *a, int grp, int gate, hipStream_t st) {
+    bl<P, PO, 1, RED, CGX, PI>(a, cls[grp][0], pd, gate, st);
+    bl<P, PO, 2, RED, CGX, PI>(a, cls[grp][1], pd, gate, st);
+    bl<P, PO, 4, RED, CGX, PI>(a, cls[grp][2], pd, gate, st);
+    bl<P, PO, 8, RED, CGX, PI>(a, cls[grp][3], pd, gate, st);
   }
   // t = D_oe p (|t|^2); alpha; the fused step x += alpha p, r -= alpha
   // (m^2 p - D_eo t), |r|^2; beta; p = r + beta p with |p|^2
   void iterate(hipStream_t st) {
-    bl_all<PS, PS, true>(s1, 0, 0, st);
+    bl_all<PS, PS, true, false, PI1>(s1, 0, 0, st);
     if (PS != QAMD_F32) bl_all<QAMD_F32, QAMD_F32, true>(f1, 1, 0, st);
     hipLaunchKernelGGL(k_bat_fin_alpha, dim3(nprob), dim3(256), 0, st, red_set(1), red_set(3), pd);
     bl_all<PS, QAMD_F32, true, true>(s2, 0, 0, st);
     if (PS != QAMD_F32) bl_all<QAMD_F32, QAMD_F32, true, true>(f2, 1, 0, st);
     hipLaunchKernelGGL(k_bat_fin_cg, dim3(nprob), dim3(256), 0, st, red_set(0), pd);
-    hipLaunchKernelGGL((k_bat_p<SH>), dim3(all.nblocks), dim3(256), 0, st, pd, all, red_set(3));
+    hipLaunchKernelGGL(k_bat_p, dim3(all.nblocks), dim3(256), 0, st, pd, all, red_set(3));
   }
   void reliable(hipStream_t st) {
     hipLaunchKernelGGL(k_bat_accum, dim3(all.nblocks), dim3(256), 0, st, pd, all);
@@ -375,7 +364,7 @@ struct Runner {
     hipLaunchKernelGGL(k_bat_fin_true, dim3(nprob), dim3(256), 0, st, red_set(0), pd);
   }
   void reset(hipStream_t st) {
-    hipLaunchKernelGGL((k_bat_reset<SH>), dim3(all.nblocks), dim3(256), 0, st, pd, all, red_set(3));
+    hipLaunchKernelGGL(k_bat_reset, dim3(all.nblocks), dim3(256), 0, st, pd, all, red_set(3));
   }
   void release() {
     dev_free(s1);
@@ -397,7 +386,7 @@ int Runner<PS>::run(const qamd_stag_problem *pr, int n, std::vector<ProbDev> &H,
   const bool verbose = getenv("QAMD_VERBOSE") != nullptr;
   nprob = n;
   // kernel arguments (the scratch was reserved before, so its sets are final)
-  std::vector<MArgs<PS, PS>> a1(n);
+  std::vector<MArgs<PS, PS, PI1>> a1(n);
   std::vector<MArgs<PS, QAMD_F32>> a2(n);
   std::vector<MArgs<QAMD_F32, QAMD_F32>> c1(n), c2(n);
   std::vector<MArgs<QAMD_F64, QAMD_F64>> b1(n), b2(n);
@@ -406,9 +395,8 @@ int Runner<PS>::run(const qamd_stag_problem *pr, int n, std::vector<ProbDev> &H,
     const int nr = H[i].nrhs;
     const double m2 = H[i].m2;
     if (!w.l32) {
-      const qamd_field *pin = HALF ? w.ph : w.p;
-      a1[i] = margs<PS, PS>(pr[i].ls, 1, pin->f.data, pin->f.norm, w.t->f.data, w.t->f.norm, nullptr, nullptr, 0.0,
-                            1.0, nr, red_set(1));
+      a1[i] = margs<PS, PS, PI1>(pr[i].ls, 1, w.p->f.data, nullptr, w.t->f.data, w.t->f.norm, nullptr, nullptr,
+                                 0.0, 1.0, nr, red_set(1));
       a2[i] = margs<PS, QAMD_F32>(pr[i].ls, 0, w.t->f.data, w.t->f.norm, nullptr, nullptr, w.p->f.data,
                                   nullptr, m2, -1.0, nr, red_set(0));
       a2[i].cgx = (float2 *)w.xs->f.data;
@@ -681,7 +669,6 @@ int qamd_stag_cg_mixed_batch(const qamd_stag_problem *pr, int n, double rsq, int
       qamd_field_destroy(w.p);
       qamd_field_destroy(w.xs);
       qamd_field_destroy(w.t);
-      qamd_field_destroy(w.ph);
       qamd_links_destroy(w.l32);
     }
   };
@@ -741,8 +728,7 @@ int qamd_stag_cg_mixed_batch(const qamd_stag_problem *pr, int n, double rsq, int
       w.p = qamd_field_create(lat, QAMD_F32, 3, nr, 0);
       w.xs = qamd_field_create(lat, QAMD_F32, 3, nr, 0);
       w.t = qamd_field_create(lat, pt, 3, nr, 1);
-      w.ph = (pt == QAMD_F32) ? nullptr : qamd_field_create(lat, ps, 3, nr, 0);
-      ok = w.r64 && w.Ax && w.t64 && w.r && w.p && w.xs && w.t && (pt == QAMD_F32 || w.ph);
+      ok = w.r64 && w.Ax && w.t64 && w.r && w.p && w.xs && w.t;
       ok = ok && !qamd_field_zero(pr[i].x) && !qamd_field_zero(w.xs) && !qamd_field_zero(w.p);
       if (!ok) break;
       ProbDev &D = H[i];
@@ -753,8 +739,6 @@ int qamd_stag_cg_mixed_batch(const qamd_stag_problem *pr, int n, double rsq, int
       D.r = (float2 *)w.r->f.data;
       D.p = (float2 *)w.p->f.data;
       D.xs = (float2 *)w.xs->f.data;
-      D.ph = w.ph ? w.ph->f.data : nullptr;
-      D.ph_n = w.ph ? w.ph->f.norm : nullptr;
     }
     if (!ok) break;
     pd = upload(H);
