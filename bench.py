@@ -588,6 +588,21 @@ def main():
                      "GB/s": round(byts / (ms * 1e-3) / 1e9, 1),
                      "hbm_frac": round(byts / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
         print("dslash %s: %s" % (key, prof[key]), file=sys.stderr, flush=True)
+    # the block CG's D_oe p with 16-bit links: FP32 input, output in the
+    # links' format (bytes: links + FP32 in + 16-bit out)
+    if 3 in precs:
+        yin = Q.Field(lat, Q.F32, 3, nbk, Q.ODD).gaussian(5, 100)
+        out = Q.Field(lat, Q.Q16, 3, nbk, Q.EVEN)
+        Q.time_dslash(LQ, out, yin, args.warmup)
+        ms = Q.time_dslash(LQ, out, yin, max(1, args.steps // 4))
+        bps = link_bytes(LQ, 3) + nbk * (VEC_BYTES[1] + VEC_BYTES[3])
+        key = "q16_f32in_block%d" % nbk
+        prof[key] = {"ms_per_call": round(ms, 5), "ms_per_rhs": round(ms / nbk, 5), "bytes_per_site": bps,
+                     "GFLOP/s": round(nbk * FLOPS_ASQTAD * vh / (ms * 1e-3) / 1e9, 1),
+                     "GB/s": round(bps * vh / (ms * 1e-3) / 1e9, 1),
+                     "hbm_frac": round(bps * vh / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        print("dslash %s: %s" % (key, prof[key]), file=sys.stderr, flush=True)
+        del yin, out
 
     # Wilson EO Dslash: FP64 one rhs, and 16 rhs sharing every link load
     # (algorithmic bytes per site: 8 links + nrhs x (input + output spinor);

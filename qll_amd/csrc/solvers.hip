@@ -434,8 +434,7 @@ static int check_vec(const qamd_field *f, int prec, int parity, const char *what
 }
 
 // Ap = m^2 p - D_eo D_oe p with the reductions of dslashp2
-// (lib/stagDslash.c:546-575): |t|^2 -> nrm[0], |p|^2 -> nrm[3].  pin is the
-// Dslash input copy of p (an FP16 shadow for FP16 links), pw the full p.
+// (lib/stagDslash.c:546-575): |t|^2 -> nrm[0], |p|^2 -> nrm[3].
 struct FinNormal {
   SolverDev *S;
   __device__ void operator()(const double *t) const {
@@ -443,11 +442,11 @@ struct FinNormal {
   }
 };
 
-static int apply_normal(const qamd_links *l, SolverDev *S, qamd_field *Ap, qamd_field *pin, qamd_field *pw,
-                        qamd_field *t, double m2, const int *skip) {
+static int apply_normal(const qamd_links *l, SolverDev *S, qamd_field *Ap, qamd_field *p, qamd_field *t, double m2,
+                        const int *skip) {
   // partial sums of |t|^2 in set 1 and of |Ap|^2, |p|^2 in set 2, one finisher
-  if (stag_dslash_launch(l, t->f, pin->f, 0.0, nullptr, 1.0, nullptr, stream(), skip, 1, false)) return -1;
-  if (stag_dslash_launch(l, Ap->f, t->f, m2, &pw->f, -1.0, nullptr, stream(), skip, 2, false)) return -1;
+  if (stag_dslash_launch(l, t->f, p->f, 0.0, nullptr, 1.0, nullptr, stream(), skip, 1, false)) return -1;
+  if (stag_dslash_launch(l, Ap->f, t->f, m2, &p->f, -1.0, nullptr, stream(), skip, 2, false)) return -1;
   int nb = (int)((l->lat->g.vh + 255) / 256);
   hipLaunchKernelGGL((k_finish2<2, 2, FinNormal>), dim3(1), dim3(FT), 0, stream(), red_set(1), red_set(2), nb,
                      FinNormal{S}, skip);
@@ -571,7 +570,7 @@ int qamd_stag_cg(const qamd_links *l, qamd_field *x, const qamd_field *b, double
     while (!Sh->done) {
       for (int k = 0; k < BATCH && !err; k++) {
         err |= cg_p(lat, p, r, st.S);
-        err |= apply_normal(l, st.S, Ap, p, p, t, mass * mass, &st.S->done);
+        err |= apply_normal(l, st.S, Ap, p, t, mass * mass, &st.S->done);
         err |= cg_xr(lat, x, p, r, Ap, st.S);
       }
       if (err || pull_state(st)) {
@@ -666,7 +665,7 @@ static int cg_mixed_core(const qamd_links *l64, const qamd_links *ls, qamd_field
       if (x_zero) {
         if (qamd_field_zero(Ax)) return -1;
         x_zero = false;
-      } else if (apply_normal(l64, st.S, Ax, x, x, t64, m2, nullptr)) {
+      } else if (apply_normal(l64, st.S, Ax, x, t64, m2, nullptr)) {
         return -1;
       }
       const RedScratch rr = rset[kit & 1];
@@ -949,7 +948,7 @@ static int cgms_core(const qamd_links *l, qamd_field *const x[], const qamd_fiel
     }
     while (!Sh->done && !err) {
       for (int k = 0; k < BATCH && !err; k++) {
-        err |= apply_normal(l, st.S, Ap, ps[0], ps[0], t, mass * mass, &st.S->done);
+        err |= apply_normal(l, st.S, Ap, ps[0], t, mass * mass, &st.S->done);
         RedScratch rs0 = red_set(0);
         int nb = (int)grid1(lat).x;
         switch (l->prec) {
