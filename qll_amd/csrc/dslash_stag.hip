@@ -515,25 +515,16 @@ static void launch_t(const StagArgs<P, PO, PI> &A, bool red, hipStream_t st) {
       hipLaunchKernelGGL((k_stag_dslash<P, PO, NT, false, true, 0, PI, 0, RG>), dim3(nb), dim3(256), 0, st, A); \
   } while (0)
       constexpr int NG = 8 / LinkG<typename Prec<P>::S>::G;   // 3-hop groups
-      if constexpr (NG == 8) {   // FP64
-        switch (mall_groups<typename Prec<P>::S>(A.g)) {
-          case 4: QAMD_LRG(4); break;
-          case 2: QAMD_LRG(2); break;
-          case 1: QAMD_LRG(1); break;
-          default: QAMD_LRG(0);
-        }
-      } else {
-        switch (mall_groups<typename Prec<P>::S>(A.g)) {
-          case 4:
-            if constexpr (NG >= 4) {
-              QAMD_LRG(4);
-              break;
-            }
-            [[fallthrough]];
-          case 2: QAMD_LRG(2); break;
-          case 1: QAMD_LRG(1); break;
-          default: QAMD_LRG(0);
-        }
+      switch (mall_groups<typename Prec<P>::S>(A.g)) {          // at most NG
+        case 4:
+          if constexpr (NG >= 4) {
+            QAMD_LRG(4);
+            break;
+          }
+          [[fallthrough]];
+        case 2: QAMD_LRG(2); break;
+        case 1: QAMD_LRG(1); break;
+        default: QAMD_LRG(0);
       }
 #undef QAMD_LRG
       return;
