@@ -261,9 +261,8 @@ __device__ __forceinline__ void fb_load_slot(const typename Prec<P>::S *base, in
 //   acc = sum_mu U_mu(x) in(x+mu) - U_mu(x-mu)^+ in(x-mu)
 //       + sum_mu L_mu(x) in(x+3mu) - L_mu(x-3mu)^+ in(x-3mu)
 // The forward links are streamed from this site's block; a backward link is
-// read at the neighbour, through the caches (the neighbour's other slots
-// serve the sites on its other sides).  REC: long third rows rebuilt from
-// the first two.
+// read at the neighbour (through the caches for FP32 and 16-bit sets, see
+// below).  REC: long third rows rebuilt from the first two.
 template <int P, int PO, int PI, int FBT, bool REC>
 __device__ __forceinline__ void stag_acc_fb(const StagArgs<P, PO, PI> &A, const Site &s, int cb, int64_t vp,
                                             typename Prec<P>::C (&acc)[3]) {
@@ -293,7 +292,11 @@ __device__ __forceinline__ void stag_acc_fb(const StagArgs<P, PO, PI> &A, const 
     const bool lng = REC && sl >= 4;
     const int nb = nbr_cb(A.g, s, mu, -hop);
     C m[9];
-    fb_load_slot<P, false>(link_base<S, 8>(A.links_o, nb), sl, lng, m);
+    // backward links through the caches for FP32 and 16-bit sets (the set
+    // stays in the MALL from call to call: 32^4, non-temporal instead, FP32
+    // 0.085 -> 0.100 ms, Q16 0.050 -> 0.059 ms), non-temporal for FP64 sets,
+    // which do not fit it (0.188-0.195 -> 0.180-0.188 ms)
+    fb_load_slot<P, P == QAMD_F64>(link_base<S, 8>(A.links_o, nb), sl, lng, m);
     C v[3];
     stag_vin(A, vp, nb, sl, v);
     if (lng) long_row3<C, R>(m, (A.lsign_o[nb] >> (sl - 4)) & 1);
