@@ -41,6 +41,12 @@ typedef struct qamd_links qamd_links;       /* doublestored Dslash links   */
 const char *qamd_last_error(void);
 int qamd_init(void);
 int qamd_sync(void);
+/* Device blocks of 64 MB or more that the library frees (work fields of the
+ * solvers, fields the caller destroys) are kept, up to 48 GB, and handed out
+ * again for allocations of the same size: repeated solves then skip
+ * hipMalloc, which has taken up to ~1 s for multi-GB blocks.  This frees
+ * them (no reference counterpart; a failed allocation does it too). */
+int qamd_release_cache(void);
 int qamd_device_name(char *buf, int len);
 void *qamd_malloc(size_t bytes);
 void qamd_free(void *p);

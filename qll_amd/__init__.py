@@ -99,6 +99,7 @@ def lib():
                                              C.POINTER(SolveInfo)]),
             "qamd_set_tuning": (C.c_int, [C.c_int]),
             "qamd_set_mall_share": (C.c_int, [C.c_int, C.c_int]),
+            "qamd_release_cache": (C.c_int, []),
             "qamd_stag_sloppy_error": (C.c_int, [_vp, _vp, C.c_double, C.POINTER(C.c_double)]),
             "qamd_stag_cg_mixed_batch": (C.c_int, [C.c_void_p, C.c_int, C.c_double, C.c_int, C.c_double, C.c_int,
                                                    C.POINTER(SolveInfo)]),
@@ -374,6 +375,11 @@ def set_tuning(mode):
     """Launch-shape autotuning: True/1 on (default), False/0 off, k >= 2 forces
     candidate k - 2 (tests)."""
     _check(lib().qamd_set_tuning(int(mode)))
+
+
+def release_cache():
+    """Free the device blocks the library keeps for reuse."""
+    _check(lib().qamd_release_cache())
 
 
 def set_mall_share(mb64=-1, mb=-1):

@@ -8,7 +8,9 @@
 #include <string.h>
 #include <stdlib.h>
 #include <map>
+#include <mutex>
 #include <string>
+#include <unordered_map>
 
 namespace qamd {
 
@@ -101,20 +103,71 @@ bool slab_split(SlabOrder &s, int ny, int nz) {
   return true;
 }
 
+// Device blocks of 64 MB and more are kept for reuse when freed, up to 48 GB
+// in all: once device memory has been cycled through, a hipMalloc of a
+// multi-GB work field has taken 0.6-0.9 s on MI355X, and the solvers
+// allocate their work fields per call (the 16-rhs Wilson CGNE at 32^3x64
+// about 25 GB per solve: 0.23 s solves took 0.4-1.0 s).  A cached block is
+// handed out again only for an allocation of exactly its size.  Freeing one
+// first waits for the library stream, as hipFree did, so no queued kernel
+// still uses it when it is handed out again.  A failed hipMalloc, and
+// qamd_release_cache, free the cached blocks.
+constexpr size_t CACHE_MIN = (size_t)64 << 20;
+constexpr size_t CACHE_MAX = (size_t)48 << 30;
+static std::mutex g_cache_mu;
+static std::unordered_map<void *, size_t> g_big;   // live blocks of CACHE_MIN or more
+static std::multimap<size_t, void *> g_cache;      // freed blocks by size
+static size_t g_cache_bytes = 0;
+
+static void release_cache_locked() {
+  for (auto &kv : g_cache) (void)hipFree(kv.second);
+  g_cache.clear();
+  g_cache_bytes = 0;
+}
+
 void *dev_alloc(size_t bytes) {
   if (ensure_init()) return nullptr;
   void *p = nullptr;
   if (bytes == 0) bytes = 256;
+  std::lock_guard<std::mutex> lk(g_cache_mu);
+  if (bytes >= CACHE_MIN) {
+    auto it = g_cache.find(bytes);
+    if (it != g_cache.end()) {
+      p = it->second;
+      g_cache.erase(it);
+      g_cache_bytes -= bytes;
+      g_big[p] = bytes;
+      return p;
+    }
+  }
   hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess && !g_cache.empty()) {   // make room and try once more
+    (void)hipGetLastError();
+    release_cache_locked();
+    e = hipMalloc(&p, bytes);
+  }
   if (e != hipSuccess) {
     set_error("hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
     return nullptr;
   }
+  if (bytes >= CACHE_MIN) g_big[p] = bytes;
   return p;
 }
 
 void dev_free(void *p) {
-  if (p) (void)hipFree(p);
+  if (!p) return;
+  std::lock_guard<std::mutex> lk(g_cache_mu);
+  auto it = g_big.find(p);
+  if (it != g_big.end()) {
+    const size_t bytes = it->second;
+    g_big.erase(it);
+    if (g_cache_bytes + bytes <= CACHE_MAX && hipStreamSynchronize(g_stream) == hipSuccess) {
+      g_cache.emplace(bytes, p);
+      g_cache_bytes += bytes;
+      return;
+    }
+  }
+  (void)hipFree(p);
 }
 
 RedScratch red_set(int set) {
@@ -167,6 +220,13 @@ extern "C" {
 const char *qamd_last_error(void) { return g_err; }
 
 int qamd_init(void) { return ensure_init(); }
+
+int qamd_release_cache(void) {
+  std::lock_guard<std::mutex> lk(g_cache_mu);
+  if (g_stream) (void)hipStreamSynchronize(g_stream);
+  release_cache_locked();
+  return 0;
+}
 
 int qamd_sync(void) {
   if (ensure_init()) return -1;
