@@ -41,7 +41,7 @@ typedef struct qamd_links qamd_links;       /* doublestored Dslash links   */
 const char *qamd_last_error(void);
 int qamd_init(void);
 int qamd_sync(void);
-/* Device blocks of 64 MB or more that the library frees (work fields of the
+/* Device blocks of 8 MB or more that the library frees (work fields of the
  * solvers, fields the caller destroys) are kept, up to 48 GB, and handed out
  * again for allocations of the same size: repeated solves then skip
  * hipMalloc, which has taken up to ~1 s for multi-GB blocks.  This frees

@@ -103,7 +103,7 @@ bool slab_split(SlabOrder &s, int ny, int nz) {
   return true;
 }
 
-// Device blocks of 64 MB and more are kept for reuse when freed, up to 48 GB
+// Device blocks of 8 MB and more are kept for reuse when freed, up to 48 GB
 // in all: once device memory has been cycled through, a hipMalloc of a
 // multi-GB work field has taken 0.6-0.9 s on MI355X, and the solvers
 // allocate their work fields per call (the 16-rhs Wilson CGNE at 32^3x64
@@ -111,8 +111,9 @@ bool slab_split(SlabOrder &s, int ny, int nz) {
 // handed out again only for an allocation of exactly its size.  Freeing one
 // first waits for the library stream, as hipFree did, so no queued kernel
 // still uses it when it is handed out again.  A failed hipMalloc, and
-// qamd_release_cache, free the cached blocks.
-constexpr size_t CACHE_MIN = (size_t)64 << 20;
+// qamd_release_cache, free the cached blocks.  (The 8 MB floor takes in the
+// single-rhs solvers' 12-25 MB vectors at 32^4: mixed Q16 CG ~1 % faster.)
+constexpr size_t CACHE_MIN = (size_t)8 << 20;
 constexpr size_t CACHE_MAX = (size_t)48 << 30;
 static std::mutex g_cache_mu;
 static std::unordered_map<void *, size_t> g_big;   // live blocks of CACHE_MIN or more

@@ -1,4 +1,4 @@
-"""GPU: reuse of freed device blocks (64 MB and more) and their release."""
+"""GPU: reuse of freed device blocks (8 MB and more) and their release."""
 import pytest
 
 pytestmark = pytest.mark.gpu
