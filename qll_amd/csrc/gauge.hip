@@ -212,17 +212,23 @@ __device__ void fat_staple_acc(M3 &A, double a, const GView &U, const GView &X, 
   madd_nn(A, a, t2, U.load(nu, U.shifted(e, mu, 1)));
 }
 
+// Outputs are stored non-temporally: a pass writes 1-3 KB per site, which
+// would otherwise take the die's L2 from the gauge field and staples the
+// neighbouring sites are still to read.
+__device__ __forceinline__ void nt_store2(double2 *p, const double2 &v) {
+  __builtin_nontemporal_store(dvec2_t{v.x, v.y}, reinterpret_cast<dvec2_t *>(p));
+}
 __device__ __forceinline__ void fat_store(double2 *base, const Geom &g, int f, int par, int cb, const M3 &m) {
   double2 *p = base + ((int64_t)(f * 2 + par) * 9) * g.vh_pad + cb;
 #pragma unroll
-  for (int e = 0; e < 9; e++) p[e * g.vh_pad] = m.e[e];
+  for (int e = 0; e < 9; e++) nt_store2(p + e * g.vh_pad, m.e[e]);
 }
 __device__ __forceinline__ void fat_accum(double2 *base, const Geom &g, int f, int par, int cb, const M3 &m) {
   double2 *p = base + ((int64_t)(f * 2 + par) * 9) * g.vh_pad + cb;
 #pragma unroll
   for (int e = 0; e < 9; e++) {
-    double2 v = p[e * g.vh_pad];
-    p[e * g.vh_pad] = make_double2(v.x + m.e[e].x, v.y + m.e[e].y);
+    const dvec2_t v = __builtin_nontemporal_load(reinterpret_cast<const dvec2_t *>(p + e * g.vh_pad));
+    nt_store2(p + e * g.vh_pad, make_double2(v.x + m.e[e].x, v.y + m.e[e].y));
   }
 }
 
@@ -653,9 +659,11 @@ int qamd_fat7(qamd_gauge *fl, qamd_gauge *ll, const double coef[6], const qamd_g
     // passes 2 and 3 read 12-24 staple fields per site besides U, so their
     // t-window outgrows a die's L2 first: they sweep each die's piece in
     // sub-pieces of at most 1024 (pass 2) and 2048 (pass 3) sites per t-slice
-    // (32^4: pieces of 4096 split 4 and 2 ways in y; fetched 8.1 -> 7.5 M KiB
-    // per smearing, 4 % faster).  QAMD_FAT7_SUB=ny1,nz1,ny2,nz2,ny3,nz3 sets
-    // the splits of the three passes instead.
+    // (32^4: pieces of 4096 split 4 and 2 ways in y, 2-4 % faster per pass
+    // than whole pieces; pass 2 split 2 ways runs within 1 % of 4 ways but
+    // fetches 8 % more; pass 1 is fastest whole).
+    // QAMD_FAT7_SUB=ny1,nz1,ny2,nz2,ny3,nz3 sets the splits of the three
+    // passes instead.
     SlabOrder sp[3] = {so, so, so};
     if (so.P > 0) {
       const char *fs = getenv("QAMD_FAT7_SUB");
