@@ -35,9 +35,10 @@ template <int P> struct WArgs {
   double *red_out;
   const int *skip;
   int dag;     // 1: the hermitian conjugate g5 H g5 (every projector sign flipped)
-  int xcd;     // XCD-aware block order (each die takes one contiguous range of sites)
+  int xcd;     // block order: 1 XCD-aware (each die takes one contiguous range of sites), 2 slab sweep
   int tiled;   // sites in the tiled order of tl (else checkerboard order)
   Tiling tl;
+  SlabOrder slab;   // xcd 2: the runs of 256 / nrhs sites in slab-sweep order
   const uint8_t *lsign;   // set: third rows rebuilt (determinant sign bit per site and term)
   Field rv;               // EPI 2: the residual updated in place, r -= alpha out
   const double *alpha;    // EPI 2: per-rhs step (device)
@@ -139,7 +140,7 @@ __global__ void __launch_bounds__(256) k_wilson(WArgs<P> A) {
   const Field fout = A.out;
   const Geom g = A.g;
   const int nr = fin.nrhs;
-  const int blk = A.xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+  const int blk = A.xcd == 2 ? slab_block(blockIdx.x, A.slab) : A.xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
   int64_t gid = (int64_t)blk * blockDim.x + threadIdx.x;
   const int li = (int)(gid / nr), r = (int)(gid - (int64_t)li * nr);
   constexpr int G = LinkG<S>::G;
@@ -374,6 +375,7 @@ constexpr WShape WSHAPES[] = {
     {1, 0, {4, 3, 3, 3}}, {1, 0, {3, 3, 2, 2}}, {1, 0, {2, 2, 2, 4}}, {1, 0, {5, 2, 2, 2}},
     {0, 1, {0, 0, 0, 0}}, {1, 1, {0, 0, 0, 0}}, {1, 1, {3, 3, 2, 2}}, {1, 1, {5, 2, 2, 2}},
     {1, 1, {3, 3, 3, 3}}, {1, 1, {2, 2, 2, 2}}, {1, 1, {4, 2, 2, 2}}, {1, 1, {4, 3, 3, 2}},
+    {2, 0, {0, 0, 0, 0}}, {2, 1, {0, 0, 0, 0}},
 };
 constexpr int NWSHAPE = sizeof(WSHAPES) / sizeof(WSHAPES[0]);
 // site tiling (log2 extents 32 x 4 x 4 x 4) of every reducing launch on a lattice that admits it
@@ -403,6 +405,7 @@ template <int P> struct WRun {
     WArgs<P> B = R.A;
     const WShape &sh = WSHAPES[R.shape[c]];
     B.xcd = sh.xcd;
+    if (sh.xcd == 2 && !slab_order(B.g, B.slab, 256 / B.in.nrhs)) return -1;
     B.tiled = sh.tile[0] ? 1 : 0;
     if (B.tiled && !B.tl.init(B.g.L, sh.tile)) return -1;
     if (R.epi == 1 || R.epi == 2) {   // per-rhs sums into B.rs, finished by the caller
@@ -480,6 +483,10 @@ static int wlaunch(const qamd_links *l, Field &out, const Field &in, double a, c
   for (int k = 0; k < NWSHAPE; k++) {
     const WShape &sh = WSHAPES[k];
     if (sh.lds && in.nrhs < 8) continue;
+    if (sh.xcd == 2) {   // blocks are runs of whole sites, in planes of whole runs
+      SlabOrder so;
+      if (256 % in.nrhs != 0 || !slab_order(A.g, so, 256 / in.nrhs)) continue;
+    }
     if (rsum) {
       bool fixed = true;
       for (int i = 0; i < 4; i++) fixed = fixed && sh.tile[i] == WRTILE[i];

@@ -246,7 +246,7 @@ def test_wilson_reducing_launches_fixed_tiling(gpu, prec, nrhs):
         L = L.converted(prec)
     yin = gpu.Field(lat, prec, 12, nrhs, 1).set_lex(ys)
     res, norms = [], []
-    for mode in [0] + list(range(2, 18)) + [1]:   # default, each candidate, tuned
+    for mode in [0] + list(range(2, 20)) + [1]:   # default, each candidate, tuned
         gpu.set_tuning(mode)
         out = gpu.Field(lat, prec, 12, nrhs, 0)
         norms.append(gpu.dslash(L, out, yin, b=-1.0, want_norm=True))
@@ -265,13 +265,14 @@ def test_wilson_reducing_launches_fixed_tiling(gpu, prec, nrhs):
     assert abs(norms[0] - nref) / nref < 4 * TOL[prec]
 
 
-@pytest.mark.parametrize("prec,nrhs", [(0, 1), (1, 16), (3, 8), (0, 8)])
-def test_wilson_every_launch_shape(gpu, prec, nrhs):
+@pytest.mark.parametrize("prec,nrhs,dims", [(0, 1, (8, 8, 8, 8)), (1, 16, (8, 8, 8, 8)), (3, 8, (8, 8, 8, 8)),
+                                             (0, 8, (8, 8, 8, 8)), (0, 1, (32, 16, 8, 8))])
+def test_wilson_every_launch_shape(gpu, prec, nrhs, dims):
     """Every Wilson launch shape the autotuner can pick (block order, links
-    staged in LDS, tiled site orders) gives the same output bit for bit, and
-    the oracle's Dslash; with the |out|^2 reduction too."""
-    print("case: wilson shapes prec=%d nrhs=%d" % (prec, nrhs), flush=True)
-    dims = (8, 8, 8, 8)
+    staged in LDS, tiled site orders, slab sweep) gives the same output bit
+    for bit, and the oracle's Dslash; with the |out|^2 reduction too.
+    32x16x8x8 admits the slab sweep for one rhs (x-y planes of 256 sites)."""
+    print("case: wilson shapes prec=%d nrhs=%d dims=%s" % (prec, nrhs, dims), flush=True)
     links = O.wilson_doublestore(dims, O.random_su3_field(dims, 33))
     ys = np.stack([O.gauss_vector(dims, 70 + r, 300, parity=-1, nreal=24) for r in range(nrhs)])
     lat = gpu.Lattice(dims)
@@ -280,7 +281,7 @@ def test_wilson_every_launch_shape(gpu, prec, nrhs):
         L = L.converted(prec)
     yin = gpu.Field(lat, prec, 12, nrhs, 1).set_lex(ys)
     res, norms = [], []
-    for mode in [0] + list(range(2, 18)) + [1]:   # default, each candidate, tuned
+    for mode in [0] + list(range(2, 20)) + [1]:   # default, each candidate, tuned
         gpu.set_tuning(mode)
         out = gpu.Field(lat, prec, 12, nrhs, 0)
         gpu.dslash(L, out, yin, b=-1.0)

@@ -166,7 +166,7 @@ def test_wilson_rebuilt_and_full(gpu, prec, nrhs):
         assert L.reconstructs == rec
         yin = gpu.Field(lat, prec, 12, nrhs, 1).set_lex(ys)
         res = []
-        for mode in [0] + list(range(2, 18)) + [1]:
+        for mode in [0] + list(range(2, 20)) + [1]:
             gpu.set_tuning(mode)
             out = gpu.Field(lat, prec, 12, nrhs, 0)
             gpu.dslash(L, out, yin, b=-1.0)
